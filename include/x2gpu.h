@@ -1,0 +1,179 @@
+/*
+ * x2gpu -- MI355X (gfx950) relaxation engine for finite-difference diatomic
+ * Hartree-Fock (the x2dhf method).  Plain C ABI: every entry point takes plain
+ * pointers and sizes, returns an int status (X2G_OK = 0) and never throws.
+ * All array pointers named *_d are device pointers in HBM.
+ *
+ * Storage conventions (identical to the reference):
+ *   - a function f(nu,mu) on an nni x nmu grid is column-major, nu fastest:
+ *     f[(mu-1)*nni + nu-1]  (unpadded, "nni*nmu" arrays);
+ *   - a relaxation work array is padded to (nni+8) x (nmu+8) with
+ *     W(nu+4,mu+4) = f(nu,mu) (inout.F90:30-36).
+ *
+ * Reference interfaces replaced (paths relative to the x2dhf source tree):
+ *   x2g_mcsor          <- mcsor(isym,fun,lhs,rhs,b,d,indx1,..)   src/sormcsor.F90:143
+ *                         and mcsor_pthread_(isym,nthreads,wk2,lhs,rhs)
+ *                                                               src/c/mcsor_pthread.c:32
+ *   x2g_mcsor_host     <- the same, with host arrays (copies in and out)
+ *   x2g_putin/putout   <- putin/putout, putinc_/putoutc_         src/inout.F90:23,165,195,266
+ *   x2g_relax_batch    <- coulExch_pthread_() + relax_single_pot_pthread_
+ *                                                               src/c/coulExch_pthread.c:19
+ *                         and the macro-iteration loops of orbSOR/orbMCSOR,
+ *                         coulExchSOR/coulExchMCSOR            src/relaxOrbs.F90:92-139,
+ *                                                               src/relaxPots.F90:97-104
+ *   x2g_lincomb        <- the daxpy/prod chains of fockHF       src/fock.F90:88-184
+ *   x2g_wdot           <- ddot(wgt,.) quadratures of norm/ortho/EaHF/coulij/exchij
+ *                         src/normOrtho.F90:41-50, src/integrals.F90:11-30,372-399
+ *   x2g_oneel          <- oneelij / oneelii                    src/integrals.F90:406,37
+ *   x2g_moments        <- coulMom / exchMom                    src/scfUtils.F90:490,615
+ *   x2g_pot_tail       <- coulAsympt / exchAsympt (vcoul/vexch) src/scfUtils.F90:275,310
+ *   x2g_setci          <- setCiOrb                             src/scfUtils.F90:380
+ */
+#ifndef X2GPU_H
+#define X2GPU_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define X2G_OK 0
+#define X2G_EINVAL (-1)  /* bad argument (size, pointer, unsupported grid) */
+#define X2G_EHIP (-2)    /* HIP runtime error */
+#define X2G_ENOMEM (-3)  /* device allocation failed */
+
+typedef struct x2g_ctx x2g_ctx;   /* device + stream + workspace */
+typedef struct x2g_grid x2g_grid; /* one (nni,nmu) grid and its coefficient arrays */
+
+/* ---- library / context ------------------------------------------------- */
+int x2g_version(void);
+const char *x2g_last_error(void);
+/* Create a context on HIP device `device` with its own non-blocking stream. */
+int x2g_ctx_create(int device, x2g_ctx **out);
+int x2g_ctx_destroy(x2g_ctx *ctx);
+int x2g_ctx_sync(x2g_ctx *ctx);
+/* Return the context's hipStream_t (as void*). */
+void *x2g_ctx_stream(x2g_ctx *ctx);
+
+/* ---- device memory ------------------------------------------------------ */
+int x2g_malloc(x2g_ctx *ctx, size_t bytes, void **dptr);
+int x2g_free(x2g_ctx *ctx, void *dptr);
+int x2g_h2d(x2g_ctx *ctx, void *dst_d, const void *src_h, size_t bytes);  /* synchronous */
+int x2g_d2h(x2g_ctx *ctx, void *dst_h, const void *src_d, size_t bytes);  /* synchronous */
+int x2g_d2d(x2g_ctx *ctx, void *dst_d, const void *src_d, size_t bytes);  /* stream-ordered */
+/* dst[i] = value for i < n (fill kernel, never a memset node). */
+int x2g_fill(x2g_ctx *ctx, double *dst_d, size_t n, double value);
+
+/* ---- grid --------------------------------------------------------------- */
+/* Upload one grid.  coef = {dmu1[4], dmu2[4], dni1[4], dni2[4], exeven[5],
+ * diag, hni, hmu, r} (26 doubles, initSuppl initCBAllocArrays.F90:1979-2041);
+ * suppl = the 14 nni*nmu coefficient arrays in the reference's i4b order
+ * (borb, bpot, d, e, f0, f1, f2, f3, f4, g, wjac1, wjac2, wgt1, wgt2);
+ * vec = {vxi[nmu], vxisq[nmu], veta[nni], vetasq[nni]}. */
+int x2g_grid_create(x2g_ctx *ctx, int nni, int nmu, const double *coef, const double *suppl,
+                    const double *vec, x2g_grid **out);
+int x2g_grid_destroy(x2g_grid *g);
+/* Device pointer of supplementary array k (0..13, i4b order) of grid g. */
+const double *x2g_grid_suppl(x2g_grid *g, int k);
+
+/* ---- relaxation ---------------------------------------------------------- */
+/* Sweep engine for all later calls of this process: 0 = LDS-pipelined strip
+ * kernel (default; falls back to 1 for grids too wide for the LDS ring),
+ * 1 = one launch per colour pass.  strip_rows > 0 fixes the strip height
+ * (rounded up to a multiple of 4), <= 0 lets the library choose. */
+int x2g_set_sweep_engine(int engine, int strip_rows);
+
+/* putin (inout.F90:23): embed fun_d (nni*nmu) into the padded work_d and fill
+ * the mirror halos by parity isym (+1 even, -1 odd). */
+int x2g_putin(x2g_grid *g, int isym, const double *fun_d, double *work_d);
+/* putout (inout.F90:165): copy the interior of work_d back into fun_d. */
+int x2g_putout(x2g_grid *g, double *fun_d, const double *work_d);
+
+/* MCSOR drop-in (sormcsor.F90:143): nsweeps five-colour sweeps of the
+ * 17-point cross stencil on the padded work_d, each followed by the boundary
+ * extrapolation of the mu=1, nu=1 and nu=nni lines.  bkind selects b(mu):
+ * 0 = borb (orbitals), 1 = bpot (potentials).  work_d is updated in place. */
+int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
+              const double *rhs_d, double omega, int nsweeps);
+/* Host-pointer drop-in for mcsor/sor: work (padded), lhs, rhs on the host;
+ * copies in, relaxes on the device, copies work back.  PCIe-inclusive. */
+int x2g_mcsor_host(x2g_grid *g, int isym, int bkind, double *work, const double *lhs,
+                   const double *rhs, double omega, int nsweeps);
+
+/* One relaxation problem: maxsor1 macro-iterations of
+ *   [orbital tail factors] putin -> maxsor2 sweeps -> putout [orbital tail]
+ * on fun_d (relaxOrbs.F90:92-139 / relaxPots.F90:97-104).
+ * tail_fock1_d != NULL marks an orbital: its tail rows mu=nmu-3..nmu are
+ * recomputed after each macro-iteration from the decay factors of
+ * fock1/f1 (orbAsymptGet/Set, relaxOrbs.F90:104-124).
+ * prec: 0 = FP64 sweeps; 1 = FP32 correction sweeps on A e = rhs - A u with
+ * the residual and the update u += e formed in FP64 (mixed precision). */
+typedef struct {
+    double *fun_d;
+    const double *lhs_d;
+    const double *rhs_d;
+    const double *tail_fock1_d;
+    double omega;
+    int isym;
+    int bkind;
+    int prec;
+    int grid_index; /* index into the grids[] array of x2g_relax_batch */
+} x2g_relax_item;
+
+/* Relax n problems, possibly on different grids, together: every sweep of
+ * every problem is issued in one fused launch. */
+int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
+                    int maxsor1, int maxsor2);
+
+/* FP64 residual norm^2 sum (rhs - A u)^2 over the relaxed interior of each
+ * item, and sum rhs^2 (for relative residuals).  out[2*i], out[2*i+1]. */
+int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
+                       double *out_h);
+
+/* ---- elementwise / quadrature ------------------------------------------- */
+/* out = base_scale*base + sum_k c_k * a_k * b_k * cc_k (b_k, cc_k may be NULL;
+ * base may be NULL).  out may alias any input (pure elementwise). */
+typedef struct {
+    double c;
+    const double *a_d;
+    const double *b_d;
+    const double *cc_d;
+} x2g_term;
+int x2g_lincomb(x2g_grid *g, double *out_d, const double *base_d, double base_scale,
+                const x2g_term *terms, int nterms);
+
+/* out_h[k] = sum_i w_k[i] * a_k[i] * b_k[i] * cc_k[i] over nni*nmu points for
+ * k < nterms (b_k, cc_k may be NULL); deterministic two-pass FP64 reduction. */
+typedef struct {
+    const double *w_d;
+    const double *a_d;
+    const double *b_d;
+    const double *cc_d;
+} x2g_dot;
+int x2g_wdot(x2g_grid *g, const x2g_dot *terms, int nterms, double *out_h);
+
+/* oneelij (integrals.F90:406): <psi2 | T + m^2 e + f0 | psi1> with wgt1, where T
+ * is the 9-point (nu,mu) Laplacian of psi1 (diffnu/diffmu, utils.f90:599-695)
+ * on the mirror-padded psi1 (parity isym). out_h[0] = total, out_h[1] = the
+ * kinetic part <psi2|T + m^2 e|psi1>, out_h[2] = <psi2|f0|psi1>. */
+int x2g_oneel(x2g_grid *g, const double *psi1_d, const double *psi2_d, int isym, int m,
+              double *out_h);
+
+/* Multipole moments (coulMom/exchMom, scfUtils.F90:490-850):
+ * out_h[k-1] = sum wgt2*f4*psi1*psi2 * N_kq P_k^q(cos th) r^k, k = 1..mpole,
+ * q = dm (0 for Coulomb moments). */
+int x2g_moments(x2g_grid *g, const double *psi1_d, const double *psi2_d, int dm, int mpole,
+                double *out_h);
+
+/* Dirichlet tail mu=nmu-3..nmu of a potential from its multipole moments
+ * (coulAsympt/exchAsympt, scfUtils.F90:275-329).  kind 0: Coulomb
+ * (mom = cmulti of the orbital, 1/r monopole included); kind 1: exchange with
+ * |Delta m| = dm (mom = exc(di,qu,oc,he,5..8)). */
+int x2g_pot_tail(x2g_grid *g, double *pot_d, int kind, int dm, int mpole, const double *mom);
+
+/* setCiOrb (scfUtils.F90:380-406): impose g (ihsym=+1) or u (-1) symmetry. */
+int x2g_setci(x2g_grid *g, double *psi_d, int ihsym);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,402 @@
+"""Parity of the HIP kernels (through the C ABI) with the reference's own
+routines (golden vectors) and with the CPU oracle on seeded inputs.
+
+Tolerances: FP64 kernels agree with the reference to 1e-12 relative to the
+array's largest magnitude (the device forms fused multiply-adds, the
+reference does not); FP32 sweeps agree with the FP32 oracle to 2e-5.
+"""
+import numpy as np
+import pytest
+
+from conftest import grid_coef, load_golden
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+REL64 = 1e-12
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from x2dhf_amd import gpu
+    c = gpu.Context(0)
+    yield c
+    gpu.load().x2g_set_sweep_engine(0, -1)
+    c.close()
+
+
+def make_grid(ctx, z):
+    from x2dhf_amd import gpu
+    nni, nmu = int(z["sizes"][0]), int(z["sizes"][1])
+    return gpu.Grid(ctx, nni, nmu, grid_coef(z), z["suppl"], z["vxi"], z["veta"])
+
+
+def relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def engines():
+    # (engine, strip rows): strip kernel with the library's choice, with a
+    # forced small strip (several strips per problem), and the per-pass kernels
+    return [(0, -1), (0, 32), (1, -1)]
+
+
+@pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
+@pytest.mark.parametrize("pfx", ["k1_", "k2_"])
+@pytest.mark.parametrize("eng", engines())
+def test_mcsor_vs_reference(ctx, case, pfx, eng):
+    from x2dhf_amd import gpu
+    z = load_golden(case)
+    if pfx + "win" not in z:
+        pytest.skip("single potential")
+    print(f"case={case} {pfx} engine={eng}", flush=True)
+    gpu.check(ctx.lib.x2g_set_sweep_engine(*eng))
+    g = make_grid(ctx, z)
+    isym = int(z[pfx + "isym"][0])
+    omega, nsw = float(z[pfx + "scal"][0]), int(z[pfx + "scal"][1])
+    lhs, rhs = ctx.upload(z[pfx + "lhs"]), ctx.upload(z[pfx + "rhs"])
+    for name, ns in (("wout_mcsor", nsw), ("wout_mcsor1", 1)):
+        w = ctx.upload(z[pfx + "win"])
+        gpu.check(ctx.lib.x2g_mcsor(g.h, isym, 1, w.ptr, lhs.ptr, rhs.ptr, omega, ns))
+        e = relerr(w.get(), z[pfx + name])
+        assert e < REL64, (name, e)
+        w.free()
+    g.destroy()
+
+
+def test_putin_putout_vs_reference(ctx):
+    from x2dhf_amd import gpu
+    z = load_golden("n2_61x91")
+    g = make_grid(ctx, z)
+    for pfx, k in (("k1_", 0), ("k2_", 1)):
+        isym = int(z[pfx + "isym"][0])
+        f = ctx.upload(z["pots_asympt"][k])
+        w = ctx.alloc(g.npad)
+        gpu.check(ctx.lib.x2g_putin(g.h, isym, f.ptr, w.ptr))
+        assert np.array_equal(w.get(), z[pfx + "win"])
+        out = ctx.alloc(g.n)
+        gpu.check(ctx.lib.x2g_putout(g.h, out.ptr, w.ptr))
+        assert np.array_equal(out.get(), z["pots_asympt"][k])
+    g.destroy()
+
+
+def _setci_all(ctx, g, z, orbs):
+    """Impose the g/u symmetry of every orbital (setCi) as the reference does
+    before the SCF loop (doSCF.F90:334-336)."""
+    from x2dhf_amd import gpu
+    # N2 decks: orbitals 1 pi u, 1 sigma g, 1 sigma u, 1 sigma g, 1 sigma u, 1 sigma g
+    ihomo = {1: [1], 6: [-1, 1, -1, 1, -1, 1]}[len(orbs)]
+    for k, o in enumerate(orbs):
+        gpu.check(ctx.lib.x2g_setci(g.h, o.ptr, ihomo[k]))
+
+
+@pytest.mark.parametrize("eng", engines())
+def test_potential_batch_vs_reference(ctx, eng):
+    """All Coulomb and exchange potentials of the N2 pi orbital (7 problems, both
+    parities, Delta m = 0, 1, 2) relaxed in one batch = the reference's
+    coulExchMCSOR (relaxPots.F90:196-322)."""
+    from x2dhf_amd import gpu
+    gpu.check(ctx.lib.x2g_set_sweep_engine(*eng))
+    for case in ("h2_61x91", "n2_61x91"):
+        z = load_golden(case)
+        print(f"case={case} engine={eng}", flush=True)
+        g = make_grid(ctx, z)
+        norb = int(z["sizes"][2])
+        orbs = [ctx.upload(z["orb_prep"][k]) for k in range(norb)]
+        _setci_all(ctx, g, z, orbs)
+        f3, e, gg = z["suppl"][7], z["suppl"][3], z["suppl"][9]
+        omega = float(z["scal"][5])
+        items, keep = [], []
+        orb_h = [o.get() for o in orbs]
+        for k, (in1, in2, dm, isym, _) in enumerate(z["potlist"]):
+            rhs = orb_h[in1 - 1] * orb_h[in2 - 1] * gg
+            lhs = f3 + float(dm * dm) * e if dm else f3.copy()
+            fun, l, r = ctx.upload(z["pots_asympt"][k]), ctx.upload(lhs), ctx.upload(rhs)
+            keep += [fun, l, r]
+            items.append(gpu.RelaxItem(fun.ptr, l.ptr, r.ptr, None, omega, int(isym), 1, 0, 0))
+        arr = (gpu.RelaxItem * len(items))(*items)
+        grids = (gpu._vp * 1)(g.h.value)
+        gpu.check(ctx.lib.x2g_relax_batch(ctx.h, grids, arr, len(items), 1, 10))
+        for k in range(len(items)):
+            got = keep[3 * k].get()
+            err = relerr(got, z["pots_relaxed"][k])
+            assert err < 1e-11, (case, k, err)
+        g.destroy()
+
+
+@pytest.mark.parametrize("eng", engines())
+def test_orbital_relaxation_vs_reference(ctx, eng):
+    """orbMCSOR (relaxOrbs.F90:150-241): lhs = fock1+diag, rhs = fock2, ten
+    sweeps, then the exponential tail of the orbital."""
+    from x2dhf_amd import gpu
+    gpu.check(ctx.lib.x2g_set_sweep_engine(*eng))
+    for case, isym in (("h2_61x91", 1), ("n2_61x91", -1)):
+        z = load_golden(case)
+        g = make_grid(ctx, z)
+        norb, tr = int(z["sizes"][2]), int(z["sizes"][3])
+        orbs = [ctx.upload(z["orb_prep"][k]) for k in range(norb)]
+        _setci_all(ctx, g, z, orbs)
+        diag = float(z["scal"][2])
+        lhs, rhs, f1 = ctx.upload(z["fock1"] + diag), ctx.upload(z["fock2"]), ctx.upload(z["fock1"])
+        psi = orbs[tr - 1]
+        it = gpu.RelaxItem(psi.ptr, lhs.ptr, rhs.ptr, f1.ptr, float(z["scal"][3]), isym, 0, 0, 0)
+        arr = (gpu.RelaxItem * 1)(it)
+        grids = (gpu._vp * 1)(g.h.value)
+        gpu.check(ctx.lib.x2g_relax_batch(ctx.h, grids, arr, 1, 1, 10))
+        err = relerr(psi.get(), z["orb_relaxed"])
+        assert err < 1e-11, (case, err)
+        g.destroy()
+
+
+def test_fock_lincomb_vs_reference(ctx):
+    """fockHF (fock.F90:57-184) assembled with x2g_lincomb on the relaxed potentials."""
+    from x2dhf_amd import gpu
+    # H2: fock1 = f0 + eps f1 + f2 * (occ-1) J
+    z = load_golden("h2_61x91")
+    g = make_grid(ctx, z)
+    S = [g.suppl_ptr(k) for k in range(14)]
+    J = ctx.upload(z["pots_relaxed"][0])
+    out = ctx.alloc(g.n)
+    eps = float(z["ee_prep"][0, 0])
+    terms = (gpu.Term * 2)(gpu.Term(eps, S[5], None, None), gpu.Term(1.0, S[6], J.ptr, None))
+    gpu.check(ctx.lib.x2g_lincomb(g.h, out.ptr, S[4], 1.0, terms, 2))
+    assert relerr(out.get(), z["fock1"]) < 1e-13
+    g.destroy()
+    # N2, pi orbital 1: Coulomb with occupations 3 (own shell) and 2; exchange
+    # with every sigma (gec = 1) and the pi self-exchange (Delta m = 2, gec = 1)
+    z = load_golden("n2_61x91")
+    g = make_grid(ctx, z)
+    S = [g.suppl_ptr(k) for k in range(14)]
+    orbs = [ctx.upload(z["orb_prep"][k]) for k in range(6)]
+    _setci_all(ctx, g, z, orbs)
+    P = [ctx.upload(p) for p in z["pots_relaxed"]]   # (1,1),(1,2)..(1,6), Coulomb 1
+    C = [P[6]] + [ctx.upload(z["coul_prep"][k]) for k in range(1, 6)]
+    coul = ctx.alloc(g.n)
+    t = [gpu.Term(3.0, C[0].ptr, None, None)] + [gpu.Term(2.0, C[k].ptr, None, None) for k in range(1, 6)]
+    gpu.check(ctx.lib.x2g_lincomb(g.h, coul.ptr, None, 0.0, (gpu.Term * 6)(*t), 6))
+    assert relerr(coul.get(), z["coulombptr"]) < 1e-13
+    exch = ctx.alloc(g.n)
+    t = [gpu.Term(1.0, P[k].ptr, orbs[k].ptr, None) for k in range(1, 6)] + [gpu.Term(1.0, P[0].ptr, orbs[0].ptr, None)]
+    gpu.check(ctx.lib.x2g_lincomb(g.h, exch.ptr, None, 0.0, (gpu.Term * 6)(*t), 6))
+    assert relerr(exch.get(), z["exchangeptr"]) < 1e-13
+    f1 = ctx.alloc(g.n)
+    eps = float(z["ee_prep"][0, 0])
+    t = [gpu.Term(eps, S[5], None, None), gpu.Term(1.0, S[3], None, None), gpu.Term(1.0, S[6], coul.ptr, None)]
+    gpu.check(ctx.lib.x2g_lincomb(g.h, f1.ptr, S[4], 1.0, (gpu.Term * 3)(*t), 3))
+    assert relerr(f1.get(), z["fock1"]) < 1e-13
+    f2 = ctx.alloc(g.n)
+    t = [gpu.Term(1.0, S[6], exch.ptr, None)]
+    gpu.check(ctx.lib.x2g_lincomb(g.h, f2.ptr, None, 0.0, (gpu.Term * 1)(*t), 1))
+    assert relerr(f2.get(), z["fock2"]) < 1e-13
+    g.destroy()
+
+
+@pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
+def test_quadratures_vs_reference(ctx, case):
+    """norm (normOrtho.F90:41-50), oneelij (integrals.F90:406-480) and the
+    multipole moments (scfUtils.F90:490-850) of the traced orbital."""
+    from x2dhf_amd import gpu
+    z = load_golden(case)
+    g = make_grid(ctx, z)
+    S = [g.suppl_ptr(k) for k in range(14)]
+    norb, tr = int(z["sizes"][2]), int(z["sizes"][3])
+    psi = ctx.upload(z["orb_relaxed"])
+    out = np.zeros(3)
+    d = (gpu.Dot * 1)(gpu.Dot(S[13], S[8], psi.ptr, psi.ptr))
+    gpu.check(ctx.lib.x2g_wdot(g.h, d, 1, out.ctypes.data))
+    assert abs(out[0] - z["norm_1"][tr - 1]) < 1e-13 * abs(z["norm_1"][tr - 1])
+    # one-electron integral on the orthonormalised orbital
+    isym, m = (1, 0) if case.startswith("h2") else (-1, 1)
+    po = ctx.upload(z["orb_ortho"])
+    gpu.check(ctx.lib.x2g_oneel(g.h, po.ptr, po.ptr, isym, m, out.ctypes.data))
+    assert abs(out[0] - z["oneel_1"][0]) < 1e-11 * abs(z["oneel_1"][0]), (out[0], z["oneel_1"][0])
+    # Coulomb moments of the traced orbital (even ones; odd vanish for homonuclear)
+    mom = np.zeros(8)
+    gpu.check(ctx.lib.x2g_moments(g.h, po.ptr, po.ptr, 0, 4, mom.ctypes.data))
+    ref = z["cmulti_1"][:, tr - 1]
+    for k in (1, 3):
+        assert abs(mom[k] - ref[k]) < 1e-11 * max(abs(ref[k]), 1e-3), (k, mom[k], ref[k])
+    if case.startswith("n2"):
+        # exchange moments of the pairs (1, j): pair index k2(1,j) = 1 + j(j-1)/2
+        orbs = [ctx.upload(z["orb_prep"][k]) for k in range(norb)]
+        _setci_all(ctx, g, z, orbs)
+        orbs[0] = po
+        for j in range(1, 7):
+            ipc = 1 + j * (j - 1) // 2
+            dm = 2 if j == 1 else 1
+            gpu.check(ctx.lib.x2g_moments(g.h, orbs[0].ptr, orbs[j - 1].ptr, dm, 4, mom.ctypes.data))
+            for k, name in enumerate(("excdi", "excqu", "excoc", "exche")):
+                r = z[name + "_1"][ipc - 1]
+                assert abs(mom[k] - r) < 1e-10 * max(abs(r), 1e-4), (j, name, mom[k], r)
+    g.destroy()
+
+
+@pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
+def test_potential_tails_vs_reference(ctx, case):
+    """coulAsympt / exchAsympt (scfUtils.F90:275-329) from the moments of prepSCF."""
+    from x2dhf_amd import gpu
+    z = load_golden(case)
+    g = make_grid(ctx, z)
+    nni, nmu = g.nni, g.nmu
+    for k, (in1, in2, dm, isym, _) in enumerate(z["potlist"]):
+        pot = ctx.upload(z["pots_prep"][k])
+        if in1 == in2 and dm == 0:   # Coulomb potential of orbital in1
+            mom = np.ascontiguousarray(z["cmulti_prep"][:, in1 - 1])
+            gpu.check(ctx.lib.x2g_pot_tail(g.h, pot.ptr, 0, 0, 4, mom.ctypes.data))
+        else:
+            ipc = in1 + in2 * (in2 - 1) // 2
+            mom = np.array([z[n + "_prep"][ipc - 1] for n in ("excdi", "excqu", "excoc", "exche")])
+            gpu.check(ctx.lib.x2g_pot_tail(g.h, pot.ptr, 1, int(dm), 4, mom.ctypes.data))
+        got = pot.get().reshape(nmu, nni)[nmu - 4:]
+        ref = z["pots_asympt"][k].reshape(nmu, nni)[nmu - 4:]
+        assert relerr(got, ref) < 1e-12, (case, k)
+    g.destroy()
+
+
+def test_setci(ctx):
+    from x2dhf_amd import gpu
+    z = load_golden("h2_61x91")
+    g = make_grid(ctx, z)
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal(g.n)
+    for ih in (1, -1):
+        d = ctx.upload(a)
+        gpu.check(ctx.lib.x2g_setci(g.h, d.ptr, ih))
+        A = a.reshape(g.nmu, g.nni).copy()
+        n2 = g.nni // 2 + 1
+        for ni in range(1, n2):
+            A[:, g.nni - ni] = ih * A[:, ni - 1]
+        if ih == -1:
+            A[:, n2 - 1] = 0
+        assert np.array_equal(d.get(), A.ravel())
+    g.destroy()
+
+
+# ---------------------------------------------------------------------------
+# Seeded synthetic problems at the configuration sizes (oracle comparison).
+
+def synthetic_problem(nni, nmu, seed, rinf=40.0, r=2.0):
+    """Grid-like coefficients (bpot(mu), d(nu), f3-type lhs) and a smooth rhs."""
+    rng = np.random.default_rng(seed)
+    hni = np.pi / (nni - 1)
+    xmi0 = 2.0 * rinf / r
+    hmu = np.log(xmi0 + np.sqrt(xmi0 * xmi0 - 1)) / (nmu - 1)
+    aa1 = np.array([3.0, -32.0, 168.0, -672.0])
+    aa2 = np.array([-9.0, 128.0, -1008.0, 8064.0])
+    coef16 = np.concatenate([aa1 / 840 / hmu, aa2 / 5040 / hmu ** 2, aa1 / 840 / hni, aa2 / 5040 / hni ** 2])
+    ex = np.array([210.0, -120.0, 45.0, -10.0, 1.0]) / 126.0
+    diag = -14350.0 / 5040.0 * (1 / hmu ** 2 + 1 / hni ** 2)
+    vmu = np.arange(nmu) * hmu
+    vxi = np.cosh(vmu)
+    vni = np.arange(nni) * hni
+    veta = np.cos(vni)
+    x1 = np.sqrt(np.maximum(vxi ** 2 - 1, 0))
+    w5 = np.where(np.arange(nmu) == 0, 0.0, vxi / np.where(x1 == 0, 1, x1) - 2 * x1 / vxi)
+    e1 = np.sqrt(1 - veta ** 2)
+    d1 = np.where(e1 < 1e-15, 0.0, veta / np.where(e1 == 0, 1, e1))
+    suppl = np.zeros((14, nni * nmu))
+    suppl[1] = np.repeat(w5, nni)
+    suppl[2] = np.tile(d1, nmu)
+    suppl[0] = np.repeat(np.where(np.arange(nmu) == 0, 0.0, vxi / np.where(x1 == 0, 1, x1)), nni)
+    lhs = (-2.0 / np.repeat(vxi ** 2, nni) + diag) * (1 + 0.05 * rng.random(nni * nmu))
+    X, Y = np.meshgrid(vni, vmu)
+    rhs = (np.exp(-((X - 1.0) ** 2 + (Y - 1.5) ** 2)) * np.sin(3 * X) + 0.01 * rng.standard_normal(X.shape)).ravel()
+    fun = 0.1 * rng.standard_normal(nni * nmu)
+    coef = np.concatenate([coef16, ex, [diag, hni, hmu, r]])
+    return dict(nni=nni, nmu=nmu, coef=coef, suppl=suppl, vxi=vxi, veta=veta, lhs=lhs, rhs=rhs, fun=fun)
+
+
+@pytest.mark.parametrize("shape", [(91, 121), (169, 241), (331, 601), (481, 901), (505, 901)])
+@pytest.mark.parametrize("isym", [1, -1])
+def test_relax_batch_large_vs_oracle(ctx, shape, isym):
+    """Batched relaxation (maxsor1=2 x maxsor2=10) at configuration sizes against the
+    oracle's putin -> mcsor -> putout macro-iterations.  481-wide FP64 grids run the
+    strip kernel with one pass per barrier; 505 (nni = 0 mod 5) has another colour
+    multiplier."""
+    from x2dhf_amd import gpu
+    nni, nmu = shape
+    print(f"shape={shape} isym={isym}", flush=True)
+    gpu.check(ctx.lib.x2g_set_sweep_engine(0, -1))
+    P = synthetic_problem(nni, nmu, seed=nni + nmu)
+    g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
+    fun, l, r = ctx.upload(P["fun"]), ctx.upload(P["lhs"]), ctx.upload(P["rhs"])
+    it = gpu.RelaxItem(fun.ptr, l.ptr, r.ptr, None, 1.9, isym, 1, 0, 0)
+    gpu.check(ctx.lib.x2g_relax_batch(ctx.h, (gpu._vp * 1)(g.h.value), (gpu.RelaxItem * 1)(it), 1, 2, 10))
+    got = fun.get()
+    f = P["fun"].copy()
+    for _ in range(2):
+        w = pyoracle.putin(nni, nmu, isym, f)
+        pyoracle.mcsor(nni, nmu, isym, w, P["lhs"], P["rhs"], P["suppl"][1], P["suppl"][2], P["coef"][:16],
+                       P["coef"][16:21], 1.9, 10)
+        f = pyoracle.putout(nni, nmu, w)
+    assert relerr(got, f) < 1e-11
+    g.destroy()
+
+
+def test_uneven_batch_matches_single_problems(ctx):
+    """Problems on four different grids relaxed in one batch give the same
+    result as each relaxed alone."""
+    from x2dhf_amd import gpu
+    shapes = [(91, 121), (151, 241), (61, 91), (331, 601)]
+    probs = [synthetic_problem(n, m, seed=7 + i) for i, (n, m) in enumerate(shapes)]
+    grids = [gpu.Grid(ctx, p["nni"], p["nmu"], p["coef"], p["suppl"], p["vxi"], p["veta"]) for p in probs]
+    gh = (gpu._vp * len(grids))(*[g.h.value for g in grids])
+    bufs, items = [], []
+    for i, p in enumerate(probs):
+        f, l, r = ctx.upload(p["fun"]), ctx.upload(p["lhs"]), ctx.upload(p["rhs"])
+        bufs.append((f, l, r))
+        items.append(gpu.RelaxItem(f.ptr, l.ptr, r.ptr, None, 1.85, 1 if i % 2 == 0 else -1, 1, 0, i))
+    gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, (gpu.RelaxItem * 4)(*items), 4, 1, 10))
+    together = [b[0].get() for b in bufs]
+    for i, p in enumerate(probs):
+        f = ctx.upload(p["fun"])
+        it = gpu.RelaxItem(f.ptr, bufs[i][1].ptr, bufs[i][2].ptr, None, 1.85, items[i].isym, 1, 0, 0)
+        gpu.check(ctx.lib.x2g_relax_batch(ctx.h, (gpu._vp * 1)(grids[i].h.value), (gpu.RelaxItem * 1)(it), 1, 1, 10))
+        assert np.array_equal(f.get(), together[i]), i
+    for g in grids:
+        g.destroy()
+
+
+@pytest.mark.parametrize("shape", [(91, 121), (331, 601), (505, 901)])
+def test_fp32_correction_sweeps_vs_oracle(ctx, shape):
+    """With u = 0 one mixed-precision macro-iteration is exactly FP32 MCSOR from
+    zero on rhs (the residual), so its interior must match the FP32 oracle."""
+    from x2dhf_amd import gpu
+    nni, nmu = shape
+    P = synthetic_problem(nni, nmu, seed=11)
+    g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
+    f = ctx.upload(np.zeros(nni * nmu))
+    l, r = ctx.upload(P["lhs"]), ctx.upload(P["rhs"])
+    it = gpu.RelaxItem(f.ptr, l.ptr, r.ptr, None, 1.9, 1, 1, 1, 0)
+    gpu.check(ctx.lib.x2g_relax_batch(ctx.h, (gpu._vp * 1)(g.h.value), (gpu.RelaxItem * 1)(it), 1, 1, 10))
+    got = f.get().reshape(nmu, nni)
+    w = np.zeros((nni + 8) * (nmu + 8), dtype=np.float32)
+    pyoracle.mcsor_f32(nni, nmu, 1, w, P["lhs"], P["rhs"], P["suppl"][1], P["suppl"][2], P["coef"][:16],
+                       P["coef"][16:21], 1.9, 10)
+    ref = w.reshape(nmu + 8, nni + 8)[4:nmu + 4, 4:nni + 4].astype(np.float64)
+    inter = (slice(1, nmu - 4), slice(1, nni - 1))
+    assert relerr(got[inter], ref[inter]) < 2e-5
+    g.destroy()
+
+
+def test_mixed_precision_refines_to_fp64(ctx):
+    """FP32 correction sweeps with FP64 residuals converge to the FP64 solution:
+    after enough macro-iterations the FP64 residual is at FP64 level."""
+    from x2dhf_amd import gpu
+    nni, nmu = 169, 241
+    P = synthetic_problem(nni, nmu, seed=13)
+    g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
+    gh = (gpu._vp * 1)(g.h.value)
+    l, r = ctx.upload(P["lhs"]), ctx.upload(P["rhs"])
+    res = np.zeros(2)
+    out = {}
+    for prec in (0, 1):
+        f = ctx.upload(P["fun"])
+        it = (gpu.RelaxItem * 1)(gpu.RelaxItem(f.ptr, l.ptr, r.ptr, None, 1.9, 1, 1, prec, 0))
+        gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, it, 1, 200, 10))
+        gpu.check(ctx.lib.x2g_residual_norms(ctx.h, gh, it, 1, res.ctypes.data))
+        out[prec] = (f.get(), np.sqrt(res[0] / res[1]))
+    print("relative residuals fp64 / mixed:", out[0][1], out[1][1])
+    assert out[0][1] < 1e-10 and out[1][1] < 1e-10, (out[0][1], out[1][1])
+    assert relerr(out[1][0], out[0][0]) < 1e-10
+    g.destroy()

@@ -1,0 +1,133 @@
+"""Pin the CPU oracle (oracle/x2oracle.c) before trusting it.
+
+* Against the reference's own routines: vectors written by the reference
+  mcsor/sor/putin (sormcsor.F90, inout.F90) through oracle/ref_harness.f90.
+* Against what the mathematics fixes: the 9-point stencil weights, the
+  boundary extrapolation weights and the 7-point Newton-Cotes weights on
+  polynomials, and the fixed point of the sweep (zero residual).
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from oracle import pyoracle
+
+
+def _ok(a, b):
+    return np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
+@pytest.mark.parametrize("pfx", ["k1_", "k2_"])
+def test_oracle_sweeps_bitexact_vs_reference(case, pfx):
+    z = load_golden(case)
+    if pfx + "win" not in z:
+        pytest.skip("case has a single potential")
+    nni, nmu = int(z["sizes"][0]), int(z["sizes"][1])
+    isym = int(z[pfx + "isym"][0])
+    omega, nsw = float(z[pfx + "scal"][0]), int(z[pfx + "scal"][1])
+    args = (z[pfx + "lhs"], z[pfx + "rhs"], z["suppl"][1], z["suppl"][2], z["coef"][:16], z["coef"][16:21], omega)
+    for mode, name, ns in [(0, "wout_mcsor", nsw), (0, "wout_mcsor1", 1), (1, "wout_sor", nsw)]:
+        w = pyoracle.mcsor(nni, nmu, isym, z[pfx + "win"].copy(), *args, ns, mode)
+        assert _ok(w, z[pfx + name]), (case, pfx, name, np.abs(w - z[pfx + name]).max())
+
+
+@pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
+def test_oracle_putin_vs_reference(case):
+    z = load_golden(case)
+    nni, nmu = int(z["sizes"][0]), int(z["sizes"][1])
+    for pfx, k in (("k1_", 0), ("k2_", 1)):
+        if pfx + "win" not in z:
+            continue
+        isym = int(z[pfx + "isym"][0])
+        w = pyoracle.putin(nni, nmu, isym, z["pots_asympt"][k])
+        assert _ok(w, z[pfx + "win"])
+        assert _ok(pyoracle.putout(nni, nmu, w), z["pots_asympt"][k])
+
+
+@pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
+def test_oracle_orbital_tail_vs_reference(case):
+    """The relaxed orbital's tail rows mu = nmu-3..nmu are the decay extrapolation
+    of its row nmu-4 with factors from fock1/f1 (relaxOrbs.F90:104-124)."""
+    z = load_golden(case)
+    nni, nmu = int(z["sizes"][0]), int(z["sizes"][1])
+    f1 = z["suppl"][5]
+    fa = np.where(np.abs(f1) <= np.finfo(float).eps, 0.0, z["fock1"] / np.where(f1 == 0, 1, f1))
+    vxisq, vetasq = z["vxi"] ** 2, z["veta"] ** 2
+    psi = z["orb_relaxed"]
+    got = pyoracle.orb_tail(nni, nmu, vxisq, vetasq, fa, psi)
+    assert _ok(got, psi)
+
+
+def test_stencil_weights_exact_on_polynomials():
+    """9-point first/second derivative weights (initCBAllocArrays.F90:1670-1673)
+    differentiate polynomials of degree <= 8 exactly; the boundary weights
+    exeven (2037-2041) reproduce even polynomials of degree <= 8 at the axis."""
+    z = load_golden("h2_61x91")
+    h = float(z["scal"][0])
+    dni1, dni2 = z["coef"][8:12], z["coef"][12:16]
+    ex = z["coef"][16:21]
+    x0 = 0.37
+    for deg in range(0, 9):
+        f = lambda x: (x - 0.1) ** deg
+        d1 = deg * (x0 - 0.1) ** (deg - 1) if deg >= 1 else 0.0
+        d2 = deg * (deg - 1) * (x0 - 0.1) ** (deg - 2) if deg >= 2 else 0.0
+        ks = np.array([4, 3, 2, 1])
+        g1 = np.sum(dni1 * (f(x0 - ks * h) - f(x0 + ks * h)))
+        g2 = np.sum(dni2 * (f(x0 - ks * h) + f(x0 + ks * h))) + (-14350.0 / 5040.0 / h ** 2) * f(x0)
+        assert abs(g1 - d1) < 1e-8 * max(1, abs(d1)), (deg, g1, d1)
+        assert abs(g2 - d2) < 1e-6 * max(1, abs(d2)), (deg, g2, d2)
+    for p in range(0, 5):
+        vals = np.array([(k * 1.0) ** (2 * p) for k in range(1, 6)])
+        expect = 1.0 if p == 0 else 0.0
+        assert abs(np.dot(ex, vals) - expect) < 1e-11
+
+
+def test_newton_cotes_weights_exact():
+    """The mu and nu quadrature weights built into wgt1/wgt2 are the composite
+    7-point Newton-Cotes rule (41, 216, 27, 272, 27, 216, 41)*h/140, exact for
+    polynomials of degree <= 7 (initCBAllocArrays.F90:1917-1952)."""
+    n = 61
+    h = 1.0 / (n - 1)
+    w = np.zeros(n)
+    w[0] = 41
+    for i in range(1, n, 6):
+        w[i:i + 5] = [216, 27, 272, 27, 216]
+        w[i + 5] = 41 if i + 5 == n - 1 else 82
+    w *= h / 140
+    x = np.linspace(0, 1, n)
+    for deg in range(8):
+        assert abs(np.dot(w, x ** deg) - 1.0 / (deg + 1)) < 1e-13
+
+
+def test_oracle_fixed_point_has_zero_residual():
+    """After many sweeps of a well-posed problem the MCSOR iterate satisfies
+    rhs - (L + lhs) u = 0: the residual restatement and the update formula
+    describe the same operator."""
+    z = load_golden("h2_61x91")
+    nni, nmu = int(z["sizes"][0]), int(z["sizes"][1])
+    w = z["k1_win"].copy()
+    args = (z["k1_lhs"], z["k1_rhs"], z["suppl"][1], z["suppl"][2], z["coef"][:16], z["coef"][16:21], 1.95)
+    r0 = pyoracle.residual(nni, nmu, w, z["k1_lhs"], z["k1_rhs"], z["suppl"][1], z["suppl"][2], z["coef"][:16])
+    pyoracle.mcsor(nni, nmu, 1, w, *args, 2000, 0)
+    r1 = pyoracle.residual(nni, nmu, w, z["k1_lhs"], z["k1_rhs"], z["suppl"][1], z["suppl"][2], z["coef"][:16])
+    assert np.linalg.norm(r1) < 1e-9 * np.linalg.norm(r0)
+
+
+def test_oracle_boundary_lines_even_odd():
+    nni, nmu = 31, 41
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((nni + 8) * (nmu + 8))
+    ex = np.array([210, -120, 45, -10, 1]) / 126.0
+    g = f.copy()
+    pyoracle.lib().x2o_boundary(nni, nmu, 1, pyoracle._p(g), pyoracle._p(ex))
+    W = g.reshape(nmu + 8, nni + 8)
+    # mu=1 row from mu=2..6 at the same nu
+    for nu in (2, 10, nni - 1):
+        assert abs(W[4, nu + 3] - np.dot(ex, W[5:10, nu + 3])) < 1e-12
+    # nu=1 at mu=1 uses the new mu=1 row
+    assert abs(W[4, 4] - np.dot(ex, W[4, 5:10])) < 1e-12
+    h = f.copy()
+    pyoracle.lib().x2o_boundary(nni, nmu, -1, pyoracle._p(h), pyoracle._p(ex))
+    V = h.reshape(nmu + 8, nni + 8)
+    assert np.all(V[4, 5:nni + 3] == 0) and np.all(V[4:nmu, 4] == 0) and np.all(V[4:nmu, nni + 3] == 0)

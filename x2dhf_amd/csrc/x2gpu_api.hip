@@ -1,0 +1,249 @@
+// Context, memory and grid management of the x2gpu C ABI.
+#include "x2gpu_internal.h"
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+
+namespace x2g {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int ws_reserve(x2g_ctx *ctx, int k, size_t bytes)
+{
+    if (ctx->ws[k].bytes >= bytes) return X2G_OK;
+    // earlier launches may still read the old buffer
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->ws[k].ptr) X2G_HIP(hipFree(ctx->ws[k].ptr));
+    ctx->ws[k].ptr = nullptr;
+    ctx->ws[k].bytes = 0;
+    const size_t want = std::max(bytes, ctx->ws[k].bytes + ctx->ws[k].bytes / 2);
+    if (hipMalloc(&ctx->ws[k].ptr, want) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("workspace %d: hipMalloc(%zu) failed", k, want);
+        ctx->ws[k].ptr = nullptr;
+        return X2G_ENOMEM;
+    }
+    ctx->ws[k].bytes = want;
+    return X2G_OK;
+}
+
+int pinned_reserve(x2g_ctx *ctx, size_t bytes)
+{
+    if (ctx->pinned_bytes >= bytes) return X2G_OK;
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->pinned) X2G_HIP(hipHostFree(ctx->pinned));
+    ctx->pinned = nullptr;
+    ctx->pinned_bytes = 0;
+    X2G_HIP(hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault));
+    ctx->pinned_bytes = bytes;
+    return X2G_OK;
+}
+
+}  // namespace x2g
+
+using namespace x2g;
+
+static constexpr size_t kStageBytes = 4u << 20;  // launch-descriptor staging area
+
+extern "C" int x2g_version(void) { return 100; }
+
+extern "C" const char *x2g_last_error(void) { return x2g::g_err; }
+
+extern "C" int x2g_ctx_create(int device, x2g_ctx **out)
+{
+    X2G_CHECK(out, "x2g_ctx_create: null out");
+    *out = nullptr;
+    int ndev = 0;
+    X2G_HIP(hipGetDeviceCount(&ndev));
+    X2G_CHECK(device >= 0 && device < ndev, "x2g_ctx_create: device %d of %d", device, ndev);
+    X2G_HIP(hipSetDevice(device));
+    x2g_ctx *c = new x2g_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        set_error("hipGetDeviceProperties failed");
+        return X2G_EHIP;
+    }
+    c->num_cu = prop.multiProcessorCount;
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, device) != hipSuccess) optin = 0;
+    size_t lds = std::max<size_t>((size_t)prop.sharedMemPerBlock, (size_t)std::max(optin, 0));
+    // gfx950 provides 160 KB of LDS per compute unit
+    c->max_lds = std::min<size_t>(lds, 160 * 1024);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreateWithFlags failed");
+        return X2G_EHIP;
+    }
+    int rc = ws_reserve(c, 2, kStageBytes);
+    if (!rc) rc = pinned_reserve(c, kStageBytes);
+    if (rc) {
+        x2g_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return X2G_OK;
+}
+
+extern "C" int x2g_ctx_destroy(x2g_ctx *ctx)
+{
+    if (!ctx) return X2G_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &w : ctx->ws)
+        if (w.ptr) (void)hipFree(w.ptr);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return X2G_OK;
+}
+
+extern "C" int x2g_ctx_sync(x2g_ctx *ctx)
+{
+    X2G_CHECK(ctx, "x2g_ctx_sync: null ctx");
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    return X2G_OK;
+}
+
+extern "C" void *x2g_ctx_stream(x2g_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+extern "C" int x2g_malloc(x2g_ctx *ctx, size_t bytes, void **dptr)
+{
+    X2G_CHECK(ctx && dptr, "x2g_malloc: null argument");
+    *dptr = nullptr;
+    if (bytes == 0) return X2G_OK;
+    if (hipMalloc(dptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *dptr = nullptr;
+        set_error("hipMalloc(%zu) failed", bytes);
+        return X2G_ENOMEM;
+    }
+    return X2G_OK;
+}
+
+extern "C" int x2g_free(x2g_ctx *ctx, void *dptr)
+{
+    X2G_CHECK(ctx, "x2g_free: null ctx");
+    if (!dptr) return X2G_OK;
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    X2G_HIP(hipFree(dptr));
+    return X2G_OK;
+}
+
+extern "C" int x2g_h2d(x2g_ctx *ctx, void *dst_d, const void *src_h, size_t bytes)
+{
+    X2G_CHECK(ctx && ((dst_d && src_h) || bytes == 0), "x2g_h2d: null argument");
+    if (!bytes) return X2G_OK;
+    X2G_HIP(hipMemcpyAsync(dst_d, src_h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    return X2G_OK;
+}
+
+extern "C" int x2g_d2h(x2g_ctx *ctx, void *dst_h, const void *src_d, size_t bytes)
+{
+    X2G_CHECK(ctx && ((dst_h && src_d) || bytes == 0), "x2g_d2h: null argument");
+    if (!bytes) return X2G_OK;
+    X2G_HIP(hipMemcpyAsync(dst_h, src_d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    return X2G_OK;
+}
+
+extern "C" int x2g_d2d(x2g_ctx *ctx, void *dst_d, const void *src_d, size_t bytes)
+{
+    X2G_CHECK(ctx && ((dst_d && src_d) || bytes == 0), "x2g_d2d: null argument");
+    if (!bytes) return X2G_OK;
+    X2G_HIP(hipMemcpyAsync(dst_d, src_d, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return X2G_OK;
+}
+
+extern "C" int x2g_grid_create(x2g_ctx *ctx, int nni, int nmu, const double *coef, const double *suppl,
+                               const double *vec, x2g_grid **out)
+{
+    X2G_CHECK(ctx && coef && suppl && vec && out, "x2g_grid_create: null argument");
+    X2G_CHECK(nni >= 7 && nmu >= 11, "x2g_grid_create: grid %dx%d too small", nni, nmu);
+    X2G_CHECK((long)nni * nmu < (1L << 30), "x2g_grid_create: grid too large");
+    *out = nullptr;
+    x2g_grid *g = new x2g_grid();
+    g->ctx = ctx;
+    g->nni = nni;
+    g->nmu = nmu;
+    std::memcpy(g->coef, coef, sizeof(g->coef));
+    const size_t n = (size_t)nni * nmu;
+    int rc = X2G_OK;
+    auto alloc = [&](void **p, size_t bytes) {
+        if (rc) return;
+        rc = x2g_malloc(ctx, bytes, p);
+    };
+    alloc((void **)&g->suppl, 14 * n * sizeof(double));
+    alloc((void **)&g->bmu_orb, nmu * sizeof(double));
+    alloc((void **)&g->bmu_pot, nmu * sizeof(double));
+    alloc((void **)&g->dnu, nni * sizeof(double));
+    alloc((void **)&g->bmu_orb_f, nmu * sizeof(float));
+    alloc((void **)&g->bmu_pot_f, nmu * sizeof(float));
+    alloc((void **)&g->dnu_f, nni * sizeof(float));
+    alloc((void **)&g->vxi, nmu * sizeof(double));
+    alloc((void **)&g->vxisq, nmu * sizeof(double));
+    alloc((void **)&g->veta, nni * sizeof(double));
+    alloc((void **)&g->vetasq, nni * sizeof(double));
+    if (rc) {
+        x2g_grid_destroy(g);
+        return rc;
+    }
+    // b(mu) and d(nu) carry 1-D content (initSuppl, initCBAllocArrays.F90:1779-1781)
+    std::vector<double> bo(nmu), bp(nmu), dn(nni);
+    std::vector<float> bof(nmu), bpf(nmu), dnf(nni);
+    for (int mu = 0; mu < nmu; ++mu) {
+        bo[mu] = suppl[0 * n + (size_t)mu * nni];
+        bp[mu] = suppl[1 * n + (size_t)mu * nni];
+        bof[mu] = (float)bo[mu];
+        bpf[mu] = (float)bp[mu];
+    }
+    for (int nu = 0; nu < nni; ++nu) {
+        dn[nu] = suppl[2 * n + nu];
+        dnf[nu] = (float)dn[nu];
+    }
+    if (!rc) rc = x2g_h2d(ctx, g->suppl, suppl, 14 * n * sizeof(double));
+    if (!rc) rc = x2g_h2d(ctx, g->bmu_orb, bo.data(), nmu * sizeof(double));
+    if (!rc) rc = x2g_h2d(ctx, g->bmu_pot, bp.data(), nmu * sizeof(double));
+    if (!rc) rc = x2g_h2d(ctx, g->dnu, dn.data(), nni * sizeof(double));
+    if (!rc) rc = x2g_h2d(ctx, g->bmu_orb_f, bof.data(), nmu * sizeof(float));
+    if (!rc) rc = x2g_h2d(ctx, g->bmu_pot_f, bpf.data(), nmu * sizeof(float));
+    if (!rc) rc = x2g_h2d(ctx, g->dnu_f, dnf.data(), nni * sizeof(float));
+    if (!rc) rc = x2g_h2d(ctx, g->vxi, vec, nmu * sizeof(double));
+    if (!rc) rc = x2g_h2d(ctx, g->vxisq, vec + nmu, nmu * sizeof(double));
+    if (!rc) rc = x2g_h2d(ctx, g->veta, vec + 2 * nmu, nni * sizeof(double));
+    if (!rc) rc = x2g_h2d(ctx, g->vetasq, vec + 2 * nmu + nni, nni * sizeof(double));
+    if (rc) {
+        x2g_grid_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return X2G_OK;
+}
+
+extern "C" int x2g_grid_destroy(x2g_grid *g)
+{
+    if (!g) return X2G_OK;
+    x2g_ctx *c = g->ctx;
+    void *ptrs[] = {g->suppl, g->bmu_orb, g->bmu_pot, g->dnu, g->bmu_orb_f, g->bmu_pot_f,
+                    g->dnu_f, g->vxi, g->vxisq, g->veta, g->vetasq};
+    for (void *p : ptrs)
+        if (p) x2g_free(c, p);
+    delete g;
+    return X2G_OK;
+}
+
+extern "C" const double *x2g_grid_suppl(x2g_grid *g, int k)
+{
+    if (!g || k < 0 || k >= 14) return nullptr;
+    return g->suppl + (size_t)k * g->n();
+}

@@ -1,0 +1,74 @@
+// Internal declarations of the x2gpu library (MI355X / gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "../../include/x2gpu.h"
+
+namespace x2g {
+
+void set_error(const char *fmt, ...);
+
+#define X2G_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::x2g::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,                 \
+                             hipGetErrorString(e_));                                    \
+            return X2G_EHIP;                                                            \
+        }                                                                               \
+    } while (0)
+
+#define X2G_CHECK(cond, ...)                                                            \
+    do {                                                                                \
+        if (!(cond)) {                                                                  \
+            ::x2g::set_error(__VA_ARGS__);                                              \
+            return X2G_EINVAL;                                                          \
+        }                                                                               \
+    } while (0)
+
+// Growable device workspace owned by a context (never freed while a launch
+// that uses it may still run: growth synchronises the stream first).
+struct Workspace {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace x2g
+
+struct x2g_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    size_t max_lds = 0;
+    x2g::Workspace ws[4];          // 0: padded work buffers, 1: reduction partials,
+                                   // 2: launch descriptors, 3: FP32 staging
+    void *pinned = nullptr;        // pinned host staging for descriptors/results
+    size_t pinned_bytes = 0;
+};
+
+struct x2g_grid {
+    x2g_ctx *ctx = nullptr;
+    int nni = 0, nmu = 0;
+    double coef[26];               // dmu1,dmu2,dni1,dni2 (16), exeven (5), diag, hni, hmu, r
+    double *suppl = nullptr;       // 14 * nni*nmu
+    double *bmu_orb = nullptr;     // b(mu) for orbitals  (borb(1,mu))
+    double *bmu_pot = nullptr;     // b(mu) for potentials (bpot(1,mu))
+    double *dnu = nullptr;         // d(nu)  (d(nu,1))
+    float *bmu_orb_f = nullptr, *bmu_pot_f = nullptr, *dnu_f = nullptr;
+    double *vxi = nullptr, *vxisq = nullptr, *veta = nullptr, *vetasq = nullptr;
+    size_t n() const { return (size_t)nni * nmu; }
+    size_t npad() const { return (size_t)(nni + 8) * (nmu + 8); }
+};
+
+namespace x2g {
+// Reserve ws[k] of at least `bytes` (synchronises the stream on growth).
+int ws_reserve(x2g_ctx *ctx, int k, size_t bytes);
+int pinned_reserve(x2g_ctx *ctx, size_t bytes);
+
+// Kernel-level entry points shared between translation units.
+int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w2);
+int launch_putout(x2g_grid *g, double *fun, const double *work);
+}  // namespace x2g

@@ -1,0 +1,463 @@
+// Elementwise and quadrature kernels around the relaxation: the Fock
+// potential assembly (fock.F90:57-184), the Newton-Cotes quadratures of
+// norm/ortho/EaHF/etotalHF (ddot against wgt1/wgt2), the one-electron integral
+// with the 9-point (nu,mu) Laplacian (integrals.F90:406-480,
+// utils.f90:599-695), the multipole moments (scfUtils.F90:490-850), the
+// multipole tails of the potentials (scfUtils.F90:275-329, 1039-1270) and the
+// g/u symmetrisation (scfUtils.F90:380-406).
+//
+// All reductions are FP64, two-pass and deterministic: each workgroup reduces
+// a fixed slice (wavefront shuffles, then LDS across the 4 wavefronts), and a
+// second kernel adds the per-workgroup partials in a fixed order.
+#include "x2gpu_internal.h"
+#include <algorithm>
+#include <cstring>
+
+namespace x2g {
+
+static inline int gx(long work, int block) { return (int)std::min<long>((work + block - 1) / block, 1024); }
+
+template <typename D>
+static int stage(x2g_ctx *ctx, const D *v, size_t cnt, size_t &offset, const D **out)
+{
+    const size_t bytes = cnt * sizeof(D);
+    const size_t aligned = (offset + 255) & ~size_t(255);
+    X2G_CHECK(aligned + bytes <= ctx->ws[2].bytes && aligned + bytes <= ctx->pinned_bytes,
+              "launch descriptors exceed the staging area");
+    std::memcpy((char *)ctx->pinned + aligned, v, bytes);
+    X2G_HIP(hipMemcpyAsync((char *)ctx->ws[2].ptr + aligned, (char *)ctx->pinned + aligned, bytes,
+                           hipMemcpyHostToDevice, ctx->stream));
+    *out = reinterpret_cast<const D *>((char *)ctx->ws[2].ptr + aligned);
+    offset = aligned + bytes;
+    return X2G_OK;
+}
+
+constexpr int kRedBlock = 256;   // 4 wavefronts
+constexpr int kRedBlocks = 240;  // partials per term (fixed: deterministic order)
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Block-wide sum of up to NV values per thread; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *sh)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        v[k] = wave_sum(v[k]);
+        if (lane == 0) sh[w * NV + k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = sh[k] + sh[NV + k] + sh[2 * NV + k] + sh[3 * NV + k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_fill(double *dst, size_t n, double v)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        dst[e] = v;
+}
+
+__global__ void k_lincomb(double *out, const double *base, double bscale, const x2g_term *__restrict__ t, int nt,
+                          long n)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        double v = base ? bscale * base[e] : 0.0;
+        for (int k = 0; k < nt; ++k) {
+            double x = t[k].c * t[k].a_d[e];
+            if (t[k].b_d) x *= t[k].b_d[e];
+            if (t[k].cc_d) x *= t[k].cc_d[e];
+            v += x;
+        }
+        out[e] = v;
+    }
+}
+
+// partial[k*kRedBlocks + blk] = sum over the block's slice of w*a*b*c
+__global__ void __launch_bounds__(kRedBlock) k_wdot(const x2g_dot *__restrict__ t, long n, double *partial)
+{
+    __shared__ double sh[4];
+    const x2g_dot &d = t[blockIdx.y];
+    double s[1] = {0.0};
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        double x = d.a_d[e];
+        if (d.b_d) x *= d.b_d[e];
+        if (d.cc_d) x *= d.cc_d[e];
+        s[0] += d.w_d[e] * x;
+    }
+    block_sum<1>(s, sh);
+    if (threadIdx.x == 0) partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = s[0];
+}
+
+// out[k] = sum of the nb partials of term k (fixed order, one block per term)
+__global__ void __launch_bounds__(kRedBlock) k_finish(const double *partial, int nb, int nv, double *out)
+{
+    __shared__ double sh[4];
+    double s[1] = {0.0};
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s[0] += partial[(long)blockIdx.x * nb + i];
+    block_sum<1>(s, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+    (void)nv;
+}
+
+// ---------------------------------------------------------------------------
+// One-electron integral.  W is psi1 embedded with its mirror halos (putin;
+// the rows above mu=nmu are zero).  For every grid point (nu,mu):
+//   T psi1 = sum_k dni(k,nu) W(nu+k,mu) + sum_k dmu(k,mu) W(nu,mu+k),
+//   dni(k,nu) = a2n(k) + d(nu) a1n(k),  dmu(k,mu) = a2m(k) + borb(mu) a1m(k)
+// (9-point first/second derivative weights, initSuppl
+// initCBAllocArrays.F90:1993-2032), then
+//   kin  += wgt1*psi2*(T psi1 + m^2 e psi1),   nuc += wgt1*psi2*f0*psi1.
+struct OneelArgs {
+    const double *w;
+    const double *psi1, *psi2, *wgt1, *e, *f0, *bmu, *dnu;
+    double a1n[9], a2n[9], a1m[9], a2m[9];
+    double m2;
+    int nni, nmu;
+};
+
+__global__ void __launch_bounds__(kRedBlock) k_oneel(OneelArgs A, double *partial)
+{
+    __shared__ double sh[8];
+    const int nni = A.nni, nmu = A.nmu;
+    const long s1 = nni + 8;
+    const long n = (long)nni * nmu;
+    double acc[2] = {0.0, 0.0};
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int mu = 1 + (int)(e / nni), nu = 1 + (int)(e % nni);
+        const long p = (long)(mu + 3) * s1 + nu + 3;
+        const double dn = A.dnu[nu - 1], bm = A.bmu[mu - 1];
+        double tn = 0.0, tm = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            tn += A.w[p + (k - 4)] * (A.a2n[k] + dn * A.a1n[k]);
+            tm += A.w[p + (long)(k - 4) * s1] * (A.a2m[k] + bm * A.a1m[k]);
+        }
+        double t = tn + tm;
+        const double ps1 = A.psi1[e], ps2 = A.psi2[e], wg = A.wgt1[e];
+        if (A.m2 != 0.0) t += A.m2 * A.e[e] * ps1;
+        acc[0] += wg * (t * ps2);
+        acc[1] += wg * (ps1 * A.f0[e] * ps2);
+    }
+    block_sum<2>(acc, sh);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = acc[0];
+        partial[gridDim.x + blockIdx.x] = acc[1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Normalised associated Legendre values N_kq P_k^q(x), k = 1..kmax, with the
+// Condon-Shortley phase, N_kq = sqrt((k-q)!/(k+q)!); standard upward
+// recurrence in k from P_q^q = (-1)^q (2q-1)!! (1-x^2)^(q/2).
+// `quirk3` reproduces the reference tail formula for q = 3, which raises
+// (1-x^2) to the power 2 instead of 3/2 (vexch, scfUtils.F90:1160-1163).
+__device__ inline void assoc_legendre(int q, double x, int kmax, double *out, bool quirk3)
+{
+    for (int k = 0; k < 8; ++k) out[k] = 0.0;
+    const double s2 = fabs(1.0 - x * x);
+    double pmm = 1.0;  // P_q^q
+    for (int i = 1; i <= q; ++i) pmm *= -(double)(2 * i - 1);
+    if (q > 0) {
+        double sp = (q == 3 && quirk3) ? s2 * s2 : pow(s2, 0.5 * q);
+        pmm *= sp;
+    }
+    double pkm2 = 0.0, pkm1 = pmm;  // P_{q-1}^q = 0, P_q^q
+    for (int k = q; k <= kmax; ++k) {
+        double pk;
+        if (k == q) pk = pmm;
+        else pk = ((double)(2 * k - 1) * x * pkm1 - (double)(k + q - 1) * pkm2) / (double)(k - q);
+        if (k > q) { pkm2 = pkm1; pkm1 = pk; }
+        // normalisation sqrt((k-q)!/(k+q)!)
+        double nrm = 1.0;
+        for (int j = k - q + 1; j <= k + q; ++j) nrm /= (double)j;
+        if (k >= 1) out[k - 1] = pk * sqrt(nrm);
+    }
+}
+
+// Plain Legendre P_1..P_kmax in the reference's recurrence order
+// (initLP, initCBAllocArrays.F90:1547-1552).
+__device__ inline void legendre_ref(double c, int kmax, double *out)
+{
+    for (int k = 0; k < 8; ++k) out[k] = 0.0;
+    out[0] = c;
+    out[1] = (3.0 * c * c - 1.0) * 0.5;
+    for (int n = 2; n <= kmax - 1; ++n) out[n] = ((double)(2 * n + 1) * c * out[n - 1] - (double)n * out[n - 2]) / (double)(n + 1);
+}
+
+struct MomArgs {
+    const double *psi1, *psi2, *wgt2, *f4, *vxi, *vxisq, *veta, *vetasq;
+    double r2;
+    int nni, nmu, dm, mpole;
+};
+
+__global__ void __launch_bounds__(kRedBlock) k_moments(MomArgs A, double *partial)
+{
+    __shared__ double sh[32];
+    const int nni = A.nni;
+    const long n = (long)nni * A.nmu;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const double eps = 2.220446049250313e-16;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int mu = (int)(e / nni), nu = (int)(e % nni);
+        const double rr = sqrt(A.vxisq[mu] + A.vetasq[nu] - 1.0);
+        double c = 0.0;
+        if (A.dm == 0) { if (!(fabs(rr) < eps)) c = A.vxi[mu] * A.veta[nu] / rr; }
+        else { if (fabs(rr) > eps) c = A.veta[nu] * A.vxi[mu] / rr; }
+        double P[8];
+        if (A.dm == 0) legendre_ref(c, A.mpole, P);
+        else assoc_legendre(A.dm, c, A.mpole, P, false);
+        const double xr = A.r2 * rr;
+        const double dens = A.f4[e] * (A.psi1[e] * A.psi2[e]);
+        const double wd = A.wgt2[e];
+        double xw = 1.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            xw *= xr;
+            if (k < A.mpole) acc[k] += wd * ((P[k] * xw) * dens);
+        }
+    }
+    block_sum<8>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 8; ++k) partial[(long)k * gridDim.x + blockIdx.x] = acc[k];
+}
+
+// Potential tail rows mu = nmu-3..nmu.
+struct TailArgs {
+    double *pot;
+    const double *vxi, *vxisq, *veta, *vetasq;
+    double mom[8];
+    double r2;
+    int nni, nmu, kind, dm, mpole;
+};
+
+__global__ void k_pot_tail(TailArgs A)
+{
+    const int nni = A.nni;
+    const double eps = 2.220446049250313e-16;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < 4 * nni; e += gridDim.x * blockDim.x) {
+        const int j = A.nmu - 3 + e / nni;  // 1-based mu
+        const int i = 1 + e % nni;
+        const double rr = sqrt(A.vxisq[j - 1] + A.vetasq[i - 1] - 1.0);
+        double v;
+        if (A.kind == 0) {
+            // coulAsympt + vcoul (scfUtils.F90:275-296, 1039-1068)
+            const double rr1 = 1.0 / (rr * A.r2);
+            const double c = A.veta[i - 1] * A.vxi[j - 1] / rr;
+            double P[8];
+            legendre_ref(c, A.mpole, P);
+            double pe = 0.0;
+            for (int m = 1; m <= A.mpole; ++m) pe += A.mom[m - 1] * P[m - 1] * pow(rr1, (double)(m + 1));
+            v = A.r2 * A.vxi[j - 1] * (rr1 + pe);
+        } else {
+            // exchAsympt + vexch (scfUtils.F90:310-329, 1075-1270)
+            double c = 0.0;
+            if (fabs(rr) > eps) c = A.veta[i - 1] * A.vxi[j - 1] / rr;
+            const double rrs = rr * A.r2;
+            double P[8];
+            if (A.dm == 0) legendre_ref(c, A.mpole, P);
+            else assoc_legendre(A.dm, c, A.mpole, P, true);
+            double xrn = rrs * rrs, pe = 0.0;
+            for (int m = 1; m <= A.mpole; ++m) {
+                if (m > 1) xrn *= rrs;
+                pe = pe + P[m - 1] * A.mom[m - 1] / xrn;
+            }
+            v = (A.dm > 4) ? 0.0 : A.r2 * A.vxi[j - 1] * pe;
+        }
+        A.pot[(long)(j - 1) * nni + i - 1] = v;
+    }
+}
+
+__global__ void k_setci(double *psi, int nni, int nmu, int ihsym)
+{
+    const int n2 = nni / 2 + 1;
+    const long tot = (long)(n2 - 1) * nmu;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+        const int mi = (int)(e / (n2 - 1)), ni = 1 + (int)(e % (n2 - 1));
+        const double v = psi[(long)mi * nni + ni - 1];
+        psi[(long)mi * nni + (nni - ni)] = ihsym == 1 ? v : -v;
+        if (ihsym != 1 && ni == 1) psi[(long)mi * nni + n2 - 1] = 0.0;
+    }
+}
+
+// 9-point weights (initSuppl, initCBAllocArrays.F90:1670-1673, 1993-2011):
+// a1 = {3,-32,168,-672,0,672,-168,32,-3}/(840 h), a2 = {-9,128,-1008,8064,
+// -14350,8064,-1008,128,-9}/(5040 h^2).
+static void deriv_weights(double h, double *a1, double *a2)
+{
+    static const double aa1[9] = {3.0, -32.0, 168.0, -672.0, 0.0, 672.0, -168.0, 32.0, -3.0};
+    static const double aa2[9] = {-9.0, 128.0, -1008.0, 8064.0, -14350.0, 8064.0, -1008.0, 128.0, -9.0};
+    const double w1 = 1.0 / 840.0, w2 = 1.0 / 5040.0;
+    for (int k = 0; k < 9; ++k) {
+        a2[k] = w2 * aa2[k] / (h * h);
+        a1[k] = w1 * aa1[k] / h;
+    }
+}
+
+}  // namespace x2g
+
+using namespace x2g;
+
+extern "C" int x2g_fill(x2g_ctx *ctx, double *dst_d, size_t n, double value)
+{
+    X2G_CHECK(ctx && (dst_d || n == 0), "x2g_fill: null argument");
+    if (n == 0) return X2G_OK;
+    hipLaunchKernelGGL(k_fill, dim3(gx((long)n, 256)), dim3(256), 0, ctx->stream, dst_d, n, value);
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
+extern "C" int x2g_lincomb(x2g_grid *g, double *out_d, const double *base_d, double base_scale,
+                           const x2g_term *terms, int nterms)
+{
+    X2G_CHECK(g && out_d && nterms >= 0 && (terms || nterms == 0), "x2g_lincomb: null argument");
+    for (int k = 0; k < nterms; ++k) X2G_CHECK(terms[k].a_d, "x2g_lincomb: term %d has no a", k);
+    x2g_ctx *ctx = g->ctx;
+    size_t off = 0;
+    const x2g_term *dt = nullptr;
+    if (nterms) {
+        int rc = stage(ctx, terms, (size_t)nterms, off, &dt);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_lincomb, dim3(gx((long)g->n(), 256)), dim3(256), 0, ctx->stream, out_d, base_d, base_scale,
+                       dt, nterms, (long)g->n());
+    X2G_HIP(hipGetLastError());
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    return X2G_OK;
+}
+
+static int reduce_finish(x2g_ctx *ctx, double *partial, int nv, double *out_h)
+{
+    double *dout = partial + (size_t)nv * kRedBlocks;
+    hipLaunchKernelGGL(k_finish, dim3(nv), dim3(kRedBlock), 0, ctx->stream, partial, kRedBlocks, nv, dout);
+    X2G_HIP(hipGetLastError());
+    X2G_HIP(hipMemcpyAsync(out_h, dout, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    return X2G_OK;
+}
+
+extern "C" int x2g_wdot(x2g_grid *g, const x2g_dot *terms, int nterms, double *out_h)
+{
+    X2G_CHECK(g && terms && out_h && nterms > 0, "x2g_wdot: null argument");
+    for (int k = 0; k < nterms; ++k) X2G_CHECK(terms[k].w_d && terms[k].a_d, "x2g_wdot: term %d incomplete", k);
+    x2g_ctx *ctx = g->ctx;
+    int rc = ws_reserve(ctx, 1, (size_t)nterms * (kRedBlocks + 1) * sizeof(double));
+    if (rc) return rc;
+    size_t off = 0;
+    const x2g_dot *dt;
+    if ((rc = stage(ctx, terms, (size_t)nterms, off, &dt))) return rc;
+    double *partial = (double *)ctx->ws[1].ptr;
+    hipLaunchKernelGGL(k_wdot, dim3(kRedBlocks, nterms), dim3(kRedBlock), 0, ctx->stream, dt, (long)g->n(), partial);
+    X2G_HIP(hipGetLastError());
+    return reduce_finish(ctx, partial, nterms, out_h);
+}
+
+extern "C" int x2g_oneel(x2g_grid *g, const double *psi1_d, const double *psi2_d, int isym, int m, double *out_h)
+{
+    X2G_CHECK(g && psi1_d && psi2_d && out_h, "x2g_oneel: null argument");
+    X2G_CHECK(isym == 1 || isym == -1, "x2g_oneel: isym");
+    x2g_ctx *ctx = g->ctx;
+    const size_t pb = (g->npad() * sizeof(double) + 255) & ~size_t(255);
+    int rc = ws_reserve(ctx, 0, pb);
+    if (rc) return rc;
+    if ((rc = ws_reserve(ctx, 1, 3 * (kRedBlocks + 1) * sizeof(double)))) return rc;
+    double *w = (double *)ctx->ws[0].ptr;
+    if ((rc = launch_putin(g, isym, psi1_d, w, nullptr))) return rc;
+    OneelArgs A{};
+    A.w = w;
+    A.psi1 = psi1_d;
+    A.psi2 = psi2_d;
+    A.wgt1 = g->suppl + 12 * g->n();
+    A.e = g->suppl + 3 * g->n();
+    A.f0 = g->suppl + 4 * g->n();
+    A.bmu = g->bmu_orb;
+    A.dnu = g->dnu;
+    deriv_weights(g->coef[22], A.a1n, A.a2n);  // hni
+    deriv_weights(g->coef[23], A.a1m, A.a2m);  // hmu
+    A.m2 = (double)(m * m);
+    A.nni = g->nni;
+    A.nmu = g->nmu;
+    double *partial = (double *)ctx->ws[1].ptr;
+    hipLaunchKernelGGL(k_oneel, dim3(kRedBlocks), dim3(kRedBlock), 0, ctx->stream, A, partial);
+    X2G_HIP(hipGetLastError());
+    double r[2];
+    if ((rc = reduce_finish(ctx, partial, 2, r))) return rc;
+    out_h[0] = r[0] + r[1];
+    out_h[1] = r[0];
+    out_h[2] = r[1];
+    return X2G_OK;
+}
+
+extern "C" int x2g_moments(x2g_grid *g, const double *psi1_d, const double *psi2_d, int dm, int mpole, double *out_h)
+{
+    X2G_CHECK(g && psi1_d && psi2_d && out_h, "x2g_moments: null argument");
+    X2G_CHECK(mpole >= 2 && mpole <= 8, "x2g_moments: mpole must be 2..8");
+    X2G_CHECK(dm >= 0 && dm <= 4, "x2g_moments: |Delta m| must be 0..4");
+    x2g_ctx *ctx = g->ctx;
+    int rc = ws_reserve(ctx, 1, 8 * (kRedBlocks + 1) * sizeof(double));
+    if (rc) return rc;
+    MomArgs A{};
+    A.psi1 = psi1_d;
+    A.psi2 = psi2_d;
+    A.wgt2 = g->suppl + 13 * g->n();
+    A.f4 = g->suppl + 8 * g->n();
+    A.vxi = g->vxi;
+    A.vxisq = g->vxisq;
+    A.veta = g->veta;
+    A.vetasq = g->vetasq;
+    A.r2 = 0.5 * g->coef[25];
+    A.nni = g->nni;
+    A.nmu = g->nmu;
+    A.dm = dm;
+    A.mpole = mpole;
+    double *partial = (double *)ctx->ws[1].ptr;
+    hipLaunchKernelGGL(k_moments, dim3(kRedBlocks), dim3(kRedBlock), 0, ctx->stream, A, partial);
+    X2G_HIP(hipGetLastError());
+    double r[8];
+    if ((rc = reduce_finish(ctx, partial, 8, r))) return rc;
+    for (int k = 0; k < mpole; ++k) out_h[k] = r[k];
+    return X2G_OK;
+}
+
+extern "C" int x2g_pot_tail(x2g_grid *g, double *pot_d, int kind, int dm, int mpole, const double *mom)
+{
+    X2G_CHECK(g && pot_d && mom, "x2g_pot_tail: null argument");
+    X2G_CHECK(kind == 0 || kind == 1, "x2g_pot_tail: kind");
+    X2G_CHECK(mpole >= 2 && mpole <= 8, "x2g_pot_tail: mpole must be 2..8");
+    X2G_CHECK(dm >= 0, "x2g_pot_tail: dm < 0");
+    TailArgs A{};
+    A.pot = pot_d;
+    A.vxi = g->vxi;
+    A.vxisq = g->vxisq;
+    A.veta = g->veta;
+    A.vetasq = g->vetasq;
+    for (int k = 0; k < 8; ++k) A.mom[k] = k < mpole ? mom[k] : 0.0;
+    A.r2 = 0.5 * g->coef[25];
+    A.nni = g->nni;
+    A.nmu = g->nmu;
+    A.kind = kind;
+    A.dm = dm;
+    A.mpole = mpole;
+    hipLaunchKernelGGL(k_pot_tail, dim3(gx(4L * g->nni, 128)), dim3(128), 0, g->ctx->stream, A);
+    X2G_HIP(hipGetLastError());
+    X2G_HIP(hipStreamSynchronize(g->ctx->stream));
+    return X2G_OK;
+}
+
+extern "C" int x2g_setci(x2g_grid *g, double *psi_d, int ihsym)
+{
+    X2G_CHECK(g && psi_d, "x2g_setci: null argument");
+    X2G_CHECK(ihsym == 1 || ihsym == -1, "x2g_setci: ihsym");
+    const long tot = (long)(g->nni / 2) * g->nmu;
+    hipLaunchKernelGGL(k_setci, dim3(gx(tot, 256)), dim3(256), 0, g->ctx->stream, psi_d, g->nni, g->nmu, ihsym);
+    X2G_HIP(hipGetLastError());
+    X2G_HIP(hipStreamSynchronize(g->ctx->stream));
+    return X2G_OK;
+}

@@ -1,0 +1,1085 @@
+// MCSOR relaxation of the 8th-order 17-point cross stencil on the (nu,mu)
+// prolate-spheroidal grid -- MI355X (gfx950) kernels.
+//
+// Method (unchanged from the reference, sormcsor.F90:143-386): each sweep
+// updates the relaxed interior nu=2..nni-1, mu=2..nmu-4 in five colour passes,
+//   f(p) = omega*(rhs - L f)/lhs + (1-omega)*f(p),
+// with L the 9-point d2/dmu2 + b(mu) d/dmu + d2/dnu2 + d(nu) d/dnu stencil
+// (lhs carries the diagonal), and then extrapolates the mu=1, nu=1 and
+// nu=nni boundary lines (sormcsor.F90:333-383).  The colour of (nu,mu) is
+// the reference's col-wise pass index ((mu-2)*(nni-2) + (nu-2)) mod 5
+// (initCBAllocArrays.F90:2467-2497 with the pass split of sormcsor.F90:170-297);
+// for nni = 2 (mod 5), where that pass split is not a proper colouring, the
+// geometric colouring (nu - mu) mod 5 is used instead.
+//
+// Two implementations:
+//  * k_phase / k_boundary: one launch per colour pass, in place.  Simple,
+//    used as the cross-check path and for grids too wide for the LDS ring.
+//  * k_strip: one launch per sweep.  A workgroup owns a strip of full nu
+//    rows [a, a+L) of one problem and pipelines the five colour passes
+//    through an LDS ring of 4-row blocks (pass p runs SKEW blocks behind pass
+//    p-1), recomputing a 16-row halo on each side; it reads the previous
+//    sweep from `src` and writes the final rows (plus the nu=1/nu=nni boundary
+//    values, and the mu=1 row for the first strip) to `dst`.  Every problem
+//    of a batch, on any grid, is handled by the same launch.
+#include "x2gpu_internal.h"
+#include <algorithm>
+#include <cstring>
+
+namespace x2g {
+
+__host__ __device__ inline int colour_mult(int nni)
+{
+    int a = (nni - 2) % 5;
+    return a == 0 ? 4 : a;
+}
+
+__device__ __forceinline__ int mod5(int x)
+{
+    int r = x % 5;
+    return r < 0 ? r + 5 : r;
+}
+
+template <typename T>
+struct SweepDesc {
+    const T *src;
+    T *dst;
+    const T *lhs;
+    const T *rhs;
+    const T *bmu;
+    const T *dnu;
+    T c[16];  // dmu1[4], dmu2[4], dni1[4], dni2[4]
+    T ex[5];
+    T omega, omega1;
+    int nni, nmu, isym, cA;
+    int wg_begin, L;
+};
+
+// ---------------------------------------------------------------------------
+// Point update, term order of sormcsor.F90:173-194.  `f` addresses the padded
+// array, s1 is the mu stride (nni+8).
+template <typename T>
+__device__ __forceinline__ T stencil_cc(const T *f, long p, long s1, const T *c, T b, T d)
+{
+    const long s2 = 2 * s1, s3 = 3 * s1, s4 = 4 * s1;
+    T ddmi2 = c[4] * (f[p - s4] + f[p + s4]) + c[5] * (f[p - s3] + f[p + s3]) +
+              c[6] * (f[p - s2] + f[p + s2]) + c[7] * (f[p - s1] + f[p + s1]);
+    T ddmi1 = c[0] * (f[p - s4] - f[p + s4]) + c[1] * (f[p - s3] - f[p + s3]) +
+              c[2] * (f[p - s2] - f[p + s2]) + c[3] * (f[p - s1] - f[p + s1]);
+    T ddni2 = c[12] * (f[p - 4] + f[p + 4]) + c[13] * (f[p - 3] + f[p + 3]) +
+              c[14] * (f[p - 2] + f[p + 2]) + c[15] * (f[p - 1] + f[p + 1]);
+    T ddni1 = c[8] * (f[p - 4] - f[p + 4]) + c[9] * (f[p - 3] - f[p + 3]) +
+              c[10] * (f[p - 2] - f[p + 2]) + c[11] * (f[p - 1] - f[p + 1]);
+    return ddmi2 + b * ddmi1 + ddni2 + d * ddni1;
+}
+
+// ---------------------------------------------------------------------------
+// Per-pass in-place kernel: blockIdx.y = problem.
+template <typename T>
+__global__ void k_phase(const SweepDesc<T> *__restrict__ descs, int pass)
+{
+    const SweepDesc<T> &d = descs[blockIdx.y];
+    const int nni = d.nni, nrow = nni - 2;
+    const long s1 = nni + 8;
+    const long npts = (long)nrow * (d.nmu - 5);
+    T *f = d.dst;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < npts;
+         k += (long)gridDim.x * blockDim.x) {
+        const int mu = 2 + (int)(k / nrow), nu = 2 + (int)(k % nrow);
+        if (mod5((nu - 2) + d.cA * (mu - 2)) != pass) continue;
+        const long p = (long)(mu + 3) * s1 + (nu + 3);
+        const long u = (long)(mu - 1) * nni + (nu - 1);
+        const T cc = stencil_cc(f, p, s1, d.c, d.bmu[mu - 1], d.dnu[nu - 1]);
+        f[p] = d.omega * (d.rhs[u] - cc) / d.lhs[u] + d.omega1 * f[p];
+    }
+}
+
+// Boundary lines after a sweep (sormcsor.F90:94-129 with the index sets of
+// initCBAllocArrays.F90:2823-2842).  The corner values at mu=1 are formed
+// from the freshly extrapolated mu=1 row exactly as the reference's sequential
+// order does, but recomputed per thread so no cross-thread ordering is needed.
+template <typename T>
+__device__ __forceinline__ T row1_value(const T *f, int nu, long s1, const T *ex)
+{
+    const long p = (long)(1 + 3) * s1 + (nu + 3);
+    return ex[0] * f[p + s1] + ex[1] * f[p + 2 * s1] + ex[2] * f[p + 3 * s1] +
+           ex[3] * f[p + 4 * s1] + ex[4] * f[p + 5 * s1];
+}
+
+template <typename T>
+__global__ void k_boundary(const SweepDesc<T> *__restrict__ descs)
+{
+    const SweepDesc<T> &d = descs[blockIdx.y];
+    const int nni = d.nni, nmu = d.nmu;
+    const long s1 = nni + 8;
+    T *f = d.dst;
+    const int nrow1 = nni - 2;       // mu=1 row, nu=2..nni-1
+    const int ncol = nmu - 4;        // nu=1 / nu=nni columns, mu=1..nmu-4
+    const int total = nrow1 + 2 * ncol;
+    const bool even = d.isym == 1;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        if (t < nrow1) {
+            const int nu = 2 + t;
+            f[(long)4 * s1 + nu + 3] = even ? row1_value(f, nu, s1, d.ex) : T(0);
+        } else {
+            const int q = t - nrow1;
+            const int side = q / ncol;          // 0: nu=1, 1: nu=nni
+            const int mu = 1 + q % ncol;
+            const int nu = side == 0 ? 1 : nni;
+            const int dir = side == 0 ? 1 : -1;
+            T v = T(0);
+            if (even) {
+                if (mu == 1) {
+                    v = d.ex[0] * row1_value(f, nu + dir, s1, d.ex) +
+                        d.ex[1] * row1_value(f, nu + 2 * dir, s1, d.ex) +
+                        d.ex[2] * row1_value(f, nu + 3 * dir, s1, d.ex) +
+                        d.ex[3] * row1_value(f, nu + 4 * dir, s1, d.ex) +
+                        d.ex[4] * row1_value(f, nu + 5 * dir, s1, d.ex);
+                } else {
+                    const long p = (long)(mu + 3) * s1 + (nu + 3);
+                    v = d.ex[0] * f[p + dir] + d.ex[1] * f[p + 2 * dir] + d.ex[2] * f[p + 3 * dir] +
+                        d.ex[3] * f[p + 4 * dir] + d.ex[4] * f[p + 5 * dir];
+                }
+            }
+            f[(long)(mu + 3) * s1 + nu + 3] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined strip kernel.
+//
+// Rows are grouped in blocks of 4: block b of a strip starting at row a
+// holds rows r0+4b .. r0+4b+3 with r0 = a-16.  Pass p updates blocks
+// b in [p, K-1-p] (K = L/4 + 8), i.e. rows [a-16+4p, a+L+16-4p), and at step
+// s it processes block s - SKEW*p.  Pass p of a block reads rows within one
+// block of it; with SKEW = 2 the blocks touched by different passes in one
+// step are two or more blocks apart, so a step needs a single barrier; with
+// SKEW = 1 the five passes of a step are separated by barriers.  Colours of
+// lower passes are always final and colours of higher passes still hold the
+// previous sweep when a block is processed, which is exactly the sequential
+// five-pass order restricted to the strip; the 16-row halo makes the rows
+// [a, a+L) independent of what lies beyond the strip's 20-row read window.
+//
+// LDS: a ring of RB = 4*SKEW+4 blocks (4 rows of nni+8 values each); during
+// step s blocks s-4*SKEW-1 .. s+1 are live and block s+2 is prefetched
+// through registers while the step computes.
+constexpr int kMaxLoads = 8;  // prefetch registers per thread (4*(nni+8) <= 8*blockDim)
+
+template <typename T, int SKEW>
+__global__ void __launch_bounds__(512) k_strip(const SweepDesc<T> *__restrict__ descs, int nprob)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T *ring = reinterpret_cast<T *>(smem_raw);
+    constexpr int RB = 4 * SKEW + 4;
+
+    // locate the problem owning this workgroup (descs sorted by wg_begin)
+    const int wg = blockIdx.x;
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (descs[mid].wg_begin <= wg) lo = mid; else hi = mid - 1;
+    }
+    const SweepDesc<T> &d = descs[lo];
+    const int nni = d.nni, nmu = d.nmu, N8 = nni + 8, L = d.L;
+    const int chunk = wg - d.wg_begin;
+    const int a = 2 + chunk * L;
+    const int aend = min(a + L, nmu - 3);
+    const int r0 = a - 16;
+    const int K = L / 4 + 8;
+    const int S = K + 4 * SKEW - 3;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int M = (nni - 2 + 4) / 5;      // max colour points per row
+    const bool even = d.isym == 1;
+    const T *__restrict__ src = d.src;
+    T *__restrict__ dst = d.dst;
+
+    auto lds_row = [&](int mu) -> T * {
+        const int q = mu - r0 + 4;          // >= 0 for every row touched
+        const int b = (q >> 2) - 1;
+        const int slot = (b + RB) % RB;
+        return ring + (long)(slot * 4 + (q & 3)) * N8;
+    };
+    auto load_block_direct = [&](int b) {
+        const int slot = (b + RB) % RB;
+        for (int e = tid; e < 4 * N8; e += nth) {
+            const int rr = e / N8, col = e - rr * N8;
+            const int mu = r0 + 4 * b + rr;
+            T v = T(0);
+            if (mu >= -3 && mu <= nmu + 4) v = src[(long)(mu + 3) * N8 + col];
+            ring[(long)(slot * 4 + rr) * N8 + col] = v;
+        }
+    };
+
+    load_block_direct(-1);
+    load_block_direct(0);
+    load_block_direct(1);
+    __syncthreads();
+
+    for (int s = 0; s < S; ++s) {
+        // 1. prefetch block s+2 into registers
+        const int bl = s + 2;
+        T pre[kMaxLoads];
+        const bool do_load = bl <= K;
+        if (do_load) {
+#pragma unroll
+            for (int j = 0; j < kMaxLoads; ++j) {
+                const int e = tid + j * nth;
+                T v = T(0);
+                if (e < 4 * N8) {
+                    const int rr = e / N8, col = e - rr * N8;
+                    const int mu = r0 + 4 * bl + rr;
+                    if (mu >= -3 && mu <= nmu + 4) v = src[(long)(mu + 3) * N8 + col];
+                }
+                pre[j] = v;
+            }
+        }
+
+        // 2. write out the block finalised by the last pass in step s-1
+        const int bw = s - 4 * SKEW - 1;
+        if (bw >= 4 && bw <= K - 5) {
+            for (int e = tid; e < 4 * nni; e += nth) {
+                const int rr = e / nni, nu = 1 + e - rr * nni;
+                const int mu = r0 + 4 * bw + rr;
+                if (mu < a || mu >= aend) continue;
+                const T *row = lds_row(mu);
+                T v;
+                if (nu == 1 || nu == nni) {
+                    const int dir = nu == 1 ? 1 : -1;
+                    const int c0 = nu + 3;
+                    v = even ? d.ex[0] * row[c0 + dir] + d.ex[1] * row[c0 + 2 * dir] +
+                                   d.ex[2] * row[c0 + 3 * dir] + d.ex[3] * row[c0 + 4 * dir] +
+                                   d.ex[4] * row[c0 + 5 * dir]
+                             : T(0);
+                } else {
+                    v = row[nu + 3];
+                }
+                dst[(long)(mu + 3) * N8 + nu + 3] = v;
+            }
+            // the mu=1 row once rows 2..6 are final (first strip only)
+            if (a == 2 && bw == 5) {
+                const T *rw[5] = {lds_row(2), lds_row(3), lds_row(4), lds_row(5), lds_row(6)};
+                auto r1 = [&](int nu) -> T {
+                    const int c0 = nu + 3;
+                    return d.ex[0] * rw[0][c0] + d.ex[1] * rw[1][c0] + d.ex[2] * rw[2][c0] +
+                           d.ex[3] * rw[3][c0] + d.ex[4] * rw[4][c0];
+                };
+                for (int nu = 1 + tid; nu <= nni; nu += nth) {
+                    T v = T(0);
+                    if (even) {
+                        if (nu == 1 || nu == nni) {
+                            const int dir = nu == 1 ? 1 : -1;
+                            v = d.ex[0] * r1(nu + dir) + d.ex[1] * r1(nu + 2 * dir) +
+                                d.ex[2] * r1(nu + 3 * dir) + d.ex[3] * r1(nu + 4 * dir) +
+                                d.ex[4] * r1(nu + 5 * dir);
+                        } else {
+                            v = r1(nu);
+                        }
+                    }
+                    dst[(long)4 * N8 + nu + 3] = v;
+                }
+            }
+        }
+
+        // 3. the five colour passes of this step
+        for (int pp = 0; pp < (SKEW == 1 ? 5 : 1); ++pp) {
+            const int p_lo = SKEW == 1 ? pp : 0, p_hi = SKEW == 1 ? pp + 1 : 5;
+            const int nitems = (p_hi - p_lo) * 4 * M;
+            for (int it = tid; it < nitems; it += nth) {
+                const int p = p_lo + it / (4 * M);
+                const int rr = (it / M) & 3;
+                const int t = it % M;
+                const int b = s - SKEW * p;
+                if (b < p || b > K - 1 - p) continue;
+                const int mu = r0 + 4 * b + rr;
+                if (mu < 2 || mu > nmu - 4) continue;
+                const int nu = 2 + mod5(p - d.cA * (mu - 2)) + 5 * t;
+                if (nu > nni - 1) continue;
+                const int c0 = nu + 3;
+                T *rc = lds_row(mu);
+                const T *m1 = lds_row(mu - 1), *m2 = lds_row(mu - 2), *m3 = lds_row(mu - 3),
+                        *m4 = lds_row(mu - 4);
+                const T *p1 = lds_row(mu + 1), *p2 = lds_row(mu + 2), *p3 = lds_row(mu + 3),
+                        *p4 = lds_row(mu + 4);
+                const T ddmi2 = d.c[4] * (m4[c0] + p4[c0]) + d.c[5] * (m3[c0] + p3[c0]) +
+                                d.c[6] * (m2[c0] + p2[c0]) + d.c[7] * (m1[c0] + p1[c0]);
+                const T ddmi1 = d.c[0] * (m4[c0] - p4[c0]) + d.c[1] * (m3[c0] - p3[c0]) +
+                                d.c[2] * (m2[c0] - p2[c0]) + d.c[3] * (m1[c0] - p1[c0]);
+                const T ddni2 = d.c[12] * (rc[c0 - 4] + rc[c0 + 4]) + d.c[13] * (rc[c0 - 3] + rc[c0 + 3]) +
+                                d.c[14] * (rc[c0 - 2] + rc[c0 + 2]) + d.c[15] * (rc[c0 - 1] + rc[c0 + 1]);
+                const T ddni1 = d.c[8] * (rc[c0 - 4] - rc[c0 + 4]) + d.c[9] * (rc[c0 - 3] - rc[c0 + 3]) +
+                                d.c[10] * (rc[c0 - 2] - rc[c0 + 2]) + d.c[11] * (rc[c0 - 1] - rc[c0 + 1]);
+                const T cc = ddmi2 + d.bmu[mu - 1] * ddmi1 + ddni2 + d.dnu[nu - 1] * ddni1;
+                const long u = (long)(mu - 1) * nni + (nu - 1);
+                rc[c0] = d.omega * (d.rhs[u] - cc) / d.lhs[u] + d.omega1 * rc[c0];
+            }
+            if (SKEW == 1 && pp < 4) __syncthreads();
+        }
+
+        // 4. commit the prefetched block
+        if (do_load) {
+            const int slot = (bl + RB) % RB;
+#pragma unroll
+            for (int j = 0; j < kMaxLoads; ++j) {
+                const int e = tid + j * nth;
+                if (e < 4 * N8) {
+                    const int rr = e / N8, col = e - rr * N8;
+                    ring[(long)(slot * 4 + rr) * N8 + col] = pre[j];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// putin / putout (inout.F90:23-82, 165-183), several problems per launch.
+struct PadDesc {
+    const double *fun;
+    double *w1;
+    double *w2;  // optional second copy (ping-pong partner), may be null
+    int nni, nmu, isym;
+};
+
+__global__ void k_putin(const PadDesc *__restrict__ pd)
+{
+    const PadDesc &d = pd[blockIdx.y];
+    const int nni = d.nni, nmu = d.nmu, N8 = nni + 8;
+    const long tot = (long)N8 * (nmu + 8);
+    const double s = d.isym == 1 ? 1.0 : -1.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+         e += (long)gridDim.x * blockDim.x) {
+        const int P = (int)(e / N8), Q = (int)(e - (long)P * N8);
+        const int mu = P - 3, nu = Q - 3;   // 1-based grid indices, halo range -3..n+4
+        double v = 0.0;
+        const bool nu_in = nu >= 1 && nu <= nni, mu_in = mu >= 1 && mu <= nmu;
+        if (nu_in && mu_in) {
+            v = d.fun[(long)(mu - 1) * nni + nu - 1];
+        } else if (nu_in && mu >= -3 && mu <= 0) {
+            v = s * d.fun[(long)(1 - mu) * nni + nu - 1];          // mu' = 2 - mu
+        } else if (mu_in && nu >= -3 && nu <= 0) {
+            v = s * d.fun[(long)(mu - 1) * nni + (1 - nu)];        // nu' = 2 - nu
+        } else if (mu_in && nu >= nni + 1 && nu <= nni + 4) {
+            v = s * d.fun[(long)(mu - 1) * nni + (2 * nni - nu - 1)];  // nu' = 2nni - nu
+        }
+        d.w1[e] = v;
+        if (d.w2) d.w2[e] = v;
+    }
+}
+
+__global__ void k_putout(const PadDesc *__restrict__ pd)
+{
+    const PadDesc &d = pd[blockIdx.y];
+    const int nni = d.nni, nmu = d.nmu, N8 = nni + 8;
+    const long tot = (long)nni * nmu;
+    double *fun = const_cast<double *>(d.fun);
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+         e += (long)gridDim.x * blockDim.x) {
+        const int mu = 1 + (int)(e / nni), nu = 1 + (int)(e % nni);
+        fun[e] = d.w1[(long)(mu + 3) * N8 + nu + 3];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Orbital tail (orbAsymptGet / orbAsymptSet, relaxOrbs.F90:104-124, 415-462):
+// edecay from fa = fock1/f1 (zero where |f1| <= eps), then
+// psi(nu,mu) = psi(nu,mu-1)*edecay for mu = nmu-3..nmu, sequentially in mu.
+struct TailDesc {
+    double *psi;
+    const double *fock1;
+    const double *f1;
+    const double *vxisq;
+    const double *vetasq;
+    int nni, nmu;
+};
+
+__global__ void k_orb_tail(const TailDesc *__restrict__ td)
+{
+    const TailDesc &d = td[blockIdx.y];
+    const int nni = d.nni, nmu = d.nmu;
+    const double eps = 2.220446049250313e-16;  // epsilon(1.0d0), setPrecision
+    for (int nu = blockIdx.x * blockDim.x + threadIdx.x + 1; nu <= nni; nu += gridDim.x * blockDim.x) {
+        double fa[5];
+        for (int k = 0; k < 5; ++k) {
+            const int mu = nmu - 4 + k;
+            const long u = (long)(mu - 1) * nni + nu - 1;
+            const double f1 = d.f1[u];
+            fa[k] = fabs(f1) <= eps ? 0.0 : d.fock1[u] / f1;
+        }
+        double ed[4];
+        for (int jj = 0; jj < 4; ++jj) {
+            const int mu = nmu - 3 + jj;
+            const double raiq1 = sqrt(d.vxisq[mu - 2] + d.vetasq[nu - 1] - 1.0);
+            const double raiq = sqrt(d.vxisq[mu - 1] + d.vetasq[nu - 1] - 1.0);
+            ed[jj] = pow(fabs(fa[jj + 1] / fa[jj]), 0.25) * exp(sqrt(fabs(fa[jj])) * (raiq1 - raiq));
+        }
+        for (int jj = 0; jj < 4; ++jj) {
+            const int mu = nmu - 3 + jj;
+            d.psi[(long)(mu - 1) * nni + nu - 1] = d.psi[(long)(mu - 2) * nni + nu - 1] * ed[jj];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision helpers.
+// Residual r = rhs - (L + lhs) u over the relaxed interior (FP64, from the
+// padded u), stored in FP32 for the correction sweeps; zero elsewhere.
+struct ResDesc {
+    const double *w;       // padded u (halos filled)
+    const double *lhs;
+    const double *rhs;
+    const double *bmu;
+    const double *dnu;
+    float *res_f;          // unpadded FP32 residual (may be null)
+    double *acc;           // [2] per problem: sum res^2, sum rhs^2 (may be null)
+    double c[16];
+    int nni, nmu;
+};
+
+__global__ void k_residual(const ResDesc *__restrict__ rd)
+{
+    const ResDesc &d = rd[blockIdx.y];
+    const int nni = d.nni, nmu = d.nmu;
+    const long s1 = nni + 8;
+    const long tot = (long)nni * nmu;
+    double sr = 0.0, sb = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+         e += (long)gridDim.x * blockDim.x) {
+        const int mu = 1 + (int)(e / nni), nu = 1 + (int)(e % nni);
+        double r = 0.0;
+        if (mu >= 2 && mu <= nmu - 4 && nu >= 2 && nu <= nni - 1) {
+            const long p = (long)(mu + 3) * s1 + nu + 3;
+            const double cc = stencil_cc(d.w, p, s1, d.c, d.bmu[mu - 1], d.dnu[nu - 1]);
+            r = d.rhs[e] - (cc + d.lhs[e] * d.w[p]);
+            sr += r * r;
+            sb += d.rhs[e] * d.rhs[e];
+        }
+        if (d.res_f) d.res_f[e] = (float)r;
+    }
+    if (d.acc) {
+        // wavefront reduction then one atomic per wavefront (diagnostic only)
+        for (int off = 32; off > 0; off >>= 1) {
+            sr += __shfl_down(sr, off, 64);
+            sb += __shfl_down(sb, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&d.acc[0], sr);
+            atomicAdd(&d.acc[1], sb);
+        }
+    }
+}
+
+struct CorrDesc {
+    double *w;          // padded FP64 u
+    const float *e;     // padded FP32 correction
+    int nni, nmu;
+};
+
+// u += e on the relaxed interior.
+__global__ void k_correct(const CorrDesc *__restrict__ cd)
+{
+    const CorrDesc &d = cd[blockIdx.y];
+    const int nni = d.nni, nmu = d.nmu;
+    const long s1 = nni + 8;
+    const long tot = (long)(nni - 2) * (nmu - 5);
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
+         k += (long)gridDim.x * blockDim.x) {
+        const int mu = 2 + (int)(k / (nni - 2)), nu = 2 + (int)(k % (nni - 2));
+        const long p = (long)(mu + 3) * s1 + nu + 3;
+        d.w[p] += (double)d.e[p];
+    }
+}
+
+struct CvtDesc {
+    const double *src;
+    float *dst;
+    long n;
+};
+__global__ void k_to_f32(const CvtDesc *__restrict__ cd)
+{
+    const CvtDesc &d = cd[blockIdx.y];
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x)
+        d.dst[e] = (float)d.src[e];
+}
+struct FillFDesc {
+    float *dst;
+    long n;
+};
+__global__ void k_zero_f32(const FillFDesc *__restrict__ fd)
+{
+    const FillFDesc &d = fd[blockIdx.y];
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x)
+        d.dst[e] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+static inline int grid_x_for(long work, int block) { return (int)std::min<long>((work + block - 1) / block, 1024); }
+
+// Upload a host array of descriptors to a device staging area at `offset`
+// (bytes) of workspace 2; returns the device pointer.
+template <typename D>
+static int stage_descs(x2g_ctx *ctx, const std::vector<D> &v, size_t &offset, const D **out)
+{
+    const size_t bytes = v.size() * sizeof(D);
+    const size_t aligned = (offset + 255) & ~size_t(255);
+    // the descriptor areas are sized once at context creation and never move,
+    // so pointers staged earlier in the same call stay valid
+    X2G_CHECK(aligned + bytes <= ctx->ws[2].bytes && aligned + bytes <= ctx->pinned_bytes,
+              "launch descriptors exceed the %zu-byte staging area", ctx->ws[2].bytes);
+    std::memcpy((char *)ctx->pinned + aligned, v.data(), bytes);
+    X2G_HIP(hipMemcpyAsync((char *)ctx->ws[2].ptr + aligned, (char *)ctx->pinned + aligned, bytes,
+                           hipMemcpyHostToDevice, ctx->stream));
+    *out = reinterpret_cast<const D *>((char *)ctx->ws[2].ptr + aligned);
+    offset = aligned + bytes;
+    return X2G_OK;
+}
+
+int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w2)
+{
+    std::vector<PadDesc> v{{fun, w1, w2, g->nni, g->nmu, isym}};
+    size_t off = 0;
+    const PadDesc *dd;
+    int rc = stage_descs(g->ctx, v, off, &dd);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_putin, dim3(grid_x_for((long)g->npad(), 256), 1), dim3(256), 0, g->ctx->stream, dd);
+    X2G_HIP(hipGetLastError());
+    X2G_HIP(hipStreamSynchronize(g->ctx->stream));
+    return X2G_OK;
+}
+
+int launch_putout(x2g_grid *g, double *fun, const double *work)
+{
+    std::vector<PadDesc> v{{fun, const_cast<double *>(work), nullptr, g->nni, g->nmu, 1}};
+    size_t off = 0;
+    const PadDesc *dd;
+    int rc = stage_descs(g->ctx, v, off, &dd);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_putout, dim3(grid_x_for((long)g->n(), 256), 1), dim3(256), 0, g->ctx->stream, dd);
+    X2G_HIP(hipGetLastError());
+    X2G_HIP(hipStreamSynchronize(g->ctx->stream));
+    return X2G_OK;
+}
+
+// Sweep-engine selection: 0 = pipelined strip kernel where the LDS ring fits
+// (default), 1 = per-pass kernels.  strip_rows > 0 fixes the strip height.
+// Initial values come from X2G_SWEEP=phase and X2G_STRIP_L.
+static int g_engine = -1, g_strip_rows = -1;
+static void engine_defaults()
+{
+    if (g_engine >= 0) return;
+    const char *e = getenv("X2G_SWEEP");
+    g_engine = (e && std::strcmp(e, "phase") == 0) ? 1 : 0;
+    const char *l = getenv("X2G_STRIP_L");
+    g_strip_rows = l ? atoi(l) : -1;
+}
+static int env_engine()
+{
+    engine_defaults();
+    return g_engine;
+}
+static int env_strip_l()
+{
+    engine_defaults();
+    return g_strip_rows;
+}
+
+template <typename T>
+static size_t strip_lds(int nni, int skew) { return (size_t)(4 * skew + 4) * 4 * (nni + 8) * sizeof(T); }
+
+// Choose SKEW (2, 1) or 0 = does not fit the LDS ring.
+template <typename T>
+static int choose_skew(x2g_ctx *ctx, int nni)
+{
+    if (4 * (nni + 8) > kMaxLoads * 512) return 0;
+    if (strip_lds<T>(nni, 2) <= ctx->max_lds) return 2;
+    if (strip_lds<T>(nni, 1) <= ctx->max_lds) return 1;
+    return 0;
+}
+
+// Strip height: enough workgroups to fill the chip, at least 32 rows.
+static int choose_L(x2g_ctx *ctx, long total_rows)
+{
+    const int forced = env_strip_l();
+    if (forced > 0) return (forced + 3) & ~3;
+    // aim for >= 2 workgroups per CU when the batch allows it
+    long target = 2L * ctx->num_cu;
+    long L = total_rows / std::max<long>(target, 1);
+    L = std::max<long>(32, std::min<long>(128, L));
+    return (int)((L + 3) & ~3L);
+}
+
+template <typename T>
+struct SweepPlan {
+    std::vector<SweepDesc<T>> even, odd;  // src=A,dst=B and src=B,dst=A
+    int skew = 0;
+    int total_wg = 0;
+    size_t lds = 0;
+};
+
+// Launch one sweep of every problem in `descs` (device copy dd, host copy hv).
+template <typename T>
+static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<SweepDesc<T>> &hv,
+                        int skew, int total_wg, size_t lds, bool in_place)
+{
+    const int n = (int)hv.size();
+    if (!in_place && skew > 0) {
+        const int nth = 512;
+        if (skew == 2) {
+            hipLaunchKernelGGL((k_strip<T, 2>), dim3(total_wg), dim3(nth), lds, ctx->stream, dd, n);
+        } else {
+            hipLaunchKernelGGL((k_strip<T, 1>), dim3(total_wg), dim3(nth), lds, ctx->stream, dd, n);
+        }
+        X2G_HIP(hipGetLastError());
+        return X2G_OK;
+    }
+    long maxpts = 0;
+    for (auto &d : hv) maxpts = std::max<long>(maxpts, (long)(d.nni - 2) * (d.nmu - 5));
+    for (int pass = 0; pass < 5; ++pass) {
+        hipLaunchKernelGGL((k_phase<T>), dim3(grid_x_for(maxpts, 256), n), dim3(256), 0, ctx->stream, dd, pass);
+        X2G_HIP(hipGetLastError());
+    }
+    long maxb = 0;
+    for (auto &d : hv) maxb = std::max<long>(maxb, (long)(d.nni - 2) + 2L * (d.nmu - 4));
+    hipLaunchKernelGGL((k_boundary<T>), dim3(grid_x_for(maxb, 256), n), dim3(256), 0, ctx->stream, dd);
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
+static int set_lds_attr(x2g_ctx *ctx)
+{
+    static bool done = false;
+    if (done) return X2G_OK;
+    const int mx = (int)ctx->max_lds;
+    X2G_HIP(hipFuncSetAttribute((const void *)k_strip<double, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    X2G_HIP(hipFuncSetAttribute((const void *)k_strip<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    done = true;
+    return X2G_OK;
+}
+
+template <typename T>
+static void fill_desc(SweepDesc<T> &d, x2g_grid *g, const T *lhs, const T *rhs, const T *bmu,
+                      const T *dnu, int isym, double omega)
+{
+    for (int i = 0; i < 16; ++i) d.c[i] = (T)g->coef[i];
+    for (int i = 0; i < 5; ++i) d.ex[i] = (T)g->coef[16 + i];
+    d.omega = (T)omega;
+    d.omega1 = (T)(1.0 - omega);
+    d.lhs = lhs;
+    d.rhs = rhs;
+    d.bmu = bmu;
+    d.dnu = dnu;
+    d.nni = g->nni;
+    d.nmu = g->nmu;
+    d.isym = isym;
+    d.cA = colour_mult(g->nni);
+    d.wg_begin = 0;
+    d.L = 0;
+}
+
+// Build the even/odd descriptor lists of a batch; a_bufs/b_bufs are the
+// ping-pong padded arrays.  Strip engine when every problem fits the ring.
+template <typename T>
+static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vector<T *> &a_bufs,
+                       std::vector<T *> &b_bufs, SweepPlan<T> &plan)
+{
+    int skew = env_engine() == 1 ? 0 : 2;
+    int maxnni = 0;
+    long rows = 0;
+    for (auto &d : base) {
+        maxnni = std::max(maxnni, d.nni);
+        rows += d.nmu - 5;
+    }
+    if (skew) skew = choose_skew<T>(ctx, maxnni);
+    plan.skew = skew;
+    const int L = choose_L(ctx, rows);
+    int wg = 0;
+    for (auto &d : base) {
+        d.L = L;
+        d.wg_begin = wg;
+        wg += (d.nmu - 5 + L - 1) / L;
+    }
+    plan.total_wg = wg;
+    plan.lds = skew ? strip_lds<T>(maxnni, skew) : 0;
+    plan.even = base;
+    plan.odd = base;
+    for (size_t i = 0; i < base.size(); ++i) {
+        if (skew) {
+            plan.even[i].src = a_bufs[i];
+            plan.even[i].dst = b_bufs[i];
+            plan.odd[i].src = b_bufs[i];
+            plan.odd[i].dst = a_bufs[i];
+        } else {
+            plan.even[i].src = plan.even[i].dst = a_bufs[i];
+            plan.odd[i].src = plan.odd[i].dst = a_bufs[i];
+        }
+    }
+    return X2G_OK;
+}
+
+}  // namespace x2g
+
+using namespace x2g;
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" int x2g_set_sweep_engine(int engine, int strip_rows)
+{
+    X2G_CHECK(engine == 0 || engine == 1, "x2g_set_sweep_engine: engine must be 0 or 1");
+    engine_defaults();
+    g_engine = engine;
+    g_strip_rows = strip_rows;
+    return X2G_OK;
+}
+
+extern "C" int x2g_putin(x2g_grid *g, int isym, const double *fun_d, double *work_d)
+{
+    X2G_CHECK(g && fun_d && work_d, "x2g_putin: null argument");
+    X2G_CHECK(isym == 1 || isym == -1, "x2g_putin: isym must be +1 or -1");
+    return launch_putin(g, isym, fun_d, work_d, nullptr);
+}
+
+extern "C" int x2g_putout(x2g_grid *g, double *fun_d, const double *work_d)
+{
+    X2G_CHECK(g && fun_d && work_d, "x2g_putout: null argument");
+    return launch_putout(g, fun_d, work_d);
+}
+
+static int check_grid_sweepable(x2g_grid *g)
+{
+    X2G_CHECK(g->nni >= 7 && g->nmu >= 11, "grid %dx%d too small for the 17-point stencil", g->nni, g->nmu);
+    return X2G_OK;
+}
+
+extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
+                         const double *rhs_d, double omega, int nsweeps)
+{
+    X2G_CHECK(g && work_d && lhs_d && rhs_d, "x2g_mcsor: null argument");
+    X2G_CHECK(isym == 1 || isym == -1, "x2g_mcsor: isym must be +1 or -1");
+    X2G_CHECK(bkind == 0 || bkind == 1, "x2g_mcsor: bkind must be 0 or 1");
+    X2G_CHECK(nsweeps >= 0, "x2g_mcsor: nsweeps < 0");
+    int rc = check_grid_sweepable(g);
+    if (rc) return rc;
+    if (nsweeps == 0) return X2G_OK;
+    x2g_ctx *ctx = g->ctx;
+    rc = set_lds_attr(ctx);
+    if (rc) return rc;
+    // second padded buffer for the ping-pong (a copy of work)
+    const size_t bytes = g->npad() * sizeof(double);
+    rc = ws_reserve(ctx, 0, bytes);
+    if (rc) return rc;
+    double *alt = (double *)ctx->ws[0].ptr;
+    X2G_HIP(hipMemcpyAsync(alt, work_d, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    std::vector<SweepDesc<double>> base(1);
+    fill_desc<double>(base[0], g, lhs_d, rhs_d, bkind == 0 ? g->bmu_orb : g->bmu_pot, g->dnu, isym, omega);
+    std::vector<double *> A{work_d}, B{alt};
+    SweepPlan<double> plan;
+    plan_sweeps(ctx, base, A, B, plan);
+    size_t off = 0;
+    const SweepDesc<double> *de, *dodd;
+    if ((rc = stage_descs(ctx, plan.even, off, &de))) return rc;
+    if ((rc = stage_descs(ctx, plan.odd, off, &dodd))) return rc;
+    const bool in_place = plan.skew == 0;
+    for (int it = 0; it < nsweeps; ++it) {
+        const bool ev = (it % 2) == 0;
+        rc = launch_sweep<double>(ctx, ev ? de : dodd, ev ? plan.even : plan.odd, plan.skew, plan.total_wg,
+                                  plan.lds, in_place);
+        if (rc) return rc;
+    }
+    if (!in_place && (nsweeps % 2) == 1)
+        X2G_HIP(hipMemcpyAsync(work_d, alt, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    return X2G_OK;
+}
+
+extern "C" int x2g_mcsor_host(x2g_grid *g, int isym, int bkind, double *work, const double *lhs,
+                              const double *rhs, double omega, int nsweeps)
+{
+    X2G_CHECK(g && work && lhs && rhs, "x2g_mcsor_host: null argument");
+    x2g_ctx *ctx = g->ctx;
+    const size_t pb = g->npad() * sizeof(double), nb = g->n() * sizeof(double);
+    double *w_d = nullptr, *l_d = nullptr, *r_d = nullptr;
+    int rc = X2G_OK;
+    if (x2g_malloc(ctx, pb, (void **)&w_d) || x2g_malloc(ctx, nb, (void **)&l_d) ||
+        x2g_malloc(ctx, nb, (void **)&r_d)) {
+        rc = X2G_ENOMEM;
+    } else if (!(rc = x2g_h2d(ctx, w_d, work, pb)) && !(rc = x2g_h2d(ctx, l_d, lhs, nb)) &&
+               !(rc = x2g_h2d(ctx, r_d, rhs, nb)) &&
+               !(rc = x2g_mcsor(g, isym, bkind, w_d, l_d, r_d, omega, nsweeps))) {
+        rc = x2g_d2h(ctx, work, w_d, pb);
+    }
+    if (w_d) x2g_free(ctx, w_d);
+    if (l_d) x2g_free(ctx, l_d);
+    if (r_d) x2g_free(ctx, r_d);
+    return rc;
+}
+
+extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
+                               int maxsor1, int maxsor2)
+{
+    X2G_CHECK(ctx && grids && items && n >= 0, "x2g_relax_batch: null argument");
+    X2G_CHECK(maxsor1 >= 0 && maxsor2 >= 0, "x2g_relax_batch: negative sweep counts");
+    if (n == 0 || maxsor1 == 0) return X2G_OK;
+    int rc = set_lds_attr(ctx);
+    if (rc) return rc;
+    // validate and size the workspace: two padded FP64 arrays per item, plus
+    // for FP32 items two padded FP32 arrays and FP32 lhs/rhs.
+    size_t off64 = 0, off32 = 0;
+    std::vector<size_t> o64(n), o32(n);
+    for (int i = 0; i < n; ++i) {
+        const x2g_relax_item &it = items[i];
+        X2G_CHECK(it.grid_index >= 0 && grids[it.grid_index], "x2g_relax_batch: bad grid index %d", it.grid_index);
+        x2g_grid *g = grids[it.grid_index];
+        X2G_CHECK(g->ctx == ctx, "x2g_relax_batch: grid of another context");
+        X2G_CHECK(it.fun_d && it.lhs_d && it.rhs_d, "x2g_relax_batch: item %d has a null array", i);
+        X2G_CHECK(it.isym == 1 || it.isym == -1, "x2g_relax_batch: item %d isym", i);
+        X2G_CHECK(it.bkind == 0 || it.bkind == 1, "x2g_relax_batch: item %d bkind", i);
+        X2G_CHECK(it.prec == 0 || it.prec == 1, "x2g_relax_batch: item %d prec", i);
+        if ((rc = check_grid_sweepable(g))) return rc;
+        o64[i] = off64;
+        off64 += 2 * ((g->npad() * sizeof(double) + 255) & ~size_t(255));
+        o32[i] = off32;
+        if (it.prec == 1)
+            off32 += 2 * ((g->npad() * sizeof(float) + 255) & ~size_t(255)) +
+                     2 * ((g->n() * sizeof(float) + 255) & ~size_t(255));
+    }
+    if ((rc = ws_reserve(ctx, 0, off64))) return rc;
+    if (off32 && (rc = ws_reserve(ctx, 3, off32))) return rc;
+    char *w64 = (char *)ctx->ws[0].ptr;
+    char *w32 = (char *)ctx->ws[3].ptr;
+
+    std::vector<int> i64, i32;  // item indices by precision
+    for (int i = 0; i < n; ++i) (items[i].prec ? i32 : i64).push_back(i);
+
+    auto bufA = [&](int i) { return (double *)(w64 + o64[i]); };
+    auto bufB = [&](int i) {
+        x2g_grid *g = grids[items[i].grid_index];
+        return (double *)(w64 + o64[i] + ((g->npad() * sizeof(double) + 255) & ~size_t(255)));
+    };
+    auto f32p = [&](int i, int k) {  // k: 0 eA, 1 eB, 2 lhs_f, 3 res_f
+        x2g_grid *g = grids[items[i].grid_index];
+        const size_t pp = (g->npad() * sizeof(float) + 255) & ~size_t(255);
+        const size_t nn = (g->n() * sizeof(float) + 255) & ~size_t(255);
+        size_t o = o32[i] + (k <= 1 ? k * pp : 2 * pp + (k - 2) * nn);
+        return (float *)(w32 + o);
+    };
+
+    size_t doff = 0;
+    // descriptors that do not change across macro-iterations
+    std::vector<PadDesc> pin, pout;
+    std::vector<TailDesc> tails;
+    for (int i = 0; i < n; ++i) {
+        const x2g_relax_item &it = items[i];
+        x2g_grid *g = grids[it.grid_index];
+        const bool strip_or_mixed = true;
+        pin.push_back({it.fun_d, bufA(i), strip_or_mixed ? bufB(i) : nullptr, g->nni, g->nmu, it.isym});
+        if (it.tail_fock1_d)
+            tails.push_back({it.fun_d, it.tail_fock1_d, g->suppl + 5 * g->n(), g->vxisq, g->vetasq, g->nni, g->nmu});
+    }
+    const PadDesc *d_pin = nullptr;
+    const TailDesc *d_tail = nullptr;
+    if ((rc = stage_descs(ctx, pin, doff, &d_pin))) return rc;
+    if (!tails.empty() && (rc = stage_descs(ctx, tails, doff, &d_tail))) return rc;
+
+    // FP64 items: sweep plan
+    SweepPlan<double> p64;
+    const SweepDesc<double> *d64e = nullptr, *d64o = nullptr;
+    std::vector<PadDesc> pout64;
+    if (!i64.empty()) {
+        std::vector<SweepDesc<double>> base(i64.size());
+        std::vector<double *> A, B;
+        for (size_t k = 0; k < i64.size(); ++k) {
+            const x2g_relax_item &it = items[i64[k]];
+            x2g_grid *g = grids[it.grid_index];
+            fill_desc<double>(base[k], g, it.lhs_d, it.rhs_d, it.bkind == 0 ? g->bmu_orb : g->bmu_pot, g->dnu,
+                              it.isym, it.omega);
+            A.push_back(bufA(i64[k]));
+            B.push_back(bufB(i64[k]));
+        }
+        plan_sweeps(ctx, base, A, B, p64);
+        if ((rc = stage_descs(ctx, p64.even, doff, &d64e))) return rc;
+        if ((rc = stage_descs(ctx, p64.odd, doff, &d64o))) return rc;
+        const bool final_in_b = p64.skew > 0 && (maxsor2 % 2) == 1;
+        for (size_t k = 0; k < i64.size(); ++k) {
+            const x2g_relax_item &it = items[i64[k]];
+            x2g_grid *g = grids[it.grid_index];
+            pout64.push_back({it.fun_d, final_in_b ? bufB(i64[k]) : bufA(i64[k]), nullptr, g->nni, g->nmu, 1});
+        }
+    }
+    const PadDesc *d_pout64 = nullptr;
+    if (!pout64.empty() && (rc = stage_descs(ctx, pout64, doff, &d_pout64))) return rc;
+
+    // FP32 items: conversion, residual, sweeps, correction
+    SweepPlan<float> p32;
+    const SweepDesc<float> *d32e = nullptr, *d32o = nullptr;
+    const ResDesc *d_res = nullptr;
+    const CorrDesc *d_corr = nullptr;
+    const CvtDesc *d_cvt = nullptr;
+    const FillFDesc *d_zero = nullptr;
+    const SweepDesc<double> *d_bnd64 = nullptr;
+    const PadDesc *d_pout32 = nullptr;
+    std::vector<SweepDesc<double>> bnd64;
+    std::vector<PadDesc> pout32;
+    if (!i32.empty()) {
+        std::vector<SweepDesc<float>> base(i32.size());
+        std::vector<float *> A, B;
+        std::vector<ResDesc> res;
+        std::vector<CorrDesc> corr;
+        std::vector<CvtDesc> cvt;
+        std::vector<FillFDesc> zero;
+        for (size_t k = 0; k < i32.size(); ++k) {
+            const int i = i32[k];
+            const x2g_relax_item &it = items[i];
+            x2g_grid *g = grids[it.grid_index];
+            fill_desc<float>(base[k], g, f32p(i, 2), f32p(i, 3), it.bkind == 0 ? g->bmu_orb_f : g->bmu_pot_f,
+                             g->dnu_f, it.isym, it.omega);
+            A.push_back(f32p(i, 0));
+            B.push_back(f32p(i, 1));
+            ResDesc r{};
+            r.w = bufA(i);
+            r.lhs = it.lhs_d;
+            r.rhs = it.rhs_d;
+            r.bmu = it.bkind == 0 ? g->bmu_orb : g->bmu_pot;
+            r.dnu = g->dnu;
+            r.res_f = f32p(i, 3);
+            r.acc = nullptr;
+            for (int q = 0; q < 16; ++q) r.c[q] = g->coef[q];
+            r.nni = g->nni;
+            r.nmu = g->nmu;
+            res.push_back(r);
+            corr.push_back({bufA(i), f32p(i, 0), g->nni, g->nmu});
+            cvt.push_back({it.lhs_d, f32p(i, 2), (long)g->n()});
+            zero.push_back({f32p(i, 0), (long)(2 * ((g->npad() * sizeof(float) + 255) & ~size_t(255)) / sizeof(float))});
+            SweepDesc<double> bd;
+            fill_desc<double>(bd, g, it.lhs_d, it.rhs_d, g->bmu_orb, g->dnu, it.isym, it.omega);
+            bd.src = bd.dst = bufA(i);
+            bnd64.push_back(bd);
+            pout32.push_back({it.fun_d, bufA(i), nullptr, g->nni, g->nmu, 1});
+        }
+        plan_sweeps(ctx, base, A, B, p32);
+        // the strip engine ping-pongs: an odd sweep count leaves the correction in B
+        if (p32.skew > 0 && (maxsor2 % 2) == 1)
+            for (size_t k = 0; k < i32.size(); ++k) corr[k].e = f32p(i32[k], 1);
+        if ((rc = stage_descs(ctx, p32.even, doff, &d32e))) return rc;
+        if ((rc = stage_descs(ctx, p32.odd, doff, &d32o))) return rc;
+        if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
+        if ((rc = stage_descs(ctx, corr, doff, &d_corr))) return rc;
+        if ((rc = stage_descs(ctx, cvt, doff, &d_cvt))) return rc;
+        if ((rc = stage_descs(ctx, zero, doff, &d_zero))) return rc;
+        if ((rc = stage_descs(ctx, bnd64, doff, &d_bnd64))) return rc;
+        if ((rc = stage_descs(ctx, pout32, doff, &d_pout32))) return rc;
+        long maxn = 0;
+        for (int i : i32) maxn = std::max<long>(maxn, (long)grids[items[i].grid_index]->n());
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_x_for(maxn, 256), (int)i32.size()), dim3(256), 0, ctx->stream, d_cvt);
+        X2G_HIP(hipGetLastError());
+    }
+
+    long maxpad = 0, maxn = 0, maxnni = 0;
+    for (int i = 0; i < n; ++i) {
+        x2g_grid *g = grids[items[i].grid_index];
+        maxpad = std::max<long>(maxpad, (long)g->npad());
+        maxn = std::max<long>(maxn, (long)g->n());
+        maxnni = std::max<long>(maxnni, g->nni);
+    }
+    for (int itr1 = 0; itr1 < maxsor1; ++itr1) {
+        // putin into both ping-pong buffers
+        hipLaunchKernelGGL(k_putin, dim3(grid_x_for(maxpad, 256), n), dim3(256), 0, ctx->stream, d_pin);
+        X2G_HIP(hipGetLastError());
+        if (!i64.empty()) {
+            for (int it2 = 0; it2 < maxsor2; ++it2) {
+                const bool ev = (it2 % 2) == 0;
+                if ((rc = launch_sweep<double>(ctx, ev ? d64e : d64o, ev ? p64.even : p64.odd, p64.skew,
+                                               p64.total_wg, p64.lds, p64.skew == 0)))
+                    return rc;
+            }
+            hipLaunchKernelGGL(k_putout, dim3(grid_x_for(maxn, 256), (int)i64.size()), dim3(256), 0, ctx->stream,
+                               d_pout64);
+            X2G_HIP(hipGetLastError());
+        }
+        if (!i32.empty()) {
+            const int n32 = (int)i32.size();
+            hipLaunchKernelGGL(k_residual, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, ctx->stream, d_res);
+            X2G_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_zero_f32, dim3(grid_x_for(2 * maxpad, 256), n32), dim3(256), 0, ctx->stream, d_zero);
+            X2G_HIP(hipGetLastError());
+            for (int it2 = 0; it2 < maxsor2; ++it2) {
+                const bool ev = (it2 % 2) == 0;
+                if ((rc = launch_sweep<float>(ctx, ev ? d32e : d32o, ev ? p32.even : p32.odd, p32.skew, p32.total_wg,
+                                              p32.lds, p32.skew == 0)))
+                    return rc;
+            }
+            hipLaunchKernelGGL(k_correct, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, ctx->stream, d_corr);
+            X2G_HIP(hipGetLastError());
+            hipLaunchKernelGGL((k_boundary<double>), dim3(grid_x_for(2 * maxn / std::max<long>(maxnni, 1) + maxnni, 256), n32),
+                               dim3(256), 0, ctx->stream, d_bnd64);
+            X2G_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_putout, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, ctx->stream, d_pout32);
+            X2G_HIP(hipGetLastError());
+        }
+        if (d_tail) {
+            hipLaunchKernelGGL(k_orb_tail, dim3(grid_x_for(maxnni, 128), (int)tails.size()), dim3(128), 0, ctx->stream,
+                               d_tail);
+            X2G_HIP(hipGetLastError());
+        }
+    }
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    return X2G_OK;
+}
+
+extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
+                                  double *out_h)
+{
+    X2G_CHECK(ctx && grids && items && out_h && n >= 0, "x2g_residual_norms: null argument");
+    if (n == 0) return X2G_OK;
+    size_t off64 = 0;
+    std::vector<size_t> o64(n);
+    for (int i = 0; i < n; ++i) {
+        x2g_grid *g = grids[items[i].grid_index];
+        o64[i] = off64;
+        off64 += (g->npad() * sizeof(double) + 255) & ~size_t(255);
+    }
+    const size_t acc_off = (off64 + 255) & ~size_t(255);
+    int rc = ws_reserve(ctx, 0, acc_off + 2 * n * sizeof(double));
+    if (rc) return rc;
+    char *w = (char *)ctx->ws[0].ptr;
+    double *acc = (double *)(w + acc_off);
+    std::vector<PadDesc> pin;
+    std::vector<ResDesc> res;
+    long maxpad = 0, maxn = 0;
+    for (int i = 0; i < n; ++i) {
+        const x2g_relax_item &it = items[i];
+        x2g_grid *g = grids[it.grid_index];
+        pin.push_back({it.fun_d, (double *)(w + o64[i]), nullptr, g->nni, g->nmu, it.isym});
+        ResDesc r{};
+        r.w = (double *)(w + o64[i]);
+        r.lhs = it.lhs_d;
+        r.rhs = it.rhs_d;
+        r.bmu = it.bkind == 0 ? g->bmu_orb : g->bmu_pot;
+        r.dnu = g->dnu;
+        r.res_f = nullptr;
+        r.acc = acc + 2 * i;
+        for (int q = 0; q < 16; ++q) r.c[q] = g->coef[q];
+        r.nni = g->nni;
+        r.nmu = g->nmu;
+        res.push_back(r);
+        maxpad = std::max<long>(maxpad, (long)g->npad());
+        maxn = std::max<long>(maxn, (long)g->n());
+    }
+    size_t doff = 0;
+    const PadDesc *d_pin;
+    const ResDesc *d_res;
+    if ((rc = stage_descs(ctx, pin, doff, &d_pin))) return rc;
+    if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
+    if ((rc = x2g_fill(ctx, acc, 2 * (size_t)n, 0.0))) return rc;
+    hipLaunchKernelGGL(k_putin, dim3(grid_x_for(maxpad, 256), n), dim3(256), 0, ctx->stream, d_pin);
+    X2G_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_residual, dim3(grid_x_for(maxn, 256), n), dim3(256), 0, ctx->stream, d_res);
+    X2G_HIP(hipGetLastError());
+    X2G_HIP(hipMemcpyAsync(out_h, acc, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    return X2G_OK;
+}
