@@ -65,7 +65,7 @@ int x2g_fill(x2g_ctx *ctx, double *dst_d, size_t n, double value);
 
 /* ---- grid --------------------------------------------------------------- */
 /* Upload one grid.  coef = {dmu1[4], dmu2[4], dni1[4], dni2[4], exeven[5],
- * diag, hni, hmu, r} (26 doubles, initSuppl initCBAllocArrays.F90:1979-2041);
+ * diag, hni, hmu, r} (25 doubles, initSuppl initCBAllocArrays.F90:1979-2041);
  * suppl = the 14 nni*nmu coefficient arrays in the reference's i4b order
  * (borb, bpot, d, e, f0, f1, f2, f3, f4, g, wjac1, wjac2, wgt1, wgt2);
  * vec = {vxi[nmu], vxisq[nmu], veta[nni], vetasq[nni]}. */

@@ -84,8 +84,9 @@ def _setci_all(ctx, g, z, orbs):
     """Impose the g/u symmetry of every orbital (setCi) as the reference does
     before the SCF loop (doSCF.F90:334-336)."""
     from x2dhf_amd import gpu
-    # N2 decks: orbitals 1 pi u, 1 sigma g, 1 sigma u, 1 sigma g, 1 sigma u, 1 sigma g
-    ihomo = {1: [1], 6: [-1, 1, -1, 1, -1, 1]}[len(orbs)]
+    # N2 decks: orbitals 1 pi u, 1 sigma g, 1 sigma u, 1 sigma g, 1 sigma u, 1 sigma g;
+    # ihomo is -1 for sigma-u and pi-g functions (initCBlocks, initCBAllocArrays.F90:981-987)
+    ihomo = {1: [1], 6: [1, 1, -1, 1, -1, 1]}[len(orbs)]
     for k, o in enumerate(orbs):
         gpu.check(ctx.lib.x2g_setci(g.h, o.ptr, ihomo[k]))
 

@@ -52,7 +52,7 @@ struct x2g_ctx {
 struct x2g_grid {
     x2g_ctx *ctx = nullptr;
     int nni = 0, nmu = 0;
-    double coef[26];               // dmu1,dmu2,dni1,dni2 (16), exeven (5), diag, hni, hmu, r
+    double coef[25];               // dmu1,dmu2,dni1,dni2 (16), exeven (5), diag, hni, hmu, r
     double *suppl = nullptr;       // 14 * nni*nmu
     double *bmu_orb = nullptr;     // b(mu) for orbitals  (borb(1,mu))
     double *bmu_pot = nullptr;     // b(mu) for potentials (bpot(1,mu))

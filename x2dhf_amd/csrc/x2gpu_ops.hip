@@ -412,7 +412,7 @@ extern "C" int x2g_moments(x2g_grid *g, const double *psi1_d, const double *psi2
     A.vxisq = g->vxisq;
     A.veta = g->veta;
     A.vetasq = g->vetasq;
-    A.r2 = 0.5 * g->coef[25];
+    A.r2 = 0.5 * g->coef[24];
     A.nni = g->nni;
     A.nmu = g->nmu;
     A.dm = dm;
@@ -439,7 +439,7 @@ extern "C" int x2g_pot_tail(x2g_grid *g, double *pot_d, int kind, int dm, int mp
     A.veta = g->veta;
     A.vetasq = g->vetasq;
     for (int k = 0; k < 8; ++k) A.mom[k] = k < mpole ? mom[k] : 0.0;
-    A.r2 = 0.5 * g->coef[25];
+    A.r2 = 0.5 * g->coef[24];
     A.nni = g->nni;
     A.nmu = g->nmu;
     A.kind = kind;

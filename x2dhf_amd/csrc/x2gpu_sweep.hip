@@ -316,7 +316,10 @@ __global__ void __launch_bounds__(512) k_strip(const SweepDesc<T> *__restrict__ 
             if (SKEW == 1 && pp < 4) __syncthreads();
         }
 
-        // 4. commit the prefetched block
+        // 4. commit the prefetched block.  Its slot held block s-4*SKEW-2,
+        // which only the mu=1 row of step 2 (first strip, bw == 5) still reads:
+        // wait for those reads before overwriting (a, bw are uniform).
+        if (a == 2 && bw == 5) __syncthreads();
         if (do_load) {
             const int slot = (bl + RB) % RB;
 #pragma unroll

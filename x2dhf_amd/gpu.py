@@ -180,7 +180,7 @@ class DeviceArray:
 class Grid:
     """x2g_grid: one (nni, nmu) grid with its coefficient arrays on the device.
 
-    coef: 26 doubles {dmu1, dmu2, dni1, dni2, exeven, diag, hni, hmu, r};
+    coef: 25 doubles {dmu1, dmu2, dni1, dni2, exeven, diag, hni, hmu, r};
     suppl: (14, nni*nmu) in the reference's i4b order; vec: vxi, vxisq, veta, vetasq.
     """
 
@@ -188,7 +188,7 @@ class Grid:
         self.ctx = ctx
         self.nni, self.nmu = int(nni), int(nmu)
         coef = np.ascontiguousarray(coef, dtype=np.float64)
-        assert coef.size == 26
+        assert coef.size == 25
         suppl = np.ascontiguousarray(suppl, dtype=np.float64).reshape(14, -1)
         assert suppl.shape[1] == nni * nmu
         vxi = np.asarray(vxi, dtype=np.float64)
