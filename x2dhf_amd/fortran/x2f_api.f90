@@ -1,0 +1,274 @@
+! C-callable entry points of the Fortran SCF driver (libx2scf.so).  Jobs live
+! in a fixed table so that several host threads may step different jobs at
+! the same time; the table itself is only modified by x2f_job_create and
+! x2f_job_destroy, which the caller serialises.
+module x2f_api
+  use, intrinsic :: iso_c_binding
+  use x2f_capi
+  use x2f_grid
+  use x2f_case
+  use x2f_scf
+  implicit none
+  private
+
+  integer, parameter :: dp = c_double
+  integer, parameter :: MAXJOBS = 256
+
+  ! job specification (mirrored by x2dhf_amd/scf.py: JobSpec)
+  type, bind(c) :: x2f_spec
+     integer(c_int) :: nni, nmu, norb, maxscf, ienterm, inoterm, nscfextra, ienterm4init
+     integer(c_int) :: maxsororb(2), maxsorpot(2), mpole, prec, schedule, device
+     real(c_double) :: z1, z2, r, rinf, omega_orb, omega_pot, recalc_mm
+     integer(c_int) :: m(MAXO), gu(MAXO), n1(MAXO), l1(MAXO), n2(MAXO), l2(MAXO)
+     real(c_double) :: occ(MAXO), c1(MAXO), c2(MAXO), za(MAXO), zb(MAXO)
+  end type x2f_spec
+
+  ! job results (mirrored by x2dhf_amd/scf.py: JobResult)
+  type, bind(c) :: x2f_result
+     integer(c_int) :: status, iterations, stopcode, norb
+     integer(c_long_long) :: nsweep_orb, nsweep_pot, bytes
+     real(c_double) :: etot, ekin, enuc, ecoul, eexch, virial, delta_e, delta_norm
+     real(c_double) :: eps(MAXO), norm(MAXO), cmulti(8, MAXO)
+  end type x2f_result
+
+  type(scf_job), allocatable, target, save :: jobs(:)
+  logical, save :: used(MAXJOBS) = .false.
+
+contains
+
+  ! Build the grid coefficient arrays on the host (tests, benchmarks).
+  integer(c_int) function x2f_grid_build(nni, nmu, z1, z2, r, rinf, coef, suppl, vec) &
+       bind(c, name='x2f_grid_build') result(rc)
+    integer(c_int), value :: nni, nmu
+    real(c_double), value :: z1, z2, r, rinf
+    real(c_double), intent(out) :: coef(25), suppl(*), vec(*)
+    type(grid_t) :: g
+    integer :: n
+    rc = int(grid_build(g, int(nni), int(nmu), z1, z2, r, rinf), c_int)
+    if (rc /= 0) return
+    n = nni*nmu
+    call grid_coef(g, coef)
+    suppl(1:14*n) = reshape(g%suppl, [14*n])
+    call grid_vec(g, vec(1:2*nmu+2*nni))
+  end function x2f_grid_build
+
+  subroutine x2f_omega_defaults(nni, nmu, z1, z2, oorb, opot) bind(c, name='x2f_omega_defaults')
+    integer(c_int), value :: nni, nmu
+    real(c_double), value :: z1, z2
+    real(c_double), intent(out) :: oorb, opot
+    call omega_defaults(int(nni), int(nmu), z1, z2, oorb, opot)
+  end subroutine x2f_omega_defaults
+
+  ! Create a job.  share > 0 makes the job use the device context of job
+  ! `share` (required for jobs stepped together by x2f_jobs_iterate_batched).
+  ! Errors: 1 bad grid, 2 unsupported case, 3 job table full, 4 bad share.
+  integer(c_int) function x2f_job_create(spec, share, handle) bind(c, name='x2f_job_create') result(rc)
+    type(x2f_spec), intent(in) :: spec
+    integer(c_int), value :: share
+    integer(c_int), intent(out) :: handle
+    integer :: h, i, ierr
+    real(dp) :: s
+    type(c_ptr) :: ctx_in
+    handle = -1
+    ctx_in = c_null_ptr
+    if (.not. allocated(jobs)) allocate(jobs(MAXJOBS))
+    if (share > 0) then
+       if (.not. valid(share)) then
+          rc = 4; return
+       endif
+       ctx_in = jobs(share)%ctx
+    endif
+    h = 0
+    do i = 1, MAXJOBS
+       if (.not. used(i)) then
+          h = i; exit
+       endif
+    enddo
+    if (h == 0) then
+       rc = 3; return
+    endif
+    associate (j => jobs(h))
+      j%c = case_t()
+      j%c%norb = spec%norb
+      j%c%z1 = spec%z1; j%c%z2 = spec%z2; j%c%r = spec%r
+      do i = 1, min(int(spec%norb), MAXO)
+         j%c%m(i) = spec%m(i); j%c%gu(i) = spec%gu(i); j%c%occ(i) = spec%occ(i)
+         ! mixing coefficients normalised to |c1|+|c2| = 1 (inputInterface.F90:1867-1874)
+         s = abs(spec%c1(i)) + abs(spec%c2(i))
+         if (s < epsilon(1.0_dp)) s = 1.0_dp
+         j%c%c1(i) = spec%c1(i)/s; j%c%c2(i) = spec%c2(i)/s
+         j%c%za(i) = spec%za(i); j%c%zb(i) = spec%zb(i)
+         j%c%n1(i) = spec%n1(i); j%c%l1(i) = spec%l1(i); j%c%n2(i) = spec%n2(i); j%c%l2(i) = spec%l2(i)
+      enddo
+      ierr = case_setup(j%c)
+      if (ierr /= 0) then
+         rc = 2; return
+      endif
+      ierr = grid_build(j%g, int(spec%nni), int(spec%nmu), spec%z1, spec%z2, spec%r, spec%rinf)
+      if (ierr /= 0) then
+         rc = 1; return
+      endif
+      j%o = scf_opts()
+      j%o%maxscf = spec%maxscf; j%o%ienterm = spec%ienterm; j%o%inoterm = spec%inoterm
+      j%o%nscfextra = spec%nscfextra; j%o%ienterm4init = spec%ienterm4init
+      j%o%maxsororb = spec%maxsororb; j%o%maxsorpot = spec%maxsorpot
+      j%o%mpole = spec%mpole; j%o%prec = spec%prec; j%o%schedule = spec%schedule
+      j%o%omega_orb = spec%omega_orb; j%o%omega_pot = spec%omega_pot; j%o%recalc_mm = spec%recalc_mm
+      ierr = job_init(j, int(spec%device), ctx_in)
+      used(h) = .true.
+      handle = h
+      rc = 0
+    end associate
+  end function x2f_job_create
+
+  logical function valid(handle)
+    integer(c_int), intent(in) :: handle
+    valid = handle >= 1 .and. handle <= MAXJOBS .and. allocated(jobs)
+    if (valid) valid = used(handle)
+  end function valid
+
+  ! Set-up pass (prepSCF).
+  integer(c_int) function x2f_job_prep(handle) bind(c, name='x2f_job_prep') result(rc)
+    integer(c_int), value :: handle
+    rc = 1
+    if (.not. valid(handle)) return
+    call prep_scf(jobs(handle))
+    rc = 0
+  end function x2f_job_prep
+
+  ! Up to niter iterations of the job's own schedule; 1 when the SCF stopped.
+  integer(c_int) function x2f_job_iterate(handle, niter) bind(c, name='x2f_job_iterate') result(rc)
+    integer(c_int), value :: handle, niter
+    integer :: k
+    logical :: st(1)
+    rc = -1
+    if (.not. valid(handle)) return
+    rc = 0
+    do k = 1, niter
+       if (jobs(handle)%o%schedule == 1) then
+          st(1) = .false.
+          call batched_iteration(jobs, [int(handle)], 1, st)
+          if (st(1)) then
+             rc = 1; exit
+          endif
+       else
+          if (scf_iteration(jobs(handle))) then
+             rc = 1; exit
+          endif
+       endif
+    enddo
+  end function x2f_job_iterate
+
+  ! One batched iteration over n jobs that share a device context; stopped(i)
+  ! is set to 1 for the jobs whose SCF stopped.
+  integer(c_int) function x2f_jobs_iterate_batched(n, handles, stopped) &
+       bind(c, name='x2f_jobs_iterate_batched') result(rc)
+    integer(c_int), value :: n
+    integer(c_int), intent(in) :: handles(n)
+    integer(c_int), intent(inout) :: stopped(n)
+    logical, allocatable :: st(:)
+    integer, allocatable :: idx(:)
+    integer :: q
+    rc = -1
+    do q = 1, n
+       if (.not. valid(handles(q))) return
+       if (.not. c_associated(jobs(handles(q))%ctx, jobs(handles(1))%ctx)) return
+    enddo
+    allocate(idx(n), st(n))
+    idx = handles
+    st = stopped /= 0
+    call batched_iteration(jobs, idx, int(n), st)
+    stopped = merge(1_c_int, 0_c_int, st)
+    deallocate(idx, st)
+    rc = 0
+  end function x2f_jobs_iterate_batched
+
+  integer(c_int) function x2f_job_finish(handle) bind(c, name='x2f_job_finish') result(rc)
+    integer(c_int), value :: handle
+    rc = 1
+    if (.not. valid(handle)) return
+    call scf_finish(jobs(handle))
+    rc = 0
+  end function x2f_job_finish
+
+  ! prep + iterations to the end + final pass
+  integer(c_int) function x2f_job_run(handle) bind(c, name='x2f_job_run') result(rc)
+    integer(c_int), value :: handle
+    rc = x2f_job_prep(handle)
+    if (rc /= 0) return
+    do
+       rc = x2f_job_iterate(handle, 1000000_c_int)
+       if (rc /= 0) exit
+    enddo
+    rc = x2f_job_finish(handle)
+  end function x2f_job_run
+
+  integer(c_int) function x2f_job_result(handle, res) bind(c, name='x2f_job_result') result(rc)
+    integer(c_int), value :: handle
+    type(x2f_result), intent(out) :: res
+    integer :: i
+    rc = 1
+    if (.not. valid(handle)) return
+    associate (j => jobs(handle))
+      res%status = merge(1_c_int, 0_c_int, j%done)
+      res%iterations = j%iscf
+      res%stopcode = j%stopcode
+      res%norb = j%c%norb
+      res%nsweep_orb = j%nsweep_orb
+      res%nsweep_pot = j%nsweep_pot
+      res%bytes = int(job_bytes(j), c_long_long)
+      res%etot = j%etot; res%ekin = j%vkt; res%enuc = j%vnt
+      res%ecoul = j%ecoul; res%eexch = j%eexch; res%virial = j%virial
+      res%delta_e = j%deltaeemax; res%delta_norm = j%deltanormmax
+      res%eps = 0; res%norm = 0; res%cmulti = 0
+      do i = 1, j%c%norb
+         res%eps(i) = j%ee(i,i)
+         res%norm(i) = j%orbnorm(i)
+         res%cmulti(:, i) = j%cmulti(:, i)
+      enddo
+    end associate
+    rc = 0
+  end function x2f_job_result
+
+  ! Copy orbital i (kind 0), Coulomb potential i (kind 1) or exchange slot i
+  ! (kind 2) to the host array out(nni*nmu).
+  integer(c_int) function x2f_job_get(handle, kind, i, out) bind(c, name='x2f_job_get') result(rc)
+    integer(c_int), value :: handle, kind, i
+    real(c_double), intent(out), target :: out(*)
+    type(c_ptr) :: p
+    rc = 1
+    if (.not. valid(handle)) return
+    associate (j => jobs(handle))
+      select case (kind)
+      case (0)
+         if (i < 1 .or. i > j%c%norb) return
+         p = j%orb(i)
+      case (1)
+         if (i < 1 .or. i > j%c%norb) return
+         p = j%coul(i)
+      case (2)
+         if (i < 1 .or. i > j%c%nslots) return
+         p = j%exch(i)
+      case default
+         return
+      end select
+      rc = x2g_d2h(j%ctx, c_loc(out), p, 8_c_size_t*j%n)
+    end associate
+  end function x2f_job_get
+
+  integer(c_long_long) function x2f_job_bytes(handle) bind(c, name='x2f_job_bytes')
+    integer(c_int), value :: handle
+    x2f_job_bytes = -1
+    if (.not. valid(handle)) return
+    x2f_job_bytes = int(job_bytes(jobs(handle)), c_long_long)
+  end function x2f_job_bytes
+
+  integer(c_int) function x2f_job_destroy(handle) bind(c, name='x2f_job_destroy') result(rc)
+    integer(c_int), value :: handle
+    rc = 1
+    if (.not. valid(handle)) return
+    call job_free(jobs(handle))
+    used(handle) = .false.
+    rc = 0
+  end function x2f_job_destroy
+end module x2f_api

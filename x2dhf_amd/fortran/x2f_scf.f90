@@ -1,0 +1,857 @@
+! SCF driver of one finite-difference HF job, with every array operation on
+! the device through the x2gpu C ABI.  The routines keep the roles and the
+! control flow of the reference's Fortran driver:
+!   prep_scf        prepSCF                     doSCF.F90:24-174
+!   scf_iteration   one pass of scf's loop      doSCF.F90:342-788
+!   orbital_step    one orbital of that pass    doSCF.F90:368-533
+!   relax_pots      coulExchSOR/MCSOR           relaxPots.F90:8-114,196-322
+!   relax_orb       orbSOR/orbMCSOR             relaxOrbs.F90:16-241
+!   fock_hf         fockHF                      fock.F90:57-184
+!   ea_hf           EaHF                        lagrangeMultipliers.F90:21-107
+!   norm / ortho    norm / ortho                normOrtho.F90:14-59,103-207
+!   pot_asympt      potAsympt                   scfUtils.F90:179-222
+!   mpole_moments   mpoleMoments/coulMom/exchMom scfUtils.F90:446-850
+!   etotal_hf       etotalHF                    totalEnergy.F90:13-245
+!   scf_finish      the tail of scf             doSCF.F90:793-858
+! Closed shells only: the off-diagonal Lagrange multipliers of such cases are
+! zero in the reference (offDiagLM is false for every pair, EabHF does
+! nothing), so EabHF has no counterpart here.
+!
+! Two schedules:
+!   schedule 0 ("plain"): the reference's order -- orbitals in reverse order,
+!     each with its own Coulomb/exchange potentials relaxed first;
+!   schedule 1 ("batched"): per iteration all potentials, then all orbitals
+!     of one or many jobs are relaxed in single fused launches; the same
+!     equations, a Jacobi instead of Gauss-Seidel order across orbitals.
+module x2f_scf
+  use, intrinsic :: iso_c_binding
+  use x2f_capi
+  use x2f_grid
+  use x2f_case
+  use x2f_guess
+  implicit none
+  private
+  public :: scf_job, scf_opts, job_init, job_free, prep_scf, scf_iteration, scf_finish, &
+       batched_iteration, MAXPOT, job_bytes
+
+  integer, parameter :: dp = c_double
+  integer, parameter :: MAXPOT = MAXO*(MAXO+1) + MAXO   ! exchange slots + Coulomb
+
+  type :: scf_opts
+     integer :: maxscf = 1000, ienterm = 10, inoterm = 8, nscfextra = 3, ienterm4init = 1
+     integer :: maxsororb(2) = [1, 10], maxsorpot(2) = [1, 10]
+     integer :: mpole = 4, prec = 0, schedule = 0
+     integer :: nscf2skip = 600, nenlast = 20, nnolast = 20, maxresets = 5
+     real(dp) :: omega_orb = -1.0_dp, omega_pot = -1.0_dp, recalc_mm = 1.15_dp
+  end type scf_opts
+
+  type :: scf_job
+     type(case_t) :: c
+     type(grid_t) :: g
+     type(scf_opts) :: o
+     type(c_ptr) :: ctx = c_null_ptr, gd = c_null_ptr
+     logical :: own_ctx = .true.
+     type(c_ptr) :: S(14)                                  ! device coefficient arrays
+     type(c_ptr) :: orb(MAXO), coul(MAXO), cpot(MAXO), xpot(MAXO)
+     type(c_ptr) :: exch(MAXO*(MAXO+1))
+     type(c_ptr) :: f1o(MAXO), f2o(MAXO), lo(MAXO)        ! per-orbital fock1, fock2, lhs
+     type(c_ptr) :: plhs(MAXPOT), prhs(MAXPOT)             ! per-potential lhs, rhs
+     integer :: pidx(MAXO, 2*MAXO+1) = 0                   ! (orbital, k) -> potential index
+     type(c_ptr) :: ones = c_null_ptr
+     integer(c_size_t) :: n = 0
+     integer :: ndev = 0
+     type(c_ptr) :: devbufs(512)
+     ! state
+     real(dp) :: ee(MAXO,MAXO) = 0, eeold(MAXO) = 0, deltaee(MAXO) = 0, orbnorm(MAXO) = 1
+     real(dp) :: cmulti(8,MAXO) = 0, exc(8,MAXO*(MAXO+1)) = 0
+     integer :: ifixorb(MAXO) = 0, itouch(MAXO) = 0, iasympt = 0
+     logical :: lfixorb = .false., lfixorb4init = .false.
+     real(dp) :: omega_orb = 0, omega_pot = 0
+     ! control state of the SCF loop
+     integer :: iscf = 0, nresets = 0, nresets4no = 0, norbenergythld = 0, norbnormthld = 0
+     integer :: noenergydec = 0, nonormdec = 0
+     logical :: lenergythld = .false., lnormthld = .false., done = .false.
+     real(dp) :: eeconv = 10, eeconvprev = 1.e35_dp, dnmaxprev = 1.e35_dp
+     real(dp) :: deltaeemax = 100, deltanormmax = 100
+     integer :: stopcode = 0       ! 1 energy, 2 norm, 3 iteration limit, 4 saturation, 5 resets
+     integer(c_long_long) :: nsweep_orb = 0, nsweep_pot = 0
+     ! results
+     real(dp) :: etot = 0, vkt = 0, vnt = 0, ecoul = 0, eexch = 0, virial = 0
+  end type scf_job
+
+  real(dp), parameter :: eps0 = epsilon(1.0_dp)
+
+contains
+
+  ! ---------------------------------------------------------------------------
+  ! set-up: grid on the device, starting functions, device buffers
+  ! ctx_in: an existing x2gpu context to share (jobs relaxed together in one
+  ! batched launch must share one), or c_null_ptr for a context of its own.
+  integer function job_init(j, device, ctx_in) result(ierr)
+    type(scf_job), intent(inout), target :: j
+    integer, intent(in) :: device
+    type(c_ptr), intent(in) :: ctx_in
+    real(dp) :: coef(25)
+    real(dp), allocatable, target :: vec(:), orbs(:,:), tmp(:)
+    integer :: i, k, ip, ii, slot
+    integer(c_size_t) :: nb
+
+    ierr = 0
+    if (c_associated(ctx_in)) then
+       j%ctx = ctx_in
+       j%own_ctx = .false.
+    else
+       call x2check(x2g_ctx_create(int(device, c_int), j%ctx), 'x2g_ctx_create')
+       j%own_ctx = .true.
+    endif
+    allocate(vec(2*j%g%nmu + 2*j%g%nni))
+    call grid_coef(j%g, coef)
+    call grid_vec(j%g, vec)
+    call x2check(x2g_grid_create(j%ctx, int(j%g%nni, c_int), int(j%g%nmu, c_int), coef, j%g%suppl, vec, j%gd), &
+         'x2g_grid_create')
+    do k = 1, 14
+       j%S(k) = x2g_grid_suppl(j%gd, int(k-1, c_int))
+    enddo
+    j%n = int(j%g%nni, c_size_t)*int(j%g%nmu, c_size_t)
+    nb = 8_c_size_t*j%n
+    j%ndev = 0
+    do i = 1, j%c%norb
+       j%orb(i) = dalloc(j); j%coul(i) = dalloc(j); j%cpot(i) = dalloc(j); j%xpot(i) = dalloc(j)
+       j%f1o(i) = dalloc(j); j%f2o(i) = dalloc(j); j%lo(i) = dalloc(j)
+    enddo
+    do slot = 1, j%c%nslots
+       j%exch(slot) = dalloc(j)
+    enddo
+    j%ones = dalloc(j)
+    call x2check(x2g_fill(j%ctx, j%ones, j%n, 1.0_dp), 'x2g_fill')
+    ! potentials of the iteration, in the SCF's orbital order
+    ip = 0
+    do ii = 1, j%c%norb
+       i = j%c%norb - ii + 1
+       do k = 1, j%c%npot(i)
+          ip = ip + 1
+          j%pidx(i,k) = ip
+          j%plhs(ip) = dalloc(j); j%prhs(ip) = dalloc(j)
+          ! lhs = f3 + dm^2 e (relaxPots.F90:87-95), fixed for the job
+          if (j%c%pdm(i,k) == 0) then
+             call lincomb0(j, j%plhs(ip), j%S(8), 1.0_dp)
+          else
+             call lincomb1(j, j%plhs(ip), j%S(8), 1.0_dp, dble(j%c%pdm(i,k)**2), j%S(4), c_null_ptr, c_null_ptr)
+          endif
+       enddo
+    enddo
+
+    ! starting orbitals and potentials (initHyd + initCoulomb + initExchange)
+    allocate(orbs(j%n, j%c%norb), tmp(j%n))
+    call guess_orbitals(j%g, j%c, orbs)
+    do i = 1, j%c%norb
+       call x2check(x2g_h2d(j%ctx, j%orb(i), c_loc(orbs(1,i)), nb), 'x2g_h2d')
+       call guess_coulomb(j%g, j%c, i, tmp)
+       call x2check(x2g_h2d(j%ctx, j%coul(i), c_loc(tmp), nb), 'x2g_h2d')
+    enddo
+    do i = 1, j%c%norb
+       do ii = i, j%c%norb
+          k = pair_index(i, ii)
+          if (j%c%ilc(k) == 0) cycle
+          call guess_exchange(j%g, j%c, i, tmp)
+          call x2check(x2g_h2d(j%ctx, j%exch(j%c%slot(k)), c_loc(tmp), nb), 'x2g_h2d')
+          if (j%c%ilc(k) == 2) call x2check(x2g_h2d(j%ctx, j%exch(j%c%slot2(k)), c_loc(tmp), nb), 'x2g_h2d')
+       enddo
+    enddo
+    deallocate(orbs, tmp, vec)
+
+    ! relaxation parameters
+    call omega_defaults(j%g%nni, j%g%nmu, j%c%z1, j%c%z2, j%omega_orb, j%omega_pot)
+    if (j%o%omega_orb > 0.0_dp) j%omega_orb = j%o%omega_orb
+    if (j%o%omega_pot > 0.0_dp) j%omega_pot = j%o%omega_pot
+    ! a hydrogenic start keeps the orbitals frozen until the orbital energies
+    ! change by less than 10^-ienterm4init (read_orbpot, inputInterface.F90:2273-2291)
+    j%lfixorb4init = j%o%ienterm4init > 0
+    j%lfixorb = j%lfixorb4init
+    j%ifixorb = merge(1, 0, j%lfixorb4init)
+    j%ee = 0
+  end function job_init
+
+  function dalloc(j) result(p)
+    type(scf_job), intent(inout) :: j
+    type(c_ptr) :: p
+    call x2check(x2g_malloc(j%ctx, 8_c_size_t*j%n, p), 'x2g_malloc')
+    call x2check(x2g_fill(j%ctx, p, j%n, 0.0_dp), 'x2g_fill')
+    j%ndev = j%ndev + 1
+    j%devbufs(j%ndev) = p
+  end function dalloc
+
+  integer(c_size_t) function job_bytes(j)
+    type(scf_job), intent(in) :: j
+    job_bytes = 8_c_size_t*j%n*int(j%ndev + 14, c_size_t)
+  end function job_bytes
+
+  subroutine job_free(j)
+    type(scf_job), intent(inout) :: j
+    integer :: i
+    integer(c_int) :: rc
+    if (.not. c_associated(j%ctx)) return
+    do i = 1, j%ndev
+       rc = x2g_free(j%ctx, j%devbufs(i))
+    enddo
+    j%ndev = 0
+    if (c_associated(j%gd)) rc = x2g_grid_destroy(j%gd)
+    if (j%own_ctx) rc = x2g_ctx_destroy(j%ctx)
+    j%ctx = c_null_ptr
+    j%gd = c_null_ptr
+  end subroutine job_free
+
+  ! ---------------------------------------------------------------------------
+  ! small wrappers of x2g_lincomb / x2g_wdot
+  subroutine lincomb0(j, out, base, s)
+    type(scf_job), intent(inout) :: j
+    type(c_ptr), intent(in) :: out, base
+    real(dp), intent(in) :: s
+    type(x2g_term) :: t(1)
+    call x2check(x2g_lincomb(j%gd, out, base, s, t, 0_c_int), 'x2g_lincomb')
+  end subroutine lincomb0
+
+  subroutine lincomb1(j, out, base, s, c, a, b, cc)
+    type(scf_job), intent(inout) :: j
+    type(c_ptr), intent(in) :: out, base, a, b, cc
+    real(dp), intent(in) :: s, c
+    type(x2g_term) :: t(1)
+    t(1) = x2g_term(c, a, b, cc)
+    call x2check(x2g_lincomb(j%gd, out, base, s, t, 1_c_int), 'x2g_lincomb')
+  end subroutine lincomb1
+
+  ! ---------------------------------------------------------------------------
+  ! fockHF (fock.F90:57-184): fock1 = f0 + eps f1 + m^2 e + f2 * sum_j coo_j J_j,
+  ! fock2 = f2 * sum_j psi_j (gec K_ij [+ gec' K'_ij] + eps_ji f4) [+ self term].
+  ! The Coulomb and exchange sums are kept per orbital for EaHF.
+  subroutine fock_hf(j, i, fock1, fock2)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i
+    type(c_ptr), intent(in) :: fock1, fock2
+    type(x2g_term) :: t(4*MAXO+4)
+    integer :: n, i1, k
+    real(dp) :: coo
+    n = 0
+    n = n + 1; t(n) = x2g_term(j%ee(i,i), j%S(6), c_null_ptr, c_null_ptr)
+    if (j%c%m(i) /= 0) then
+       n = n + 1; t(n) = x2g_term(dble(j%c%m(i)**2), j%S(4), c_null_ptr, c_null_ptr)
+    endif
+    if (j%c%nel == 1) then
+       call x2check(x2g_lincomb(j%gd, fock1, j%S(5), 1.0_dp, t, int(n, c_int)), 'fock1')
+       call x2check(x2g_fill(j%ctx, fock2, j%n, 0.0_dp), 'fock2')
+       return
+    endif
+    ! Coulomb part
+    block
+      type(x2g_term) :: tc(MAXO)
+      do i1 = 1, j%c%norb
+         coo = j%c%occ(i1)
+         if (i1 == i) coo = coo - 1.0_dp
+         tc(i1) = x2g_term(coo, j%coul(i1), c_null_ptr, c_null_ptr)
+      enddo
+      call x2check(x2g_lincomb(j%gd, j%cpot(i), c_null_ptr, 0.0_dp, tc, int(j%c%norb, c_int)), 'fock coulomb')
+    end block
+    ! exchange part
+    block
+      type(x2g_term) :: tx(4*MAXO)
+      integer :: nx
+      nx = 0
+      do i1 = 1, j%c%norb
+         k = j%c%k2(i, i1)
+         if (i1 /= i) then
+            nx = nx + 1; tx(nx) = x2g_term(j%c%gec(i,i1), j%exch(j%c%slot(k)), j%orb(i1), c_null_ptr)
+            if (j%c%ilc(k) == 2) then
+               nx = nx + 1; tx(nx) = x2g_term(j%c%gec2(i,i1), j%exch(j%c%slot2(k)), j%orb(i1), c_null_ptr)
+            endif
+            if (abs(j%ee(i1,i)) > eps0) then
+               nx = nx + 1; tx(nx) = x2g_term(j%ee(i1,i), j%S(9), j%orb(i1), c_null_ptr)
+            endif
+         else
+            if (j%c%m(i) > 0 .and. k > 0) then
+               if (j%c%ilc(k) > 0) then
+                  nx = nx + 1; tx(nx) = x2g_term(j%c%gec(i,i), j%exch(j%c%slot(k)), j%orb(i), c_null_ptr)
+               endif
+            endif
+         endif
+      enddo
+      call x2check(x2g_lincomb(j%gd, j%xpot(i), c_null_ptr, 0.0_dp, tx, int(nx, c_int)), 'fock exchange')
+    end block
+    n = n + 1; t(n) = x2g_term(1.0_dp, j%S(7), j%cpot(i), c_null_ptr)
+    call x2check(x2g_lincomb(j%gd, fock1, j%S(5), 1.0_dp, t, int(n, c_int)), 'fock1')
+    t(1) = x2g_term(1.0_dp, j%S(7), j%xpot(i), c_null_ptr)
+    call x2check(x2g_lincomb(j%gd, fock2, c_null_ptr, 0.0_dp, t, 1_c_int), 'fock2')
+  end subroutine fock_hf
+
+  ! ---------------------------------------------------------------------------
+  ! relaxation items of the potentials of orbital i (relaxPots): rhs = psi1 psi2 g
+  subroutine pot_items(j, i, items, n, gidx)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i, gidx
+    type(x2g_relax_item), intent(inout) :: items(:)
+    integer, intent(inout) :: n
+    integer :: k, ip
+    type(c_ptr) :: fun
+    do k = 1, j%c%npot(i)
+       ip = j%pidx(i,k)
+       call lincomb1(j, j%prhs(ip), c_null_ptr, 0.0_dp, 1.0_dp, j%orb(j%c%pin1(i,k)), j%orb(j%c%pin2(i,k)), j%S(10))
+       if (j%c%pslot(i,k) == 0) then
+          fun = j%coul(i)
+       else
+          fun = j%exch(j%c%pslot(i,k))
+       endif
+       n = n + 1
+       items(n) = x2g_relax_item(fun, j%plhs(ip), j%prhs(ip), c_null_ptr, j%omega_pot, &
+            int(j%c%psym(i,k), c_int), 1_c_int, int(j%o%prec, c_int), int(gidx, c_int))
+    enddo
+    j%nsweep_pot = j%nsweep_pot + int(j%c%npot(i), c_long_long)*j%o%maxsorpot(1)*j%o%maxsorpot(2)
+  end subroutine pot_items
+
+  ! the relaxation item of orbital i (orbMCSOR): lhs = fock1 + diag, rhs = fock2
+  subroutine orb_item(j, i, items, n, gidx)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i, gidx
+    type(x2g_relax_item), intent(inout) :: items(:)
+    integer, intent(inout) :: n
+    call fock_hf(j, i, j%f1o(i), j%f2o(i))
+    call lincomb1(j, j%lo(i), j%f1o(i), 1.0_dp, j%g%diag, j%ones, c_null_ptr, c_null_ptr)
+    n = n + 1
+    items(n) = x2g_relax_item(j%orb(i), j%lo(i), j%f2o(i), j%f1o(i), j%omega_orb, &
+         int(j%c%isym(i), c_int), 0_c_int, int(j%o%prec, c_int), int(gidx, c_int))
+    j%nsweep_orb = j%nsweep_orb + int(j%o%maxsororb(1)*j%o%maxsororb(2), c_long_long)
+  end subroutine orb_item
+
+  subroutine relax_pots(j, i)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i
+    type(x2g_relax_item) :: items(2*MAXO+1)
+    type(c_ptr) :: grids(1)
+    integer :: n
+    n = 0
+    call pot_items(j, i, items, n, 0)
+    grids(1) = j%gd
+    call x2check(x2g_relax_batch(j%ctx, grids, items, int(n, c_int), int(j%o%maxsorpot(1), c_int), &
+         int(j%o%maxsorpot(2), c_int)), 'relax potentials')
+  end subroutine relax_pots
+
+  subroutine relax_orb(j, i)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i
+    type(x2g_relax_item) :: items(1)
+    type(c_ptr) :: grids(1)
+    integer :: n
+    if (j%ifixorb(i) == 1) return
+    n = 0
+    call orb_item(j, i, items, n, 0)
+    grids(1) = j%gd
+    call x2check(x2g_relax_batch(j%ctx, grids, items, 1_c_int, int(j%o%maxsororb(1), c_int), &
+         int(j%o%maxsororb(2), c_int)), 'relax orbital')
+  end subroutine relax_orb
+
+  ! ---------------------------------------------------------------------------
+  ! norm (normOrtho.F90:14-59)
+  subroutine norm(j, i)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i
+    type(x2g_dot) :: d(1)
+    real(dp) :: x(1)
+    d(1) = x2g_dot(j%S(14), j%orb(i), j%orb(i), j%S(9))
+    call x2check(x2g_wdot(j%gd, d, 1_c_int, x), 'norm')
+    j%orbnorm(i) = x(1)
+    call lincomb0(j, j%orb(i), j%orb(i), 1.0_dp/sqrt(x(1)))
+  end subroutine norm
+
+  ! ortho (normOrtho.F90:103-190): Gram-Schmidt against the higher orbitals of
+  ! the same symmetry, then the reference's rescaling
+  subroutine ortho(j, i1)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i1
+    type(x2g_dot) :: d(MAXO)
+    type(x2g_term) :: t(MAXO)
+    real(dp) :: ovla(MAXO), ano
+    integer :: jor, i2, i3, k
+    if (j%ifixorb(i1) /= 0) return
+    jor = 0
+    do i2 = 1, j%c%norb
+       i3 = j%c%norb - i2 + 1
+       if (i3 <= i1) cycle
+       if (j%c%m(i3) /= j%c%m(i1)) cycle
+       if (j%c%ige(i3) /= j%c%ige(i1)) cycle
+       jor = jor + 1
+       d(jor) = x2g_dot(j%S(14), j%orb(i1), j%orb(i3), j%S(9))
+       t(jor)%a_d = j%orb(i3)
+    enddo
+    if (jor == 0) return
+    call x2check(x2g_wdot(j%gd, d, int(jor, c_int), ovla), 'ortho overlaps')
+    ano = 0.0_dp
+    do k = 1, jor
+       ano = ano + ovla(k)*ovla(k)
+       t(k)%c = -ovla(k)
+       t(k)%b_d = c_null_ptr
+       t(k)%cc_d = c_null_ptr
+    enddo
+    call x2check(x2g_lincomb(j%gd, j%orb(i1), j%orb(i1), 1.0_dp, t, int(jor, c_int)), 'ortho')
+    ano = sqrt(1.0_dp + ano)/(1.0_dp - ano*ano)
+    call lincomb0(j, j%orb(i1), j%orb(i1), ano)
+  end subroutine ortho
+
+  subroutine setci(j, i)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i
+    integer :: k
+    if (.not. j%c%homonuclear) return
+    if (i == 0) then
+       do k = 1, j%c%norb
+          call x2check(x2g_setci(j%gd, j%orb(k), int(j%c%ihomo(k), c_int)), 'setci')
+       enddo
+    else
+       call x2check(x2g_setci(j%gd, j%orb(i), int(j%c%ihomo(i), c_int)), 'setci')
+    endif
+  end subroutine setci
+
+  ! EaHF (lagrangeMultipliers.F90:21-107) with the Coulomb and exchange sums of
+  ! the last fock_hf of orbital i
+  subroutine ea_hf(j, i)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i
+    real(dp) :: one(3), two(2)
+    type(x2g_dot) :: d(2)
+    call x2check(x2g_oneel(j%gd, j%orb(i), j%orb(i), int(j%c%isym(i), c_int), int(j%c%m(i), c_int), one), 'oneel')
+    d(1) = x2g_dot(j%S(14), j%orb(i), j%cpot(i), j%orb(i))
+    d(2) = x2g_dot(j%S(14), j%orb(i), j%xpot(i), c_null_ptr)
+    call x2check(x2g_wdot(j%gd, d, 2_c_int, two), 'EaHF')
+    j%ee(i,i) = one(1) + (two(1) - two(2))
+  end subroutine ea_hf
+
+  ! ---------------------------------------------------------------------------
+  ! mpoleMoments: Coulomb moments of touched orbitals (odd ones vanish by
+  ! symmetry for homonuclear molecules) and the moments of every exchange pair
+  subroutine mpole_moments(j)
+    type(scf_job), intent(inout) :: j
+    integer :: i, k
+    real(dp) :: mom(8)
+    if (j%c%nel == 1) return
+    do i = 1, j%c%norb
+       if (j%itouch(i) == 0) cycle
+       call x2check(x2g_moments(j%gd, j%orb(i), j%orb(i), 0_c_int, int(j%o%mpole, c_int), mom), 'coulMom')
+       j%cmulti(1:j%o%mpole, i) = mom(1:j%o%mpole)
+       if (j%c%homonuclear) then
+          j%cmulti(1, i) = 0.0_dp; j%cmulti(3, i) = 0.0_dp
+          j%cmulti(5, i) = 0.0_dp; j%cmulti(7, i) = 0.0_dp
+       endif
+    enddo
+    do k = 1, j%c%nmm
+       call x2check(x2g_moments(j%gd, j%orb(j%c%mm1(k)), j%orb(j%c%mm2(k)), int(j%c%mmdel(k), c_int), &
+            int(j%o%mpole, c_int), mom), 'exchMom')
+       j%exc(1:j%o%mpole, j%c%mmslot(k)) = mom(1:j%o%mpole)
+    enddo
+  end subroutine mpole_moments
+
+  ! potAsympt: multipole tails of the potentials of orbital i1
+  subroutine pot_asympt(j, i1)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i1
+    integer :: i2, k, idel
+    if (j%itouch(i1) == 1) &
+         call x2check(x2g_pot_tail(j%gd, j%coul(i1), 0_c_int, 0_c_int, int(j%o%mpole, c_int), j%cmulti(:, i1)), &
+         'coulAsympt')
+    do i2 = 1, j%c%norb
+       if (j%itouch(i1) == 0 .and. j%itouch(i2) == 0) cycle
+       k = j%c%k2(i1, i2)
+       if (k == 0) cycle
+       if (j%c%ilc(k) == 0) cycle
+       idel = abs(j%c%m(i1) - j%c%m(i2))
+       if (i1 == i2) idel = 2*j%c%m(i1)
+       call x2check(x2g_pot_tail(j%gd, j%exch(j%c%slot(k)), 1_c_int, int(idel, c_int), int(j%o%mpole, c_int), &
+            j%exc(:, j%c%slot(k))), 'exchAsympt')
+       if (j%c%ilc(k) == 2) then
+          idel = j%c%m(i2) + j%c%m(i1)
+          call x2check(x2g_pot_tail(j%gd, j%exch(j%c%slot2(k)), 1_c_int, int(idel, c_int), int(j%o%mpole, c_int), &
+               j%exc(:, j%c%slot2(k))), 'exchAsympt')
+       endif
+    enddo
+  end subroutine pot_asympt
+
+  ! ---------------------------------------------------------------------------
+  ! etotalHF (totalEnergy.F90:13-245)
+  subroutine etotal_hf(j)
+    type(scf_job), intent(inout) :: j
+    type(x2g_dot) :: d(MAXO*MAXO*2 + MAXO)
+    real(dp) :: v(MAXO*MAXO*2 + MAXO), one(3), vk(MAXO), vn(MAXO), oneel(MAXO)
+    real(dp) :: woneel, wdcoul, wndc, wex1, wex2, evt, epott
+    integer :: i1, i2, nd, k, idx
+    ! one-electron parts
+    do i1 = 1, j%c%norb
+       call x2check(x2g_oneel(j%gd, j%orb(i1), j%orb(i1), int(j%c%isym(i1), c_int), int(j%c%m(i1), c_int), one), &
+            'oneelii')
+       oneel(i1) = one(1); vk(i1) = one(2); vn(i1) = one(3)
+    enddo
+    ! all two-electron quadratures in one batch, combined below in the reference's order
+    nd = 0
+    do i1 = 1, j%c%norb
+       nd = nd + 1; d(nd) = x2g_dot(j%S(14), j%orb(i1), j%coul(i1), j%orb(i1))            ! J_ii
+       if (j%c%m(i1) > 0) then
+          k = j%c%k2(i1, i1)
+          nd = nd + 1; d(nd) = x2g_dot(j%S(14), j%orb(i1), j%exch(j%c%slot(k)), j%orb(i1)) ! K_ii (dm = 2m)
+       endif
+       do i2 = i1+1, j%c%norb
+          k = j%c%k2(i1, i2)
+          nd = nd + 1; d(nd) = x2g_dot(j%S(14), j%orb(i1), j%coul(i2), j%orb(i1))        ! coulij(i2,i1)
+          nd = nd + 1; d(nd) = x2g_dot(j%S(14), j%orb(i1), j%exch(j%c%slot(k)), j%orb(i2))
+          if (j%c%ilc(k) == 2) then
+             nd = nd + 1; d(nd) = x2g_dot(j%S(14), j%orb(i1), j%exch(j%c%slot2(k)), j%orb(i2))
+          endif
+       enddo
+    enddo
+    call x2check(x2g_wdot(j%gd, d, int(nd, c_int), v), 'etotal')
+    woneel = 0; wdcoul = 0; wndc = 0; wex1 = 0; wex2 = 0
+    j%vkt = 0; j%vnt = 0
+    idx = 0
+    do i1 = 1, j%c%norb
+       woneel = woneel + j%c%occ(i1)*oneel(i1)
+       j%vkt = j%vkt + j%c%occ(i1)*vk(i1)
+       j%vnt = j%vnt + j%c%occ(i1)*vn(i1)
+       idx = idx + 1
+       if (j%c%occ(i1) > 1.0_dp) wdcoul = wdcoul + j%c%occ(i1)*(j%c%occ(i1) - 1.0_dp)*v(idx)/2.0_dp
+       if (j%c%m(i1) > 0) idx = idx + 1
+       do i2 = i1+1, j%c%norb
+          idx = idx + 2
+          k = j%c%k2(i1, i2)
+          if (j%c%ilc(k) == 2) idx = idx + 1
+       enddo
+    enddo
+    idx = 0
+    do i1 = 1, j%c%norb
+       idx = idx + 1
+       if (j%c%m(i1) > 0) then
+          idx = idx + 1
+          if (abs(j%c%occ(i1) - 1.0_dp) > 1.e-7_dp) wex1 = wex1 + j%c%oxself(i1)*v(idx)
+       endif
+       do i2 = i1+1, j%c%norb
+          k = j%c%k2(i1, i2)
+          idx = idx + 1
+          wndc = wndc + j%c%occ(i1)*j%c%occ(i2)*v(idx)
+          idx = idx + 1
+          wex1 = wex1 + j%c%ox1(i1,i2)*v(idx)
+          if (j%c%ilc(k) == 2) then
+             idx = idx + 1
+             wex2 = wex2 + j%c%ox2(i1,i2)*v(idx)
+          endif
+       enddo
+    enddo
+    evt = woneel + wdcoul + wndc - wex1 - wex2
+    epott = j%vnt + wdcoul + wndc - wex1 - wex2
+    j%virial = (epott + j%c%z1*j%c%z2/j%c%r)/j%vkt
+    j%etot = evt + j%c%z1*j%c%z2/j%c%r
+    j%ecoul = wdcoul + wndc
+    j%eexch = -wex1 - wex2
+  end subroutine etotal_hf
+
+  ! ---------------------------------------------------------------------------
+  ! prepSCF for a start from guessed functions
+  subroutine prep_scf(j)
+    type(scf_job), intent(inout) :: j
+    integer :: i, ii
+    do i = 1, j%c%norb
+       call fock_hf(j, i, j%f1o(i), j%f2o(i))
+    enddo
+    do ii = 1, j%c%norb
+       i = j%c%norb + 1 - ii
+       call norm(j, i)
+       call ortho(j, i)
+    enddo
+    j%itouch = 1
+    call mpole_moments(j)
+    do i = j%c%norb, 1, -1
+       call ea_hf(j, i)
+       j%eeold(i) = j%ee(i,i)
+    enddo
+    call etotal_hf(j)
+    ! scf's own initialisation (doSCF.F90:265-334)
+    j%iscf = 0
+    j%nresets = 0; j%nresets4no = 0; j%norbenergythld = 0; j%norbnormthld = 0
+    j%eeconv = 10.0_dp
+    j%itouch = 0
+    j%deltaeemax = 100.0_dp
+    j%deltaee = 0.0_dp
+    do i = 1, j%c%norb
+       j%eeold(i) = j%ee(i,i)
+    enddo
+    j%deltanormmax = 100.0_dp
+    j%lenergythld = .false.; j%lnormthld = .false.
+    j%eeconvprev = 1.e35_dp; j%dnmaxprev = 1.e35_dp
+    j%noenergydec = 0; j%nonormdec = 0
+    j%done = .false.; j%stopcode = 0
+    call setci(j, 0)
+  end subroutine prep_scf
+
+  ! one orbital of the plain schedule (doSCF.F90:368-533)
+  subroutine orbital_step(j, i)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i
+    j%itouch(i) = 1
+    if (j%iasympt /= 0 .and. j%ifixorb(i) == 0) then
+       call pot_asympt(j, i)
+       j%iasympt = j%iasympt - 1
+    endif
+    call relax_pots(j, i)
+    if (.not. j%lfixorb) call relax_orb(j, i)
+    if (j%ifixorb(i) == 0) call norm(j, i)
+    call setci(j, i)
+    if (j%c%norb > 1 .and. j%ifixorb(i) == 0) call ortho(j, i)
+    if (j%ifixorb(i) /= 0) call fock_hf(j, i, j%f1o(i), j%f2o(i))
+    call ea_hf(j, i)
+    call energy_change(j, i)
+  end subroutine orbital_step
+
+  subroutine energy_change(j, i)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i
+    real(dp) :: thld
+    thld = 10.0_dp**dble(-j%o%ienterm)
+    j%deltaee(i) = j%ee(i,i) - j%eeold(i)
+    if (abs(j%deltaee(i)) < eps0) j%deltaee(i) = 2.0_dp*thld
+    j%eeold(i) = j%ee(i,i)
+  end subroutine energy_change
+
+  ! one SCF iteration of the plain schedule; returns .true. when the SCF stops
+  logical function scf_iteration(j) result(stop_now)
+    type(scf_job), intent(inout) :: j
+    integer :: ii
+    j%iscf = j%iscf + 1
+    j%iasympt = j%c%norb
+    do ii = 1, j%c%norb
+       call orbital_step(j, j%c%norb - ii + 1)
+    enddo
+    stop_now = convergence_control(j)
+  end function scf_iteration
+
+  ! the end-of-iteration bookkeeping of scf (doSCF.F90:546-744)
+  logical function convergence_control(j) result(stop_now)
+    type(scf_job), intent(inout) :: j
+    real(dp) :: ethld, nthld, thren4init
+    integer :: nei
+    stop_now = .false.
+    ethld = 10.0_dp**dble(-j%o%ienterm)
+    nthld = 10.0_dp**dble(-j%o%inoterm)
+    thren4init = 10.0_dp**dble(-j%o%ienterm4init)
+
+    j%deltaeemax = 0.0_dp
+    do nei = 1, j%c%norb
+       if (abs(j%deltaee(nei)) >= abs(j%deltaeemax)) j%deltaeemax = j%deltaee(nei)
+    enddo
+    j%deltanormmax = 0.0_dp
+    do nei = 1, j%c%norb
+       if (j%ifixorb(nei) == 0) then
+          if (abs(j%orbnorm(nei) - 1.0_dp) >= j%deltanormmax) then
+             j%deltanormmax = abs(j%orbnorm(nei) - 1.0_dp)
+             if (abs(j%deltanormmax) < eps0) j%deltanormmax = eps0
+          endif
+       endif
+    enddo
+    if (abs(j%eeconv) < eps0) j%eeconv = j%deltaeemax
+
+    ! saturation (no decrease for nenlast/nnolast iterations after nscf2skip)
+    if (j%iscf > j%o%nscf2skip) then
+       if (abs(j%deltaeemax) >= abs(j%eeconvprev)) then
+          j%noenergydec = j%noenergydec + 1
+       else
+          j%eeconvprev = j%deltaeemax
+          j%noenergydec = 0
+       endif
+       if (abs(j%deltanormmax) >= abs(j%dnmaxprev)) then
+          j%nonormdec = j%nonormdec + 1
+       else
+          j%dnmaxprev = j%deltanormmax
+          j%nonormdec = 0
+       endif
+       if (j%noenergydec > j%o%nenlast .or. j%nonormdec > j%o%nnolast) then
+          j%stopcode = 4; stop_now = .true.; return
+       endif
+    else
+       j%eeconvprev = 100.0_dp
+       j%dnmaxprev = j%deltanormmax
+    endif
+    if (j%iscf == j%o%maxscf) then
+       j%stopcode = 3; stop_now = .true.; return
+    endif
+
+    if (j%lfixorb4init .and. abs(j%deltaeemax) < thren4init) then
+       if (j%norbenergythld == j%o%nscfextra) then
+          j%ifixorb = 0
+          j%lfixorb = .false.
+          j%lfixorb4init = .false.
+          j%eeconv = 10.0_dp
+          j%deltaeemax = 100.0_dp
+       else
+          j%norbenergythld = j%norbenergythld + 1
+       endif
+    endif
+
+    if (.not. j%lenergythld) then
+       if (abs(j%deltaeemax) < ethld) then
+          j%lenergythld = .true.
+          j%eeconv = j%deltaeemax
+          j%iasympt = j%c%norb
+          j%itouch = 0
+          j%norbenergythld = 0
+       endif
+    else
+       if (abs(j%deltaeemax) < ethld) then
+          if (j%norbenergythld == j%o%nscfextra) then
+             j%stopcode = 1; stop_now = .true.; return
+          else
+             j%norbenergythld = j%norbenergythld + 1
+          endif
+       elseif (abs(j%deltanormmax) < nthld) then
+          if (j%norbnormthld == j%o%nscfextra) then
+             j%stopcode = 2; stop_now = .true.; return
+          else
+             j%norbnormthld = j%norbnormthld + 1
+          endif
+       else
+          j%lenergythld = .false.
+          if (j%nresets > j%o%maxresets) then
+             j%stopcode = 5; stop_now = .true.; return
+          else
+             j%nresets = j%nresets + 1
+          endif
+       endif
+    endif
+
+    if (.not. j%lfixorb) then
+       if (.not. j%lnormthld) then
+          if (abs(j%deltanormmax) < nthld) then
+             j%lnormthld = .true.
+             j%iasympt = j%c%norb
+             j%itouch = 0
+             j%norbnormthld = 0
+          endif
+       else
+          if (abs(j%deltanormmax) < nthld) then
+             if (j%norbnormthld == j%o%nscfextra) then
+                j%stopcode = 2; stop_now = .true.; return
+             else
+                j%norbnormthld = j%norbnormthld + 1
+             endif
+          else
+             j%lnormthld = .false.
+             if (j%nresets4no > j%o%maxresets) then
+                j%stopcode = 5; stop_now = .true.; return
+             else
+                j%nresets4no = j%nresets4no + 1
+             endif
+          endif
+       endif
+    endif
+
+    ! refresh the multipole moments when the energy change dropped enough
+    if (abs(j%eeconv/j%deltaeemax) > j%o%recalc_mm) then
+       j%eeconv = j%deltaeemax
+       call mpole_moments(j)
+    endif
+    j%iasympt = j%c%norb
+    j%itouch = 0
+  end function convergence_control
+
+  ! the final pass of scf (doSCF.F90:793-858)
+  subroutine scf_finish(j)
+    type(scf_job), intent(inout) :: j
+    real(dp) :: saved(MAXO)
+    integer :: i, ii
+    if (j%c%norb > 1) then
+       saved = j%orbnorm
+       do ii = 1, j%c%norb
+          i = j%c%norb + 1 - ii
+          call norm(j, i)
+          call ortho(j, i)
+       enddo
+       j%orbnorm = saved
+    endif
+    do i = j%c%norb, 1, -1
+       call ea_hf(j, i)
+    enddo
+    call mpole_moments(j)
+    call etotal_hf(j)
+    j%done = .true.
+  end subroutine scf_finish
+
+  ! ---------------------------------------------------------------------------
+  ! One iteration of the batched schedule over several jobs.  Per job it keeps
+  ! the same equations and control as the plain schedule, but relaxes all
+  ! potentials of all jobs in one fused launch sequence, then all orbitals of
+  ! all jobs, before the per-orbital normalisation, orthogonalisation and
+  ! orbital energies.  Returns per job whether it stopped.
+  subroutine batched_iteration(js, idx, nj, stopped)
+    type(scf_job), intent(inout), target :: js(:)
+    integer, intent(in) :: idx(:), nj
+    logical, intent(inout) :: stopped(:)
+    type(x2g_relax_item), allocatable :: items(:)
+    type(c_ptr), allocatable :: grids(:)
+    integer :: q, h, i, ii, n, nmax, ms1, ms2
+    nmax = 0
+    do q = 1, nj
+       if (stopped(q)) cycle
+       h = idx(q)
+       nmax = nmax + sum(js(h)%c%npot(1:js(h)%c%norb)) + js(h)%c%norb
+    enddo
+    if (nmax == 0) return
+    allocate(items(nmax), grids(nj))
+    ! 1. tails and potentials of every orbital of every job
+    n = 0
+    do q = 1, nj
+       h = idx(q)
+       grids(q) = js(h)%gd
+       if (stopped(q)) cycle
+       js(h)%iscf = js(h)%iscf + 1
+       js(h)%iasympt = js(h)%c%norb
+       do ii = 1, js(h)%c%norb
+          i = js(h)%c%norb - ii + 1
+          js(h)%itouch(i) = 1
+          if (js(h)%ifixorb(i) == 0) then
+             call pot_asympt(js(h), i)
+             js(h)%iasympt = js(h)%iasympt - 1
+          endif
+          call pot_items(js(h), i, items, n, q-1)
+       enddo
+    enddo
+    ms1 = 0; ms2 = 0
+    do q = 1, nj
+       if (stopped(q)) cycle
+       h = idx(q)
+       ms1 = max(ms1, js(h)%o%maxsorpot(1)); ms2 = max(ms2, js(h)%o%maxsorpot(2))
+    enddo
+    call x2check(x2g_relax_batch(js(idx(1))%ctx, grids, items, int(n, c_int), int(ms1, c_int), &
+         int(ms2, c_int)), 'batched potentials')
+    ! 2. Fock operators and all orbitals
+    n = 0
+    ms1 = 0; ms2 = 0
+    do q = 1, nj
+       h = idx(q)
+       if (stopped(q) .or. js(h)%lfixorb) cycle
+       do ii = 1, js(h)%c%norb
+          i = js(h)%c%norb - ii + 1
+          if (js(h)%ifixorb(i) == 1) cycle
+          call orb_item(js(h), i, items, n, q-1)
+       enddo
+       ms1 = max(ms1, js(h)%o%maxsororb(1)); ms2 = max(ms2, js(h)%o%maxsororb(2))
+    enddo
+    if (n > 0) call x2check(x2g_relax_batch(js(idx(1))%ctx, grids, items, int(n, c_int), int(ms1, c_int), &
+         int(ms2, c_int)), 'batched orbitals')
+    ! 3. normalisation, symmetry, orthogonalisation, orbital energies, control
+    do q = 1, nj
+       if (stopped(q)) cycle
+       h = idx(q)
+       do ii = 1, js(h)%c%norb
+          i = js(h)%c%norb - ii + 1
+          if (js(h)%ifixorb(i) == 0) call norm(js(h), i)
+          call setci(js(h), i)
+          if (js(h)%c%norb > 1 .and. js(h)%ifixorb(i) == 0) call ortho(js(h), i)
+          if (js(h)%ifixorb(i) /= 0) call fock_hf(js(h), i, js(h)%f1o(i), js(h)%f2o(i))
+          call ea_hf(js(h), i)
+          call energy_change(js(h), i)
+       enddo
+       stopped(q) = convergence_control(js(h))
+    enddo
+    deallocate(items, grids)
+  end subroutine batched_iteration
+end module x2f_scf
