@@ -53,6 +53,13 @@ int x2g_ctx_destroy(x2g_ctx *ctx);
 int x2g_ctx_sync(x2g_ctx *ctx);
 /* Return the context's hipStream_t (as void*). */
 void *x2g_ctx_stream(x2g_ctx *ctx);
+/* Time every sweep launch of this context with HIP events on its stream
+ * (on != 0 starts and clears the counters, 0 stops). */
+int x2g_stats_enable(x2g_ctx *ctx, int on);
+/* out[6] = {launches, milliseconds, algorithmic bytes} of the FP64 sweeps,
+ * then of the FP32 sweeps.  Algorithmic bytes per relaxed point and sweep:
+ * 4 words (read u, write u, read lhs, read rhs). */
+int x2g_stats_get(x2g_ctx *ctx, double *out);
 
 /* ---- device memory ------------------------------------------------------ */
 int x2g_malloc(x2g_ctx *ctx, size_t bytes, void **dptr);

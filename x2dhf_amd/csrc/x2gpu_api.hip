@@ -48,11 +48,104 @@ int pinned_reserve(x2g_ctx *ctx, size_t bytes)
     return X2G_OK;
 }
 
+int stage_bytes(x2g_ctx *ctx, const void *src, size_t bytes, const void **dev_out)
+{
+    const size_t cap = std::min(ctx->ws[2].bytes, ctx->pinned_bytes);
+    X2G_CHECK(bytes <= cap, "launch descriptors (%zu bytes) exceed the staging ring", bytes);
+    size_t off = (ctx->stage_off + 255) & ~size_t(255);
+    if (off + bytes > cap) {
+        // wrap: every copy and launch that used the ring has to be done
+        X2G_HIP(hipStreamSynchronize(ctx->stream));
+        off = 0;
+    }
+    std::memcpy((char *)ctx->pinned + off, src, bytes);
+    X2G_HIP(hipMemcpyAsync((char *)ctx->ws[2].ptr + off, (char *)ctx->pinned + off, bytes,
+                           hipMemcpyHostToDevice, ctx->stream));
+    *dev_out = (const char *)ctx->ws[2].ptr + off;
+    ctx->stage_off = off + bytes;
+    return X2G_OK;
+}
+
+static hipEvent_t take_event(x2g_ctx *ctx)
+{
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int stats_begin(x2g_ctx *ctx, hipEvent_t *a, hipEvent_t *b)
+{
+    *a = *b = nullptr;
+    if (!ctx->stats_on) return X2G_OK;
+    *a = take_event(ctx);
+    *b = take_event(ctx);
+    if (!*a || !*b) {
+        set_error("hipEventCreate failed");
+        return X2G_EHIP;
+    }
+    X2G_HIP(hipEventRecord(*a, ctx->stream));
+    return X2G_OK;
+}
+
+int stats_end(x2g_ctx *ctx, hipEvent_t a, hipEvent_t b, double bytes, int prec)
+{
+    if (!ctx->stats_on || !a) return X2G_OK;
+    X2G_HIP(hipEventRecord(b, ctx->stream));
+    ctx->pending.push_back({a, b, bytes, prec});
+    if (ctx->pending.size() >= 4096) return stats_flush(ctx);
+    return X2G_OK;
+}
+
+int stats_flush(x2g_ctx *ctx)
+{
+    if (ctx->pending.empty()) return X2G_OK;
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto &p : ctx->pending) {
+        float ms = 0.0f;
+        X2G_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        ctx->st_launch[p.prec] += 1;
+        ctx->st_ms[p.prec] += ms;
+        ctx->st_bytes[p.prec] += p.bytes;
+        ctx->ev_pool.push_back(p.a);
+        ctx->ev_pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+    return X2G_OK;
+}
+
 }  // namespace x2g
 
 using namespace x2g;
 
-static constexpr size_t kStageBytes = 4u << 20;  // launch-descriptor staging area
+extern "C" int x2g_stats_enable(x2g_ctx *ctx, int on)
+{
+    X2G_CHECK(ctx, "x2g_stats_enable: null ctx");
+    int rc = stats_flush(ctx);
+    if (rc) return rc;
+    ctx->stats_on = on != 0;
+    for (int k = 0; k < 2; ++k) ctx->st_launch[k] = ctx->st_ms[k] = ctx->st_bytes[k] = 0;
+    return X2G_OK;
+}
+
+extern "C" int x2g_stats_get(x2g_ctx *ctx, double *out)
+{
+    X2G_CHECK(ctx && out, "x2g_stats_get: null argument");
+    int rc = stats_flush(ctx);
+    if (rc) return rc;
+    for (int k = 0; k < 2; ++k) {
+        out[3 * k + 0] = ctx->st_launch[k];
+        out[3 * k + 1] = ctx->st_ms[k];
+        out[3 * k + 2] = ctx->st_bytes[k];
+    }
+    return X2G_OK;
+}
+
+static constexpr size_t kStageBytes = 16u << 20;  // launch-descriptor staging ring
 
 extern "C" int x2g_version(void) { return 100; }
 
@@ -99,6 +192,11 @@ extern "C" int x2g_ctx_destroy(x2g_ctx *ctx)
 {
     if (!ctx) return X2G_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : ctx->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     for (auto &w : ctx->ws)
         if (w.ptr) (void)hipFree(w.ptr);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);

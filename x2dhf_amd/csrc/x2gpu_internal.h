@@ -47,6 +47,14 @@ struct x2g_ctx {
                                    // 2: launch descriptors, 3: FP32 staging
     void *pinned = nullptr;        // pinned host staging for descriptors/results
     size_t pinned_bytes = 0;
+    size_t stage_off = 0;          // ring offset into ws[2] / pinned
+    // sweep-kernel timing (x2g_stats_enable): event pairs around every sweep
+    // launch, resolved at x2g_stats_get or when the pending list is long
+    bool stats_on = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Pending { hipEvent_t a, b; double bytes; int prec; };
+    std::vector<Pending> pending;
+    double st_launch[2] = {0, 0}, st_ms[2] = {0, 0}, st_bytes[2] = {0, 0};
 };
 
 struct x2g_grid {
@@ -67,8 +75,24 @@ namespace x2g {
 // Reserve ws[k] of at least `bytes` (synchronises the stream on growth).
 int ws_reserve(x2g_ctx *ctx, int k, size_t bytes);
 int pinned_reserve(x2g_ctx *ctx, size_t bytes);
+// Copy `bytes` of launch descriptors to device memory through the context's
+// staging ring (pinned host buffer -> stream-ordered copy).  A region is
+// reused only after the stream has been synchronised past its last use.
+int stage_bytes(x2g_ctx *ctx, const void *src, size_t bytes, const void **dev_out);
+template <typename D>
+inline int stage_n(x2g_ctx *ctx, const D *v, size_t n, const D **out)
+{
+    const void *p = nullptr;
+    int rc = stage_bytes(ctx, v, n * sizeof(D), &p);
+    *out = reinterpret_cast<const D *>(p);
+    return rc;
+}
 
 // Kernel-level entry points shared between translation units.
 int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w2);
 int launch_putout(x2g_grid *g, double *fun, const double *work);
+// Sweep timing hooks (no-ops unless statistics are enabled on the context).
+int stats_begin(x2g_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
+int stats_end(x2g_ctx *ctx, hipEvent_t a, hipEvent_t b, double bytes, int prec);
+int stats_flush(x2g_ctx *ctx);
 }  // namespace x2g

@@ -18,18 +18,9 @@ namespace x2g {
 static inline int gx(long work, int block) { return (int)std::min<long>((work + block - 1) / block, 1024); }
 
 template <typename D>
-static int stage(x2g_ctx *ctx, const D *v, size_t cnt, size_t &offset, const D **out)
+static int stage(x2g_ctx *ctx, const D *v, size_t cnt, size_t & /*unused*/, const D **out)
 {
-    const size_t bytes = cnt * sizeof(D);
-    const size_t aligned = (offset + 255) & ~size_t(255);
-    X2G_CHECK(aligned + bytes <= ctx->ws[2].bytes && aligned + bytes <= ctx->pinned_bytes,
-              "launch descriptors exceed the staging area");
-    std::memcpy((char *)ctx->pinned + aligned, v, bytes);
-    X2G_HIP(hipMemcpyAsync((char *)ctx->ws[2].ptr + aligned, (char *)ctx->pinned + aligned, bytes,
-                           hipMemcpyHostToDevice, ctx->stream));
-    *out = reinterpret_cast<const D *>((char *)ctx->ws[2].ptr + aligned);
-    offset = aligned + bytes;
-    return X2G_OK;
+    return stage_n(ctx, v, cnt, out);
 }
 
 constexpr int kRedBlock = 256;   // 4 wavefronts
@@ -329,7 +320,6 @@ extern "C" int x2g_lincomb(x2g_grid *g, double *out_d, const double *base_d, dou
     hipLaunchKernelGGL(k_lincomb, dim3(gx((long)g->n(), 256)), dim3(256), 0, ctx->stream, out_d, base_d, base_scale,
                        dt, nterms, (long)g->n());
     X2G_HIP(hipGetLastError());
-    X2G_HIP(hipStreamSynchronize(ctx->stream));
     return X2G_OK;
 }
 
@@ -447,7 +437,6 @@ extern "C" int x2g_pot_tail(x2g_grid *g, double *pot_d, int kind, int dm, int mp
     A.mpole = mpole;
     hipLaunchKernelGGL(k_pot_tail, dim3(gx(4L * g->nni, 128)), dim3(128), 0, g->ctx->stream, A);
     X2G_HIP(hipGetLastError());
-    X2G_HIP(hipStreamSynchronize(g->ctx->stream));
     return X2G_OK;
 }
 
@@ -458,6 +447,5 @@ extern "C" int x2g_setci(x2g_grid *g, double *psi_d, int ihsym)
     const long tot = (long)(g->nni / 2) * g->nmu;
     hipLaunchKernelGGL(k_setci, dim3(gx(tot, 256)), dim3(256), 0, g->ctx->stream, psi_d, g->nni, g->nmu, ihsym);
     X2G_HIP(hipGetLastError());
-    X2G_HIP(hipStreamSynchronize(g->ctx->stream));
     return X2G_OK;
 }

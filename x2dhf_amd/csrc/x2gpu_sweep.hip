@@ -25,6 +25,7 @@
 #include "x2gpu_internal.h"
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 namespace x2g {
 
@@ -520,23 +521,11 @@ __global__ void k_zero_f32(const FillFDesc *__restrict__ fd)
 
 static inline int grid_x_for(long work, int block) { return (int)std::min<long>((work + block - 1) / block, 1024); }
 
-// Upload a host array of descriptors to a device staging area at `offset`
-// (bytes) of workspace 2; returns the device pointer.
+// Upload a host array of descriptors through the context's staging ring.
 template <typename D>
-static int stage_descs(x2g_ctx *ctx, const std::vector<D> &v, size_t &offset, const D **out)
+static int stage_descs(x2g_ctx *ctx, const std::vector<D> &v, size_t & /*unused*/, const D **out)
 {
-    const size_t bytes = v.size() * sizeof(D);
-    const size_t aligned = (offset + 255) & ~size_t(255);
-    // the descriptor areas are sized once at context creation and never move,
-    // so pointers staged earlier in the same call stay valid
-    X2G_CHECK(aligned + bytes <= ctx->ws[2].bytes && aligned + bytes <= ctx->pinned_bytes,
-              "launch descriptors exceed the %zu-byte staging area", ctx->ws[2].bytes);
-    std::memcpy((char *)ctx->pinned + aligned, v.data(), bytes);
-    X2G_HIP(hipMemcpyAsync((char *)ctx->ws[2].ptr + aligned, (char *)ctx->pinned + aligned, bytes,
-                           hipMemcpyHostToDevice, ctx->stream));
-    *out = reinterpret_cast<const D *>((char *)ctx->ws[2].ptr + aligned);
-    offset = aligned + bytes;
-    return X2G_OK;
+    return stage_n(ctx, v.data(), v.size(), out);
 }
 
 int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w2)
@@ -548,7 +537,6 @@ int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w
     if (rc) return rc;
     hipLaunchKernelGGL(k_putin, dim3(grid_x_for((long)g->npad(), 256), 1), dim3(256), 0, g->ctx->stream, dd);
     X2G_HIP(hipGetLastError());
-    X2G_HIP(hipStreamSynchronize(g->ctx->stream));
     return X2G_OK;
 }
 
@@ -561,7 +549,6 @@ int launch_putout(x2g_grid *g, double *fun, const double *work)
     if (rc) return rc;
     hipLaunchKernelGGL(k_putout, dim3(grid_x_for((long)g->n(), 256), 1), dim3(256), 0, g->ctx->stream, dd);
     X2G_HIP(hipGetLastError());
-    X2G_HIP(hipStreamSynchronize(g->ctx->stream));
     return X2G_OK;
 }
 
@@ -627,6 +614,14 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
                         int skew, int total_wg, size_t lds, bool in_place)
 {
     const int n = (int)hv.size();
+    // algorithmic bytes of one sweep: read u, write u, read lhs and rhs per
+    // relaxed point (b and d are 1-D and neighbour reads are reuse)
+    double bytes = 0.0;
+    for (auto &d : hv) bytes += 4.0 * sizeof(T) * (double)(d.nni - 2) * (double)(d.nmu - 5);
+    const int prec = sizeof(T) == 8 ? 0 : 1;
+    hipEvent_t ea, eb;
+    int rc = stats_begin(ctx, &ea, &eb);
+    if (rc) return rc;
     if (!in_place && skew > 0) {
         const int nth = 512;
         if (skew == 2) {
@@ -635,7 +630,7 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
             hipLaunchKernelGGL((k_strip<T, 1>), dim3(total_wg), dim3(nth), lds, ctx->stream, dd, n);
         }
         X2G_HIP(hipGetLastError());
-        return X2G_OK;
+        return stats_end(ctx, ea, eb, bytes, prec);
     }
     long maxpts = 0;
     for (auto &d : hv) maxpts = std::max<long>(maxpts, (long)(d.nni - 2) * (d.nmu - 5));
@@ -647,12 +642,14 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
     for (auto &d : hv) maxb = std::max<long>(maxb, (long)(d.nni - 2) + 2L * (d.nmu - 4));
     hipLaunchKernelGGL((k_boundary<T>), dim3(grid_x_for(maxb, 256), n), dim3(256), 0, ctx->stream, dd);
     X2G_HIP(hipGetLastError());
-    return X2G_OK;
+    return stats_end(ctx, ea, eb, bytes, prec);
 }
 
+static std::mutex g_attr_mutex;
 static int set_lds_attr(x2g_ctx *ctx)
 {
     static bool done = false;
+    std::lock_guard<std::mutex> lock(g_attr_mutex);
     if (done) return X2G_OK;
     const int mx = (int)ctx->max_lds;
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<double, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
@@ -795,7 +792,6 @@ extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const
     }
     if (!in_place && (nsweeps % 2) == 1)
         X2G_HIP(hipMemcpyAsync(work_d, alt, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    X2G_HIP(hipStreamSynchronize(ctx->stream));
     return X2G_OK;
 }
 
@@ -1029,7 +1025,6 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             X2G_HIP(hipGetLastError());
         }
     }
-    X2G_HIP(hipStreamSynchronize(ctx->stream));
     return X2G_OK;
 }
 

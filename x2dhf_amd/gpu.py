@@ -46,6 +46,8 @@ _PROTOS = {
     "x2g_ctx_destroy": (_int, [_vp]),
     "x2g_ctx_sync": (_int, [_vp]),
     "x2g_ctx_stream": (_vp, [_vp]),
+    "x2g_stats_enable": (_int, [_vp, _int]),
+    "x2g_stats_get": (_int, [_vp, _vp]),
     "x2g_malloc": (_int, [_vp, _szt, C.POINTER(_vp)]),
     "x2g_free": (_int, [_vp, _vp]),
     "x2g_h2d": (_int, [_vp, _vp, _vp, _szt]),
