@@ -83,10 +83,12 @@ int x2g_grid_destroy(x2g_grid *g);
 const double *x2g_grid_suppl(x2g_grid *g, int k);
 
 /* ---- relaxation ---------------------------------------------------------- */
-/* Sweep engine for all later calls of this process: 0 = LDS-pipelined strip
- * kernel (default; falls back to 1 for grids too wide for the LDS ring),
- * 1 = one launch per colour pass.  strip_rows > 0 fixes the strip height
- * (rounded up to a multiple of 4), <= 0 lets the library choose. */
+/* Sweep engine for all later calls of this process: 0 = default: the
+ * pipelined strip sweep with a fixed lane map and residue-split lhs/rhs where
+ * it fits, else 2, else 1; 1 = one launch per colour pass; 2 = the first
+ * pipelined strip kernel (natural lhs/rhs; falls back to 1 for grids too wide
+ * for its LDS ring).  strip_rows > 0 fixes the strip height (rounded up to a
+ * multiple of 4), <= 0 lets the library choose. */
 int x2g_set_sweep_engine(int engine, int strip_rows);
 
 /* putin (inout.F90:23): embed fun_d (nni*nmu) into the padded work_d and fill

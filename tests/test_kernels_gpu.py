@@ -36,9 +36,10 @@ def relerr(a, b):
 
 
 def engines():
-    # (engine, strip rows): strip kernel with the library's choice, with a
-    # forced small strip (several strips per problem), and the per-pass kernels
-    return [(0, -1), (0, 32), (1, -1)]
+    # (engine, strip rows): the default sweep kernel with the library's strip
+    # height and with a forced small strip (several strips per problem), the
+    # first strip kernel likewise, and the per-pass kernels
+    return [(0, -1), (0, 24), (2, -1), (2, 32), (1, -1)]
 
 
 @pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
@@ -309,15 +310,16 @@ def synthetic_problem(nni, nmu, seed, rinf=40.0, r=2.0):
 
 @pytest.mark.parametrize("shape", [(91, 121), (169, 241), (331, 601), (481, 901), (505, 901)])
 @pytest.mark.parametrize("isym", [1, -1])
-def test_relax_batch_large_vs_oracle(ctx, shape, isym):
+@pytest.mark.parametrize("engine", [0, 2])
+def test_relax_batch_large_vs_oracle(ctx, shape, isym, engine):
     """Batched relaxation (maxsor1=2 x maxsor2=10) at configuration sizes against the
-    oracle's putin -> mcsor -> putout macro-iterations.  481-wide FP64 grids run the
-    strip kernel with one pass per barrier; 505 (nni = 0 mod 5) has another colour
-    multiplier."""
+    oracle's putin -> mcsor -> putout macro-iterations.  FP64 grids wider than the
+    default kernel's LDS ring (481, 505) run the first strip kernel with one pass
+    per barrier; 505 (nni = 0 mod 5) has another colour multiplier."""
     from x2dhf_amd import gpu
     nni, nmu = shape
-    print(f"shape={shape} isym={isym}", flush=True)
-    gpu.check(ctx.lib.x2g_set_sweep_engine(0, -1))
+    print(f"shape={shape} isym={isym} engine={engine}", flush=True)
+    gpu.check(ctx.lib.x2g_set_sweep_engine(engine, -1))
     P = synthetic_problem(nni, nmu, seed=nni + nmu)
     g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
     fun, l, r = ctx.upload(P["fun"]), ctx.upload(P["lhs"]), ctx.upload(P["rhs"])
@@ -338,6 +340,7 @@ def test_uneven_batch_matches_single_problems(ctx):
     """Problems on four different grids relaxed in one batch give the same
     result as each relaxed alone."""
     from x2dhf_amd import gpu
+    gpu.check(ctx.lib.x2g_set_sweep_engine(0, -1))
     shapes = [(91, 121), (151, 241), (61, 91), (331, 601)]
     probs = [synthetic_problem(n, m, seed=7 + i) for i, (n, m) in enumerate(shapes)]
     grids = [gpu.Grid(ctx, p["nni"], p["nmu"], p["coef"], p["suppl"], p["vxi"], p["veta"]) for p in probs]
@@ -359,11 +362,14 @@ def test_uneven_batch_matches_single_problems(ctx):
 
 
 @pytest.mark.parametrize("shape", [(91, 121), (331, 601), (505, 901)])
-def test_fp32_correction_sweeps_vs_oracle(ctx, shape):
+@pytest.mark.parametrize("engine", [0, 2])
+def test_fp32_correction_sweeps_vs_oracle(ctx, shape, engine):
     """With u = 0 one mixed-precision macro-iteration is exactly FP32 MCSOR from
     zero on rhs (the residual), so its interior must match the FP32 oracle."""
     from x2dhf_amd import gpu
     nni, nmu = shape
+    print(f"shape={shape} engine={engine}", flush=True)
+    gpu.check(ctx.lib.x2g_set_sweep_engine(engine, -1))
     P = synthetic_problem(nni, nmu, seed=11)
     g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
     f = ctx.upload(np.zeros(nni * nmu))
@@ -384,6 +390,7 @@ def test_mixed_precision_refines_to_fp64(ctx):
     """FP32 correction sweeps with FP64 residuals converge to the FP64 solution:
     after enough macro-iterations the FP64 residual is at FP64 level."""
     from x2dhf_amd import gpu
+    gpu.check(ctx.lib.x2g_set_sweep_engine(0, -1))
     nni, nmu = 169, 241
     P = synthetic_problem(nni, nmu, seed=13)
     g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])

@@ -337,6 +337,316 @@ __global__ void __launch_bounds__(512) k_strip(const SweepDesc<T> *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Residue-split layout of lhs / rhs for k_sweep.  Relaxed point (nu, mu),
+// nu = 2..nni-1, mu = 2..nmu-4, is stored at
+//   (mu-2)*5*M + ((nu-2) mod 5)*M + (nu-2) div 5,   M = ceil((nni-2)/5),
+// so that the points of one colour pass of one row are contiguous.
+__host__ __device__ inline int split_m(int nni) { return (nni - 2 + 4) / 5; }
+__host__ __device__ inline long split_size(int nni, int nmu) { return (long)(nmu - 5) * 5 * split_m(nni); }
+__device__ __forceinline__ long split_index(int nu, int mu, int M)
+{
+    const int q = nu - 2;
+    return (long)(mu - 2) * 5 * M + (long)(q % 5) * M + q / 5;
+}
+
+template <typename TO>
+struct PackDesc {
+    const double *src;  // unpadded nni*nmu array
+    TO *dst;            // residue-split array
+    int nni, nmu;
+};
+
+template <typename TO>
+__global__ void k_pack(const PackDesc<TO> *__restrict__ pd)
+{
+    const PackDesc<TO> &d = pd[blockIdx.y];
+    const int nni = d.nni, nrow = nni - 2, M = split_m(nni);
+    const long npts = (long)nrow * (d.nmu - 5);
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < npts; k += (long)gridDim.x * blockDim.x) {
+        const int mu = 2 + (int)(k / nrow), nu = 2 + (int)(k % nrow);
+        d.dst[split_index(nu, mu, M)] = (TO)d.src[(long)(mu - 1) * nni + (nu - 1)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_sweep: the pipelined strip sweep (same block schedule as k_strip with
+// SKEW = 2) with a fixed lane -> work map.  Lane tid < 5*TG serves colour
+// pass p = tid / TG and the colour points t = tid % TG + j*TG (j < NT) of each
+// of the four rows of its current block, so the row offsets are compile-time
+// constants and the ring addresses follow from three slot bases per step.
+// lhs / rhs come from the residue-split layout (contiguous per pass and row)
+// one step ahead of their use; b(mu) and d(nu) are staged in LDS.
+//
+// LDS: ring of 12 blocks of 4 rows (nni+8 values each), then d(nu) [nni+8],
+// b(mu) for the strip's rows [L+32] and the coefficients [18].
+template <typename T>
+__host__ __device__ inline size_t sweep_lds_elems(int nni, int L)
+{
+    return (size_t)12 * 4 * (nni + 8) + (nni + 8) + (L + 32) + 18;
+}
+
+// Global-address-space views of descriptor pointers: loads through them are
+// global_load (vector memory counter only) instead of flat loads, which would
+// also occupy the LDS counter and make every LDS wait wait for prefetches.
+template <typename T>
+using gmem = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ const gmem<T> *as_global(const T *p) { return (const gmem<T> *)p; }
+template <typename T>
+__device__ __forceinline__ gmem<T> *as_global(T *p) { return (gmem<T> *)p; }
+
+template <typename T, int NT>
+__global__ void __launch_bounds__(512) k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
+{
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int SKEW = 2, RB = 4 * SKEW + 4;
+    T *ring = reinterpret_cast<T *>(smem_raw);
+
+    const int wg = blockIdx.x;
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (descs[mid].wg_begin <= wg) lo = mid; else hi = mid - 1;
+    }
+    const SweepDesc<T> &d = descs[lo];
+    const int nni = d.nni, nmu = d.nmu, N8 = nni + 8, L = d.L;
+    const int chunk = wg - d.wg_begin;
+    const int a = 2 + chunk * L;
+    const int aend = min(a + L, nmu - 3);
+    const int r0 = a - 16;
+    const int K = L / 4 + 8;
+    const int S = K + 4 * SKEW - 3;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int M = split_m(nni);
+    const int cA = d.cA;
+    const bool even = d.isym == 1;
+    const gmem<T> *__restrict__ src = as_global(d.src);
+    gmem<T> *__restrict__ dst = as_global(d.dst);
+    const gmem<T> *__restrict__ lhs = as_global(d.lhs);
+    const gmem<T> *__restrict__ rhs = as_global(d.rhs);
+    const gmem<T> *__restrict__ dnu_g = as_global(d.dnu);
+    const gmem<T> *__restrict__ bmu_g = as_global(d.bmu);
+    T *dnu_s = ring + RB * 4 * N8;
+    T *bmu_s = dnu_s + N8;
+    T *coef_s = bmu_s + L + 32;  // c[16], omega, omega1
+
+    auto block_row = [&](int b, int rr) -> T * { return ring + (long)(((b + RB) % RB) * 4 + rr) * N8; };
+    auto load_block_direct = [&](int b) {
+        T *base = block_row(b, 0);
+        for (int e = tid; e < 4 * N8; e += nth) {
+            const int rr = e / N8, col = e - rr * N8;
+            const int mu = r0 + 4 * b + rr;
+            T v = T(0);
+            if (mu >= -3 && mu <= nmu + 4) v = src[(long)(mu + 3) * N8 + col];
+            base[(long)rr * N8 + col] = v;
+        }
+    };
+
+    if (tid < 16) coef_s[tid] = d.c[tid];
+    if (tid == 16) coef_s[16] = d.omega;
+    if (tid == 17) coef_s[17] = d.omega1;
+    for (int e = tid; e < nni; e += nth) dnu_s[e] = dnu_g[e];
+    for (int e = tid; e < L + 32; e += nth) {
+        const int mu = r0 + e;
+        bmu_s[e] = (mu >= 1 && mu <= nmu) ? bmu_g[mu - 1] : T(0);
+    }
+    load_block_direct(-1);
+    load_block_direct(0);
+    load_block_direct(1);
+
+    // lane role
+    const int p = tid / TG;  // >= 5: lane only loads and stores
+    const int tl = tid - p * TG;
+    const bool worker = p < 5;
+    int sb = (RB - 2 * p % RB) % RB;                  // ring slot of block s - 2p
+    int off0 = mod5(p - cA * (r0 - 8 * p - 2));        // (nu-2) mod 5 of pass p in row r0 + 4(s - 2p)
+
+    auto rows_active = [&](int s) {
+        const int b = s - SKEW * p;
+        return worker && b >= p && b <= K - 1 - p;
+    };
+    // lhs / rhs of the points of step s (off0s: colour offset of its first row)
+    auto load_lr = [&](int s, int off0s, T (&lv)[4][NT], T (&rv)[4][NT]) {
+        const bool act = rows_active(s);
+        const int mu0 = r0 + 4 * (s - SKEW * p);
+        int off = off0s;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            if (rr) off = off - cA < 0 ? off - cA + 5 : off - cA;
+            const int mu = mu0 + rr;
+            const bool rowok = act && mu >= 2 && mu <= nmu - 4;
+            const long rbase = (long)(mu - 2) * 5 * M + (long)off * M;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const int t = tl + j * TG;
+                T l = T(1), r = T(0);
+                if (rowok && 2 + off + 5 * t <= nni - 1) {
+                    l = lhs[rbase + t];
+                    r = rhs[rbase + t];
+                }
+                lv[rr][j] = l;
+                rv[rr][j] = r;
+            }
+        }
+    };
+
+    T lv[4][NT], rv[4][NT];
+    load_lr(0, off0, lv, rv);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // per-lane copies of the stencil coefficients (read from LDS, so they
+    // live in vector registers rather than being re-fetched as scalars)
+    T cf[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) cf[q] = coef_s[q];
+    const T om = coef_s[16], om1 = coef_s[17];
+
+    for (int s = 0; s < S; ++s) {
+        // 1. lhs / rhs of step s+1 and block s+2 into registers (in this order:
+        // the wait before the commit below then covers both, and the compute
+        // of the next step never waits on loads issued in its own step)
+        const int off0n = off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA;
+        T ln[4][NT], rn[4][NT];
+        load_lr(s + 1, off0n, ln, rn);
+        const int bl = s + 2;
+        T pre[kMaxLoads];
+        const bool do_load = bl <= K;
+        if (do_load) {
+#pragma unroll
+            for (int j = 0; j < kMaxLoads; ++j) {
+                const int e = tid + j * nth;
+                T v = T(0);
+                if (e < 4 * N8) {
+                    const int rr = e / N8, col = e - rr * N8;
+                    const int mu = r0 + 4 * bl + rr;
+                    if (mu >= -3 && mu <= nmu + 4) v = src[(long)(mu + 3) * N8 + col];
+                }
+                pre[j] = v;
+            }
+        }
+
+        // 2. write out the block finalised by the last pass in step s-1
+        const int bw = s - 4 * SKEW - 1;
+        if (bw >= 4 && bw <= K - 5) {
+            for (int e = tid; e < 4 * nni; e += nth) {
+                const int rr = e / nni, nu = 1 + e - rr * nni;
+                const int mu = r0 + 4 * bw + rr;
+                if (mu < a || mu >= aend) continue;
+                const T *row = block_row(bw, rr);
+                T v;
+                if (nu == 1 || nu == nni) {
+                    const int dir = nu == 1 ? 1 : -1;
+                    const int c0 = nu + 3;
+                    v = even ? d.ex[0] * row[c0 + dir] + d.ex[1] * row[c0 + 2 * dir] +
+                                   d.ex[2] * row[c0 + 3 * dir] + d.ex[3] * row[c0 + 4 * dir] +
+                                   d.ex[4] * row[c0 + 5 * dir]
+                             : T(0);
+                } else {
+                    v = row[nu + 3];
+                }
+                dst[(long)(mu + 3) * N8 + nu + 3] = v;
+            }
+            // the mu=1 row once rows 2..6 are final (first strip only)
+            if (a == 2 && bw == 5) {
+                const int q2 = 2 - r0;  // rows 2..6 sit at strip offsets 16..20
+                const T *rw[5] = {block_row(q2 >> 2, q2 & 3), block_row((q2 + 1) >> 2, (q2 + 1) & 3),
+                                  block_row((q2 + 2) >> 2, (q2 + 2) & 3), block_row((q2 + 3) >> 2, (q2 + 3) & 3),
+                                  block_row((q2 + 4) >> 2, (q2 + 4) & 3)};
+                auto r1 = [&](int nu) -> T {
+                    const int c0 = nu + 3;
+                    return d.ex[0] * rw[0][c0] + d.ex[1] * rw[1][c0] + d.ex[2] * rw[2][c0] +
+                           d.ex[3] * rw[3][c0] + d.ex[4] * rw[4][c0];
+                };
+                for (int nu = 1 + tid; nu <= nni; nu += nth) {
+                    T v = T(0);
+                    if (even) {
+                        if (nu == 1 || nu == nni) {
+                            const int dir = nu == 1 ? 1 : -1;
+                            v = d.ex[0] * r1(nu + dir) + d.ex[1] * r1(nu + 2 * dir) +
+                                d.ex[2] * r1(nu + 3 * dir) + d.ex[3] * r1(nu + 4 * dir) +
+                                d.ex[4] * r1(nu + 5 * dir);
+                        } else {
+                            v = r1(nu);
+                        }
+                    }
+                    dst[(long)4 * N8 + nu + 3] = v;
+                }
+            }
+        }
+
+        // 3. pass p on block s - 2p
+        if (rows_active(s)) {
+            const int b = s - SKEW * p;
+            const int sm = sb == 0 ? RB - 1 : sb - 1, sp = sb == RB - 1 ? 0 : sb + 1;
+            int R[12];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                R[q] = (sm * 4 + q) * N8;
+                R[4 + q] = (sb * 4 + q) * N8;
+                R[8 + q] = (sp * 4 + q) * N8;
+            }
+            const int mu0 = r0 + 4 * b;
+            int off = off0;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                if (rr) off = off - cA < 0 ? off - cA + 5 : off - cA;
+                const int mu = mu0 + rr;
+                if (mu >= 2 && mu <= nmu - 4) {
+                    const T bm = bmu_s[mu - r0];
+#pragma unroll
+                    for (int j = 0; j < NT; ++j) {
+                        const int nu = 2 + off + 5 * (tl + j * TG);
+                        if (nu <= nni - 1) {
+                            const int c0 = nu + 3;
+                            T *rc = ring + R[rr + 4] + c0;
+                            const T *m1 = ring + R[rr + 3] + c0, *m2 = ring + R[rr + 2] + c0,
+                                    *m3 = ring + R[rr + 1] + c0, *m4 = ring + R[rr] + c0;
+                            const T *p1 = ring + R[rr + 5] + c0, *p2 = ring + R[rr + 6] + c0,
+                                    *p3 = ring + R[rr + 7] + c0, *p4 = ring + R[rr + 8] + c0;
+                            const T ddmi2 = cf[4] * (*m4 + *p4) + cf[5] * (*m3 + *p3) +
+                                            cf[6] * (*m2 + *p2) + cf[7] * (*m1 + *p1);
+                            const T ddmi1 = cf[0] * (*m4 - *p4) + cf[1] * (*m3 - *p3) +
+                                            cf[2] * (*m2 - *p2) + cf[3] * (*m1 - *p1);
+                            const T ddni2 = cf[12] * (rc[-4] + rc[4]) + cf[13] * (rc[-3] + rc[3]) +
+                                            cf[14] * (rc[-2] + rc[2]) + cf[15] * (rc[-1] + rc[1]);
+                            const T ddni1 = cf[8] * (rc[-4] - rc[4]) + cf[9] * (rc[-3] - rc[3]) +
+                                            cf[10] * (rc[-2] - rc[2]) + cf[11] * (rc[-1] - rc[1]);
+                            const T cc = ddmi2 + bm * ddmi1 + ddni2 + dnu_s[nu - 1] * ddni1;
+                            rc[0] = om * (rv[rr][j] - cc) / lv[rr][j] + om1 * rc[0];
+                        }
+                    }
+                }
+            }
+        }
+
+        // 4. commit the prefetched block (its slot held block s-10, which only
+        // the mu=1 row of step 2 may still read: a, bw are uniform)
+        if (a == 2 && bw == 5) __syncthreads();
+        if (do_load) {
+            T *base = block_row(bl, 0);
+#pragma unroll
+            for (int j = 0; j < kMaxLoads; ++j) {
+                const int e = tid + j * nth;
+                if (e < 4 * N8) base[e] = pre[j];
+            }
+        }
+        sb = sb == RB - 1 ? 0 : sb + 1;
+        off0 = off0n;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                lv[rr][j] = ln[rr][j];
+                rv[rr][j] = rn[rr][j];
+            }
+        // every load of this step has landed before the next one starts, so
+        // the next compute phase never waits on that step's own prefetches
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // putin / putout (inout.F90:23-82, 165-183), several problems per launch.
 struct PadDesc {
     const double *fun;
@@ -434,7 +744,8 @@ struct ResDesc {
     const double *rhs;
     const double *bmu;
     const double *dnu;
-    float *res_f;          // unpadded FP32 residual (may be null)
+    float *res_f;          // FP32 residual (may be null)
+    int res_split;         // res_f in the residue-split layout (else unpadded nni*nmu)
     double *acc;           // [2] per problem: sum res^2, sum rhs^2 (may be null)
     double c[16];
     int nni, nmu;
@@ -457,8 +768,9 @@ __global__ void k_residual(const ResDesc *__restrict__ rd)
             r = d.rhs[e] - (cc + d.lhs[e] * d.w[p]);
             sr += r * r;
             sb += d.rhs[e] * d.rhs[e];
+            if (d.res_f && d.res_split) d.res_f[split_index(nu, mu, split_m(nni))] = (float)r;
         }
-        if (d.res_f) d.res_f[e] = (float)r;
+        if (d.res_f && !d.res_split) d.res_f[e] = (float)r;
     }
     if (d.acc) {
         // wavefront reduction then one atomic per wavefront (diagnostic only)
@@ -552,15 +864,18 @@ int launch_putout(x2g_grid *g, double *fun, const double *work)
     return X2G_OK;
 }
 
-// Sweep-engine selection: 0 = pipelined strip kernel where the LDS ring fits
-// (default), 1 = per-pass kernels.  strip_rows > 0 fixes the strip height.
-// Initial values come from X2G_SWEEP=phase and X2G_STRIP_L.
+// Sweep-engine selection: 0 = k_sweep where it applies, else k_strip, else
+// the per-pass kernels (default); 1 = per-pass kernels; 2 = k_strip where the
+// LDS ring fits.  strip_rows > 0 fixes the strip height.  Initial values come
+// from X2G_SWEEP=phase|strip and X2G_STRIP_L.
 static int g_engine = -1, g_strip_rows = -1;
 static void engine_defaults()
 {
     if (g_engine >= 0) return;
     const char *e = getenv("X2G_SWEEP");
-    g_engine = (e && std::strcmp(e, "phase") == 0) ? 1 : 0;
+    g_engine = 0;
+    if (e && std::strcmp(e, "phase") == 0) g_engine = 1;
+    if (e && std::strcmp(e, "strip") == 0) g_engine = 2;
     const char *l = getenv("X2G_STRIP_L");
     g_strip_rows = l ? atoi(l) : -1;
 }
@@ -588,7 +903,7 @@ static int choose_skew(x2g_ctx *ctx, int nni)
     return 0;
 }
 
-// Strip height: enough workgroups to fill the chip, at least 32 rows.
+// Strip height for k_strip: enough workgroups to fill the chip, at least 32 rows.
 static int choose_L(x2g_ctx *ctx, long total_rows)
 {
     const int forced = env_strip_l();
@@ -600,19 +915,75 @@ static int choose_L(x2g_ctx *ctx, long total_rows)
     return (int)((L + 3) & ~3L);
 }
 
+// Strip height for k_sweep from a cost model: a workgroup of strip height L
+// runs L/4 + 13 pipeline steps (the 16-row halos and the pipeline fill), the
+// launch runs ceil(W / slots) rounds of workgroups, W = sum over problems of
+// ceil(rows / L), slots = resident workgroups (CUs x workgroups per CU).
+template <typename T>
+static int choose_sweep_L(x2g_ctx *ctx, const std::vector<int> &rows, int maxnni)
+{
+    const int forced = env_strip_l();
+    if (forced > 0) return (forced + 3) & ~3;
+    int maxrows = 0;
+    for (int r : rows) maxrows = std::max(maxrows, r);
+    double best = 1e300;
+    int bestL = 32;
+    for (int L = 24; L <= std::max(24, ((maxrows + 3) & ~3)); L += 4) {
+        const size_t lds = sweep_lds_elems<T>(maxnni, L) * sizeof(T);
+        if (lds > ctx->max_lds) break;
+        const long per_cu = std::max<long>(1, (long)(ctx->max_lds / lds));
+        const long slots = (long)ctx->num_cu * std::min<long>(per_cu, 2);
+        long W = 0;
+        for (int r : rows) W += (r + L - 1) / L;
+        const double cost = (double)((W + slots - 1) / slots) * (L / 4.0 + 13.0);
+        if (cost < best - 1e-9) {
+            best = cost;
+            bestL = L;
+        }
+    }
+    return bestL;
+}
+
+// kind: 0 per-pass kernels (in place), 1 k_strip, 2 k_sweep (residue-split lhs/rhs)
 template <typename T>
 struct SweepPlan {
     std::vector<SweepDesc<T>> even, odd;  // src=A,dst=B and src=B,dst=A
+    int kind = 0;
     int skew = 0;
     int total_wg = 0;
+    int nth = 0, TG = 0, NT = 0;
     size_t lds = 0;
 };
+
+// Does k_sweep apply to a batch whose widest grid has nni points?
+template <typename T>
+static bool sweep_fits(x2g_ctx *ctx, int maxnni)
+{
+    const int M = split_m(maxnni);
+    if (M > 2 * 102) return false;                       // NT <= 2 with TG <= 102
+    if (4 * (maxnni + 8) > kMaxLoads * 512) return false;
+    return sweep_lds_elems<T>(maxnni, 24) * sizeof(T) <= ctx->max_lds;
+}
+
+// Which kernel family a batch of this precision and width uses (engine setting
+// and LDS limits); the caller lays out lhs / rhs accordingly.
+template <typename T>
+static int sweep_kind(x2g_ctx *ctx, int maxnni)
+{
+    const int e = env_engine();
+    if (e == 1) return 0;
+    if (e == 0 && sweep_fits<T>(ctx, maxnni)) return 2;
+    return choose_skew<T>(ctx, maxnni) > 0 ? 1 : 0;
+}
 
 // Launch one sweep of every problem in `descs` (device copy dd, host copy hv).
 template <typename T>
 static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<SweepDesc<T>> &hv,
-                        int skew, int total_wg, size_t lds, bool in_place)
+                        const SweepPlan<T> &plan)
 {
+    const int skew = plan.skew, total_wg = plan.total_wg;
+    const size_t lds = plan.lds;
+    const bool in_place = plan.kind == 0;
     const int n = (int)hv.size();
     // algorithmic bytes of one sweep: read u, write u, read lhs and rhs per
     // relaxed point (b and d are 1-D and neighbour reads are reuse)
@@ -622,6 +993,15 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
     hipEvent_t ea, eb;
     int rc = stats_begin(ctx, &ea, &eb);
     if (rc) return rc;
+    if (plan.kind == 2) {
+        if (plan.NT == 1) {
+            hipLaunchKernelGGL((k_sweep<T, 1>), dim3(total_wg), dim3(plan.nth), lds, ctx->stream, dd, n, plan.TG);
+        } else {
+            hipLaunchKernelGGL((k_sweep<T, 2>), dim3(total_wg), dim3(plan.nth), lds, ctx->stream, dd, n, plan.TG);
+        }
+        X2G_HIP(hipGetLastError());
+        return stats_end(ctx, ea, eb, bytes, prec);
+    }
     if (!in_place && skew > 0) {
         const int nth = 512;
         if (skew == 2) {
@@ -656,6 +1036,10 @@ static int set_lds_attr(x2g_ctx *ctx)
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    X2G_HIP(hipFuncSetAttribute((const void *)k_sweep<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    X2G_HIP(hipFuncSetAttribute((const void *)k_sweep<double, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    X2G_HIP(hipFuncSetAttribute((const void *)k_sweep<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    X2G_HIP(hipFuncSetAttribute((const void *)k_sweep<float, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     done = true;
     return X2G_OK;
 }
@@ -686,16 +1070,33 @@ template <typename T>
 static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vector<T *> &a_bufs,
                        std::vector<T *> &b_bufs, SweepPlan<T> &plan)
 {
-    int skew = env_engine() == 1 ? 0 : 2;
     int maxnni = 0;
     long rows = 0;
+    std::vector<int> prow;
     for (auto &d : base) {
         maxnni = std::max(maxnni, d.nni);
         rows += d.nmu - 5;
+        prow.push_back(d.nmu - 5);
     }
-    if (skew) skew = choose_skew<T>(ctx, maxnni);
-    plan.skew = skew;
-    const int L = choose_L(ctx, rows);
+    plan.kind = sweep_kind<T>(ctx, maxnni);
+    int L = 0;
+    if (plan.kind == 2) {
+        const int M = split_m(maxnni);
+        plan.NT = (M + 101) / 102;
+        plan.TG = (M + plan.NT - 1) / plan.NT;
+        plan.nth = ((5 * plan.TG + 63) / 64) * 64;
+        plan.skew = 2;
+        L = choose_sweep_L<T>(ctx, prow, maxnni);
+        plan.lds = sweep_lds_elems<T>(maxnni, L) * sizeof(T);
+    } else if (plan.kind == 1) {
+        plan.skew = choose_skew<T>(ctx, maxnni);
+        L = choose_L(ctx, rows);
+        plan.lds = strip_lds<T>(maxnni, plan.skew);
+    } else {
+        plan.skew = 0;
+        L = choose_L(ctx, rows);
+        plan.lds = 0;
+    }
     int wg = 0;
     for (auto &d : base) {
         d.L = L;
@@ -703,11 +1104,10 @@ static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vecto
         wg += (d.nmu - 5 + L - 1) / L;
     }
     plan.total_wg = wg;
-    plan.lds = skew ? strip_lds<T>(maxnni, skew) : 0;
     plan.even = base;
     plan.odd = base;
     for (size_t i = 0; i < base.size(); ++i) {
-        if (skew) {
+        if (plan.kind != 0) {
             plan.even[i].src = a_bufs[i];
             plan.even[i].dst = b_bufs[i];
             plan.odd[i].src = b_bufs[i];
@@ -720,6 +1120,23 @@ static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vecto
     return X2G_OK;
 }
 
+// Residue-split copies of lhs and rhs for a k_sweep batch.
+template <typename TO>
+static int launch_pack(x2g_ctx *ctx, const std::vector<PackDesc<TO>> &v)
+{
+    if (v.empty()) return X2G_OK;
+    const PackDesc<TO> *dd;
+    int rc = stage_n(ctx, v.data(), v.size(), &dd);
+    if (rc) return rc;
+    long maxpts = 0;
+    for (auto &d : v) maxpts = std::max<long>(maxpts, (long)(d.nni - 2) * (d.nmu - 5));
+    hipLaunchKernelGGL((k_pack<TO>), dim3(grid_x_for(maxpts, 256), (int)v.size()), dim3(256), 0, ctx->stream, dd);
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
+static inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
 }  // namespace x2g
 
 using namespace x2g;
@@ -729,7 +1146,7 @@ using namespace x2g;
 
 extern "C" int x2g_set_sweep_engine(int engine, int strip_rows)
 {
-    X2G_CHECK(engine == 0 || engine == 1, "x2g_set_sweep_engine: engine must be 0 or 1");
+    X2G_CHECK(engine >= 0 && engine <= 2, "x2g_set_sweep_engine: engine must be 0, 1 or 2");
     engine_defaults();
     g_engine = engine;
     g_strip_rows = strip_rows;
@@ -768,26 +1185,37 @@ extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const
     x2g_ctx *ctx = g->ctx;
     rc = set_lds_attr(ctx);
     if (rc) return rc;
-    // second padded buffer for the ping-pong (a copy of work)
+    // second padded buffer for the ping-pong (a copy of work), then the
+    // residue-split lhs / rhs when k_sweep runs
     const size_t bytes = g->npad() * sizeof(double);
-    rc = ws_reserve(ctx, 0, bytes);
+    const size_t sbytes = align256(split_size(g->nni, g->nmu) * sizeof(double));
+    rc = ws_reserve(ctx, 0, align256(bytes) + 2 * sbytes);
     if (rc) return rc;
-    double *alt = (double *)ctx->ws[0].ptr;
+    char *w0 = (char *)ctx->ws[0].ptr;
+    double *alt = (double *)w0;
     X2G_HIP(hipMemcpyAsync(alt, work_d, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     std::vector<SweepDesc<double>> base(1);
     fill_desc<double>(base[0], g, lhs_d, rhs_d, bkind == 0 ? g->bmu_orb : g->bmu_pot, g->dnu, isym, omega);
     std::vector<double *> A{work_d}, B{alt};
     SweepPlan<double> plan;
     plan_sweeps(ctx, base, A, B, plan);
+    if (plan.kind == 2) {
+        double *lp = (double *)(w0 + align256(bytes)), *rp = (double *)(w0 + align256(bytes) + sbytes);
+        std::vector<PackDesc<double>> pk{{lhs_d, lp, g->nni, g->nmu}, {rhs_d, rp, g->nni, g->nmu}};
+        if ((rc = launch_pack(ctx, pk))) return rc;
+        for (auto *v : {&plan.even, &plan.odd}) {
+            (*v)[0].lhs = lp;
+            (*v)[0].rhs = rp;
+        }
+    }
     size_t off = 0;
     const SweepDesc<double> *de, *dodd;
     if ((rc = stage_descs(ctx, plan.even, off, &de))) return rc;
     if ((rc = stage_descs(ctx, plan.odd, off, &dodd))) return rc;
-    const bool in_place = plan.skew == 0;
+    const bool in_place = plan.kind == 0;
     for (int it = 0; it < nsweeps; ++it) {
         const bool ev = (it % 2) == 0;
-        rc = launch_sweep<double>(ctx, ev ? de : dodd, ev ? plan.even : plan.odd, plan.skew, plan.total_wg,
-                                  plan.lds, in_place);
+        rc = launch_sweep<double>(ctx, ev ? de : dodd, ev ? plan.even : plan.odd, plan);
         if (rc) return rc;
     }
     if (!in_place && (nsweeps % 2) == 1)
@@ -840,11 +1268,12 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         X2G_CHECK(it.prec == 0 || it.prec == 1, "x2g_relax_batch: item %d prec", i);
         if ((rc = check_grid_sweepable(g))) return rc;
         o64[i] = off64;
-        off64 += 2 * ((g->npad() * sizeof(double) + 255) & ~size_t(255));
+        off64 += 2 * align256(g->npad() * sizeof(double));
+        if (it.prec == 0) off64 += 2 * align256(split_size(g->nni, g->nmu) * sizeof(double));
         o32[i] = off32;
         if (it.prec == 1)
-            off32 += 2 * ((g->npad() * sizeof(float) + 255) & ~size_t(255)) +
-                     2 * ((g->n() * sizeof(float) + 255) & ~size_t(255));
+            off32 += 2 * align256(g->npad() * sizeof(float)) +
+                     2 * align256(std::max<size_t>(g->n(), split_size(g->nni, g->nmu)) * sizeof(float));
     }
     if ((rc = ws_reserve(ctx, 0, off64))) return rc;
     if (off32 && (rc = ws_reserve(ctx, 3, off32))) return rc;
@@ -859,10 +1288,16 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         x2g_grid *g = grids[items[i].grid_index];
         return (double *)(w64 + o64[i] + ((g->npad() * sizeof(double) + 255) & ~size_t(255)));
     };
+    auto splitp = [&](int i, int k) {  // FP64 residue-split lhs (k=0) / rhs (k=1)
+        x2g_grid *g = grids[items[i].grid_index];
+        const size_t pp = align256(g->npad() * sizeof(double));
+        const size_t ss = align256(split_size(g->nni, g->nmu) * sizeof(double));
+        return (double *)(w64 + o64[i] + 2 * pp + k * ss);
+    };
     auto f32p = [&](int i, int k) {  // k: 0 eA, 1 eB, 2 lhs_f, 3 res_f
         x2g_grid *g = grids[items[i].grid_index];
-        const size_t pp = (g->npad() * sizeof(float) + 255) & ~size_t(255);
-        const size_t nn = (g->n() * sizeof(float) + 255) & ~size_t(255);
+        const size_t pp = align256(g->npad() * sizeof(float));
+        const size_t nn = align256(std::max<size_t>(g->n(), split_size(g->nni, g->nmu)) * sizeof(float));
         size_t o = o32[i] + (k <= 1 ? k * pp : 2 * pp + (k - 2) * nn);
         return (float *)(w32 + o);
     };
@@ -900,9 +1335,24 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             B.push_back(bufB(i64[k]));
         }
         plan_sweeps(ctx, base, A, B, p64);
+        if (p64.kind == 2) {
+            // lhs / rhs do not change during the call: one residue-split copy
+            std::vector<PackDesc<double>> pk;
+            for (size_t k = 0; k < i64.size(); ++k) {
+                const x2g_relax_item &it = items[i64[k]];
+                x2g_grid *g = grids[it.grid_index];
+                pk.push_back({it.lhs_d, splitp(i64[k], 0), g->nni, g->nmu});
+                pk.push_back({it.rhs_d, splitp(i64[k], 1), g->nni, g->nmu});
+                for (auto *v : {&p64.even, &p64.odd}) {
+                    (*v)[k].lhs = splitp(i64[k], 0);
+                    (*v)[k].rhs = splitp(i64[k], 1);
+                }
+            }
+            if ((rc = launch_pack(ctx, pk))) return rc;
+        }
         if ((rc = stage_descs(ctx, p64.even, doff, &d64e))) return rc;
         if ((rc = stage_descs(ctx, p64.odd, doff, &d64o))) return rc;
-        const bool final_in_b = p64.skew > 0 && (maxsor2 % 2) == 1;
+        const bool final_in_b = p64.kind != 0 && (maxsor2 % 2) == 1;
         for (size_t k = 0; k < i64.size(); ++k) {
             const x2g_relax_item &it = items[i64[k]];
             x2g_grid *g = grids[it.grid_index];
@@ -945,6 +1395,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             r.bmu = it.bkind == 0 ? g->bmu_orb : g->bmu_pot;
             r.dnu = g->dnu;
             r.res_f = f32p(i, 3);
+            r.res_split = 0;
             r.acc = nullptr;
             for (int q = 0; q < 16; ++q) r.c[q] = g->coef[q];
             r.nni = g->nni;
@@ -960,9 +1411,12 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             pout32.push_back({it.fun_d, bufA(i), nullptr, g->nni, g->nmu, 1});
         }
         plan_sweeps(ctx, base, A, B, p32);
-        // the strip engine ping-pongs: an odd sweep count leaves the correction in B
-        if (p32.skew > 0 && (maxsor2 % 2) == 1)
+        // the strip engines ping-pong: an odd sweep count leaves the correction in B
+        if (p32.kind != 0 && (maxsor2 % 2) == 1)
             for (size_t k = 0; k < i32.size(); ++k) corr[k].e = f32p(i32[k], 1);
+        const bool split32 = p32.kind == 2;
+        if (split32)
+            for (auto &r : res) r.res_split = 1;
         if ((rc = stage_descs(ctx, p32.even, doff, &d32e))) return rc;
         if ((rc = stage_descs(ctx, p32.odd, doff, &d32o))) return rc;
         if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
@@ -971,10 +1425,20 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         if ((rc = stage_descs(ctx, zero, doff, &d_zero))) return rc;
         if ((rc = stage_descs(ctx, bnd64, doff, &d_bnd64))) return rc;
         if ((rc = stage_descs(ctx, pout32, doff, &d_pout32))) return rc;
-        long maxn = 0;
-        for (int i : i32) maxn = std::max<long>(maxn, (long)grids[items[i].grid_index]->n());
-        hipLaunchKernelGGL(k_to_f32, dim3(grid_x_for(maxn, 256), (int)i32.size()), dim3(256), 0, ctx->stream, d_cvt);
-        X2G_HIP(hipGetLastError());
+        if (split32) {
+            std::vector<PackDesc<float>> pk;
+            for (int i : i32) {
+                x2g_grid *g = grids[items[i].grid_index];
+                pk.push_back({items[i].lhs_d, f32p(i, 2), g->nni, g->nmu});
+            }
+            if ((rc = launch_pack(ctx, pk))) return rc;
+        } else {
+            long maxn = 0;
+            for (int i : i32) maxn = std::max<long>(maxn, (long)grids[items[i].grid_index]->n());
+            hipLaunchKernelGGL(k_to_f32, dim3(grid_x_for(maxn, 256), (int)i32.size()), dim3(256), 0, ctx->stream,
+                               d_cvt);
+            X2G_HIP(hipGetLastError());
+        }
     }
 
     long maxpad = 0, maxn = 0, maxnni = 0;
@@ -991,8 +1455,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         if (!i64.empty()) {
             for (int it2 = 0; it2 < maxsor2; ++it2) {
                 const bool ev = (it2 % 2) == 0;
-                if ((rc = launch_sweep<double>(ctx, ev ? d64e : d64o, ev ? p64.even : p64.odd, p64.skew,
-                                               p64.total_wg, p64.lds, p64.skew == 0)))
+                if ((rc = launch_sweep<double>(ctx, ev ? d64e : d64o, ev ? p64.even : p64.odd, p64)))
                     return rc;
             }
             hipLaunchKernelGGL(k_putout, dim3(grid_x_for(maxn, 256), (int)i64.size()), dim3(256), 0, ctx->stream,
@@ -1007,8 +1470,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             X2G_HIP(hipGetLastError());
             for (int it2 = 0; it2 < maxsor2; ++it2) {
                 const bool ev = (it2 % 2) == 0;
-                if ((rc = launch_sweep<float>(ctx, ev ? d32e : d32o, ev ? p32.even : p32.odd, p32.skew, p32.total_wg,
-                                              p32.lds, p32.skew == 0)))
+                if ((rc = launch_sweep<float>(ctx, ev ? d32e : d32o, ev ? p32.even : p32.odd, p32)))
                     return rc;
             }
             hipLaunchKernelGGL(k_correct, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, ctx->stream, d_corr);

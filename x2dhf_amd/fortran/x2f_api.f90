@@ -256,6 +256,15 @@ contains
     end associate
   end function x2f_job_get
 
+  ! Device context of a job (x2g_ctx*, for x2g_stats_* and x2g_ctx_sync);
+  ! null for an invalid handle.
+  type(c_ptr) function x2f_job_ctx(handle) bind(c, name='x2f_job_ctx')
+    integer(c_int), value :: handle
+    x2f_job_ctx = c_null_ptr
+    if (.not. valid(handle)) return
+    x2f_job_ctx = jobs(handle)%ctx
+  end function x2f_job_ctx
+
   integer(c_long_long) function x2f_job_bytes(handle) bind(c, name='x2f_job_bytes')
     integer(c_int), value :: handle
     x2f_job_bytes = -1

@@ -73,6 +73,8 @@ def load():
     L.x2f_job_get.restype = _i
     L.x2f_job_bytes.argtypes = [_i]
     L.x2f_job_bytes.restype = C.c_longlong
+    L.x2f_job_ctx.argtypes = [_i]
+    L.x2f_job_ctx.restype = vp
     _lib = L
     return L
 
@@ -201,6 +203,10 @@ class Job:
 
     def bytes(self):
         return int(self.L.x2f_job_bytes(self.h))
+
+    def ctx(self):
+        """The job's x2g_ctx* (shared by every job created with share=)."""
+        return self.L.x2f_job_ctx(self.h)
 
     def close(self):
         if self.h:
