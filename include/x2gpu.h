@@ -28,6 +28,19 @@
  *   x2g_moments        <- coulMom / exchMom                    src/scfUtils.F90:490,615
  *   x2g_pot_tail       <- coulAsympt / exchAsympt (vcoul/vexch) src/scfUtils.F90:275,310
  *   x2g_setci          <- setCiOrb                             src/scfUtils.F90:380
+ *
+ * Batched forms (many problems, possibly on different grids, per launch; the
+ * scalars stay on the device until x2g_gather):
+ *   x2g_lincomb_batch  <- fockHF and the relaxPots right-hand sides for all
+ *                         orbitals of several jobs  src/fock.F90:88-184, src/relaxPots.F90:62
+ *   x2g_normortho_batch<- norm + setCiOrb + ortho of one orbital of each job
+ *                                                  src/normOrtho.F90:14-59,103-190
+ *   x2g_eahf_batch     <- EaHF (diagonal Lagrange multipliers)
+ *                                                  src/lagrangeMultipliers.F90:21-107
+ *   x2g_moments_batch  <- mpoleMoments (coulMom/exchMom of every orbital/pair)
+ *                                                  src/scfUtils.F90:446-850
+ *   x2g_pot_tail_batch <- potAsympt (coulAsympt/exchAsympt of every potential)
+ *                                                  src/scfUtils.F90:209-329
  */
 #ifndef X2GPU_H
 #define X2GPU_H
@@ -181,6 +194,69 @@ int x2g_pot_tail(x2g_grid *g, double *pot_d, int kind, int dm, int mpole, const 
 
 /* setCiOrb (scfUtils.F90:380-406): impose g (ihsym=+1) or u (-1) symmetry. */
 int x2g_setci(x2g_grid *g, double *psi_d, int ihsym);
+
+/* ---- batched operations -------------------------------------------------- */
+/* out = base_scale*base + sum of terms[first_term .. first_term+nterms-1], on
+ * grids[grid_index]; all ops of a call run in one launch, so no op may read
+ * another op's output. */
+typedef struct {
+    double *out_d;
+    const double *base_d; /* may be NULL */
+    double base_scale;
+    int first_term, nterms;
+    int grid_index;
+} x2g_lincomb_op;
+int x2g_lincomb_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_lincomb_op *ops, int nops,
+                      const x2g_term *terms, int nterms_total);
+
+/* norm + setCiOrb + ortho of one orbital per item (normOrtho.F90:14-59,
+ * 103-190), with the quadrature weights wgt2*f4 of the item's grid:
+ *   s = <psi|psi> (written to norm_out_d when not NULL);
+ *   psi <- setci(psi) (ihomo = +1/-1; 0 = no symmetry);
+ *   ovla_k = <psi|other_k>/sqrt(s);  a = sum ovla_k^2;
+ *   psi <- sqrt(1+a)/(1-a^2) * (psi/sqrt(s) - sum_k ovla_k other_k)
+ * (the reference's sequence folded into one update; nothers = 0 leaves the
+ * plain normalisation).  normalise = 0: only the symmetry is imposed. */
+#define X2G_MAX_ORTHO 16
+typedef struct {
+    double *psi_d;
+    const double *others_d[X2G_MAX_ORTHO];
+    int nothers;
+    int ihomo;
+    int normalise;
+    int grid_index;
+    double *norm_out_d;
+} x2g_normortho_item;
+int x2g_normortho_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_normortho_item *items, int n);
+
+/* ee = <psi|h|psi> + <psi|cpot psi> - <psi|xpot> (EaHF, lagrangeMultipliers.F90:
+ * 21-107; h as x2g_oneel) written to ee_out_d, one item per orbital. */
+typedef struct {
+    const double *psi_d, *cpot_d, *xpot_d;
+    int isym, m, grid_index;
+    double *ee_out_d;
+} x2g_eahf_item;
+int x2g_eahf_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_eahf_item *items, int n);
+
+/* Multipole moments (as x2g_moments) of n pairs; out_h[8*k + q] (synchronises). */
+typedef struct {
+    const double *psi1_d, *psi2_d;
+    int dm, grid_index;
+} x2g_moment_item;
+int x2g_moments_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_moment_item *items, int n, int mpole,
+                      double *out_h);
+
+/* Potential tails (as x2g_pot_tail) of n potentials in one launch. */
+typedef struct {
+    double *pot_d;
+    int kind, dm, grid_index;
+    double mom[8];
+} x2g_tail_item;
+int x2g_pot_tail_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_tail_item *items, int n, int mpole);
+
+/* Copy counts[k] doubles from each src_d[k] into out_h, packed in order, with
+ * one device-to-host transfer (synchronises). */
+int x2g_gather(x2g_ctx *ctx, const double *const *src_d, const int *counts, int n, double *out_h);
 
 #ifdef __cplusplus
 }

@@ -74,3 +74,66 @@ def test_small_molecules_match_reference(name, maker):
     assert abs(res["etot"] - ref["etot"]) < TOL_E
     # the listing's final table is in input order
     assert np.abs(res["eps"] - np.array(ref["eps"])).max() < 1e-9
+
+
+def run_batched(jobs):
+    """Step jobs (sharing one context) with batched iterations to the end."""
+    from x2dhf_amd import scf
+    for j in jobs:
+        j.prep()
+    stopped = [False] * len(jobs)
+    while not all(stopped):
+        stopped = scf.iterate_batched(jobs, stopped)
+    for j in jobs:
+        j.finish()
+    return [j.result() for j in jobs]
+
+
+@pytest.mark.parametrize("name,maker", [("n2_91x121", "n2_spec"), ("fh_91x121", "fh_spec")])
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-8)])
+def test_batched_schedule_converges_to_reference(name, maker, prec, tol):
+    """The batched schedule (all potentials, then all orbitals, Jacobi across
+    orbitals, device-resident scalars) reaches the reference's total energy:
+    within 1e-10 Eh in FP64 and 1e-8 Eh with FP32 correction sweeps."""
+    from x2dhf_amd import scf
+    ref = golden(name)
+    spec = getattr(scf, maker)(91, 121, 40.0, ienterm=12, inoterm=12, maxscf=3000, schedule=1, prec=prec)
+    job = scf.Job(spec)
+    try:
+        res = run_batched([job])[0]
+    finally:
+        job.close()
+    print(f"{name} batched prec={prec}: E = {res['etot']:.14f} (ref {ref['etot']:.14f}) "
+          f"dE = {res['etot'] - ref['etot']:.2e}, iterations {res['iterations']}, stop '{res['stop']}'")
+    assert abs(res["etot"] - ref["etot"]) < tol
+    assert np.abs(res["eps"] - np.array(ref["eps"])).max() < 10 * tol
+
+
+def test_jobs_give_the_same_result_alone_and_in_a_mix():
+    """Three different molecules on three grids: stepped together in one batch
+    (one context, fused launches over all of them) each job ends with exactly
+    the result it has when run alone."""
+    from x2dhf_amd import scf
+    specs = [scf.h2_spec(61, 91, 40.0, schedule=1, ienterm=11, inoterm=11, maxscf=2000),
+             scf.n2_spec(91, 121, 40.0, schedule=1, ienterm=11, inoterm=11, maxscf=2000),
+             scf.fh_spec(91, 121, 40.0, schedule=1, ienterm=11, inoterm=11, maxscf=2000, prec=1)]
+    alone = []
+    for s in specs:
+        j = scf.Job(s)
+        try:
+            alone.append(run_batched([j])[0])
+        finally:
+            j.close()
+    jobs = []
+    try:
+        for s in specs:
+            jobs.append(scf.Job(s, share=jobs[0] if jobs else None))
+        mixed = run_batched(jobs)
+    finally:
+        for j in reversed(jobs):
+            j.close()
+    for a, m in zip(alone, mixed):
+        print(f"alone E={a['etot']:.15f} it={a['iterations']}   mix E={m['etot']:.15f} it={m['iterations']}")
+        assert a["iterations"] == m["iterations"]
+        assert a["etot"] == m["etot"]
+        assert np.array_equal(a["eps"], m["eps"])

@@ -48,6 +48,19 @@ int pinned_reserve(x2g_ctx *ctx, size_t bytes)
     return X2G_OK;
 }
 
+int pinned_out_reserve(x2g_ctx *ctx, size_t bytes)
+{
+    if (ctx->pinned_out_bytes >= bytes) return X2G_OK;
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->pinned_out) X2G_HIP(hipHostFree(ctx->pinned_out));
+    ctx->pinned_out = nullptr;
+    ctx->pinned_out_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, 65536);
+    X2G_HIP(hipHostMalloc(&ctx->pinned_out, want, hipHostMallocDefault));
+    ctx->pinned_out_bytes = want;
+    return X2G_OK;
+}
+
 int stage_bytes(x2g_ctx *ctx, const void *src, size_t bytes, const void **dev_out)
 {
     const size_t cap = std::min(ctx->ws[2].bytes, ctx->pinned_bytes);
@@ -200,6 +213,7 @@ extern "C" int x2g_ctx_destroy(x2g_ctx *ctx)
     for (auto &w : ctx->ws)
         if (w.ptr) (void)hipFree(w.ptr);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->pinned_out) (void)hipHostFree(ctx->pinned_out);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return X2G_OK;

@@ -43,10 +43,14 @@ struct x2g_ctx {
     hipStream_t stream = nullptr;
     int num_cu = 0;
     size_t max_lds = 0;
-    x2g::Workspace ws[4];          // 0: padded work buffers, 1: reduction partials,
-                                   // 2: launch descriptors, 3: FP32 staging
+    x2g::Workspace ws[6];          // 0: padded work buffers, 1: reduction partials,
+                                   // 2: launch descriptors, 3: FP32 staging,
+                                   // 4: padded buffers of batched quadratures,
+                                   // 5: batched reduction partials
     void *pinned = nullptr;        // pinned host staging for descriptors/results
     size_t pinned_bytes = 0;
+    void *pinned_out = nullptr;    // pinned host buffer for gathered results
+    size_t pinned_out_bytes = 0;
     size_t stage_off = 0;          // ring offset into ws[2] / pinned
     // sweep-kernel timing (x2g_stats_enable): event pairs around every sweep
     // launch, resolved at x2g_stats_get or when the pending list is long
@@ -91,6 +95,15 @@ inline int stage_n(x2g_ctx *ctx, const D *v, size_t n, const D **out)
 // Kernel-level entry points shared between translation units.
 int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w2);
 int launch_putout(x2g_grid *g, double *fun, const double *work);
+// putin of several functions (each on its own grid) in one launch
+struct PadItem {
+    const double *fun;
+    double *w;
+    int nni, nmu, isym;
+};
+int launch_putin_items(x2g_ctx *ctx, const std::vector<PadItem> &items);
+// pinned host buffer of at least `bytes` for device -> host results
+int pinned_out_reserve(x2g_ctx *ctx, size_t bytes);
 // Sweep timing hooks (no-ops unless statistics are enabled on the context).
 int stats_begin(x2g_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
 int stats_end(x2g_ctx *ctx, hipEvent_t a, hipEvent_t b, double bytes, int prec);

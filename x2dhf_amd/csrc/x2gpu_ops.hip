@@ -189,7 +189,7 @@ struct MomArgs {
     int nni, nmu, dm, mpole;
 };
 
-__global__ void __launch_bounds__(kRedBlock) k_moments(MomArgs A, double *partial)
+__device__ __forceinline__ void moments_body(const MomArgs &A, double *partial)
 {
     __shared__ double sh[32];
     const int nni = A.nni;
@@ -220,6 +220,14 @@ __global__ void __launch_bounds__(kRedBlock) k_moments(MomArgs A, double *partia
         for (int k = 0; k < 8; ++k) partial[(long)k * gridDim.x + blockIdx.x] = acc[k];
 }
 
+__global__ void __launch_bounds__(kRedBlock) k_moments(MomArgs A, double *partial) { moments_body(A, partial); }
+
+// blockIdx.y = pair; partial[(pair*8 + k)*gridDim.x + block]
+__global__ void __launch_bounds__(kRedBlock) k_moments_batch(const MomArgs *__restrict__ A, double *partial)
+{
+    moments_body(A[blockIdx.y], partial + (long)blockIdx.y * 8 * gridDim.x);
+}
+
 // Potential tail rows mu = nmu-3..nmu.
 struct TailArgs {
     double *pot;
@@ -229,7 +237,7 @@ struct TailArgs {
     int nni, nmu, kind, dm, mpole;
 };
 
-__global__ void k_pot_tail(TailArgs A)
+__device__ __forceinline__ void pot_tail_body(const TailArgs &A)
 {
     const int nni = A.nni;
     const double eps = 2.220446049250313e-16;
@@ -266,6 +274,9 @@ __global__ void k_pot_tail(TailArgs A)
     }
 }
 
+__global__ void k_pot_tail(TailArgs A) { pot_tail_body(A); }
+__global__ void k_pot_tail_batch(const TailArgs *__restrict__ A) { pot_tail_body(A[blockIdx.y]); }
+
 __global__ void k_setci(double *psi, int nni, int nmu, int ihsym)
 {
     const int n2 = nni / 2 + 1;
@@ -276,6 +287,198 @@ __global__ void k_setci(double *psi, int nni, int nmu, int ihsym)
         psi[(long)mi * nni + (nni - ni)] = ihsym == 1 ? v : -v;
         if (ihsym != 1 && ni == 1) psi[(long)mi * nni + n2 - 1] = 0.0;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched kernels: blockIdx.y selects the problem.  The reductions keep the
+// slicing of the single-problem kernels (kRedBlocks grid-strided slices, then
+// a fixed-order sum), so a batched quadrature equals the single one bit for bit.
+
+struct LcOp {
+    double *out;
+    const double *base;
+    double bscale;
+    int first, nterms;
+    long n;
+};
+
+__global__ void k_lincomb_batch(const LcOp *__restrict__ ops, const x2g_term *__restrict__ terms)
+{
+    const LcOp &o = ops[blockIdx.y];
+    const x2g_term *t = terms + o.first;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < o.n; e += (long)gridDim.x * blockDim.x) {
+        double v = o.base ? o.bscale * o.base[e] : 0.0;
+        for (int k = 0; k < o.nterms; ++k) {
+            double x = t[k].c * t[k].a_d[e];
+            if (t[k].b_d) x *= t[k].b_d[e];
+            if (t[k].cc_d) x *= t[k].cc_d[e];
+            v += x;
+        }
+        o.out[e] = v;
+    }
+}
+
+// Sum of the nb partials at p (fixed order), valid in thread 0.
+__device__ __forceinline__ double sum_partials(const double *p, int nb, double *sh)
+{
+    double s[1] = {0.0};
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s[0] += p[i];
+    block_sum<1>(s, sh);
+    __syncthreads();  // sh is reused by the caller's next sum
+    return s[0];
+}
+
+constexpr int kMaxOrtho = X2G_MAX_ORTHO;
+struct NoDesc {
+    double *psi;
+    const double *w, *f;
+    const double *oth[kMaxOrtho];
+    double *norm_out;
+    long n;
+    int nni, nmu, noth, ihomo, normalise;
+};
+// partial layout: ((item*(kMaxOrtho+1) + k)*kRedBlocks + block), k = 0 norm,
+// k = 1+q overlap with other q; coef layout: item*(kMaxOrtho+1) + k
+
+// stage 0: s = sum w*(psi*psi*f); stage 1 (blockIdx.z = q): sum w*(psi*oth_q*f)
+__global__ void __launch_bounds__(kRedBlock) k_no_dot(const NoDesc *__restrict__ nd, double *partial, int stage)
+{
+    __shared__ double sh[4];
+    const NoDesc &d = nd[blockIdx.y];
+    if (!d.normalise) return;
+    const int q = blockIdx.z;
+    if (stage == 1 && q >= d.noth) return;
+    const double *b = stage == 0 ? d.psi : d.oth[q];
+    double s[1] = {0.0};
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
+        double x = d.psi[e];
+        x *= b[e];
+        x *= d.f[e];
+        s[0] += d.w[e] * x;
+    }
+    block_sum<1>(s, sh);
+    const int k = stage == 0 ? 0 : 1 + q;
+    if (threadIdx.x == 0) partial[((long)blockIdx.y * (kMaxOrtho + 1) + k) * gridDim.x + blockIdx.x] = s[0];
+}
+
+__global__ void k_no_setci(const NoDesc *__restrict__ nd)
+{
+    const NoDesc &d = nd[blockIdx.y];
+    if (d.ihomo == 0) return;
+    const int nni = d.nni, n2 = nni / 2 + 1;
+    const long tot = (long)(n2 - 1) * d.nmu;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+        const int mi = (int)(e / (n2 - 1)), ni = 1 + (int)(e % (n2 - 1));
+        const double v = d.psi[(long)mi * nni + ni - 1];
+        d.psi[(long)mi * nni + (nni - ni)] = d.ihomo == 1 ? v : -v;
+        if (d.ihomo != 1 && ni == 1) d.psi[(long)mi * nni + n2 - 1] = 0.0;
+    }
+}
+
+// one block per item: the update coefficients of k_no_apply
+__global__ void __launch_bounds__(kRedBlock) k_no_finish(const NoDesc *__restrict__ nd, const double *partial, int nb,
+                                                         double *coef)
+{
+    __shared__ double sh[4];
+    __shared__ double ov[kMaxOrtho + 1];
+    const NoDesc &d = nd[blockIdx.x];
+    if (!d.normalise) return;
+    const double *p = partial + (long)blockIdx.x * (kMaxOrtho + 1) * nb;
+    for (int k = 0; k <= d.noth; ++k) {
+        const double v = sum_partials(p + (long)k * nb, nb, sh);
+        if (threadIdx.x == 0) ov[k] = v;
+    }
+    if (threadIdx.x == 0) {
+        const double s = ov[0];
+        if (d.norm_out) *d.norm_out = s;
+        const double rs = 1.0 / sqrt(s);
+        double a = 0.0;
+        for (int k = 1; k <= d.noth; ++k) {
+            ov[k] = ov[k] * rs;
+            a += ov[k] * ov[k];
+        }
+        const double ano = d.noth > 0 ? sqrt(1.0 + a) / (1.0 - a * a) : 1.0;
+        double *c = coef + (long)blockIdx.x * (kMaxOrtho + 1);
+        c[0] = ano * rs;
+        for (int k = 1; k <= d.noth; ++k) c[k] = -ano * ov[k];
+    }
+}
+
+__global__ void k_no_apply(const NoDesc *__restrict__ nd, const double *coef)
+{
+    const NoDesc &d = nd[blockIdx.y];
+    if (!d.normalise) return;
+    const double *c = coef + (long)blockIdx.y * (kMaxOrtho + 1);
+    const int no = d.noth;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
+        double v = c[0] * d.psi[e];
+        for (int k = 0; k < no; ++k) v += c[1 + k] * d.oth[k][e];
+        d.psi[e] = v;
+    }
+}
+
+// EaHF of one orbital: the accumulators of k_oneel (kinetic, nuclear) and of
+// the two x2g_wdot terms <psi|cpot psi> and <psi|xpot> in one pass.
+struct EaDesc {
+    OneelArgs a;
+    const double *cpot, *xpot, *wgt2;
+    double *ee_out;
+};
+
+__global__ void __launch_bounds__(kRedBlock) k_eahf(const EaDesc *__restrict__ ed, double *partial)
+{
+    __shared__ double sh[16];
+    const EaDesc &D = ed[blockIdx.y];
+    const OneelArgs &A = D.a;
+    const int nni = A.nni, nmu = A.nmu;
+    const long s1 = nni + 8;
+    const long n = (long)nni * nmu;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int mu = 1 + (int)(e / nni), nu = 1 + (int)(e % nni);
+        const long p = (long)(mu + 3) * s1 + nu + 3;
+        const double dn = A.dnu[nu - 1], bm = A.bmu[mu - 1];
+        double tn = 0.0, tm = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            tn += A.w[p + (k - 4)] * (A.a2n[k] + dn * A.a1n[k]);
+            tm += A.w[p + (long)(k - 4) * s1] * (A.a2m[k] + bm * A.a1m[k]);
+        }
+        double t = tn + tm;
+        const double ps1 = A.psi1[e], ps2 = A.psi2[e], wg = A.wgt1[e];
+        if (A.m2 != 0.0) t += A.m2 * A.e[e] * ps1;
+        acc[0] += wg * (t * ps2);
+        acc[1] += wg * (ps1 * A.f0[e] * ps2);
+        double x = ps1 * D.cpot[e];
+        x *= ps1;
+        acc[2] += D.wgt2[e] * x;
+        acc[3] += D.wgt2[e] * (ps1 * D.xpot[e]);
+    }
+    block_sum<4>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) partial[((long)blockIdx.y * 4 + k) * gridDim.x + blockIdx.x] = acc[k];
+}
+
+__global__ void __launch_bounds__(kRedBlock) k_eahf_finish(const EaDesc *__restrict__ ed, const double *partial, int nb)
+{
+    __shared__ double sh[4];
+    __shared__ double r[4];
+    const double *p = partial + (long)blockIdx.x * 4 * nb;
+    for (int k = 0; k < 4; ++k) {
+        const double v = sum_partials(p + (long)k * nb, nb, sh);
+        if (threadIdx.x == 0) r[k] = v;
+    }
+    if (threadIdx.x == 0) *ed[blockIdx.x].ee_out = (r[0] + r[1]) + (r[2] - r[3]);
+}
+
+struct GatherDesc {
+    const double *src;
+    int count, offset;
+};
+__global__ void k_gather(const GatherDesc *__restrict__ gd, double *dst)
+{
+    const GatherDesc &g = gd[blockIdx.x];
+    for (int i = threadIdx.x; i < g.count; i += blockDim.x) dst[g.offset + i] = g.src[i];
 }
 
 // 9-point weights (initSuppl, initCBAllocArrays.F90:1670-1673, 1993-2011):
@@ -447,5 +650,275 @@ extern "C" int x2g_setci(x2g_grid *g, double *psi_d, int ihsym)
     const long tot = (long)(g->nni / 2) * g->nmu;
     hipLaunchKernelGGL(k_setci, dim3(gx(tot, 256)), dim3(256), 0, g->ctx->stream, psi_d, g->nni, g->nmu, ihsym);
     X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched C ABI
+
+static int check_grids(x2g_ctx *ctx, x2g_grid *const *grids, int gi)
+{
+    X2G_CHECK(gi >= 0 && grids[gi], "bad grid index %d", gi);
+    X2G_CHECK(grids[gi]->ctx == ctx, "grid of another context");
+    return X2G_OK;
+}
+
+extern "C" int x2g_lincomb_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_lincomb_op *ops, int nops,
+                                 const x2g_term *terms, int nterms_total)
+{
+    X2G_CHECK(ctx && grids && (ops || nops == 0) && nops >= 0 && nterms_total >= 0, "x2g_lincomb_batch: null argument");
+    if (nops == 0) return X2G_OK;
+    std::vector<LcOp> v(nops);
+    long maxn = 0;
+    for (int i = 0; i < nops; ++i) {
+        const x2g_lincomb_op &o = ops[i];
+        int rc = check_grids(ctx, grids, o.grid_index);
+        if (rc) return rc;
+        X2G_CHECK(o.out_d && o.nterms >= 0 && o.first_term >= 0 && o.first_term + o.nterms <= nterms_total,
+                  "x2g_lincomb_batch: op %d has a bad term range", i);
+        for (int k = 0; k < o.nterms; ++k)
+            X2G_CHECK(terms[o.first_term + k].a_d, "x2g_lincomb_batch: term %d has no a", o.first_term + k);
+        v[i] = {o.out_d, o.base_d, o.base_scale, o.first_term, o.nterms, (long)grids[o.grid_index]->n()};
+        maxn = std::max(maxn, v[i].n);
+    }
+    size_t off = 0;
+    const LcOp *dops;
+    const x2g_term *dt = nullptr;
+    int rc = stage(ctx, v.data(), v.size(), off, &dops);
+    if (rc) return rc;
+    if (nterms_total && (rc = stage(ctx, terms, (size_t)nterms_total, off, &dt))) return rc;
+    hipLaunchKernelGGL(k_lincomb_batch, dim3(gx(maxn, 256), nops), dim3(256), 0, ctx->stream, dops, dt);
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
+extern "C" int x2g_normortho_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_normortho_item *items, int n)
+{
+    X2G_CHECK(ctx && grids && (items || n == 0) && n >= 0, "x2g_normortho_batch: null argument");
+    if (n == 0) return X2G_OK;
+    std::vector<NoDesc> v(n);
+    long maxn = 0, maxhalf = 0;
+    int maxoth = 0;
+    for (int i = 0; i < n; ++i) {
+        const x2g_normortho_item &it = items[i];
+        int rc = check_grids(ctx, grids, it.grid_index);
+        if (rc) return rc;
+        x2g_grid *g = grids[it.grid_index];
+        X2G_CHECK(it.psi_d, "x2g_normortho_batch: item %d has no orbital", i);
+        X2G_CHECK(it.nothers >= 0 && it.nothers <= kMaxOrtho, "x2g_normortho_batch: item %d nothers", i);
+        X2G_CHECK(it.ihomo >= -1 && it.ihomo <= 1, "x2g_normortho_batch: item %d ihomo", i);
+        NoDesc &d = v[i];
+        d.psi = it.psi_d;
+        d.w = g->suppl + 13 * g->n();  // wgt2
+        d.f = g->suppl + 8 * g->n();   // f4
+        for (int k = 0; k < kMaxOrtho; ++k) d.oth[k] = k < it.nothers ? it.others_d[k] : nullptr;
+        for (int k = 0; k < it.nothers; ++k) X2G_CHECK(d.oth[k], "x2g_normortho_batch: item %d other %d", i, k);
+        d.norm_out = it.norm_out_d;
+        d.n = (long)g->n();
+        d.nni = g->nni;
+        d.nmu = g->nmu;
+        d.noth = it.normalise ? it.nothers : 0;
+        d.ihomo = it.ihomo;
+        d.normalise = it.normalise ? 1 : 0;
+        maxn = std::max(maxn, d.n);
+        maxhalf = std::max<long>(maxhalf, (long)(g->nni / 2) * g->nmu);
+        maxoth = std::max(maxoth, d.noth);
+    }
+    const size_t per = (size_t)(kMaxOrtho + 1) * (kRedBlocks + 1);
+    int rc = ws_reserve(ctx, 5, (size_t)n * per * sizeof(double));
+    if (rc) return rc;
+    double *partial = (double *)ctx->ws[5].ptr;
+    double *coef = partial + (size_t)n * (kMaxOrtho + 1) * kRedBlocks;
+    size_t off = 0;
+    const NoDesc *dd;
+    if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
+    hipLaunchKernelGGL(k_no_dot, dim3(kRedBlocks, n, 1), dim3(kRedBlock), 0, ctx->stream, dd, partial, 0);
+    X2G_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_no_setci, dim3(gx(maxhalf, 256), n), dim3(256), 0, ctx->stream, dd);
+    X2G_HIP(hipGetLastError());
+    if (maxoth > 0) {
+        hipLaunchKernelGGL(k_no_dot, dim3(kRedBlocks, n, maxoth), dim3(kRedBlock), 0, ctx->stream, dd, partial, 1);
+        X2G_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_no_finish, dim3(n), dim3(kRedBlock), 0, ctx->stream, dd, partial, kRedBlocks, coef);
+    X2G_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_no_apply, dim3(gx(maxn, 256), n), dim3(256), 0, ctx->stream, dd, coef);
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
+extern "C" int x2g_eahf_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_eahf_item *items, int n)
+{
+    X2G_CHECK(ctx && grids && (items || n == 0) && n >= 0, "x2g_eahf_batch: null argument");
+    if (n == 0) return X2G_OK;
+    size_t padtot = 0;
+    std::vector<size_t> poff(n);
+    for (int i = 0; i < n; ++i) {
+        const x2g_eahf_item &it = items[i];
+        int rc = check_grids(ctx, grids, it.grid_index);
+        if (rc) return rc;
+        X2G_CHECK(it.psi_d && it.cpot_d && it.xpot_d && it.ee_out_d, "x2g_eahf_batch: item %d incomplete", i);
+        X2G_CHECK(it.isym == 1 || it.isym == -1, "x2g_eahf_batch: item %d isym", i);
+        poff[i] = padtot;
+        padtot += (grids[it.grid_index]->npad() * sizeof(double) + 255) & ~size_t(255);
+    }
+    int rc = ws_reserve(ctx, 4, padtot);
+    if (!rc) rc = ws_reserve(ctx, 5, (size_t)n * 4 * kRedBlocks * sizeof(double));
+    if (rc) return rc;
+    char *pads = (char *)ctx->ws[4].ptr;
+    double *partial = (double *)ctx->ws[5].ptr;
+    std::vector<PadItem> pin(n);
+    std::vector<EaDesc> v(n);
+    for (int i = 0; i < n; ++i) {
+        const x2g_eahf_item &it = items[i];
+        x2g_grid *g = grids[it.grid_index];
+        double *w = (double *)(pads + poff[i]);
+        pin[i] = {it.psi_d, w, g->nni, g->nmu, it.isym};
+        EaDesc &d = v[i];
+        OneelArgs &A = d.a;
+        A = OneelArgs{};
+        A.w = w;
+        A.psi1 = it.psi_d;
+        A.psi2 = it.psi_d;
+        A.wgt1 = g->suppl + 12 * g->n();
+        A.e = g->suppl + 3 * g->n();
+        A.f0 = g->suppl + 4 * g->n();
+        A.bmu = g->bmu_orb;
+        A.dnu = g->dnu;
+        deriv_weights(g->coef[22], A.a1n, A.a2n);
+        deriv_weights(g->coef[23], A.a1m, A.a2m);
+        A.m2 = (double)(it.m * it.m);
+        A.nni = g->nni;
+        A.nmu = g->nmu;
+        d.cpot = it.cpot_d;
+        d.xpot = it.xpot_d;
+        d.wgt2 = g->suppl + 13 * g->n();
+        d.ee_out = it.ee_out_d;
+    }
+    if ((rc = launch_putin_items(ctx, pin))) return rc;
+    size_t off = 0;
+    const EaDesc *dd;
+    if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
+    hipLaunchKernelGGL(k_eahf, dim3(kRedBlocks, n), dim3(kRedBlock), 0, ctx->stream, dd, partial);
+    X2G_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_eahf_finish, dim3(n), dim3(kRedBlock), 0, ctx->stream, dd, partial, kRedBlocks);
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
+extern "C" int x2g_moments_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_moment_item *items, int n, int mpole,
+                                 double *out_h)
+{
+    X2G_CHECK(ctx && grids && (items || n == 0) && n >= 0 && out_h, "x2g_moments_batch: null argument");
+    X2G_CHECK(mpole >= 2 && mpole <= 8, "x2g_moments_batch: mpole must be 2..8");
+    if (n == 0) return X2G_OK;
+    std::vector<MomArgs> v(n);
+    for (int i = 0; i < n; ++i) {
+        const x2g_moment_item &it = items[i];
+        int rc = check_grids(ctx, grids, it.grid_index);
+        if (rc) return rc;
+        X2G_CHECK(it.psi1_d && it.psi2_d, "x2g_moments_batch: item %d incomplete", i);
+        X2G_CHECK(it.dm >= 0 && it.dm <= 4, "x2g_moments_batch: item %d |Delta m| must be 0..4", i);
+        x2g_grid *g = grids[it.grid_index];
+        MomArgs &A = v[i];
+        A = MomArgs{};
+        A.psi1 = it.psi1_d;
+        A.psi2 = it.psi2_d;
+        A.wgt2 = g->suppl + 13 * g->n();
+        A.f4 = g->suppl + 8 * g->n();
+        A.vxi = g->vxi;
+        A.vxisq = g->vxisq;
+        A.veta = g->veta;
+        A.vetasq = g->vetasq;
+        A.r2 = 0.5 * g->coef[24];
+        A.nni = g->nni;
+        A.nmu = g->nmu;
+        A.dm = it.dm;
+        A.mpole = mpole;
+    }
+    const int nv = 8 * n;
+    int rc = ws_reserve(ctx, 5, (size_t)nv * (kRedBlocks + 1) * sizeof(double));
+    if (!rc) rc = pinned_out_reserve(ctx, (size_t)nv * sizeof(double));
+    if (rc) return rc;
+    double *partial = (double *)ctx->ws[5].ptr;
+    double *dout = partial + (size_t)nv * kRedBlocks;
+    size_t off = 0;
+    const MomArgs *dd;
+    if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
+    hipLaunchKernelGGL(k_moments_batch, dim3(kRedBlocks, n), dim3(kRedBlock), 0, ctx->stream, dd, partial);
+    X2G_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finish, dim3(nv), dim3(kRedBlock), 0, ctx->stream, partial, kRedBlocks, nv, dout);
+    X2G_HIP(hipGetLastError());
+    X2G_HIP(hipMemcpyAsync(ctx->pinned_out, dout, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    const double *r = (const double *)ctx->pinned_out;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 8; ++k) out_h[8 * i + k] = k < mpole ? r[8 * i + k] : 0.0;
+    return X2G_OK;
+}
+
+extern "C" int x2g_pot_tail_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_tail_item *items, int n, int mpole)
+{
+    X2G_CHECK(ctx && grids && (items || n == 0) && n >= 0, "x2g_pot_tail_batch: null argument");
+    X2G_CHECK(mpole >= 2 && mpole <= 8, "x2g_pot_tail_batch: mpole must be 2..8");
+    if (n == 0) return X2G_OK;
+    std::vector<TailArgs> v(n);
+    int maxnni = 0;
+    for (int i = 0; i < n; ++i) {
+        const x2g_tail_item &it = items[i];
+        int rc = check_grids(ctx, grids, it.grid_index);
+        if (rc) return rc;
+        X2G_CHECK(it.pot_d, "x2g_pot_tail_batch: item %d has no potential", i);
+        X2G_CHECK(it.kind == 0 || it.kind == 1, "x2g_pot_tail_batch: item %d kind", i);
+        X2G_CHECK(it.dm >= 0, "x2g_pot_tail_batch: item %d dm < 0", i);
+        x2g_grid *g = grids[it.grid_index];
+        TailArgs &A = v[i];
+        A = TailArgs{};
+        A.pot = it.pot_d;
+        A.vxi = g->vxi;
+        A.vxisq = g->vxisq;
+        A.veta = g->veta;
+        A.vetasq = g->vetasq;
+        for (int k = 0; k < 8; ++k) A.mom[k] = k < mpole ? it.mom[k] : 0.0;
+        A.r2 = 0.5 * g->coef[24];
+        A.nni = g->nni;
+        A.nmu = g->nmu;
+        A.kind = it.kind;
+        A.dm = it.dm;
+        A.mpole = mpole;
+        maxnni = std::max(maxnni, g->nni);
+    }
+    size_t off = 0;
+    const TailArgs *dd;
+    int rc = stage(ctx, v.data(), v.size(), off, &dd);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pot_tail_batch, dim3(gx(4L * maxnni, 128), n), dim3(128), 0, ctx->stream, dd);
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
+extern "C" int x2g_gather(x2g_ctx *ctx, const double *const *src_d, const int *counts, int n, double *out_h)
+{
+    X2G_CHECK(ctx && (n == 0 || (src_d && counts && out_h)) && n >= 0, "x2g_gather: null argument");
+    if (n == 0) return X2G_OK;
+    std::vector<GatherDesc> v(n);
+    int tot = 0;
+    for (int i = 0; i < n; ++i) {
+        X2G_CHECK(src_d[i] && counts[i] >= 0, "x2g_gather: item %d", i);
+        v[i] = {src_d[i], counts[i], tot};
+        tot += counts[i];
+    }
+    int rc = ws_reserve(ctx, 1, (size_t)std::max(tot, 1) * sizeof(double));
+    if (!rc) rc = pinned_out_reserve(ctx, (size_t)std::max(tot, 1) * sizeof(double));
+    if (rc) return rc;
+    double *dbuf = (double *)ctx->ws[1].ptr;
+    size_t off = 0;
+    const GatherDesc *dd;
+    if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
+    hipLaunchKernelGGL(k_gather, dim3(n), dim3(64), 0, ctx->stream, dd, dbuf);
+    X2G_HIP(hipGetLastError());
+    X2G_HIP(hipMemcpyAsync(ctx->pinned_out, dbuf, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    X2G_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out_h, ctx->pinned_out, (size_t)tot * sizeof(double));
     return X2G_OK;
 }

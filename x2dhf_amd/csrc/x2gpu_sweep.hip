@@ -852,6 +852,23 @@ int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w
     return X2G_OK;
 }
 
+int launch_putin_items(x2g_ctx *ctx, const std::vector<PadItem> &items)
+{
+    if (items.empty()) return X2G_OK;
+    std::vector<PadDesc> v;
+    long maxpad = 0;
+    for (auto &it : items) {
+        v.push_back({it.fun, it.w, nullptr, it.nni, it.nmu, it.isym});
+        maxpad = std::max<long>(maxpad, (long)(it.nni + 8) * (it.nmu + 8));
+    }
+    const PadDesc *dd;
+    int rc = stage_n(ctx, v.data(), v.size(), &dd);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_putin, dim3(grid_x_for(maxpad, 256), (int)v.size()), dim3(256), 0, ctx->stream, dd);
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
 int launch_putout(x2g_grid *g, double *fun, const double *work)
 {
     std::vector<PadDesc> v{{fun, const_cast<double *>(work), nullptr, g->nni, g->nmu, 1}};

@@ -33,6 +33,52 @@ module x2f_capi
      type(c_ptr) :: cc_d = c_null_ptr
   end type x2g_dot
 
+  integer, parameter :: X2G_MAX_ORTHO = 16
+
+  type, bind(c) :: x2g_lincomb_op
+     type(c_ptr) :: out_d = c_null_ptr
+     type(c_ptr) :: base_d = c_null_ptr
+     real(c_double) :: base_scale = 0.0_c_double
+     integer(c_int) :: first_term = 0
+     integer(c_int) :: nterms = 0
+     integer(c_int) :: grid_index = 0
+  end type x2g_lincomb_op
+
+  type, bind(c) :: x2g_normortho_item
+     type(c_ptr) :: psi_d = c_null_ptr
+     type(c_ptr) :: others_d(X2G_MAX_ORTHO) = c_null_ptr
+     integer(c_int) :: nothers = 0
+     integer(c_int) :: ihomo = 0
+     integer(c_int) :: normalise = 1
+     integer(c_int) :: grid_index = 0
+     type(c_ptr) :: norm_out_d = c_null_ptr
+  end type x2g_normortho_item
+
+  type, bind(c) :: x2g_eahf_item
+     type(c_ptr) :: psi_d = c_null_ptr
+     type(c_ptr) :: cpot_d = c_null_ptr
+     type(c_ptr) :: xpot_d = c_null_ptr
+     integer(c_int) :: isym = 1
+     integer(c_int) :: m = 0
+     integer(c_int) :: grid_index = 0
+     type(c_ptr) :: ee_out_d = c_null_ptr
+  end type x2g_eahf_item
+
+  type, bind(c) :: x2g_moment_item
+     type(c_ptr) :: psi1_d = c_null_ptr
+     type(c_ptr) :: psi2_d = c_null_ptr
+     integer(c_int) :: dm = 0
+     integer(c_int) :: grid_index = 0
+  end type x2g_moment_item
+
+  type, bind(c) :: x2g_tail_item
+     type(c_ptr) :: pot_d = c_null_ptr
+     integer(c_int) :: kind = 0
+     integer(c_int) :: dm = 0
+     integer(c_int) :: grid_index = 0
+     real(c_double) :: mom(8) = 0.0_c_double
+  end type x2g_tail_item
+
   interface
      function x2g_last_error() bind(c, name='x2g_last_error')
        import :: c_ptr
@@ -159,6 +205,59 @@ module x2f_capi
        integer(c_int), value :: kind, dm, mpole
        real(c_double), intent(in) :: mom(*)
      end function x2g_pot_tail
+
+     integer(c_int) function x2g_lincomb_batch(ctx, grids, ops, nops, terms, nterms) &
+          bind(c, name='x2g_lincomb_batch')
+       import :: c_int, c_ptr, x2g_lincomb_op, x2g_term
+       type(c_ptr), value :: ctx
+       type(c_ptr), intent(in) :: grids(*)
+       type(x2g_lincomb_op), intent(in) :: ops(*)
+       integer(c_int), value :: nops
+       type(x2g_term), intent(in) :: terms(*)
+       integer(c_int), value :: nterms
+     end function x2g_lincomb_batch
+
+     integer(c_int) function x2g_normortho_batch(ctx, grids, items, n) bind(c, name='x2g_normortho_batch')
+       import :: c_int, c_ptr, x2g_normortho_item
+       type(c_ptr), value :: ctx
+       type(c_ptr), intent(in) :: grids(*)
+       type(x2g_normortho_item), intent(in) :: items(*)
+       integer(c_int), value :: n
+     end function x2g_normortho_batch
+
+     integer(c_int) function x2g_eahf_batch(ctx, grids, items, n) bind(c, name='x2g_eahf_batch')
+       import :: c_int, c_ptr, x2g_eahf_item
+       type(c_ptr), value :: ctx
+       type(c_ptr), intent(in) :: grids(*)
+       type(x2g_eahf_item), intent(in) :: items(*)
+       integer(c_int), value :: n
+     end function x2g_eahf_batch
+
+     integer(c_int) function x2g_moments_batch(ctx, grids, items, n, mpole, out) bind(c, name='x2g_moments_batch')
+       import :: c_int, c_ptr, c_double, x2g_moment_item
+       type(c_ptr), value :: ctx
+       type(c_ptr), intent(in) :: grids(*)
+       type(x2g_moment_item), intent(in) :: items(*)
+       integer(c_int), value :: n, mpole
+       real(c_double), intent(out) :: out(*)
+     end function x2g_moments_batch
+
+     integer(c_int) function x2g_pot_tail_batch(ctx, grids, items, n, mpole) bind(c, name='x2g_pot_tail_batch')
+       import :: c_int, c_ptr, x2g_tail_item
+       type(c_ptr), value :: ctx
+       type(c_ptr), intent(in) :: grids(*)
+       type(x2g_tail_item), intent(in) :: items(*)
+       integer(c_int), value :: n, mpole
+     end function x2g_pot_tail_batch
+
+     integer(c_int) function x2g_gather(ctx, src, counts, n, out) bind(c, name='x2g_gather')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       type(c_ptr), intent(in) :: src(*)
+       integer(c_int), intent(in) :: counts(*)
+       integer(c_int), value :: n
+       real(c_double), intent(out) :: out(*)
+     end function x2g_gather
 
      integer(c_int) function x2g_setci(g, psi, ihsym) bind(c, name='x2g_setci')
        import :: c_int, c_ptr

@@ -45,6 +45,12 @@ module x2f_scf
      real(dp) :: omega_orb = -1.0_dp, omega_pot = -1.0_dp, recalc_mm = 1.15_dp
   end type scf_opts
 
+  type :: lc_batch
+     type(x2g_lincomb_op), allocatable :: ops(:)
+     type(x2g_term), allocatable :: terms(:)
+     integer :: nops = 0, nterms = 0
+  end type lc_batch
+
   type :: scf_job
      type(case_t) :: c
      type(grid_t) :: g
@@ -75,6 +81,9 @@ module x2f_scf
      real(dp) :: deltaeemax = 100, deltanormmax = 100
      integer :: stopcode = 0       ! 1 energy, 2 norm, 3 iteration limit, 4 saturation, 5 resets
      integer(c_long_long) :: nsweep_orb = 0, nsweep_pot = 0
+     ! device record written by the batched schedule: <psi|psi> before
+     ! normalisation (1..MAXO) and orbital energies (MAXO+1..2*MAXO)
+     type(c_ptr) :: rec = c_null_ptr
      ! results
      real(dp) :: etot = 0, vkt = 0, vnt = 0, ecoul = 0, eexch = 0, virial = 0
   end type scf_job
@@ -124,6 +133,10 @@ contains
     enddo
     j%ones = dalloc(j)
     call x2check(x2g_fill(j%ctx, j%ones, j%n, 1.0_dp), 'x2g_fill')
+    call x2check(x2g_malloc(j%ctx, int(16*MAXO, c_size_t), j%rec), 'x2g_malloc')
+    call x2check(x2g_fill(j%ctx, j%rec, int(2*MAXO, c_size_t), 0.0_dp), 'x2g_fill')
+    j%ndev = j%ndev + 1
+    j%devbufs(j%ndev) = j%rec
     ! potentials of the iteration, in the SCF's orbital order
     ip = 0
     do ii = 1, j%c%norb
@@ -224,58 +237,61 @@ contains
   ! fockHF (fock.F90:57-184): fock1 = f0 + eps f1 + m^2 e + f2 * sum_j coo_j J_j,
   ! fock2 = f2 * sum_j psi_j (gec K_ij [+ gec' K'_ij] + eps_ji f4) [+ self term].
   ! The Coulomb and exchange sums are kept per orbital for EaHF.
+  ! fock_terms returns the Coulomb-sum terms (tc), the exchange-sum terms (tx)
+  ! and the fock1 terms other than f2 * Coulomb sum (t1).
+  subroutine fock_terms(j, i, tc, ntc, tx, ntx, t1, nt1)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i
+    type(x2g_term), intent(out) :: tc(:), tx(:), t1(:)
+    integer, intent(out) :: ntc, ntx, nt1
+    integer :: i1, k
+    real(dp) :: coo
+    nt1 = 1
+    t1(1) = x2g_term(j%ee(i,i), j%S(6), c_null_ptr, c_null_ptr)
+    if (j%c%m(i) /= 0) then
+       nt1 = nt1 + 1; t1(nt1) = x2g_term(dble(j%c%m(i)**2), j%S(4), c_null_ptr, c_null_ptr)
+    endif
+    ntc = 0; ntx = 0
+    if (j%c%nel == 1) return
+    do i1 = 1, j%c%norb
+       coo = j%c%occ(i1)
+       if (i1 == i) coo = coo - 1.0_dp
+       ntc = ntc + 1; tc(ntc) = x2g_term(coo, j%coul(i1), c_null_ptr, c_null_ptr)
+    enddo
+    do i1 = 1, j%c%norb
+       k = j%c%k2(i, i1)
+       if (i1 /= i) then
+          ntx = ntx + 1; tx(ntx) = x2g_term(j%c%gec(i,i1), j%exch(j%c%slot(k)), j%orb(i1), c_null_ptr)
+          if (j%c%ilc(k) == 2) then
+             ntx = ntx + 1; tx(ntx) = x2g_term(j%c%gec2(i,i1), j%exch(j%c%slot2(k)), j%orb(i1), c_null_ptr)
+          endif
+          if (abs(j%ee(i1,i)) > eps0) then
+             ntx = ntx + 1; tx(ntx) = x2g_term(j%ee(i1,i), j%S(9), j%orb(i1), c_null_ptr)
+          endif
+       else
+          if (j%c%m(i) > 0 .and. k > 0) then
+             if (j%c%ilc(k) > 0) then
+                ntx = ntx + 1; tx(ntx) = x2g_term(j%c%gec(i,i), j%exch(j%c%slot(k)), j%orb(i), c_null_ptr)
+             endif
+          endif
+       endif
+    enddo
+  end subroutine fock_terms
+
   subroutine fock_hf(j, i, fock1, fock2)
     type(scf_job), intent(inout) :: j
     integer, intent(in) :: i
     type(c_ptr), intent(in) :: fock1, fock2
-    type(x2g_term) :: t(4*MAXO+4)
-    integer :: n, i1, k
-    real(dp) :: coo
-    n = 0
-    n = n + 1; t(n) = x2g_term(j%ee(i,i), j%S(6), c_null_ptr, c_null_ptr)
-    if (j%c%m(i) /= 0) then
-       n = n + 1; t(n) = x2g_term(dble(j%c%m(i)**2), j%S(4), c_null_ptr, c_null_ptr)
-    endif
+    type(x2g_term) :: t(4), tc(MAXO), tx(4*MAXO)
+    integer :: n, ntc, ntx
+    call fock_terms(j, i, tc, ntc, tx, ntx, t, n)
     if (j%c%nel == 1) then
        call x2check(x2g_lincomb(j%gd, fock1, j%S(5), 1.0_dp, t, int(n, c_int)), 'fock1')
        call x2check(x2g_fill(j%ctx, fock2, j%n, 0.0_dp), 'fock2')
        return
     endif
-    ! Coulomb part
-    block
-      type(x2g_term) :: tc(MAXO)
-      do i1 = 1, j%c%norb
-         coo = j%c%occ(i1)
-         if (i1 == i) coo = coo - 1.0_dp
-         tc(i1) = x2g_term(coo, j%coul(i1), c_null_ptr, c_null_ptr)
-      enddo
-      call x2check(x2g_lincomb(j%gd, j%cpot(i), c_null_ptr, 0.0_dp, tc, int(j%c%norb, c_int)), 'fock coulomb')
-    end block
-    ! exchange part
-    block
-      type(x2g_term) :: tx(4*MAXO)
-      integer :: nx
-      nx = 0
-      do i1 = 1, j%c%norb
-         k = j%c%k2(i, i1)
-         if (i1 /= i) then
-            nx = nx + 1; tx(nx) = x2g_term(j%c%gec(i,i1), j%exch(j%c%slot(k)), j%orb(i1), c_null_ptr)
-            if (j%c%ilc(k) == 2) then
-               nx = nx + 1; tx(nx) = x2g_term(j%c%gec2(i,i1), j%exch(j%c%slot2(k)), j%orb(i1), c_null_ptr)
-            endif
-            if (abs(j%ee(i1,i)) > eps0) then
-               nx = nx + 1; tx(nx) = x2g_term(j%ee(i1,i), j%S(9), j%orb(i1), c_null_ptr)
-            endif
-         else
-            if (j%c%m(i) > 0 .and. k > 0) then
-               if (j%c%ilc(k) > 0) then
-                  nx = nx + 1; tx(nx) = x2g_term(j%c%gec(i,i), j%exch(j%c%slot(k)), j%orb(i), c_null_ptr)
-               endif
-            endif
-         endif
-      enddo
-      call x2check(x2g_lincomb(j%gd, j%xpot(i), c_null_ptr, 0.0_dp, tx, int(nx, c_int)), 'fock exchange')
-    end block
+    call x2check(x2g_lincomb(j%gd, j%cpot(i), c_null_ptr, 0.0_dp, tc, int(ntc, c_int)), 'fock coulomb')
+    call x2check(x2g_lincomb(j%gd, j%xpot(i), c_null_ptr, 0.0_dp, tx, int(ntx, c_int)), 'fock exchange')
     n = n + 1; t(n) = x2g_term(1.0_dp, j%S(7), j%cpot(i), c_null_ptr)
     call x2check(x2g_lincomb(j%gd, fock1, j%S(5), 1.0_dp, t, int(n, c_int)), 'fock1')
     t(1) = x2g_term(1.0_dp, j%S(7), j%xpot(i), c_null_ptr)
@@ -283,17 +299,69 @@ contains
   end subroutine fock_hf
 
   ! ---------------------------------------------------------------------------
+  ! Collected linear combinations, launched together by x2g_lincomb_batch.
+  subroutine lc_add(b, out, base, s, t, nt, gidx)
+    type(lc_batch), intent(inout) :: b
+    type(c_ptr), intent(in) :: out, base
+    real(dp), intent(in) :: s
+    type(x2g_term), intent(in) :: t(:)
+    integer, intent(in) :: nt, gidx
+    type(x2g_lincomb_op), allocatable :: o2(:)
+    type(x2g_term), allocatable :: t2(:)
+    if (.not. allocated(b%ops)) allocate(b%ops(64), b%terms(256))
+    if (b%nops == size(b%ops)) then
+       allocate(o2(2*size(b%ops))); o2(1:b%nops) = b%ops(1:b%nops)
+       call move_alloc(o2, b%ops)
+    endif
+    if (b%nterms + nt > size(b%terms)) then
+       allocate(t2(2*(b%nterms + nt))); t2(1:b%nterms) = b%terms(1:b%nterms)
+       call move_alloc(t2, b%terms)
+    endif
+    b%nops = b%nops + 1
+    b%ops(b%nops) = x2g_lincomb_op(out, base, s, int(b%nterms, c_int), int(nt, c_int), int(gidx, c_int))
+    if (nt > 0) b%terms(b%nterms+1:b%nterms+nt) = t(1:nt)
+    b%nterms = b%nterms + nt
+  end subroutine lc_add
+
+  subroutine lc_run(b, ctx, grids)
+    type(lc_batch), intent(inout) :: b
+    type(c_ptr), intent(in) :: ctx
+    type(c_ptr), intent(in) :: grids(:)
+    if (b%nops == 0) return
+    call x2check(x2g_lincomb_batch(ctx, grids, b%ops, int(b%nops, c_int), b%terms, int(b%nterms, c_int)), &
+         'x2g_lincomb_batch')
+    b%nops = 0
+    b%nterms = 0
+  end subroutine lc_run
+
+  ! c_ptr advanced by nbytes
+  function ptr_off(p, nbytes) result(q)
+    type(c_ptr), intent(in) :: p
+    integer, intent(in) :: nbytes
+    type(c_ptr) :: q
+    q = transfer(transfer(p, 0_c_intptr_t) + int(nbytes, c_intptr_t), q)
+  end function ptr_off
+
+  ! ---------------------------------------------------------------------------
   ! relaxation items of the potentials of orbital i (relaxPots): rhs = psi1 psi2 g
-  subroutine pot_items(j, i, items, n, gidx)
+  subroutine pot_items(j, i, items, n, gidx, lb)
     type(scf_job), intent(inout) :: j
     integer, intent(in) :: i, gidx
     type(x2g_relax_item), intent(inout) :: items(:)
     integer, intent(inout) :: n
+    type(lc_batch), intent(inout), optional :: lb   ! collect the rhs products instead of launching them
     integer :: k, ip
     type(c_ptr) :: fun
+    type(x2g_term) :: t(1)
     do k = 1, j%c%npot(i)
        ip = j%pidx(i,k)
-       call lincomb1(j, j%prhs(ip), c_null_ptr, 0.0_dp, 1.0_dp, j%orb(j%c%pin1(i,k)), j%orb(j%c%pin2(i,k)), j%S(10))
+       if (present(lb)) then
+          t(1) = x2g_term(1.0_dp, j%orb(j%c%pin1(i,k)), j%orb(j%c%pin2(i,k)), j%S(10))
+          call lc_add(lb, j%prhs(ip), c_null_ptr, 0.0_dp, t, 1, gidx)
+       else
+          call lincomb1(j, j%prhs(ip), c_null_ptr, 0.0_dp, 1.0_dp, j%orb(j%c%pin1(i,k)), j%orb(j%c%pin2(i,k)), &
+               j%S(10))
+       endif
        if (j%c%pslot(i,k) == 0) then
           fun = j%coul(i)
        else
@@ -424,36 +492,61 @@ contains
 
   ! ---------------------------------------------------------------------------
   ! mpoleMoments: Coulomb moments of touched orbitals (odd ones vanish by
-  ! symmetry for homonuclear molecules) and the moments of every exchange pair
+  ! symmetry for homonuclear molecules) and the moments of every exchange pair,
+  ! all in one batched reduction
   subroutine mpole_moments(j)
     type(scf_job), intent(inout) :: j
-    integer :: i, k
-    real(dp) :: mom(8)
+    type(x2g_moment_item), allocatable :: it(:)
+    real(dp), allocatable :: out(:)
+    integer, allocatable :: who(:)
+    type(c_ptr) :: grids(1)
+    integer :: i, k, n, mp
     if (j%c%nel == 1) return
+    allocate(it(j%c%norb + j%c%nmm), who(j%c%norb + j%c%nmm))
+    n = 0
     do i = 1, j%c%norb
        if (j%itouch(i) == 0) cycle
-       call x2check(x2g_moments(j%gd, j%orb(i), j%orb(i), 0_c_int, int(j%o%mpole, c_int), mom), 'coulMom')
-       j%cmulti(1:j%o%mpole, i) = mom(1:j%o%mpole)
-       if (j%c%homonuclear) then
-          j%cmulti(1, i) = 0.0_dp; j%cmulti(3, i) = 0.0_dp
-          j%cmulti(5, i) = 0.0_dp; j%cmulti(7, i) = 0.0_dp
-       endif
+       n = n + 1
+       it(n) = x2g_moment_item(j%orb(i), j%orb(i), 0_c_int, 0_c_int)
+       who(n) = i
     enddo
     do k = 1, j%c%nmm
-       call x2check(x2g_moments(j%gd, j%orb(j%c%mm1(k)), j%orb(j%c%mm2(k)), int(j%c%mmdel(k), c_int), &
-            int(j%o%mpole, c_int), mom), 'exchMom')
-       j%exc(1:j%o%mpole, j%c%mmslot(k)) = mom(1:j%o%mpole)
+       n = n + 1
+       it(n) = x2g_moment_item(j%orb(j%c%mm1(k)), j%orb(j%c%mm2(k)), int(j%c%mmdel(k), c_int), 0_c_int)
+       who(n) = -k
+    enddo
+    if (n == 0) return
+    allocate(out(8*n))
+    grids(1) = j%gd
+    mp = j%o%mpole
+    call x2check(x2g_moments_batch(j%ctx, grids, it, int(n, c_int), int(mp, c_int), out), 'mpoleMoments')
+    do k = 1, n
+       if (who(k) > 0) then
+          i = who(k)
+          j%cmulti(1:mp, i) = out(8*(k-1)+1:8*(k-1)+mp)
+          if (j%c%homonuclear) then
+             j%cmulti(1, i) = 0.0_dp; j%cmulti(3, i) = 0.0_dp
+             j%cmulti(5, i) = 0.0_dp; j%cmulti(7, i) = 0.0_dp
+          endif
+       else
+          j%exc(1:mp, j%c%mmslot(-who(k))) = out(8*(k-1)+1:8*(k-1)+mp)
+       endif
     enddo
   end subroutine mpole_moments
 
-  ! potAsympt: multipole tails of the potentials of orbital i1
-  subroutine pot_asympt(j, i1)
+  ! potAsympt: the multipole-tail items of the potentials of orbital i1; a
+  ! slot already queued (done) is not queued again
+  subroutine pot_tail_items(j, i1, ti, nt, gidx, done)
     type(scf_job), intent(inout) :: j
-    integer, intent(in) :: i1
+    integer, intent(in) :: i1, gidx
+    type(x2g_tail_item), intent(inout) :: ti(:)
+    integer, intent(inout) :: nt
+    logical, intent(inout) :: done(:)
     integer :: i2, k, idel
-    if (j%itouch(i1) == 1) &
-         call x2check(x2g_pot_tail(j%gd, j%coul(i1), 0_c_int, 0_c_int, int(j%o%mpole, c_int), j%cmulti(:, i1)), &
-         'coulAsympt')
+    if (j%itouch(i1) == 1) then
+       nt = nt + 1
+       ti(nt) = x2g_tail_item(j%coul(i1), 0_c_int, 0_c_int, int(gidx, c_int), j%cmulti(:, i1))
+    endif
     do i2 = 1, j%c%norb
        if (j%itouch(i1) == 0 .and. j%itouch(i2) == 0) cycle
        k = j%c%k2(i1, i2)
@@ -461,14 +554,37 @@ contains
        if (j%c%ilc(k) == 0) cycle
        idel = abs(j%c%m(i1) - j%c%m(i2))
        if (i1 == i2) idel = 2*j%c%m(i1)
-       call x2check(x2g_pot_tail(j%gd, j%exch(j%c%slot(k)), 1_c_int, int(idel, c_int), int(j%o%mpole, c_int), &
-            j%exc(:, j%c%slot(k))), 'exchAsympt')
+       if (.not. done(j%c%slot(k))) then
+          nt = nt + 1
+          ti(nt) = x2g_tail_item(j%exch(j%c%slot(k)), 1_c_int, int(idel, c_int), int(gidx, c_int), &
+               j%exc(:, j%c%slot(k)))
+          done(j%c%slot(k)) = .true.
+       endif
        if (j%c%ilc(k) == 2) then
-          idel = j%c%m(i2) + j%c%m(i1)
-          call x2check(x2g_pot_tail(j%gd, j%exch(j%c%slot2(k)), 1_c_int, int(idel, c_int), int(j%o%mpole, c_int), &
-               j%exc(:, j%c%slot2(k))), 'exchAsympt')
+          if (.not. done(j%c%slot2(k))) then
+             idel = j%c%m(i2) + j%c%m(i1)
+             nt = nt + 1
+             ti(nt) = x2g_tail_item(j%exch(j%c%slot2(k)), 1_c_int, int(idel, c_int), int(gidx, c_int), &
+                  j%exc(:, j%c%slot2(k)))
+             done(j%c%slot2(k)) = .true.
+          endif
        endif
     enddo
+  end subroutine pot_tail_items
+
+  ! potAsympt of orbital i1 in one launch
+  subroutine pot_asympt(j, i1)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: i1
+    type(x2g_tail_item) :: ti(1 + 2*MAXO)
+    logical :: done(MAXO*(MAXO+1))
+    type(c_ptr) :: grids(1)
+    integer :: nt
+    nt = 0
+    done = .false.
+    call pot_tail_items(j, i1, ti, nt, 0, done)
+    grids(1) = j%gd
+    call x2check(x2g_pot_tail_batch(j%ctx, grids, ti, int(nt, c_int), int(j%o%mpole, c_int)), 'potAsympt')
   end subroutine pot_asympt
 
   ! ---------------------------------------------------------------------------
@@ -786,43 +902,82 @@ contains
     integer, intent(in) :: idx(:), nj
     logical, intent(inout) :: stopped(:)
     type(x2g_relax_item), allocatable :: items(:)
-    type(c_ptr), allocatable :: grids(:)
-    integer :: q, h, i, ii, n, nmax, ms1, ms2
-    nmax = 0
-    do q = 1, nj
-       if (stopped(q)) cycle
-       h = idx(q)
-       nmax = nmax + sum(js(h)%c%npot(1:js(h)%c%norb)) + js(h)%c%norb
-    enddo
-    if (nmax == 0) return
-    allocate(items(nmax), grids(nj))
-    ! 1. tails and potentials of every orbital of every job
-    n = 0
+    type(x2g_tail_item), allocatable :: tails(:)
+    type(x2g_normortho_item), allocatable :: no(:)
+    type(x2g_eahf_item), allocatable :: ea(:)
+    type(c_ptr), allocatable :: grids(:), recs(:)
+    integer(c_int), allocatable :: cnt(:)
+    integer, allocatable :: recq(:)
+    real(dp), allocatable :: rech(:)
+    logical, allocatable :: done(:)
+    type(lc_batch) :: lb
+    type(x2g_term) :: t1(8), tc(MAXO), tx(4*MAXO)
+    type(c_ptr) :: ctx
+    integer :: q, h, i, ii, n, nt, nmax, ntmax, ms1, ms2, maxnorb, mpmax, i2, i3, jor, nno, nea, nrec
+    integer :: n1, ntc, ntx, base
+    nmax = 0; ntmax = 0; maxnorb = 0; mpmax = 2
+    allocate(grids(nj))
     do q = 1, nj
        h = idx(q)
        grids(q) = js(h)%gd
        if (stopped(q)) cycle
+       nmax = nmax + sum(js(h)%c%npot(1:js(h)%c%norb)) + js(h)%c%norb
+       ntmax = ntmax + js(h)%c%norb + 2*js(h)%c%nslots
+       maxnorb = max(maxnorb, js(h)%c%norb)
+       mpmax = max(mpmax, js(h)%o%mpole)
+    enddo
+    if (nmax == 0) return
+    ctx = js(idx(1))%ctx
+    allocate(items(nmax), tails(max(1, ntmax)))
+
+    ! 1. tails and right-hand sides of the potentials of every orbital of every
+    ! job, then all potentials relaxed together
+    n = 0; nt = 0
+    do q = 1, nj
+       if (stopped(q)) cycle
+       h = idx(q)
        js(h)%iscf = js(h)%iscf + 1
        js(h)%iasympt = js(h)%c%norb
+       allocate(done(max(1, js(h)%c%nslots)))
+       done = .false.
        do ii = 1, js(h)%c%norb
           i = js(h)%c%norb - ii + 1
           js(h)%itouch(i) = 1
           if (js(h)%ifixorb(i) == 0) then
-             call pot_asympt(js(h), i)
+             call pot_tail_items(js(h), i, tails, nt, q-1, done)
              js(h)%iasympt = js(h)%iasympt - 1
           endif
-          call pot_items(js(h), i, items, n, q-1)
+          call pot_items(js(h), i, items, n, q-1, lb)
        enddo
+       deallocate(done)
     enddo
+    if (nt > 0) call x2check(x2g_pot_tail_batch(ctx, grids, tails, int(nt, c_int), int(mpmax, c_int)), &
+         'batched potAsympt')
+    call lc_run(lb, ctx, grids)
     ms1 = 0; ms2 = 0
     do q = 1, nj
        if (stopped(q)) cycle
        h = idx(q)
        ms1 = max(ms1, js(h)%o%maxsorpot(1)); ms2 = max(ms2, js(h)%o%maxsorpot(2))
     enddo
-    call x2check(x2g_relax_batch(js(idx(1))%ctx, grids, items, int(n, c_int), int(ms1, c_int), &
-         int(ms2, c_int)), 'batched potentials')
-    ! 2. Fock operators and all orbitals
+    call x2check(x2g_relax_batch(ctx, grids, items, int(n, c_int), int(ms1, c_int), int(ms2, c_int)), &
+         'batched potentials')
+
+    ! 2. Fock operators (Coulomb and exchange sums, then fock1, fock2 and the
+    ! relaxation lhs = fock1 + diag), then all orbitals relaxed together
+    do q = 1, nj
+       h = idx(q)
+       if (stopped(q) .or. js(h)%lfixorb) cycle
+       if (js(h)%c%nel == 1) cycle
+       do ii = 1, js(h)%c%norb
+          i = js(h)%c%norb - ii + 1
+          if (js(h)%ifixorb(i) == 1) cycle
+          call fock_terms(js(h), i, tc, ntc, tx, ntx, t1, n1)
+          call lc_add(lb, js(h)%cpot(i), c_null_ptr, 0.0_dp, tc, ntc, q-1)
+          call lc_add(lb, js(h)%xpot(i), c_null_ptr, 0.0_dp, tx, ntx, q-1)
+       enddo
+    enddo
+    call lc_run(lb, ctx, grids)
     n = 0
     ms1 = 0; ms2 = 0
     do q = 1, nj
@@ -831,27 +986,110 @@ contains
        do ii = 1, js(h)%c%norb
           i = js(h)%c%norb - ii + 1
           if (js(h)%ifixorb(i) == 1) cycle
-          call orb_item(js(h), i, items, n, q-1)
+          call fock_terms(js(h), i, tc, ntc, tx, ntx, t1, n1)
+          if (js(h)%c%nel == 1) then
+             call lc_add(lb, js(h)%f1o(i), js(h)%S(5), 1.0_dp, t1, n1, q-1)
+             call lc_add(lb, js(h)%f2o(i), c_null_ptr, 0.0_dp, t1, 0, q-1)
+          else
+             n1 = n1 + 1; t1(n1) = x2g_term(1.0_dp, js(h)%S(7), js(h)%cpot(i), c_null_ptr)
+             call lc_add(lb, js(h)%f1o(i), js(h)%S(5), 1.0_dp, t1, n1, q-1)
+             tc(1) = x2g_term(1.0_dp, js(h)%S(7), js(h)%xpot(i), c_null_ptr)
+             call lc_add(lb, js(h)%f2o(i), c_null_ptr, 0.0_dp, tc, 1, q-1)
+          endif
+          n1 = n1 + 1; t1(n1) = x2g_term(js(h)%g%diag, js(h)%ones, c_null_ptr, c_null_ptr)
+          call lc_add(lb, js(h)%lo(i), js(h)%S(5), 1.0_dp, t1, n1, q-1)
+          n = n + 1
+          items(n) = x2g_relax_item(js(h)%orb(i), js(h)%lo(i), js(h)%f2o(i), js(h)%f1o(i), js(h)%omega_orb, &
+               int(js(h)%c%isym(i), c_int), 0_c_int, int(js(h)%o%prec, c_int), int(q-1, c_int))
+          js(h)%nsweep_orb = js(h)%nsweep_orb + int(js(h)%o%maxsororb(1)*js(h)%o%maxsororb(2), c_long_long)
        enddo
        ms1 = max(ms1, js(h)%o%maxsororb(1)); ms2 = max(ms2, js(h)%o%maxsororb(2))
     enddo
-    if (n > 0) call x2check(x2g_relax_batch(js(idx(1))%ctx, grids, items, int(n, c_int), int(ms1, c_int), &
-         int(ms2, c_int)), 'batched orbitals')
-    ! 3. normalisation, symmetry, orthogonalisation, orbital energies, control
+    call lc_run(lb, ctx, grids)
+    if (n > 0) call x2check(x2g_relax_batch(ctx, grids, items, int(n, c_int), int(ms1, c_int), int(ms2, c_int)), &
+         'batched orbitals')
+
+    ! 3. normalisation, symmetry and orthogonalisation, orbital position by
+    ! orbital position (each orbital against the higher ones of its symmetry)
+    allocate(no(nj), ea(nj*maxnorb))
+    do ii = 1, maxnorb
+       nno = 0
+       do q = 1, nj
+          h = idx(q)
+          if (stopped(q) .or. js(h)%c%norb < ii) cycle
+          associate (j => js(h))
+            i = j%c%norb - ii + 1
+            if (j%ifixorb(i) /= 0 .and. .not. j%c%homonuclear) cycle
+            nno = nno + 1
+            no(nno) = x2g_normortho_item()
+            no(nno)%psi_d = j%orb(i)
+            no(nno)%normalise = merge(1_c_int, 0_c_int, j%ifixorb(i) == 0)
+            no(nno)%ihomo = 0
+            if (j%c%homonuclear) no(nno)%ihomo = int(j%c%ihomo(i), c_int)
+            no(nno)%grid_index = int(q-1, c_int)
+            no(nno)%norm_out_d = ptr_off(j%rec, 8*(i-1))
+            jor = 0
+            if (j%c%norb > 1 .and. j%ifixorb(i) == 0) then
+               do i2 = 1, j%c%norb
+                  i3 = j%c%norb - i2 + 1
+                  if (i3 <= i) cycle
+                  if (j%c%m(i3) /= j%c%m(i)) cycle
+                  if (j%c%ige(i3) /= j%c%ige(i)) cycle
+                  jor = jor + 1
+                  no(nno)%others_d(jor) = j%orb(i3)
+               enddo
+            endif
+            no(nno)%nothers = int(jor, c_int)
+          end associate
+       enddo
+       if (nno > 0) call x2check(x2g_normortho_batch(ctx, grids, no, int(nno, c_int)), 'batched normOrtho')
+       ! frozen orbitals refresh their Fock operators at this point of the order
+       do q = 1, nj
+          h = idx(q)
+          if (stopped(q) .or. js(h)%c%norb < ii) cycle
+          i = js(h)%c%norb - ii + 1
+          if (js(h)%ifixorb(i) /= 0) call fock_hf(js(h), i, js(h)%f1o(i), js(h)%f2o(i))
+       enddo
+    enddo
+
+    ! 4. orbital energies of every orbital of every job
+    nea = 0
+    do q = 1, nj
+       h = idx(q)
+       if (stopped(q)) cycle
+       do i = 1, js(h)%c%norb
+          nea = nea + 1
+          ea(nea) = x2g_eahf_item(js(h)%orb(i), js(h)%cpot(i), js(h)%xpot(i), int(js(h)%c%isym(i), c_int), &
+               int(js(h)%c%m(i), c_int), int(q-1, c_int), ptr_off(js(h)%rec, 8*(MAXO+i-1)))
+       enddo
+    enddo
+    call x2check(x2g_eahf_batch(ctx, grids, ea, int(nea, c_int)), 'batched EaHF')
+
+    ! 5. one transfer of the norms and orbital energies, then each job's
+    ! bookkeeping in the reference's orbital order
+    allocate(recs(nj), cnt(nj), recq(nj))
+    nrec = 0
     do q = 1, nj
        if (stopped(q)) cycle
+       nrec = nrec + 1
+       recs(nrec) = js(idx(q))%rec
+       cnt(nrec) = int(2*MAXO, c_int)
+       recq(nrec) = q
+    enddo
+    allocate(rech(2*MAXO*nrec))
+    call x2check(x2g_gather(ctx, recs, cnt, int(nrec, c_int), rech), 'gather')
+    do n = 1, nrec
+       q = recq(n)
        h = idx(q)
+       base = 2*MAXO*(n-1)
        do ii = 1, js(h)%c%norb
           i = js(h)%c%norb - ii + 1
-          if (js(h)%ifixorb(i) == 0) call norm(js(h), i)
-          call setci(js(h), i)
-          if (js(h)%c%norb > 1 .and. js(h)%ifixorb(i) == 0) call ortho(js(h), i)
-          if (js(h)%ifixorb(i) /= 0) call fock_hf(js(h), i, js(h)%f1o(i), js(h)%f2o(i))
-          call ea_hf(js(h), i)
+          if (js(h)%ifixorb(i) == 0) js(h)%orbnorm(i) = rech(base + i)
+          js(h)%ee(i,i) = rech(base + MAXO + i)
           call energy_change(js(h), i)
        enddo
        stopped(q) = convergence_control(js(h))
     enddo
-    deallocate(items, grids)
+    deallocate(items, tails, grids, no, ea, recs, cnt, recq, rech)
   end subroutine batched_iteration
 end module x2f_scf

@@ -38,6 +38,37 @@ class Dot(C.Structure):
     _fields_ = [("w_d", _vp), ("a_d", _vp), ("b_d", _vp), ("cc_d", _vp)]
 
 
+MAX_ORTHO = 16
+
+
+class LincombOp(C.Structure):
+    """x2g_lincomb_op"""
+    _fields_ = [("out_d", _vp), ("base_d", _vp), ("base_scale", _dbl), ("first_term", _int), ("nterms", _int),
+                ("grid_index", _int)]
+
+
+class NormOrthoItem(C.Structure):
+    """x2g_normortho_item"""
+    _fields_ = [("psi_d", _vp), ("others_d", _vp * MAX_ORTHO), ("nothers", _int), ("ihomo", _int),
+                ("normalise", _int), ("grid_index", _int), ("norm_out_d", _vp)]
+
+
+class EaHFItem(C.Structure):
+    """x2g_eahf_item"""
+    _fields_ = [("psi_d", _vp), ("cpot_d", _vp), ("xpot_d", _vp), ("isym", _int), ("m", _int),
+                ("grid_index", _int), ("ee_out_d", _vp)]
+
+
+class MomentItem(C.Structure):
+    """x2g_moment_item"""
+    _fields_ = [("psi1_d", _vp), ("psi2_d", _vp), ("dm", _int), ("grid_index", _int)]
+
+
+class TailItem(C.Structure):
+    """x2g_tail_item"""
+    _fields_ = [("pot_d", _vp), ("kind", _int), ("dm", _int), ("grid_index", _int), ("mom", _dbl * 8)]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "x2g_version": (_int, []),
@@ -70,6 +101,12 @@ _PROTOS = {
     "x2g_moments": (_int, [_vp, _vp, _vp, _int, _int, _vp]),
     "x2g_pot_tail": (_int, [_vp, _vp, _int, _int, _int, _vp]),
     "x2g_setci": (_int, [_vp, _vp, _int]),
+    "x2g_lincomb_batch": (_int, [_vp, _vp, C.POINTER(LincombOp), _int, C.POINTER(Term), _int]),
+    "x2g_normortho_batch": (_int, [_vp, _vp, C.POINTER(NormOrthoItem), _int]),
+    "x2g_eahf_batch": (_int, [_vp, _vp, C.POINTER(EaHFItem), _int]),
+    "x2g_moments_batch": (_int, [_vp, _vp, C.POINTER(MomentItem), _int, _int, _vp]),
+    "x2g_pot_tail_batch": (_int, [_vp, _vp, C.POINTER(TailItem), _int, _int]),
+    "x2g_gather": (_int, [_vp, _vp, _vp, _int, _vp]),
 }
 
 _lib = None
