@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MCSOR relaxation path on one MI355X.
+
+Workload: closed-shell N2 (Z1 = Z2 = 7, R = 2.068 bohr, 6 orbitals: 1 pi-u,
+5 sigma), prolate spheroidal grid 331 x 601, R_inf = 40 bohr, hydrogenic LCAO
+start, SCF to orbital-energy changes below 1e-12 (the reference's own N2
+deck, tests/golden/decks/config2_n2_331x601.data; 337 x 601 rounds to 331 in
+the reference).  Every SCF iteration relaxes 22 Coulomb/exchange potentials
+and 6 orbitals with 10 MCSOR sweeps each.
+
+One step = one wave of --batch requests of that workload served to
+convergence by the continuous-batching server (x2dhf_amd.serve): FP32 MCSOR
+correction sweeps with FP64 residuals, normalisation, orthogonalisation,
+Lagrange multipliers and energies, all jobs of the wave stepped together.
+value = converged solves per hour over the timed steps.  Reported beside it:
+the plain FP64 GPU path (reference orbital order, one job, FP64 sweeps) on the
+same device, p50/p99 time-to-converged-energy, |E - E_ref| against the
+reference's FP64 CPU energy, the roofline of the dominant sweep kernel from
+HIP events around its launches, and the reference's own CPU build timed on
+this host's cores.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DECK = os.path.join(HERE, "tests", "golden", "decks", "config2_n2_331x601.data")
+GOLDEN = os.path.join(HERE, "tests", "golden", "scf_config2_n2_331x601.json")
+REF_EXE = os.path.join(HERE, "oracle", "_ref", "x2dhf_omp")
+TRAFFIC = os.path.join(HERE, "profiles", "traffic_sweep_n2_331x601.json")
+HBM_PEAK_GBS = 8000.0
+METRIC = "converged SCF solves/hr + p50/p99 latency vs plain FP64 GPU at ΔE<1e-8 Eh; GB/s"
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1, help="timed waves")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed waves")
+    ap.add_argument("--batch", type=int, default=16, help="requests per wave (all admitted together)")
+    ap.add_argument("--prec", type=int, default=1, choices=(0, 1), help="1: FP32 correction sweeps, 0: FP64")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU solve")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
+    ap.add_argument("--cpu-iters", default="61,81", help="SCF iterations of the two timed reference runs")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run a copy kernel of known size and a few sweeps (for rocprofv3 --pmc)")
+    return ap.parse_args()
+
+
+def n2_spec(prec, schedule, maxscf=8000):
+    from x2dhf_amd import scf
+    return scf.n2_spec(331, 601, 40.0, ienterm=12, inoterm=12, maxscf=maxscf, prec=prec, schedule=schedule)
+
+
+def plain_fp64():
+    """One solve of the plain FP64 GPU path (reference order, FP64 sweeps)."""
+    from x2dhf_amd import scf
+    job = scf.Job(n2_spec(0, 0))
+    try:
+        t0 = time.perf_counter()
+        res = job.run()
+        return time.perf_counter() - t0, res
+    finally:
+        job.close()
+
+
+def cpu_baseline(total_iters, iters):
+    """The reference program (built from its sources by oracle/ref.mk) on the
+    same deck: two runs of k1 < k2 SCF iterations (both past the frozen-orbital
+    start), per-iteration time from their difference, extrapolated to the
+    reference's own iteration count of the converged run."""
+    k1, k2 = iters
+    threads = min(16, os.cpu_count() or 1)
+    sample = (f"reference x2dhf (OpenMP build of its own sources) on the N2 331x601 deck, runs of {k1} and {k2} "
+              f"SCF iterations; per-iteration time x {total_iters} iterations of its converged run + set-up")
+    if not os.path.exists(REF_EXE):
+        return {"value": None, "unit": "solves/hr", "cores": threads, "kind": "reference",
+                "sample": "reference binary not built here (oracle/_ref/x2dhf_omp)"}
+    deck = open(DECK).read()
+    times = []
+    with tempfile.TemporaryDirectory() as d:
+        for k in (k1, k2):
+            path = os.path.join(d, f"deck{k}")
+            with open(path, "w") as f:
+                f.write(deck.replace("scf 8000 ", f"scf {k} "))
+            t0 = time.perf_counter()
+            with open(path) as fin, open(os.path.join(d, f"out{k}.lst"), "w") as fout:
+                subprocess.run([REF_EXE], stdin=fin, stdout=fout, stderr=subprocess.STDOUT, cwd=d, check=True,
+                               env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+            times.append(time.perf_counter() - t0)
+            print(f"# cpu baseline: {k} iterations in {times[-1]:.2f} s ({threads} threads)", file=sys.stderr,
+                  flush=True)
+    per_it = (times[1] - times[0]) / (k2 - k1)
+    setup = max(times[0] - k1 * per_it, 0.0)
+    solve = setup + total_iters * per_it
+    return {"value": 3600.0 / solve, "unit": "solves/hr", "cores": threads, "kind": "reference",
+            "sample": sample, "seconds_per_iteration": per_it, "estimated_solve_s": solve}
+
+
+def traffic_probe():
+    """For the rocprofv3 counter runs: a copy kernel of known size (x2g_lincomb
+    out = 1.0*a on a 1201 x 1201 grid: 11.5 MB read + 11.5 MB written per
+    launch, three launches), then one relaxation call (5 sweeps) in FP32 and
+    one in FP64 of the 22 potentials of an N2 331x601 iteration."""
+    import numpy as np
+    from x2dhf_amd import gpu, scf
+    ctx = gpu.Context(0)
+    rng = np.random.default_rng(0)
+    cb = scf.grid_build(1201, 1201, 7.0, 7.0, 2.068, 40.0)
+    big = gpu.Grid(ctx, 1201, 1201, *cb)
+    a, out = ctx.upload(rng.standard_normal(big.n)), ctx.alloc(big.n)
+    copy = (gpu.Term * 1)(gpu.Term(1.0, a.ptr, None, None))
+    for _ in range(3):
+        gpu.check(ctx.lib.x2g_lincomb(big.h, out.ptr, None, 0.0, copy, 1))
+    gpu.check(ctx.lib.x2g_ctx_sync(ctx.h))
+    coef, suppl, vxi, veta = scf.grid_build(331, 601, 7.0, 7.0, 2.068, 40.0)
+    g = gpu.Grid(ctx, 331, 601, coef, suppl, vxi, veta)
+    gh = (gpu._vp * 1)(g.h.value)
+    items, keep = [], []
+    for k in range(22):
+        f, l, r = ctx.upload(0.01 * rng.standard_normal(g.n)), ctx.upload(suppl[7]), ctx.upload(
+            rng.standard_normal(g.n))
+        keep += [f, l, r]
+        items.append(gpu.RelaxItem(f.ptr, l.ptr, r.ptr, None, 1.9, 1, 1, 1, 0))
+    gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, (gpu.RelaxItem * 22)(*items), 22, 1, 5))
+    for it in items:
+        it.prec = 0
+    gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, (gpu.RelaxItem * 22)(*items), 22, 1, 5))
+    gpu.check(ctx.lib.x2g_ctx_sync(ctx.h))
+    g.destroy()
+    big.destroy()
+    ctx.close()
+    print("traffic probe done", flush=True)
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (use --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe()
+        return
+    import ctypes as C
+    from x2dhf_amd import gpu, serve
+    E_ref = json.load(open(GOLDEN))["etot"]
+    ref_iters = len(json.load(open(GOLDEN))["iterations"])
+
+    plain = None
+    if not args.no_plain:
+        t_plain, res = plain_fp64()
+        plain = {"solve_s": t_plain, "solves_per_hr": 3600.0 / t_plain, "iterations": res["iterations"],
+                 "abs_dE": abs(res["etot"] - E_ref)}
+        print(f"# plain FP64 GPU: {t_plain:.2f} s, {res['iterations']} iterations, |dE| {plain['abs_dE']:.2e}",
+              file=sys.stderr, flush=True)
+
+    server = serve.Server(max_active=args.batch)
+    lib = server.lib
+
+    def wave():
+        t = server.now()
+        reqs = [server.submit(n2_spec(args.prec, 1), arrival=t) for _ in range(args.batch)]
+        server.run()
+        return reqs
+
+    for w in range(args.warmup):
+        wave()
+        print(f"# warmup wave {w + 1} done", file=sys.stderr, flush=True)
+    gpu.check(lib.x2g_ctx_sync(server.ctx.h))
+    gpu.check(lib.x2g_stats_enable(server.ctx.h, 1))
+    t0 = time.perf_counter()
+    timed = []
+    for s in range(args.steps):
+        timed += wave()
+        print(f"# timed wave {s + 1} done", file=sys.stderr, flush=True)
+    gpu.check(lib.x2g_ctx_sync(server.ctx.h))
+    elapsed = time.perf_counter() - t0
+    st = (C.c_double * 6)()
+    gpu.check(lib.x2g_stats_get(server.ctx.h, st))
+    server.close()
+
+    lat = [r.latency for r in timed]
+    dE = max(abs(r.result["etot"] - E_ref) for r in timed)
+    value = len(timed) / elapsed * 3600.0
+    k = 1 if st[4] >= st[1] else 0            # dominant sweep precision by time
+    launches, ms, nbytes = st[3 * k], st[3 * k + 1], st[3 * k + 2]
+    avg_us = ms / launches * 1e3 if launches else None
+    per_launch = nbytes / launches if launches else None
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms else None
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        tr = json.load(open(TRAFFIC))
+        traffic = tr.get("fp32" if k == 1 else "fp64", {}).get("hbm_bytes_per_launch")
+    cpu = None if args.no_cpu else cpu_baseline(ref_iters, [int(x) for x in args.cpu_iters.split(",")])
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/hr",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.prec == 1 else "f64",
+        "data": "synthetic: the reference's N2 deck (hydrogenic LCAO start), identical requests",
+        "config": {"workload": "N2 R=2.068 bohr, 6 closed-shell orbitals, grid 331x601, R_inf=40, "
+                               "HF SCF to orbital-energy change < 1e-12 (10 MCSOR sweeps per relaxation)",
+                   "requests_per_step": args.batch, "served": "continuous batching, all requests of a wave "
+                   "arrive together", "precision": "FP32 MCSOR correction sweeps, FP64 residual/norm/"
+                   "orthogonalisation/Lagrange multipliers/energy" if args.prec else "FP64"},
+        "p50_latency_s": serve.percentile(lat, 50),
+        "p99_latency_s": serve.percentile(lat, 99),
+        "max_abs_dE_vs_fp64_oracle": dE,
+        "plain_fp64_gpu": plain,
+        "speedup_vs_plain_fp64_gpu": value / plain["solves_per_hr"] if plain else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "kernel": "k_sweep<float>" if k == 1 else "k_sweep<double>",
+                     "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": avg_us,
+                     "launches": launches},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,8 @@ int x2g_ctx_destroy(x2g_ctx *ctx);
 int x2g_ctx_sync(x2g_ctx *ctx);
 /* Return the context's hipStream_t (as void*). */
 void *x2g_ctx_stream(x2g_ctx *ctx);
+/* Free and total device memory of the context's GPU (bytes). */
+int x2g_mem_info(x2g_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
 /* Time every sweep launch of this context with HIP events on its stream
  * (on != 0 starts and clears the counters, 0 stops). */
 int x2g_stats_enable(x2g_ctx *ctx, int on);

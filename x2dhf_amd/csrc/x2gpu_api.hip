@@ -228,6 +228,14 @@ extern "C" int x2g_ctx_sync(x2g_ctx *ctx)
 
 extern "C" void *x2g_ctx_stream(x2g_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
+extern "C" int x2g_mem_info(x2g_ctx *ctx, size_t *free_bytes, size_t *total_bytes)
+{
+    X2G_CHECK(ctx && free_bytes && total_bytes, "x2g_mem_info: null argument");
+    X2G_HIP(hipSetDevice(ctx->device));
+    X2G_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return X2G_OK;
+}
+
 extern "C" int x2g_malloc(x2g_ctx *ctx, size_t bytes, void **dptr)
 {
     X2G_CHECK(ctx && dptr, "x2g_malloc: null argument");

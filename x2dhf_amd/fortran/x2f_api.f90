@@ -66,8 +66,6 @@ contains
     type(x2f_spec), intent(in) :: spec
     integer(c_int), value :: share
     integer(c_int), intent(out) :: handle
-    integer :: h, i, ierr
-    real(dp) :: s
     type(c_ptr) :: ctx_in
     handle = -1
     ctx_in = c_null_ptr
@@ -78,6 +76,54 @@ contains
        endif
        ctx_in = jobs(share)%ctx
     endif
+    rc = create_job(spec, ctx_in, handle)
+  end function x2f_job_create
+
+  ! Create a job on an x2g_ctx owned by the caller (it must outlive the job);
+  ! jobs on one context can be stepped together.  Errors as x2f_job_create.
+  integer(c_int) function x2f_job_create_ctx(spec, ctx, handle) bind(c, name='x2f_job_create_ctx') result(rc)
+    type(x2f_spec), intent(in) :: spec
+    type(c_ptr), value :: ctx
+    integer(c_int), intent(out) :: handle
+    handle = -1
+    if (.not. c_associated(ctx)) then
+       rc = 4; return
+    endif
+    if (.not. allocated(jobs)) allocate(jobs(MAXJOBS))
+    rc = create_job(spec, ctx, handle)
+  end function x2f_job_create_ctx
+
+  ! Device bytes a job of this specification holds (its arrays; the shared
+  ! library workspace is not included), without creating it; -1 for a
+  ! specification that x2f_job_create would reject.
+  integer(c_long_long) function x2f_spec_bytes(spec) bind(c, name='x2f_spec_bytes')
+    type(x2f_spec), intent(in) :: spec
+    type(case_t) :: c
+    integer :: i, ierr, npotall
+    integer(c_long_long) :: n
+    x2f_spec_bytes = -1
+    if (mod(spec%nni - 1, 6) /= 0 .or. mod(spec%nmu - 1, 6) /= 0 .or. spec%nni < 13 .or. spec%nmu < 13) return
+    c = case_t()
+    c%norb = spec%norb
+    c%z1 = spec%z1; c%z2 = spec%z2; c%r = spec%r
+    do i = 1, min(int(spec%norb), MAXO)
+       c%m(i) = spec%m(i); c%gu(i) = spec%gu(i); c%occ(i) = spec%occ(i)
+    enddo
+    ierr = case_setup(c)
+    if (ierr /= 0) return
+    npotall = sum(c%npot(1:c%norb))
+    n = int(spec%nni, c_long_long)*int(spec%nmu, c_long_long)
+    ! the arrays of job_init (orbitals, potentials, Fock parts, lhs/rhs of
+    ! every potential, ones, record) plus the 14 grid coefficient arrays
+    x2f_spec_bytes = 8_c_long_long*n*int(7*c%norb + c%nslots + 2*npotall + 2 + 14, c_long_long)
+  end function x2f_spec_bytes
+
+  integer(c_int) function create_job(spec, ctx_in, handle) result(rc)
+    type(x2f_spec), intent(in) :: spec
+    type(c_ptr), intent(in) :: ctx_in
+    integer(c_int), intent(out) :: handle
+    integer :: h, i, ierr
+    real(dp) :: s
     h = 0
     do i = 1, MAXJOBS
        if (.not. used(i)) then
@@ -119,7 +165,7 @@ contains
       handle = h
       rc = 0
     end associate
-  end function x2f_job_create
+  end function create_job
 
   logical function valid(handle)
     integer(c_int), intent(in) :: handle

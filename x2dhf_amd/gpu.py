@@ -77,6 +77,7 @@ _PROTOS = {
     "x2g_ctx_destroy": (_int, [_vp]),
     "x2g_ctx_sync": (_int, [_vp]),
     "x2g_ctx_stream": (_vp, [_vp]),
+    "x2g_mem_info": (_int, [_vp, C.POINTER(_szt), C.POINTER(_szt)]),
     "x2g_stats_enable": (_int, [_vp, _int]),
     "x2g_stats_get": (_int, [_vp, _vp]),
     "x2g_malloc": (_int, [_vp, _szt, C.POINTER(_vp)]),

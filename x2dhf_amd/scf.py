@@ -75,6 +75,10 @@ def load():
     L.x2f_job_bytes.restype = C.c_longlong
     L.x2f_job_ctx.argtypes = [_i]
     L.x2f_job_ctx.restype = vp
+    L.x2f_job_create_ctx.argtypes = [C.POINTER(JobSpec), vp, C.POINTER(_i)]
+    L.x2f_job_create_ctx.restype = _i
+    L.x2f_spec_bytes.argtypes = [C.POINTER(JobSpec)]
+    L.x2f_spec_bytes.restype = C.c_longlong
     _lib = L
     return L
 
@@ -152,15 +156,29 @@ def fh_spec(nni=481, nmu=901, rinf=50.0, r=1.733, **kw):
     return make_spec(9.0, 1.0, r, nni, nmu, rinf, orbs, **kw)
 
 
+def spec_bytes(spec):
+    """Device bytes a job of this specification holds (its own arrays), without
+    creating it; raises ValueError for a specification the driver rejects."""
+    b = int(load().x2f_spec_bytes(C.byref(spec)))
+    if b < 0:
+        raise ValueError("unsupported job specification")
+    return b
+
+
 class Job:
     """One SCF job on the GPU.  share: another Job whose device context this
-    job uses (jobs stepped together by iterate_batched must share one)."""
+    job uses; ctx: an x2gpu context (gpu.Context) owned by the caller.  Jobs
+    stepped together by iterate_batched must use one context."""
 
-    def __init__(self, spec, share=None):
+    def __init__(self, spec, share=None, ctx=None):
         self.L = load()
         self.spec = spec
         h = _i()
-        rc = self.L.x2f_job_create(C.byref(spec), share.h if share is not None else 0, C.byref(h))
+        if ctx is not None:
+            rc = self.L.x2f_job_create_ctx(C.byref(spec), ctx.h.value if hasattr(ctx.h, "value") else ctx.h,
+                                           C.byref(h))
+        else:
+            rc = self.L.x2f_job_create(C.byref(spec), share.h if share is not None else 0, C.byref(h))
         if rc != 0:
             raise ValueError(f"x2f_job_create failed with code {rc}")
         self.h = h.value
