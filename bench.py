@@ -101,11 +101,12 @@ def cpu_baseline(total_iters, iters):
             "sample": sample, "seconds_per_iteration": per_it, "estimated_solve_s": solve}
 
 
-def traffic_probe():
+def traffic_probe(batch):
     """For the rocprofv3 counter runs: a copy kernel of known size (x2g_lincomb
     out = 1.0*a on a 1201 x 1201 grid: 11.5 MB read + 11.5 MB written per
     launch, three launches), then one relaxation call (5 sweeps) in FP32 and
-    one in FP64 of the 22 potentials of an N2 331x601 iteration."""
+    one in FP64 of --batch x 22 potentials on N2 331x601 grids: the sweep
+    launches of --batch jobs' potential relaxation in the benchmark."""
     import numpy as np
     from x2dhf_amd import gpu, scf
     ctx = gpu.Context(0)
@@ -118,20 +119,23 @@ def traffic_probe():
         gpu.check(ctx.lib.x2g_lincomb(big.h, out.ptr, None, 0.0, copy, 1))
     gpu.check(ctx.lib.x2g_ctx_sync(ctx.h))
     coef, suppl, vxi, veta = scf.grid_build(331, 601, 7.0, 7.0, 2.068, 40.0)
-    g = gpu.Grid(ctx, 331, 601, coef, suppl, vxi, veta)
-    gh = (gpu._vp * 1)(g.h.value)
+    gs = [gpu.Grid(ctx, 331, 601, coef, suppl, vxi, veta) for _ in range(batch)]
+    gh = (gpu._vp * batch)(*[g.h.value for g in gs])
     items, keep = [], []
-    for k in range(22):
-        f, l, r = ctx.upload(0.01 * rng.standard_normal(g.n)), ctx.upload(suppl[7]), ctx.upload(
-            rng.standard_normal(g.n))
-        keep += [f, l, r]
-        items.append(gpu.RelaxItem(f.ptr, l.ptr, r.ptr, None, 1.9, 1, 1, 1, 0))
-    gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, (gpu.RelaxItem * 22)(*items), 22, 1, 5))
+    lhs = ctx.upload(suppl[7])
+    for q in range(batch):
+        for k in range(22):
+            f, r = ctx.upload(0.01 * rng.standard_normal(gs[q].n)), ctx.upload(rng.standard_normal(gs[q].n))
+            keep += [f, r]
+            items.append(gpu.RelaxItem(f.ptr, lhs.ptr, r.ptr, None, 1.9, 1, 1, 1, q))
+    n = len(items)
+    gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, (gpu.RelaxItem * n)(*items), n, 1, 5))
     for it in items:
         it.prec = 0
-    gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, (gpu.RelaxItem * 22)(*items), 22, 1, 5))
+    gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, (gpu.RelaxItem * n)(*items), n, 1, 5))
     gpu.check(ctx.lib.x2g_ctx_sync(ctx.h))
-    g.destroy()
+    for g in gs:
+        g.destroy()
     big.destroy()
     ctx.close()
     print("traffic probe done", flush=True)
@@ -145,7 +149,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.traffic_probe:
-        traffic_probe()
+        traffic_probe(args.batch)
         return
     import ctypes as C
     from x2dhf_amd import gpu, serve

@@ -52,6 +52,7 @@ struct SweepDesc {
     T c[16];  // dmu1[4], dmu2[4], dni1[4], dni2[4]
     T ex[5];
     T omega, omega1;
+    T *trash;  // k_sweep: target of the stores of inactive lanes (kTrashRegions x 512 values, never read)
     int nni, nmu, isym, cA;
     int wg_begin, L;
 };
@@ -166,6 +167,8 @@ __global__ void k_boundary(const SweepDesc<T> *__restrict__ descs)
 // step s blocks s-4*SKEW-1 .. s+1 are live and block s+2 is prefetched
 // through registers while the step computes.
 constexpr int kMaxLoads = 8;  // prefetch registers per thread (4*(nni+8) <= 8*blockDim)
+// k_sweep copies a block with NS = 5 (one colour point per lane and row) or
+// 8 (two) slots per thread: 4*(nni+8) <= NS*blockDim
 
 template <typename T, int SKEW>
 __global__ void __launch_bounds__(512) k_strip(const SweepDesc<T> *__restrict__ descs, int nprob)
@@ -353,6 +356,7 @@ template <typename TO>
 struct PackDesc {
     const double *src;  // unpadded nni*nmu array
     TO *dst;            // residue-split array
+    double omega;       // > 0: store omega/src (the scaled inverse diagonal), else src
     int nni, nmu;
 };
 
@@ -364,7 +368,8 @@ __global__ void k_pack(const PackDesc<TO> *__restrict__ pd)
     const long npts = (long)nrow * (d.nmu - 5);
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < npts; k += (long)gridDim.x * blockDim.x) {
         const int mu = 2 + (int)(k / nrow), nu = 2 + (int)(k % nrow);
-        d.dst[split_index(nu, mu, M)] = (TO)d.src[(long)(mu - 1) * nni + (nu - 1)];
+        const double v = d.src[(long)(mu - 1) * nni + (nu - 1)];
+        d.dst[split_index(nu, mu, M)] = (TO)(d.omega > 0.0 ? d.omega / v : v);
     }
 }
 
@@ -372,17 +377,23 @@ __global__ void k_pack(const PackDesc<TO> *__restrict__ pd)
 // k_sweep: the pipelined strip sweep (same block schedule as k_strip with
 // SKEW = 2) with a fixed lane -> work map.  Lane tid < 5*TG serves colour
 // pass p = tid / TG and the colour points t = tid % TG + j*TG (j < NT) of each
-// of the four rows of its current block, so the row offsets are compile-time
-// constants and the ring addresses follow from three slot bases per step.
+// of the four rows of its current block.  The ring rows have a compile-time
+// stride W >= nni+16, so a point reads its 17 stencil values with immediate
+// offsets from three block bases, and every point of a step is processed
+// without branches: points outside the pass's block range or the grid are
+// computed on in-bounds garbage and written to an LDS sink.
 // lhs / rhs come from the residue-split layout (contiguous per pass and row)
-// one step ahead of their use; b(mu) and d(nu) are staged in LDS.
+// ahead of their use.  The split "lhs" array holds omega/lhs, so a point
+// update is  f = (rhs - L f) * (omega/lhs) + (1-omega) f;  the d/dmu part of
+// L f uses per-row weights  c2_k +- b(mu) c1_k  (k = 1..4).
 //
-// LDS: ring of 12 blocks of 4 rows (nni+8 values each), then d(nu) [nni+8],
-// b(mu) for the strip's rows [L+32] and the coefficients [18].
-template <typename T>
-__host__ __device__ inline size_t sweep_lds_elems(int nni, int L)
+// LDS (elements of T): d(nu) at index nu-1 [W] (so a point's column base also
+// addresses its d(nu)), the row weights of the ring's rows [48 x 8, written
+// with each block], the coefficients [32], the ring of 12 blocks of 4 rows
+// [48 W], the sink [one per thread].  Independent of the strip height.
+__host__ __device__ inline size_t sweep_lds_elems(int W, int nth)
 {
-    return (size_t)12 * 4 * (nni + 8) + (nni + 8) + (L + 32) + 18;
+    return (size_t)W + 48 * 8 + 32 + (size_t)48 * W + (size_t)nth;
 }
 
 // Global-address-space views of descriptor pointers: loads through them are
@@ -395,12 +406,14 @@ __device__ __forceinline__ const gmem<T> *as_global(const T *p) { return (const 
 template <typename T>
 __device__ __forceinline__ gmem<T> *as_global(T *p) { return (gmem<T> *)p; }
 
-template <typename T, int NT>
-__global__ void __launch_bounds__(512) k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
+template <typename T, int NT, int PIPE, int W>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && W <= 352 ? 4 : 1)))
+k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
 {
     extern __shared__ unsigned char smem_raw[];
     constexpr int SKEW = 2, RB = 4 * SKEW + 4;
-    T *ring = reinterpret_cast<T *>(smem_raw);
+    constexpr int NS = NT == 1 ? 5 : 8;  // block-copy / write-out slots per thread
+    T *const dnu_s = reinterpret_cast<T *>(smem_raw);
 
     const int wg = blockIdx.x;
     int lo = 0, hi = nprob - 1;
@@ -424,35 +437,56 @@ __global__ void __launch_bounds__(512) k_sweep(const SweepDesc<T> *__restrict__ 
     gmem<T> *__restrict__ dst = as_global(d.dst);
     const gmem<T> *__restrict__ lhs = as_global(d.lhs);
     const gmem<T> *__restrict__ rhs = as_global(d.rhs);
-    const gmem<T> *__restrict__ dnu_g = as_global(d.dnu);
+    T *const ab_r = dnu_s + W;                 // row weights, 8 per ring row
+    T *const coef_s = ab_r + 48 * 8;           // c[8..15] (nu weights), 1-omega, c[0..7]
+    T *const ring = coef_s + 32;
+    T *const sink = ring + 48 * W;
+
+    auto block_row = [&](int b, int rr) -> T * { return ring + (((b + RB) % RB) * 4 + rr) * W; };
+
+    // Row weights of row mu: distance k = 4,3,2,1 -> c2 = c[4..7], c1 = c[0..3];
+    // ab[0..3] = c2 + b(mu) c1 (rows mu-k), ab[4..7] = c2 - b(mu) c1 (rows mu+k).
+    // Thread tid < 32 forms entry tid & 7 of row tid >> 3 of a block.
     const gmem<T> *__restrict__ bmu_g = as_global(d.bmu);
-    T *dnu_s = ring + RB * 4 * N8;
-    T *bmu_s = dnu_s + N8;
-    T *coef_s = bmu_s + L + 32;  // c[16], omega, omega1
-
-    auto block_row = [&](int b, int rr) -> T * { return ring + (long)(((b + RB) % RB) * 4 + rr) * N8; };
-    auto load_block_direct = [&](int b) {
-        T *base = block_row(b, 0);
-        for (int e = tid; e < 4 * N8; e += nth) {
-            const int rr = e / N8, col = e - rr * N8;
-            const int mu = r0 + 4 * b + rr;
-            T v = T(0);
-            if (mu >= -3 && mu <= nmu + 4) v = src[(long)(mu + 3) * N8 + col];
-            base[(long)rr * N8 + col] = v;
+    // this thread's two weights, loaded once (a per-lane load of the
+    // descriptor inside the step loop would break the exact wait counts)
+    const T rw_c2 = d.c[4 + (tid & 3)], rw_c1 = d.c[tid & 3];
+    auto row_weight = [&](T b) -> T { return (tid & 7) < 4 ? rw_c2 + b * rw_c1 : rw_c2 - b * rw_c1; };
+    // prologue: coefficients, d(nu), blocks -1, 0, 1 and their row weights
+    if (tid < 8) coef_s[tid] = d.c[8 + tid];
+    if (tid == 8) coef_s[8] = d.omega1;
+    {
+        const gmem<T> *dnu_g = as_global(d.dnu);
+        for (int e = tid; e < nni; e += nth) dnu_s[e] = dnu_g[e];
+        for (int b = -1; b <= 1; ++b) {
+            if (tid < 32) {
+                const int mu = r0 + 4 * b + (tid >> 3);
+                const T bv = (mu >= 1 && mu <= nmu) ? bmu_g[mu - 1] : T(0);
+                ab_r[((b + RB) % RB) * 32 + tid] = row_weight(bv);
+            }
         }
-    };
-
-    if (tid < 16) coef_s[tid] = d.c[tid];
-    if (tid == 16) coef_s[16] = d.omega;
-    if (tid == 17) coef_s[17] = d.omega1;
-    for (int e = tid; e < nni; e += nth) dnu_s[e] = dnu_g[e];
-    for (int e = tid; e < L + 32; e += nth) {
-        const int mu = r0 + e;
-        bmu_s[e] = (mu >= 1 && mu <= nmu) ? bmu_g[mu - 1] : T(0);
+        for (int b = -1; b <= 1; ++b) {
+            T *base = block_row(b, 0);
+            for (int e = tid; e < 4 * N8; e += nth) {
+                const int rr = e / N8, col = e - rr * N8;
+                const int mu = r0 + 4 * b + rr;
+                T v = T(0);
+                if (mu >= -3 && mu <= nmu + 4) v = src[(long)(mu + 3) * N8 + col];
+                base[rr * W + col] = v;
+            }
+        }
     }
-    load_block_direct(-1);
-    load_block_direct(0);
-    load_block_direct(1);
+
+    // per-thread slots of the block copies (element e = tid + j*nth of the
+    // 4 x N8 block) and of the write-out (element e of the 4 x nni rows):
+    // the row of slot j in bits 3j..3j+2 (4 = no element), computed once
+    unsigned lrows = 0, wrows = 0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        const int e = tid + j * nth;
+        lrows |= (unsigned)(e < 4 * N8 ? e / N8 : 4) << (3 * j);
+        wrows |= (unsigned)(e < 4 * nni ? e / nni : 4) << (3 * j);
+    }
 
     // lane role
     const int p = tid / TG;  // >= 5: lane only loads and stores
@@ -460,189 +494,283 @@ __global__ void __launch_bounds__(512) k_sweep(const SweepDesc<T> *__restrict__ 
     const bool worker = p < 5;
     int sb = (RB - 2 * p % RB) % RB;                  // ring slot of block s - 2p
     int off0 = mod5(p - cA * (r0 - 8 * p - 2));        // (nu-2) mod 5 of pass p in row r0 + 4(s - 2p)
+    const long M5 = 5L * M;
 
     auto rows_active = [&](int s) {
         const int b = s - SKEW * p;
         return worker && b >= p && b <= K - 1 - p;
     };
-    // lhs / rhs of the points of step s (off0s: colour offset of its first row)
+    // lhs / rhs of the points of step s (off0s: colour offset of its first
+    // row).  PIPE 2: every lane issues every load (inactive ones read element
+    // 0), so each wave issues the same memory operations in every step and
+    // the compiler's waits count exactly the loads they need.  An inactive
+    // point's values are never used, so they are not masked.
     auto load_lr = [&](int s, int off0s, T (&lv)[4][NT], T (&rv)[4][NT]) {
         const bool act = rows_active(s);
         const int mu0 = r0 + 4 * (s - SKEW * p);
+        const long i0 = (long)(mu0 - 2) * M5 + tl;
         int off = off0s;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
             if (rr) off = off - cA < 0 ? off - cA + 5 : off - cA;
             const int mu = mu0 + rr;
             const bool rowok = act && mu >= 2 && mu <= nmu - 4;
-            const long rbase = (long)(mu - 2) * 5 * M + (long)off * M;
+            const long ib = i0 + rr * M5 + (long)off * M;
 #pragma unroll
             for (int j = 0; j < NT; ++j) {
                 const int t = tl + j * TG;
-                T l = T(1), r = T(0);
-                if (rowok && 2 + off + 5 * t <= nni - 1) {
-                    l = lhs[rbase + t];
-                    r = rhs[rbase + t];
+                const bool ok = rowok && 2 + off + 5 * t <= nni - 1;
+                if (PIPE == 2) {
+                    const long idx = ok ? ib + j * TG : 0;
+                    lv[rr][j] = lhs[idx];
+                    rv[rr][j] = rhs[idx];
+                } else if (ok) {
+                    lv[rr][j] = lhs[ib + j * TG];
+                    rv[rr][j] = rhs[ib + j * TG];
                 }
-                lv[rr][j] = l;
-                rv[rr][j] = r;
             }
         }
     };
 
-    T lv[4][NT], rv[4][NT];
-    load_lr(0, off0, lv, rv);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
+    __syncthreads();  // the prologue's LDS contents are in place
     // per-lane copies of the stencil coefficients (read from LDS, so they
     // live in vector registers rather than being re-fetched as scalars)
-    T cf[16];
+    T cf[8];  // c[8..15]: d/dnu (c[8..11]) and d2/dnu2 (c[12..15]) weights
 #pragma unroll
-    for (int q = 0; q < 16; ++q) cf[q] = coef_s[q];
-    const T om = coef_s[16], om1 = coef_s[17];
+    for (int q = 0; q < 8; ++q) cf[q] = coef_s[q];
+    const T om1 = coef_s[8];
 
-    for (int s = 0; s < S; ++s) {
-        // 1. lhs / rhs of step s+1 and block s+2 into registers (in this order:
-        // the wait before the commit below then covers both, and the compute
-        // of the next step never waits on loads issued in its own step)
-        const int off0n = off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA;
-        T ln[4][NT], rn[4][NT];
-        load_lr(s + 1, off0n, ln, rn);
-        const int bl = s + 2;
-        T pre[kMaxLoads];
-        const bool do_load = bl <= K;
-        if (do_load) {
-#pragma unroll
-            for (int j = 0; j < kMaxLoads; ++j) {
-                const int e = tid + j * nth;
-                T v = T(0);
-                if (e < 4 * N8) {
-                    const int rr = e / N8, col = e - rr * N8;
-                    const int mu = r0 + 4 * bl + rr;
-                    if (mu >= -3 && mu <= nmu + 4) v = src[(long)(mu + 3) * N8 + col];
-                }
-                pre[j] = v;
+    // Block copies and write-out.  PIPE 2: every lane issues every load and
+    // store (out-of-range loads read a clamped address and are discarded,
+    // stores of inactive lanes go to the context's sink), for the same reason.
+    gmem<T> *__restrict__ trash = as_global(d.trash) + (long)(wg % kTrashRegions) * 512;
+    // the block's values, plus b(mu) of row tid >> 3 of the block for the
+    // row weights (threads tid < 32 use it)
+    auto load_block = [&](int bl, T (&pr)[NS], T &bv) {
+        const int mub = r0 + 4 * bl;
+        const long base = (long)(mub + 3) * N8;
+        {
+            const int mu = mub + ((tid >> 3) & 3);
+            const bool ok = tid < 32 && bl <= K && mu >= 1 && mu <= nmu;
+            if (PIPE == 2) {
+                const T v = bmu_g[ok ? mu - 1 : 0];
+                bv = v;  // rows outside the grid have b = 0: selected at commit
+            } else {
+                bv = ok ? bmu_g[mu - 1] : T(0);
             }
         }
-
-        // 2. write out the block finalised by the last pass in step s-1
-        const int bw = s - 4 * SKEW - 1;
-        if (bw >= 4 && bw <= K - 5) {
-            for (int e = tid; e < 4 * nni; e += nth) {
-                const int rr = e / nni, nu = 1 + e - rr * nni;
-                const int mu = r0 + 4 * bw + rr;
-                if (mu < a || mu >= aend) continue;
-                const T *row = block_row(bw, rr);
-                T v;
-                if (nu == 1 || nu == nni) {
-                    const int dir = nu == 1 ? 1 : -1;
-                    const int c0 = nu + 3;
-                    v = even ? d.ex[0] * row[c0 + dir] + d.ex[1] * row[c0 + 2 * dir] +
-                                   d.ex[2] * row[c0 + 3 * dir] + d.ex[3] * row[c0 + 4 * dir] +
-                                   d.ex[4] * row[c0 + 5 * dir]
-                             : T(0);
-                } else {
-                    v = row[nu + 3];
-                }
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int rr = (lrows >> (3 * j)) & 7;
+            const int mu = mub + rr;
+            // rows outside the padded array are never read from the ring and
+            // slots without an element are not committed: no select needed
+            const bool ok = rr < 4 && bl <= K && mu >= -3 && mu <= nmu + 4;
+            if (PIPE == 2) pr[j] = src[ok ? base + tid + j * nth : 0];
+            else pr[j] = ok ? src[base + tid + j * nth] : T(0);
+        }
+    };
+    auto commit_block = [&](int bl, const T (&pr)[NS], T bv) {
+        if (bl > K) return;
+        if (tid < 32) {
+            const int mu = r0 + 4 * bl + (tid >> 3);
+            ab_r[((bl + RB) % RB) * 32 + tid] = row_weight(mu >= 1 && mu <= nmu ? bv : T(0));
+        }
+        T *bp = block_row(bl, 0);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int rr = (lrows >> (3 * j)) & 7;
+            if (rr < 4) bp[tid + j * nth + rr * (W - N8)] = pr[j];
+        }
+    };
+    auto write_out = [&](int bw) {
+        const bool wact = bw >= 4 && bw <= K - 5;
+        const T *wbase = block_row(bw, 0);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int rr = (wrows >> (3 * j)) & 7;
+            const int mu = r0 + 4 * bw + rr;
+            const bool ok = wact && rr < 4 && mu >= a && mu < aend;
+            const int nu = ok ? 1 + tid + j * nth - rr * nni : 2;
+            const T *row = wbase + (ok ? rr : 0) * W;
+            T v;
+            if (nu == 1 || nu == nni) {
+                const int dir = nu == 1 ? 1 : -1;
+                const int c0 = nu + 3;
+                v = even ? d.ex[0] * row[c0 + dir] + d.ex[1] * row[c0 + 2 * dir] + d.ex[2] * row[c0 + 3 * dir] +
+                               d.ex[3] * row[c0 + 4 * dir] + d.ex[4] * row[c0 + 5 * dir]
+                         : T(0);
+            } else {
+                v = row[nu + 3];
+            }
+            if (PIPE == 2) {
+                gmem<T> *addr = ok ? dst + ((long)(mu + 3) * N8 + nu + 3) : trash + tid;
+                *addr = v;
+            } else if (ok) {
                 dst[(long)(mu + 3) * N8 + nu + 3] = v;
             }
-            // the mu=1 row once rows 2..6 are final (first strip only)
-            if (a == 2 && bw == 5) {
-                const int q2 = 2 - r0;  // rows 2..6 sit at strip offsets 16..20
-                const T *rw[5] = {block_row(q2 >> 2, q2 & 3), block_row((q2 + 1) >> 2, (q2 + 1) & 3),
-                                  block_row((q2 + 2) >> 2, (q2 + 2) & 3), block_row((q2 + 3) >> 2, (q2 + 3) & 3),
-                                  block_row((q2 + 4) >> 2, (q2 + 4) & 3)};
-                auto r1 = [&](int nu) -> T {
-                    const int c0 = nu + 3;
-                    return d.ex[0] * rw[0][c0] + d.ex[1] * rw[1][c0] + d.ex[2] * rw[2][c0] +
-                           d.ex[3] * rw[3][c0] + d.ex[4] * rw[4][c0];
-                };
-                for (int nu = 1 + tid; nu <= nni; nu += nth) {
-                    T v = T(0);
-                    if (even) {
-                        if (nu == 1 || nu == nni) {
-                            const int dir = nu == 1 ? 1 : -1;
-                            v = d.ex[0] * r1(nu + dir) + d.ex[1] * r1(nu + 2 * dir) +
-                                d.ex[2] * r1(nu + 3 * dir) + d.ex[3] * r1(nu + 4 * dir) +
-                                d.ex[4] * r1(nu + 5 * dir);
-                        } else {
-                            v = r1(nu);
-                        }
-                    }
-                    dst[(long)4 * N8 + nu + 3] = v;
-                }
-            }
         }
-
-        // 3. pass p on block s - 2p
-        if (rows_active(s)) {
-            const int b = s - SKEW * p;
-            const int sm = sb == 0 ? RB - 1 : sb - 1, sp = sb == RB - 1 ? 0 : sb + 1;
-            int R[12];
+    };
+    // pass p on block s - 2p with the lhs / rhs of step s; within an active
+    // pass the points are processed without branches (points outside the grid
+    // compute on in-bounds values and write to the sink).
+    // The points of one pass in the four rows of a block do not read each
+    // other (a point's neighbours have other colours), so the stencil values
+    // of GR rows are gathered first and their arithmetic follows, which keeps
+    // the LDS reads in flight together (GR bounded by the register budget).
+    constexpr int GR = 1;
+    auto compute = [&](int s, const T (&lv)[4][NT], const T (&rv)[4][NT]) {
+        const int b = s - SKEW * p;
+        const bool act = worker && b >= p && b <= K - 1 - p;
+        if (!act) return;  // idle lanes and passes outside their block range
+        const int sm = sb == 0 ? RB - 1 : sb - 1, sp = sb == RB - 1 ? 0 : sb + 1;
+        const T *const bm = ring + sm * 4 * W;   // block b-1
+        T *const b0 = ring + sb * 4 * W;         // block b
+        const T *const bp = ring + sp * 4 * W;   // block b+1
+        const int mu0 = r0 + 4 * b;
+        const T *ab = ab_r + sb * 32;           // row weights of block b
+        int offs[4];
+        offs[0] = off0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                R[q] = (sm * 4 + q) * N8;
-                R[4 + q] = (sb * 4 + q) * N8;
-                R[8 + q] = (sp * 4 + q) * N8;
-            }
-            const int mu0 = r0 + 4 * b;
-            int off = off0;
+        for (int rr = 1; rr < 4; ++rr) offs[rr] = offs[rr - 1] - cA < 0 ? offs[rr - 1] - cA + 5 : offs[rr - 1] - cA;
 #pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-                if (rr) off = off - cA < 0 ? off - cA + 5 : off - cA;
+        for (int g = 0; g < 4; g += GR) {
+            int cbv[GR][NT];
+            bool okv[GR][NT];
+            T x[GR][NT][17];  // 0..8: own row at nu-4..nu+4; 9..12: rows mu-1..mu-4; 13..16: rows mu+1..mu+4
+#pragma unroll
+            for (int q = 0; q < GR; ++q) {
+                const int rr = g + q;
                 const int mu = mu0 + rr;
-                if (mu >= 2 && mu <= nmu - 4) {
-                    const T bm = bmu_s[mu - r0];
+                const bool rowok = act && mu >= 2 && mu <= nmu - 4;
 #pragma unroll
-                    for (int j = 0; j < NT; ++j) {
-                        const int nu = 2 + off + 5 * (tl + j * TG);
-                        if (nu <= nni - 1) {
-                            const int c0 = nu + 3;
-                            T *rc = ring + R[rr + 4] + c0;
-                            const T *m1 = ring + R[rr + 3] + c0, *m2 = ring + R[rr + 2] + c0,
-                                    *m3 = ring + R[rr + 1] + c0, *m4 = ring + R[rr] + c0;
-                            const T *p1 = ring + R[rr + 5] + c0, *p2 = ring + R[rr + 6] + c0,
-                                    *p3 = ring + R[rr + 7] + c0, *p4 = ring + R[rr + 8] + c0;
-                            const T ddmi2 = cf[4] * (*m4 + *p4) + cf[5] * (*m3 + *p3) +
-                                            cf[6] * (*m2 + *p2) + cf[7] * (*m1 + *p1);
-                            const T ddmi1 = cf[0] * (*m4 - *p4) + cf[1] * (*m3 - *p3) +
-                                            cf[2] * (*m2 - *p2) + cf[3] * (*m1 - *p1);
-                            const T ddni2 = cf[12] * (rc[-4] + rc[4]) + cf[13] * (rc[-3] + rc[3]) +
-                                            cf[14] * (rc[-2] + rc[2]) + cf[15] * (rc[-1] + rc[1]);
-                            const T ddni1 = cf[8] * (rc[-4] - rc[4]) + cf[9] * (rc[-3] - rc[3]) +
-                                            cf[10] * (rc[-2] - rc[2]) + cf[11] * (rc[-1] - rc[1]);
-                            const T cc = ddmi2 + bm * ddmi1 + ddni2 + dnu_s[nu - 1] * ddni1;
-                            rc[0] = om * (rv[rr][j] - cc) / lv[rr][j] + om1 * rc[0];
-                        }
+                for (int j = 0; j < NT; ++j) {
+                    const int cb = 1 + offs[rr] + 5 * (tl + j * TG);  // = nu - 1 = (column of nu) - 4
+                    cbv[q][j] = cb;
+                    okv[q][j] = rowok && cb <= nni - 2;
+                    const T *qm = bm + cb, *qp = bp + cb, *q0 = b0 + cb;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) x[q][j][k] = q0[rr * W + k];
+#pragma unroll
+                    for (int k = 1; k <= 4; ++k) {
+                        const int rdn = rr - k, rup = rr + k;
+                        x[q][j][8 + k] = rdn < 0 ? qm[(rdn + 4) * W + 4] : q0[rdn * W + 4];
+                        x[q][j][12 + k] = rup > 3 ? qp[(rup - 4) * W + 4] : q0[rup * W + 4];
                     }
                 }
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < GR; ++q) {
+                const int rr = g + q;
+                const T am4 = ab[8 * rr + 0], am3 = ab[8 * rr + 1], am2 = ab[8 * rr + 2], am1 = ab[8 * rr + 3];
+                const T ap4 = ab[8 * rr + 4], ap3 = ab[8 * rr + 5], ap2 = ab[8 * rr + 6], ap1 = ab[8 * rr + 7];
+#pragma unroll
+                for (int j = 0; j < NT; ++j) {
+                    const T *xv = x[q][j];
+                    const T dmu = (am4 * xv[12] + ap4 * xv[16]) + (am3 * xv[11] + ap3 * xv[15]) +
+                                  ((am2 * xv[10] + ap2 * xv[14]) + (am1 * xv[9] + ap1 * xv[13]));
+                    const T ddni2 = (cf[4] * (xv[0] + xv[8]) + cf[5] * (xv[1] + xv[7])) +
+                                    (cf[6] * (xv[2] + xv[6]) + cf[7] * (xv[3] + xv[5]));
+                    const T ddni1 = (cf[0] * (xv[0] - xv[8]) + cf[1] * (xv[1] - xv[7])) +
+                                    (cf[2] * (xv[2] - xv[6]) + cf[3] * (xv[3] - xv[5]));
+                    const int cb = cbv[q][j];
+                    const T cc = (dmu + ddni2) + dnu_s[cb] * ddni1;
+                    const T v = (rv[rr][j] - cc) * lv[rr][j] + om1 * xv[4];
+                    T *w = okv[q][j] ? b0 + cb + rr * W + 4 : sink + tid;
+                    *w = v;
+                }
+            }
         }
+    };
+    // the mu=1 row once rows 2..6 are final (first strip only, at bw == 5;
+    // rows 2 and 3 sit in the slot that the commit of the same step reuses)
+    auto row_one = [&]() {
+        const int q2 = 2 - r0;  // rows 2..6 sit at strip offsets 16..20
+        const T *rw[5] = {block_row(q2 >> 2, q2 & 3), block_row((q2 + 1) >> 2, (q2 + 1) & 3),
+                          block_row((q2 + 2) >> 2, (q2 + 2) & 3), block_row((q2 + 3) >> 2, (q2 + 3) & 3),
+                          block_row((q2 + 4) >> 2, (q2 + 4) & 3)};
+        auto r1 = [&](int nu) -> T {
+            const int c0 = nu + 3;
+            return d.ex[0] * rw[0][c0] + d.ex[1] * rw[1][c0] + d.ex[2] * rw[2][c0] + d.ex[3] * rw[3][c0] +
+                   d.ex[4] * rw[4][c0];
+        };
+        for (int nu = 1 + tid; nu <= nni; nu += nth) {
+            T v = T(0);
+            if (even) {
+                if (nu == 1 || nu == nni) {
+                    const int dir = nu == 1 ? 1 : -1;
+                    v = d.ex[0] * r1(nu + dir) + d.ex[1] * r1(nu + 2 * dir) + d.ex[2] * r1(nu + 3 * dir) +
+                        d.ex[3] * r1(nu + 4 * dir) + d.ex[4] * r1(nu + 5 * dir);
+                } else {
+                    v = r1(nu);
+                }
+            }
+            dst[(long)4 * N8 + nu + 3] = v;
+        }
+    };
 
-        // 4. commit the prefetched block (its slot held block s-10, which only
-        // the mu=1 row of step 2 may still read: a, bw are uniform)
-        if (a == 2 && bw == 5) __syncthreads();
-        if (do_load) {
-            T *base = block_row(bl, 0);
-#pragma unroll
-            for (int j = 0; j < kMaxLoads; ++j) {
-                const int e = tid + j * nth;
-                if (e < 4 * N8) base[e] = pre[j];
+    if (PIPE == 2) {
+        // Block s+2 arrives in LDS at the end of step s from the register set
+        // loaded in step s-1; the lhs / rhs of step s were loaded in step s-2.
+        // Two register sets of each alternate between even and odd steps.
+        auto step = [&](int s, const T (&pc)[NS], T (&pn)[NS], const T &bc, T &bn, T (&lc)[4][NT],
+                        T (&rcv)[4][NT]) {
+            load_block(s + 3, pn, bn);
+            compute(s, lc, rcv);
+            const int off2 = (off0 + 2 * cA) % 5;
+            load_lr(s + 2, off2, lc, rcv);
+            const int bw = s - 4 * SKEW - 1;
+            if (a == 2 && bw == 5) {
+                row_one();
+                __syncthreads();
             }
+            commit_block(s + 2, pc, bc);
+            write_out(bw);
+            sb = sb == RB - 1 ? 0 : sb + 1;
+            off0 = off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA;
+            __syncthreads();
+        };
+        T pa[NS], pb[NS], ba, bb;
+        T la[4][NT], ra[4][NT], lb[4][NT], rb[4][NT];
+        load_block(2, pa, ba);
+        load_lr(0, off0, la, ra);
+        load_lr(1, off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA, lb, rb);
+        for (int s = 0; s < S; s += 2) {
+            step(s, pa, pb, ba, bb, la, ra);
+            if (s + 1 < S) step(s + 1, pb, pa, bb, ba, lb, rb);
         }
-        sb = sb == RB - 1 ? 0 : sb + 1;
-        off0 = off0n;
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr)
-#pragma unroll
-            for (int j = 0; j < NT; ++j) {
-                lv[rr][j] = ln[rr][j];
-                rv[rr][j] = rn[rr][j];
-            }
-        // every load of this step has landed before the next one starts, so
-        // the next compute phase never waits on that step's own prefetches
+    } else {
+        // One step ahead: block s+2 and the lhs / rhs of step s+1 are loaded at
+        // the start of step s and have landed before its commit, so the next
+        // compute phase never waits on them.
+        T lv[4][NT], rv[4][NT];
+        load_lr(0, off0, lv, rv);
         __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
+        for (int s = 0; s < S; ++s) {
+            T ln[4][NT], rn[4][NT], pre[NS], bpre;
+            load_lr(s + 1, off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA, ln, rn);
+            load_block(s + 2, pre, bpre);
+            compute(s, lv, rv);
+            const int bw = s - 4 * SKEW - 1;
+            if (a == 2 && bw == 5) {
+                row_one();
+                __syncthreads();
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            commit_block(s + 2, pre, bpre);
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+                for (int j = 0; j < NT; ++j) {
+                    lv[rr][j] = ln[rr][j];
+                    rv[rr][j] = rn[rr][j];
+                }
+            write_out(bw);
+            sb = sb == RB - 1 ? 0 : sb + 1;
+            off0 = off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA;
+            __syncthreads();
+        }
     }
 }
 
@@ -736,10 +864,30 @@ __global__ void k_orb_tail(const TailDesc *__restrict__ td)
 
 // ---------------------------------------------------------------------------
 // Mixed precision helpers.
-// Residual r = rhs - (L + lhs) u over the relaxed interior (FP64, from the
-// padded u), stored in FP32 for the correction sweeps; zero elsewhere.
+// u(nu, mu) of an unpadded function with the mirror halos of putin applied on
+// the fly (inout.F90:51-63): below mu = 1 and outside nu = 1..nni the value
+// is isym times the mirrored one.  The 17-point cross needs at most one mirror.
+__device__ __forceinline__ double fmirror(const double *f, int nu, int mu, int nni, double s)
+{
+    double sc = 1.0;
+    if (mu < 1) {
+        mu = 2 - mu;
+        sc = s;
+    } else if (nu < 1) {
+        nu = 2 - nu;
+        sc = s;
+    } else if (nu > nni) {
+        nu = 2 * nni - nu;
+        sc = s;
+    }
+    return sc * f[(long)(mu - 1) * nni + nu - 1];
+}
+
+// Residual r = rhs - (L + lhs) u over the relaxed interior (FP64, read from
+// the unpadded u with its mirror halos), stored in FP32 for the correction
+// sweeps; the operator's terms in the order of sormcsor.F90:173-194.
 struct ResDesc {
-    const double *w;       // padded u (halos filled)
+    const double *fun;     // unpadded u
     const double *lhs;
     const double *rhs;
     const double *bmu;
@@ -748,14 +896,14 @@ struct ResDesc {
     int res_split;         // res_f in the residue-split layout (else unpadded nni*nmu)
     double *acc;           // [2] per problem: sum res^2, sum rhs^2 (may be null)
     double c[16];
-    int nni, nmu;
+    int nni, nmu, isym;
 };
 
 __global__ void k_residual(const ResDesc *__restrict__ rd)
 {
     const ResDesc &d = rd[blockIdx.y];
     const int nni = d.nni, nmu = d.nmu;
-    const long s1 = nni + 8;
+    const double s = d.isym == 1 ? 1.0 : -1.0;
     const long tot = (long)nni * nmu;
     double sr = 0.0, sb = 0.0;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
@@ -763,9 +911,18 @@ __global__ void k_residual(const ResDesc *__restrict__ rd)
         const int mu = 1 + (int)(e / nni), nu = 1 + (int)(e % nni);
         double r = 0.0;
         if (mu >= 2 && mu <= nmu - 4 && nu >= 2 && nu <= nni - 1) {
-            const long p = (long)(mu + 3) * s1 + nu + 3;
-            const double cc = stencil_cc(d.w, p, s1, d.c, d.bmu[mu - 1], d.dnu[nu - 1]);
-            r = d.rhs[e] - (cc + d.lhs[e] * d.w[p]);
+            const double *f = d.fun, *c = d.c;
+            auto F = [&](int dn, int dm) { return fmirror(f, nu + dn, mu + dm, nni, s); };
+            const double ddmi2 = c[4] * (F(0, -4) + F(0, 4)) + c[5] * (F(0, -3) + F(0, 3)) +
+                                 c[6] * (F(0, -2) + F(0, 2)) + c[7] * (F(0, -1) + F(0, 1));
+            const double ddmi1 = c[0] * (F(0, -4) - F(0, 4)) + c[1] * (F(0, -3) - F(0, 3)) +
+                                 c[2] * (F(0, -2) - F(0, 2)) + c[3] * (F(0, -1) - F(0, 1));
+            const double ddni2 = c[12] * (F(-4, 0) + F(4, 0)) + c[13] * (F(-3, 0) + F(3, 0)) +
+                                 c[14] * (F(-2, 0) + F(2, 0)) + c[15] * (F(-1, 0) + F(1, 0));
+            const double ddni1 = c[8] * (F(-4, 0) - F(4, 0)) + c[9] * (F(-3, 0) - F(3, 0)) +
+                                 c[10] * (F(-2, 0) - F(2, 0)) + c[11] * (F(-1, 0) - F(1, 0));
+            const double cc = ddmi2 + d.bmu[mu - 1] * ddmi1 + ddni2 + d.dnu[nu - 1] * ddni1;
+            r = d.rhs[e] - (cc + d.lhs[e] * f[e]);
             sr += r * r;
             sb += d.rhs[e] * d.rhs[e];
             if (d.res_f && d.res_split) d.res_f[split_index(nu, mu, split_m(nni))] = (float)r;
@@ -786,7 +943,7 @@ __global__ void k_residual(const ResDesc *__restrict__ rd)
 }
 
 struct CorrDesc {
-    double *w;          // padded FP64 u
+    double *fun;        // unpadded FP64 u, corrected in place
     const float *e;     // padded FP32 correction
     int nni, nmu;
 };
@@ -801,8 +958,51 @@ __global__ void k_correct(const CorrDesc *__restrict__ cd)
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
          k += (long)gridDim.x * blockDim.x) {
         const int mu = 2 + (int)(k / (nni - 2)), nu = 2 + (int)(k % (nni - 2));
-        const long p = (long)(mu + 3) * s1 + nu + 3;
-        d.w[p] += (double)d.e[p];
+        d.fun[(long)(mu - 1) * nni + nu - 1] += (double)d.e[(long)(mu + 3) * s1 + nu + 3];
+    }
+}
+
+// The boundary lines of an unpadded function from its (corrected) interior,
+// as k_boundary does on the padded array (sormcsor.F90:94-129).
+struct BndDesc {
+    double *fun;
+    double ex[5];
+    int nni, nmu, isym;
+};
+
+__global__ void k_boundary_fun(const BndDesc *__restrict__ bd)
+{
+    const BndDesc &d = bd[blockIdx.y];
+    const int nni = d.nni, nmu = d.nmu;
+    double *f = d.fun;
+    auto at = [&](int nu, int mu) -> double & { return f[(long)(mu - 1) * nni + nu - 1]; };
+    auto row1 = [&](int nu) {
+        return d.ex[0] * at(nu, 2) + d.ex[1] * at(nu, 3) + d.ex[2] * at(nu, 4) + d.ex[3] * at(nu, 5) +
+               d.ex[4] * at(nu, 6);
+    };
+    const int nrow1 = nni - 2, ncol = nmu - 4, total = nrow1 + 2 * ncol;
+    const bool even = d.isym == 1;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        if (t < nrow1) {
+            const int nu = 2 + t;
+            at(nu, 1) = even ? row1(nu) : 0.0;
+        } else {
+            const int q = t - nrow1;
+            const int side = q / ncol, mu = 1 + q % ncol;
+            const int nu = side == 0 ? 1 : nni, dir = side == 0 ? 1 : -1;
+            double v = 0.0;
+            if (even) {
+                if (mu == 1) {
+                    v = d.ex[0] * row1(nu + dir) + d.ex[1] * row1(nu + 2 * dir) + d.ex[2] * row1(nu + 3 * dir) +
+                        d.ex[3] * row1(nu + 4 * dir) + d.ex[4] * row1(nu + 5 * dir);
+                } else {
+                    v = d.ex[0] * at(nu + dir, mu) + d.ex[1] * at(nu + 2 * dir, mu) +
+                        d.ex[2] * at(nu + 3 * dir, mu) + d.ex[3] * at(nu + 4 * dir, mu) +
+                        d.ex[4] * at(nu + 5 * dir, mu);
+                }
+            }
+            at(nu, mu) = v;
+        }
     }
 }
 
@@ -933,11 +1133,11 @@ static int choose_L(x2g_ctx *ctx, long total_rows)
 }
 
 // Strip height for k_sweep from a cost model: a workgroup of strip height L
-// runs L/4 + 13 pipeline steps (the 16-row halos and the pipeline fill), the
-// launch runs ceil(W / slots) rounds of workgroups, W = sum over problems of
-// ceil(rows / L), slots = resident workgroups (CUs x workgroups per CU).
-template <typename T>
-static int choose_sweep_L(x2g_ctx *ctx, const std::vector<int> &rows, int maxnni)
+// runs L/4 + 13 pipeline steps (the 16-row halos and the pipeline fill); the
+// W = sum over problems of ceil(rows / L) workgroups spread over the CUs, and
+// workgroups sharing a CU share its throughput, so the launch takes
+// ceil(W / CUs) workgroup times.
+static int choose_sweep_L(x2g_ctx *ctx, const std::vector<int> &rows)
 {
     const int forced = env_strip_l();
     if (forced > 0) return (forced + 3) & ~3;
@@ -946,13 +1146,9 @@ static int choose_sweep_L(x2g_ctx *ctx, const std::vector<int> &rows, int maxnni
     double best = 1e300;
     int bestL = 32;
     for (int L = 24; L <= std::max(24, ((maxrows + 3) & ~3)); L += 4) {
-        const size_t lds = sweep_lds_elems<T>(maxnni, L) * sizeof(T);
-        if (lds > ctx->max_lds) break;
-        const long per_cu = std::max<long>(1, (long)(ctx->max_lds / lds));
-        const long slots = (long)ctx->num_cu * std::min<long>(per_cu, 2);
-        long W = 0;
-        for (int r : rows) W += (r + L - 1) / L;
-        const double cost = (double)((W + slots - 1) / slots) * (L / 4.0 + 13.0);
+        long nwg = 0;
+        for (int r : rows) nwg += (r + L - 1) / L;
+        const double cost = (double)((nwg + ctx->num_cu - 1) / ctx->num_cu) * (L / 4.0 + 13.0);
         if (cost < best - 1e-9) {
             best = cost;
             bestL = L;
@@ -968,18 +1164,46 @@ struct SweepPlan {
     int kind = 0;
     int skew = 0;
     int total_wg = 0;
-    int nth = 0, TG = 0, NT = 0;
+    int nth = 0, TG = 0, NT = 0, W = 0;
     size_t lds = 0;
 };
 
 // Does k_sweep apply to a batch whose widest grid has nni points?
+// Ring row stride of k_sweep for a batch whose widest grid has nni points:
+// the smallest compiled class >= nni + 16 (0: none).  FP64 is compiled for
+// the classes up to 352 (wider FP64 rings do not fit the LDS).
+template <typename T>
+static int sweep_stride(int maxnni)
+{
+    static const int cls32[] = {128, 192, 256, 352, 528, 768};
+    static const int cls64[] = {128, 192, 256, 352};
+    const int *c = sizeof(T) == 4 ? cls32 : cls64;
+    const int nc = sizeof(T) == 4 ? 6 : 4;
+    for (int i = 0; i < nc; ++i)
+        if (c[i] >= maxnni + 16) return c[i];
+    return 0;
+}
+
+// Lanes of a k_sweep launch for a batch whose widest grid has nni points:
+// TG lanes per colour pass, NT colour points per lane and row.
+static void sweep_lanes(int maxnni, int &TG, int &NT, int &nth)
+{
+    const int M = split_m(maxnni);
+    NT = (M + 101) / 102;
+    TG = (M + NT - 1) / NT;
+    nth = ((5 * TG + 63) / 64) * 64;
+}
+
 template <typename T>
 static bool sweep_fits(x2g_ctx *ctx, int maxnni)
 {
-    const int M = split_m(maxnni);
-    if (M > 2 * 102) return false;                       // NT <= 2 with TG <= 102
-    if (4 * (maxnni + 8) > kMaxLoads * 512) return false;
-    return sweep_lds_elems<T>(maxnni, 24) * sizeof(T) <= ctx->max_lds;
+    int TG, NT, nth;
+    sweep_lanes(maxnni, TG, NT, nth);
+    const int W = sweep_stride<T>(maxnni);
+    if (W == 0 || NT > 2 || (NT == 2) != (W == 768)) return false;
+    const int NS = NT == 1 ? 5 : 8;
+    if (4 * (maxnni + 8) > NS * nth) return false;
+    return sweep_lds_elems(W, nth) * sizeof(T) <= ctx->max_lds;
 }
 
 // Which kernel family a batch of this precision and width uses (engine setting
@@ -994,6 +1218,32 @@ static int sweep_kind(x2g_ctx *ctx, int maxnni)
 }
 
 // Launch one sweep of every problem in `descs` (device copy dd, host copy hv).
+// k_sweep for the plan's stride class and depth (the instantiations are
+// listed once here and in set_lds_attr).
+template <typename T, int NT, int PIPE, int W>
+static void launch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, int n, const SweepPlan<T> &plan)
+{
+    hipLaunchKernelGGL((k_sweep<T, NT, PIPE, W>), dim3(plan.total_wg), dim3(plan.nth), plan.lds, ctx->stream, dd, n,
+                       plan.TG);
+}
+
+template <typename T, int PIPE>
+static bool dispatch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, int n, const SweepPlan<T> &plan)
+{
+    switch (plan.W) {
+    case 128: launch_ks<T, 1, PIPE, 128>(ctx, dd, n, plan); return true;
+    case 192: launch_ks<T, 1, PIPE, 192>(ctx, dd, n, plan); return true;
+    case 256: launch_ks<T, 1, PIPE, 256>(ctx, dd, n, plan); return true;
+    case 352: launch_ks<T, 1, PIPE, 352>(ctx, dd, n, plan); return true;
+    default: break;
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (plan.W == 528) { launch_ks<T, 1, PIPE, 528>(ctx, dd, n, plan); return true; }
+        if (plan.W == 768) { launch_ks<T, 2, PIPE, 768>(ctx, dd, n, plan); return true; }
+    }
+    return false;
+}
+
 template <typename T>
 static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<SweepDesc<T>> &hv,
                         const SweepPlan<T> &plan)
@@ -1011,11 +1261,14 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
     int rc = stats_begin(ctx, &ea, &eb);
     if (rc) return rc;
     if (plan.kind == 2) {
-        if (plan.NT == 1) {
-            hipLaunchKernelGGL((k_sweep<T, 1>), dim3(total_wg), dim3(plan.nth), lds, ctx->stream, dd, n, plan.TG);
-        } else {
-            hipLaunchKernelGGL((k_sweep<T, 2>), dim3(total_wg), dim3(plan.nth), lds, ctx->stream, dd, n, plan.TG);
-        }
+        // two-deep pipeline for FP32 batches that place several workgroups on
+        // each CU (measured faster there); one-deep elsewhere
+        bool ok;
+        if constexpr (sizeof(T) == 4)
+            ok = total_wg >= 2 * ctx->num_cu ? dispatch_ks<T, 2>(ctx, dd, n, plan) : dispatch_ks<T, 1>(ctx, dd, n, plan);
+        else
+            ok = dispatch_ks<T, 1>(ctx, dd, n, plan);
+        X2G_CHECK(ok, "no sweep kernel for ring stride %d", plan.W);
         X2G_HIP(hipGetLastError());
         return stats_end(ctx, ea, eb, bytes, prec);
     }
@@ -1053,10 +1306,16 @@ static int set_lds_attr(x2g_ctx *ctx)
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-    X2G_HIP(hipFuncSetAttribute((const void *)k_sweep<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-    X2G_HIP(hipFuncSetAttribute((const void *)k_sweep<double, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-    X2G_HIP(hipFuncSetAttribute((const void *)k_sweep<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-    X2G_HIP(hipFuncSetAttribute((const void *)k_sweep<float, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    const void *ks[] = {
+        (const void *)k_sweep<double, 1, 1, 128>, (const void *)k_sweep<double, 1, 1, 192>,
+        (const void *)k_sweep<double, 1, 1, 256>, (const void *)k_sweep<double, 1, 1, 352>,
+        (const void *)k_sweep<float, 1, 1, 128>,  (const void *)k_sweep<float, 1, 1, 192>,
+        (const void *)k_sweep<float, 1, 1, 256>,  (const void *)k_sweep<float, 1, 1, 352>,
+        (const void *)k_sweep<float, 1, 1, 528>,  (const void *)k_sweep<float, 2, 1, 768>,
+        (const void *)k_sweep<float, 1, 2, 128>,  (const void *)k_sweep<float, 1, 2, 192>,
+        (const void *)k_sweep<float, 1, 2, 256>,  (const void *)k_sweep<float, 1, 2, 352>,
+        (const void *)k_sweep<float, 1, 2, 528>,  (const void *)k_sweep<float, 2, 2, 768>};
+    for (const void *k : ks) X2G_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     done = true;
     return X2G_OK;
 }
@@ -1076,6 +1335,7 @@ static void fill_desc(SweepDesc<T> &d, x2g_grid *g, const T *lhs, const T *rhs, 
     d.nni = g->nni;
     d.nmu = g->nmu;
     d.isym = isym;
+    d.trash = (T *)g->ctx->trash;
     d.cA = colour_mult(g->nni);
     d.wg_begin = 0;
     d.L = 0;
@@ -1098,13 +1358,11 @@ static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vecto
     plan.kind = sweep_kind<T>(ctx, maxnni);
     int L = 0;
     if (plan.kind == 2) {
-        const int M = split_m(maxnni);
-        plan.NT = (M + 101) / 102;
-        plan.TG = (M + plan.NT - 1) / plan.NT;
-        plan.nth = ((5 * plan.TG + 63) / 64) * 64;
+        sweep_lanes(maxnni, plan.TG, plan.NT, plan.nth);
+        plan.W = sweep_stride<T>(maxnni);
         plan.skew = 2;
-        L = choose_sweep_L<T>(ctx, prow, maxnni);
-        plan.lds = sweep_lds_elems<T>(maxnni, L) * sizeof(T);
+        L = choose_sweep_L(ctx, prow);
+        plan.lds = sweep_lds_elems(plan.W, plan.nth) * sizeof(T);
     } else if (plan.kind == 1) {
         plan.skew = choose_skew<T>(ctx, maxnni);
         L = choose_L(ctx, rows);
@@ -1196,6 +1454,7 @@ extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const
     X2G_CHECK(isym == 1 || isym == -1, "x2g_mcsor: isym must be +1 or -1");
     X2G_CHECK(bkind == 0 || bkind == 1, "x2g_mcsor: bkind must be 0 or 1");
     X2G_CHECK(nsweeps >= 0, "x2g_mcsor: nsweeps < 0");
+    X2G_CHECK(omega > 0.0 && omega < 2.0, "x2g_mcsor: omega must lie in (0, 2)");
     int rc = check_grid_sweepable(g);
     if (rc) return rc;
     if (nsweeps == 0) return X2G_OK;
@@ -1218,7 +1477,7 @@ extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const
     plan_sweeps(ctx, base, A, B, plan);
     if (plan.kind == 2) {
         double *lp = (double *)(w0 + align256(bytes)), *rp = (double *)(w0 + align256(bytes) + sbytes);
-        std::vector<PackDesc<double>> pk{{lhs_d, lp, g->nni, g->nmu}, {rhs_d, rp, g->nni, g->nmu}};
+        std::vector<PackDesc<double>> pk{{lhs_d, lp, omega, g->nni, g->nmu}, {rhs_d, rp, 0.0, g->nni, g->nmu}};
         if ((rc = launch_pack(ctx, pk))) return rc;
         for (auto *v : {&plan.even, &plan.odd}) {
             (*v)[0].lhs = lp;
@@ -1283,16 +1542,17 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         X2G_CHECK(it.isym == 1 || it.isym == -1, "x2g_relax_batch: item %d isym", i);
         X2G_CHECK(it.bkind == 0 || it.bkind == 1, "x2g_relax_batch: item %d bkind", i);
         X2G_CHECK(it.prec == 0 || it.prec == 1, "x2g_relax_batch: item %d prec", i);
+        X2G_CHECK(it.omega > 0.0 && it.omega < 2.0, "x2g_relax_batch: item %d omega must lie in (0, 2)", i);
         if ((rc = check_grid_sweepable(g))) return rc;
-        o64[i] = off64;
-        off64 += 2 * align256(g->npad() * sizeof(double));
-        if (it.prec == 0) off64 += 2 * align256(split_size(g->nni, g->nmu) * sizeof(double));
+        o64[i] = off64;  // FP64 items: two padded buffers and the split lhs / rhs
+        if (it.prec == 0)
+            off64 += 2 * align256(g->npad() * sizeof(double)) + 2 * align256(split_size(g->nni, g->nmu) * sizeof(double));
         o32[i] = off32;
         if (it.prec == 1)
             off32 += 2 * align256(g->npad() * sizeof(float)) +
                      2 * align256(std::max<size_t>(g->n(), split_size(g->nni, g->nmu)) * sizeof(float));
     }
-    if ((rc = ws_reserve(ctx, 0, off64))) return rc;
+    if (off64 && (rc = ws_reserve(ctx, 0, off64))) return rc;
     if (off32 && (rc = ws_reserve(ctx, 3, off32))) return rc;
     char *w64 = (char *)ctx->ws[0].ptr;
     char *w32 = (char *)ctx->ws[3].ptr;
@@ -1320,20 +1580,21 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     };
 
     size_t doff = 0;
-    // descriptors that do not change across macro-iterations
+    // descriptors that do not change across macro-iterations; FP32 items work
+    // on the unpadded function directly (residual with on-the-fly mirrors,
+    // in-place correction), so only FP64 items are embedded
     std::vector<PadDesc> pin, pout;
     std::vector<TailDesc> tails;
     for (int i = 0; i < n; ++i) {
         const x2g_relax_item &it = items[i];
         x2g_grid *g = grids[it.grid_index];
-        const bool strip_or_mixed = true;
-        pin.push_back({it.fun_d, bufA(i), strip_or_mixed ? bufB(i) : nullptr, g->nni, g->nmu, it.isym});
+        if (it.prec == 0) pin.push_back({it.fun_d, bufA(i), bufB(i), g->nni, g->nmu, it.isym});
         if (it.tail_fock1_d)
             tails.push_back({it.fun_d, it.tail_fock1_d, g->suppl + 5 * g->n(), g->vxisq, g->vetasq, g->nni, g->nmu});
     }
     const PadDesc *d_pin = nullptr;
     const TailDesc *d_tail = nullptr;
-    if ((rc = stage_descs(ctx, pin, doff, &d_pin))) return rc;
+    if (!pin.empty() && (rc = stage_descs(ctx, pin, doff, &d_pin))) return rc;
     if (!tails.empty() && (rc = stage_descs(ctx, tails, doff, &d_tail))) return rc;
 
     // FP64 items: sweep plan
@@ -1358,8 +1619,8 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             for (size_t k = 0; k < i64.size(); ++k) {
                 const x2g_relax_item &it = items[i64[k]];
                 x2g_grid *g = grids[it.grid_index];
-                pk.push_back({it.lhs_d, splitp(i64[k], 0), g->nni, g->nmu});
-                pk.push_back({it.rhs_d, splitp(i64[k], 1), g->nni, g->nmu});
+                pk.push_back({it.lhs_d, splitp(i64[k], 0), it.omega, g->nni, g->nmu});
+                pk.push_back({it.rhs_d, splitp(i64[k], 1), 0.0, g->nni, g->nmu});
                 for (auto *v : {&p64.even, &p64.odd}) {
                     (*v)[k].lhs = splitp(i64[k], 0);
                     (*v)[k].rhs = splitp(i64[k], 1);
@@ -1386,10 +1647,8 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     const CorrDesc *d_corr = nullptr;
     const CvtDesc *d_cvt = nullptr;
     const FillFDesc *d_zero = nullptr;
-    const SweepDesc<double> *d_bnd64 = nullptr;
-    const PadDesc *d_pout32 = nullptr;
-    std::vector<SweepDesc<double>> bnd64;
-    std::vector<PadDesc> pout32;
+    const BndDesc *d_bnd = nullptr;
+    std::vector<BndDesc> bnd;
     if (!i32.empty()) {
         std::vector<SweepDesc<float>> base(i32.size());
         std::vector<float *> A, B;
@@ -1406,7 +1665,8 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             A.push_back(f32p(i, 0));
             B.push_back(f32p(i, 1));
             ResDesc r{};
-            r.w = bufA(i);
+            r.fun = it.fun_d;
+            r.isym = it.isym;
             r.lhs = it.lhs_d;
             r.rhs = it.rhs_d;
             r.bmu = it.bkind == 0 ? g->bmu_orb : g->bmu_pot;
@@ -1418,14 +1678,16 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             r.nni = g->nni;
             r.nmu = g->nmu;
             res.push_back(r);
-            corr.push_back({bufA(i), f32p(i, 0), g->nni, g->nmu});
+            corr.push_back({it.fun_d, f32p(i, 0), g->nni, g->nmu});
             cvt.push_back({it.lhs_d, f32p(i, 2), (long)g->n()});
             zero.push_back({f32p(i, 0), (long)(2 * ((g->npad() * sizeof(float) + 255) & ~size_t(255)) / sizeof(float))});
-            SweepDesc<double> bd;
-            fill_desc<double>(bd, g, it.lhs_d, it.rhs_d, g->bmu_orb, g->dnu, it.isym, it.omega);
-            bd.src = bd.dst = bufA(i);
-            bnd64.push_back(bd);
-            pout32.push_back({it.fun_d, bufA(i), nullptr, g->nni, g->nmu, 1});
+            BndDesc bd{};
+            bd.fun = it.fun_d;
+            for (int q = 0; q < 5; ++q) bd.ex[q] = g->coef[16 + q];
+            bd.nni = g->nni;
+            bd.nmu = g->nmu;
+            bd.isym = it.isym;
+            bnd.push_back(bd);
         }
         plan_sweeps(ctx, base, A, B, p32);
         // the strip engines ping-pong: an odd sweep count leaves the correction in B
@@ -1440,13 +1702,12 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         if ((rc = stage_descs(ctx, corr, doff, &d_corr))) return rc;
         if ((rc = stage_descs(ctx, cvt, doff, &d_cvt))) return rc;
         if ((rc = stage_descs(ctx, zero, doff, &d_zero))) return rc;
-        if ((rc = stage_descs(ctx, bnd64, doff, &d_bnd64))) return rc;
-        if ((rc = stage_descs(ctx, pout32, doff, &d_pout32))) return rc;
+        if ((rc = stage_descs(ctx, bnd, doff, &d_bnd))) return rc;
         if (split32) {
             std::vector<PackDesc<float>> pk;
             for (int i : i32) {
                 x2g_grid *g = grids[items[i].grid_index];
-                pk.push_back({items[i].lhs_d, f32p(i, 2), g->nni, g->nmu});
+                pk.push_back({items[i].lhs_d, f32p(i, 2), items[i].omega, g->nni, g->nmu});
             }
             if ((rc = launch_pack(ctx, pk))) return rc;
         } else {
@@ -1466,9 +1727,12 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         maxnni = std::max<long>(maxnni, g->nni);
     }
     for (int itr1 = 0; itr1 < maxsor1; ++itr1) {
-        // putin into both ping-pong buffers
-        hipLaunchKernelGGL(k_putin, dim3(grid_x_for(maxpad, 256), n), dim3(256), 0, ctx->stream, d_pin);
-        X2G_HIP(hipGetLastError());
+        // FP64 items: putin into both ping-pong buffers
+        if (d_pin) {
+            hipLaunchKernelGGL(k_putin, dim3(grid_x_for(maxpad, 256), (int)pin.size()), dim3(256), 0, ctx->stream,
+                               d_pin);
+            X2G_HIP(hipGetLastError());
+        }
         if (!i64.empty()) {
             for (int it2 = 0; it2 < maxsor2; ++it2) {
                 const bool ev = (it2 % 2) == 0;
@@ -1492,10 +1756,8 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             }
             hipLaunchKernelGGL(k_correct, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, ctx->stream, d_corr);
             X2G_HIP(hipGetLastError());
-            hipLaunchKernelGGL((k_boundary<double>), dim3(grid_x_for(2 * maxn / std::max<long>(maxnni, 1) + maxnni, 256), n32),
-                               dim3(256), 0, ctx->stream, d_bnd64);
-            X2G_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_putout, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, ctx->stream, d_pout32);
+            hipLaunchKernelGGL(k_boundary_fun, dim3(grid_x_for(2 * maxn / std::max<long>(maxnni, 1) + maxnni, 256), n32),
+                               dim3(256), 0, ctx->stream, d_bnd);
             X2G_HIP(hipGetLastError());
         }
         if (d_tail) {
@@ -1512,27 +1774,19 @@ extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2
 {
     X2G_CHECK(ctx && grids && items && out_h && n >= 0, "x2g_residual_norms: null argument");
     if (n == 0) return X2G_OK;
-    size_t off64 = 0;
-    std::vector<size_t> o64(n);
-    for (int i = 0; i < n; ++i) {
-        x2g_grid *g = grids[items[i].grid_index];
-        o64[i] = off64;
-        off64 += (g->npad() * sizeof(double) + 255) & ~size_t(255);
-    }
-    const size_t acc_off = (off64 + 255) & ~size_t(255);
-    int rc = ws_reserve(ctx, 0, acc_off + 2 * n * sizeof(double));
+    int rc = ws_reserve(ctx, 1, 2 * n * sizeof(double));
     if (rc) return rc;
-    char *w = (char *)ctx->ws[0].ptr;
-    double *acc = (double *)(w + acc_off);
-    std::vector<PadDesc> pin;
+    double *acc = (double *)ctx->ws[1].ptr;
     std::vector<ResDesc> res;
-    long maxpad = 0, maxn = 0;
+    long maxn = 0;
     for (int i = 0; i < n; ++i) {
         const x2g_relax_item &it = items[i];
+        X2G_CHECK(it.grid_index >= 0 && grids[it.grid_index], "x2g_residual_norms: bad grid index");
+        X2G_CHECK(it.fun_d && it.lhs_d && it.rhs_d, "x2g_residual_norms: item %d has a null array", i);
         x2g_grid *g = grids[it.grid_index];
-        pin.push_back({it.fun_d, (double *)(w + o64[i]), nullptr, g->nni, g->nmu, it.isym});
         ResDesc r{};
-        r.w = (double *)(w + o64[i]);
+        r.fun = it.fun_d;
+        r.isym = it.isym;
         r.lhs = it.lhs_d;
         r.rhs = it.rhs_d;
         r.bmu = it.bkind == 0 ? g->bmu_orb : g->bmu_pot;
@@ -1543,17 +1797,12 @@ extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2
         r.nni = g->nni;
         r.nmu = g->nmu;
         res.push_back(r);
-        maxpad = std::max<long>(maxpad, (long)g->npad());
         maxn = std::max<long>(maxn, (long)g->n());
     }
     size_t doff = 0;
-    const PadDesc *d_pin;
     const ResDesc *d_res;
-    if ((rc = stage_descs(ctx, pin, doff, &d_pin))) return rc;
     if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
     if ((rc = x2g_fill(ctx, acc, 2 * (size_t)n, 0.0))) return rc;
-    hipLaunchKernelGGL(k_putin, dim3(grid_x_for(maxpad, 256), n), dim3(256), 0, ctx->stream, d_pin);
-    X2G_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_residual, dim3(grid_x_for(maxn, 256), n), dim3(256), 0, ctx->stream, d_res);
     X2G_HIP(hipGetLastError());
     X2G_HIP(hipMemcpyAsync(out_h, acc, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
