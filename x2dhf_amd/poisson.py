@@ -1,0 +1,144 @@
+"""Batched synthetic Poisson problems: many independent Coulomb-type potential
+relaxations on different grids, relaxed together on one GPU.
+
+A problem is the Coulomb potential equation of x2dhf's relaxPots
+(relaxPots.F90:62-95, Delta m = 0, even symmetry) for a synthetic axial
+density rho: lhs = f3, rhs = rho * g, with the Dirichlet rows mu = nmu-3..nmu
+set from the density's multipole moments (coulAsympt, scfUtils.F90:275-296),
+from a zero interior.  rho is a sum of K Gaussians centred on the
+internuclear axis, exp(-|x - z_k e_z|^2 / (2 sigma_k^2)) with sigma_k in bohr,
+normalised to one electron with the orbital-norm quadrature (wgt2 * f4); a
+density of N electrons gives N times the potential of the unit one.
+
+The problems are relaxed with x2g_relax_batch (all active problems in one
+launch per sweep), FP32 correction sweeps with FP64 residuals by default,
+until ||rhs - A u|| / ||rhs|| <= tol for each; converged problems leave the
+batch.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import gpu, scf
+
+GRIDS = [(91, 121), (169, 241), (337, 601), (505, 901)]
+
+
+def make_problems(count=256, seed=0, rinf=40.0):
+    """Problem parameters: grid, R, Gaussians (z_k, sigma_k, weight_k), electrons."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        nni, nmu = GRIDS[rng.integers(len(GRIDS))]
+        r = float(rng.uniform(1.0, 6.0))
+        ng = int(rng.integers(1, 9))
+        gauss = [(float(rng.uniform(-0.6 * r, 0.6 * r)), float(rng.uniform(0.1, 2.0)), float(rng.uniform(0.2, 1.0)))
+                 for _ in range(ng)]
+        out.append(dict(nni=nni, nmu=nmu, r=r, rinf=rinf, gauss=gauss, electrons=float(rng.uniform(1.0, 20.0))))
+    return out
+
+
+def density(p, vxi, veta):
+    """The problem's density on its grid (nni*nmu, nu fastest), unnormalised."""
+    r2 = 0.5 * p["r"]
+    xi, eta = np.meshgrid(vxi, veta, indexing="ij")          # (nmu, nni)
+    z = r2 * xi * eta
+    rho2 = (r2 * r2) * np.maximum(xi * xi - 1.0, 0.0) * np.maximum(1.0 - eta * eta, 0.0)
+    d = np.zeros_like(z)
+    for zk, sk, wk in p["gauss"]:
+        d += wk * np.exp(-(rho2 + (z - zk) ** 2) / (2.0 * sk * sk))
+    return d.ravel()
+
+
+class Batch:
+    """The device side of a set of problems on one context."""
+
+    def __init__(self, problems, ctx=None, prec=1, omega=None, mpole=4):
+        self.lib = gpu.load()
+        self.ctx = ctx or gpu.Context(0)
+        self.own = ctx is None
+        self.problems = problems
+        self.prec = prec
+        self.grids, self.items, self.keep = [], [], []
+        for i, p in enumerate(problems):
+            coef, suppl, vxi, veta = scf.grid_build(p["nni"], p["nmu"], 1.0, 1.0, p["r"], p["rinf"])
+            g = gpu.Grid(self.ctx, p["nni"], p["nmu"], coef, suppl, vxi, veta)
+            self.grids.append(g)
+            d = density(p, vxi, veta)
+            nrm = float(np.sum(suppl[13] * suppl[8] * d))
+            d /= nrm
+            lhs, rhs = self.ctx.upload(suppl[7]), self.ctx.upload(d * suppl[9])
+            fun = self.ctx.upload(np.zeros(g.n))
+            ones = self.ctx.upload(np.ones(g.n))
+            dd = self.ctx.upload(d)
+            mom = np.zeros(8)
+            gpu.check(self.lib.x2g_moments(g.h, dd.ptr, ones.ptr, 0, mpole, mom.ctypes.data))
+            gpu.check(self.lib.x2g_pot_tail(g.h, fun.ptr, 0, 0, mpole, mom.ctypes.data))
+            w = omega
+            if w is None:
+                oo, op = C.c_double(), C.c_double()
+                scf.load().x2f_omega_defaults(p["nni"], p["nmu"], 1.0, 1.0, C.byref(oo), C.byref(op))
+                w = op.value
+            self.keep += [lhs, rhs, fun, ones, dd]
+            self.items.append(gpu.RelaxItem(fun.ptr, lhs.ptr, rhs.ptr, None, w, 1, 1, prec, i))
+        self.gh = (gpu._vp * len(self.grids))(*[g.h.value for g in self.grids])
+
+    def residuals(self, active):
+        its = (gpu.RelaxItem * len(active))(*[self.items[i] for i in active])
+        out = np.zeros(2 * len(active))
+        gpu.check(self.lib.x2g_residual_norms(self.ctx.h, self.gh, its, len(active), out.ctypes.data))
+        return np.sqrt(out[0::2] / np.maximum(out[1::2], 1e-300))
+
+    def solve(self, tol=1e-12, sweeps_per_call=10, check_every=5, max_calls=20000, stall_checks=40):
+        """Relax until every problem meets tol (or stops improving for
+        stall_checks residual checks, or max_calls relaxation calls); returns
+        per-problem calls and final relative residuals."""
+        n = len(self.items)
+        active = list(range(n))
+        calls = np.zeros(n, dtype=int)
+        res = np.full(n, np.inf)
+        best = np.full(n, np.inf)
+        stall = np.zeros(n, dtype=int)
+        k = 0
+        while active and k < max_calls:
+            its = (gpu.RelaxItem * len(active))(*[self.items[i] for i in active])
+            for _ in range(check_every):
+                gpu.check(self.lib.x2g_relax_batch(self.ctx.h, self.gh, its, len(active), 1, sweeps_per_call))
+            k += check_every
+            r = self.residuals(active)
+            for i, v in zip(active, r):
+                calls[i] += check_every
+                res[i] = v
+                if v < 0.9 * best[i]:
+                    best[i], stall[i] = v, 0
+                else:
+                    stall[i] += 1
+            active = [i for i in active if res[i] > tol and stall[i] < stall_checks]
+        return calls, res
+
+    def solution(self, i):
+        return self.keep[5 * i + 2].get()
+
+    def close(self):
+        for g in self.grids:
+            g.destroy()
+        if self.own:
+            self.ctx.close()
+
+
+def run(count=256, seed=0, prec=1, tol=1e-12):
+    """Build, solve and time a batch; returns a summary dict."""
+    probs = make_problems(count, seed)
+    t0 = time.perf_counter()
+    b = Batch(probs, prec=prec)
+    t1 = time.perf_counter()
+    gpu.check(b.lib.x2g_stats_enable(b.ctx.h, 1))
+    calls, res = b.solve(tol=tol)
+    gpu.check(b.lib.x2g_ctx_sync(b.ctx.h))
+    t2 = time.perf_counter()
+    st = (C.c_double * 6)()
+    gpu.check(b.lib.x2g_stats_get(b.ctx.h, st))
+    b.close()
+    return dict(problems=count, setup_s=t1 - t0, solve_s=t2 - t1, max_residual=float(res.max()),
+                calls=calls.tolist(), stats=list(st))
