@@ -193,11 +193,6 @@ extern "C" int x2g_ctx_create(int device, x2g_ctx **out)
     }
     int rc = ws_reserve(c, 2, kStageBytes);
     if (!rc) rc = pinned_reserve(c, kStageBytes);
-    if (!rc && hipMalloc(&c->trash, (size_t)kTrashRegions * 512 * sizeof(double)) != hipSuccess) {
-        (void)hipGetLastError();
-        set_error("hipMalloc of the context's store sink failed");
-        rc = X2G_ENOMEM;
-    }
     if (rc) {
         x2g_ctx_destroy(c);
         return rc;
@@ -219,7 +214,6 @@ extern "C" int x2g_ctx_destroy(x2g_ctx *ctx)
         if (w.ptr) (void)hipFree(w.ptr);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->pinned_out) (void)hipHostFree(ctx->pinned_out);
-    if (ctx->trash) (void)hipFree(ctx->trash);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return X2G_OK;

@@ -38,8 +38,6 @@ struct Workspace {
 
 }  // namespace x2g
 
-constexpr int kTrashRegions = 128;  // workgroup w of a launch writes region w mod 128
-
 struct x2g_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -51,8 +49,6 @@ struct x2g_ctx {
                                    // 5: batched reduction partials
     void *pinned = nullptr;        // pinned host staging for descriptors/results
     size_t pinned_bytes = 0;
-    void *trash = nullptr;         // kTrashRegions x 512 doubles of device memory that kernels
-                                   // may write and never read (stores of inactive lanes)
     void *pinned_out = nullptr;    // pinned host buffer for gathered results
     size_t pinned_out_bytes = 0;
     size_t stage_off = 0;          // ring offset into ws[2] / pinned

@@ -52,7 +52,10 @@ struct SweepDesc {
     T c[16];  // dmu1[4], dmu2[4], dni1[4], dni2[4]
     T ex[5];
     T omega, omega1;
-    T *trash;  // k_sweep: target of the stores of inactive lanes (kTrashRegions x 512 values, never read)
+    const T *lr;          // k_sweep: (omega/lhs, rhs) pairs in the residue-split order
+    unsigned pad_bytes;   // bytes of src (and of dst)
+    unsigned lr_bytes;    // bytes of lr
+    int P;                // row stride of src / dst (nni+8 except for k_sweep)
     int nni, nmu, isym, cA;
     int wg_begin, L;
 };
@@ -355,9 +358,9 @@ __device__ __forceinline__ long split_index(int nu, int mu, int M)
 template <typename TO>
 struct PackDesc {
     const double *src;  // unpadded nni*nmu array
-    TO *dst;            // residue-split array
+    TO *dst;            // residue-split array of pairs: component `comp` of pair i
     double omega;       // > 0: store omega/src (the scaled inverse diagonal), else src
-    int nni, nmu;
+    int nni, nmu, comp;
 };
 
 template <typename TO>
@@ -369,7 +372,7 @@ __global__ void k_pack(const PackDesc<TO> *__restrict__ pd)
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < npts; k += (long)gridDim.x * blockDim.x) {
         const int mu = 2 + (int)(k / nrow), nu = 2 + (int)(k % nrow);
         const double v = d.src[(long)(mu - 1) * nni + (nu - 1)];
-        d.dst[split_index(nu, mu, M)] = (TO)(d.omega > 0.0 ? d.omega / v : v);
+        d.dst[2 * split_index(nu, mu, M) + d.comp] = (TO)(d.omega > 0.0 ? d.omega / v : v);
     }
 }
 
@@ -382,10 +385,21 @@ __global__ void k_pack(const PackDesc<TO> *__restrict__ pd)
 // offsets from three block bases, and every point of a step is processed
 // without branches: points outside the pass's block range or the grid are
 // computed on in-bounds garbage and written to an LDS sink.
-// lhs / rhs come from the residue-split layout (contiguous per pass and row)
-// ahead of their use.  The split "lhs" array holds omega/lhs, so a point
-// update is  f = (rhs - L f) * (omega/lhs) + (1-omega) f;  the d/dmu part of
-// L f uses per-row weights  c2_k +- b(mu) c1_k  (k = 1..4).
+//
+// Global memory goes through buffer resources (raw buffers: a load at an
+// offset outside the array returns 0 and a store there is dropped).  Every
+// lane issues every load and store with a 32-bit offset, inactive lanes with
+// an out-of-range one, so each wave issues the same memory operations in
+// every step and the compiler's waits count exactly the loads they need.
+// src / dst have a row stride P = nni+8 rounded up to 16 bytes: a block of
+// 4 rows is a contiguous run of 16-byte chunks, copied into the ring with one
+// 16-byte load and one LDS store per chunk (NSV chunks per thread) and
+// written back the same way, pad columns included (src and dst hold the same
+// pads, which the sweep never changes).  lhs / rhs come as one array of
+// (omega/lhs, rhs) pairs in the residue-split order (contiguous per pass and
+// row), one pair load per point.  A point update is
+//   f = (rhs - L f) * (omega/lhs) + (1-omega) f,
+// the d/dmu part of L f with per-row weights  c2_k +- b(mu) c1_k  (k = 1..4).
 //
 // LDS (elements of T): d(nu) at index nu-1 [W] (so a point's column base also
 // addresses its d(nu)), the row weights of the ring's rows [48 x 8, written
@@ -396,24 +410,63 @@ __host__ __device__ inline size_t sweep_lds_elems(int W, int nth)
     return (size_t)W + 48 * 8 + 32 + (size_t)48 * W + (size_t)nth;
 }
 
-// Global-address-space views of descriptor pointers: loads through them are
-// global_load (vector memory counter only) instead of flat loads, which would
-// also occupy the LDS counter and make every LDS wait wait for prefetches.
+// elements of T per 16-byte chunk, and chunks per thread in a block copy (a
+// block is 4P elements with P <= nni+11, and a launch has at least
+// max(64, nni-2) threads)
 template <typename T>
-using gmem = __attribute__((address_space(1))) T;
+constexpr int sweep_vec() { return 16 / (int)sizeof(T); }
 template <typename T>
-__device__ __forceinline__ const gmem<T> *as_global(const T *p) { return (const gmem<T> *)p; }
+constexpr int sweep_nsv() { return sizeof(T) == 4 ? 2 : 3; }
+// row stride of k_sweep's padded arrays: nni+8 rounded up to 16 bytes
+__host__ __device__ inline int sweep_row_stride(int nni, int vec) { return (nni + 8 + vec - 1) / vec * vec; }
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr unsigned kOOB = 0x80000000u;  // a byte offset outside every array (arrays are < 2 GiB)
+constexpr int kRsrcWord3 = 0x00020000;  // buffer resource word 3 for raw buffers on gfx9-family GPUs
+
+// Byte offset `off` where ok, else one outside every array (an offset with
+// the top bit set; `off` is formed either way, which keeps the address
+// arithmetic free of branches).
+__device__ __forceinline__ unsigned oob_unless(bool ok, int off) { return (unsigned)off | (ok ? 0u : kOOB); }
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kRsrcWord3);
+}
+__device__ __forceinline__ float buf_load1(__amdgpu_buffer_rsrc_t r, unsigned off, float)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ double buf_load1(__amdgpu_buffer_rsrc_t r, unsigned off, double)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
 template <typename T>
-__device__ __forceinline__ gmem<T> *as_global(T *p) { return (gmem<T> *)p; }
+struct Pair {
+    T a, b;
+};
+__device__ __forceinline__ Pair<float> buf_load2(__amdgpu_buffer_rsrc_t r, unsigned off, float)
+{
+    return __builtin_bit_cast(Pair<float>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ Pair<double> buf_load2(__amdgpu_buffer_rsrc_t r, unsigned off, double)
+{
+    return __builtin_bit_cast(Pair<double>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+template <typename T>
+struct Chunk {
+    T v[16 / sizeof(T)];
+};
 
 template <typename T, int NT, int PIPE, int W>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && W <= 352 ? 4 : 1)))
 k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char ks_smem[];
     constexpr int SKEW = 2, RB = 4 * SKEW + 4;
-    constexpr int NS = NT == 1 ? 5 : 8;  // block-copy / write-out slots per thread
-    T *const dnu_s = reinterpret_cast<T *>(smem_raw);
+    constexpr int VEC = sweep_vec<T>(), NSV = sweep_nsv<T>();
+    T *const dnu_s = reinterpret_cast<T *>(ks_smem);
 
     const int wg = blockIdx.x;
     int lo = 0, hi = nprob - 1;
@@ -422,7 +475,7 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
         if (descs[mid].wg_begin <= wg) lo = mid; else hi = mid - 1;
     }
     const SweepDesc<T> &d = descs[lo];
-    const int nni = d.nni, nmu = d.nmu, N8 = nni + 8, L = d.L;
+    const int nni = d.nni, nmu = d.nmu, P = d.P, L = d.L;
     const int chunk = wg - d.wg_begin;
     const int a = 2 + chunk * L;
     const int aend = min(a + L, nmu - 3);
@@ -433,59 +486,108 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
     const int M = split_m(nni);
     const int cA = d.cA;
     const bool even = d.isym == 1;
-    const gmem<T> *__restrict__ src = as_global(d.src);
-    gmem<T> *__restrict__ dst = as_global(d.dst);
-    const gmem<T> *__restrict__ lhs = as_global(d.lhs);
-    const gmem<T> *__restrict__ rhs = as_global(d.rhs);
+    const __amdgpu_buffer_rsrc_t rs_src = buf_rsrc(d.src, d.pad_bytes);
+    const __amdgpu_buffer_rsrc_t rs_dst = buf_rsrc(d.dst, d.pad_bytes);
+    const __amdgpu_buffer_rsrc_t rs_lr = buf_rsrc(d.lr, d.lr_bytes);
+    const __amdgpu_buffer_rsrc_t rs_bmu = buf_rsrc(d.bmu, (unsigned)(nmu * sizeof(T)));
     T *const ab_r = dnu_s + W;                 // row weights, 8 per ring row
-    T *const coef_s = ab_r + 48 * 8;           // c[8..15] (nu weights), 1-omega, c[0..7]
+    T *const coef_s = ab_r + 48 * 8;           // c[8..15] (nu weights), 1-omega
     T *const ring = coef_s + 32;
     T *const sink = ring + 48 * W;
 
-    auto block_row = [&](int b, int rr) -> T * { return ring + (((b + RB) % RB) * 4 + rr) * W; };
+    auto slot_of = [&](int b) { return (b + RB) % RB; };  // b >= -RB
+
+    // Chunk q = tid + j*nth of a block (elements VEC*q .. VEC*q+VEC-1 of its
+    // 4 x P values): its LDS offset rr*W + col within a ring block (-1: no
+    // such chunk), and in cinfo its row rr (bits 0..2), the element of the
+    // chunk holding nu = 1 (bits 3..5) and nu = nni (bits 6..8) of that row
+    // (7: none).  Fixed for the whole strip.
+    int cl[NSV], cinfo[NSV];
+#pragma unroll
+    for (int j = 0; j < NSV; ++j) {
+        const int e0 = (tid + j * nth) * VEC;
+        const int rr = e0 / P, col = e0 - rr * P;
+        const bool v = e0 < 4 * P;
+        const int b1 = 4 - col, b2 = nni + 3 - col;
+        cl[j] = v ? rr * W + col : -1;
+        cinfo[j] = (v ? rr : 4) | ((v && b1 >= 0 && b1 < VEC ? b1 : 7) << 3) |
+                   ((v && b2 >= 0 && b2 < VEC ? b2 : 7) << 6);
+    }
 
     // Row weights of row mu: distance k = 4,3,2,1 -> c2 = c[4..7], c1 = c[0..3];
     // ab[0..3] = c2 + b(mu) c1 (rows mu-k), ab[4..7] = c2 - b(mu) c1 (rows mu+k).
-    // Thread tid < 32 forms entry tid & 7 of row tid >> 3 of a block.
-    const gmem<T> *__restrict__ bmu_g = as_global(d.bmu);
-    // this thread's two weights, loaded once (a per-lane load of the
-    // descriptor inside the step loop would break the exact wait counts)
+    // Thread tid < 32 forms entry tid & 7 of row tid >> 3 of a block; its two
+    // coefficients are loaded once.
     const T rw_c2 = d.c[4 + (tid & 3)], rw_c1 = d.c[tid & 3];
     auto row_weight = [&](T b) -> T { return (tid & 7) < 4 ? rw_c2 + b * rw_c1 : rw_c2 - b * rw_c1; };
+
+    // the chunks of block bl, plus b(mu) of row (tid >> 3) & 3 of the block
+    // for the row weights (rows outside the grid read b = 0)
+    auto load_block = [&](int bl, u32x4 (&pr)[NSV], T &bv) {
+        const int mub = r0 + 4 * bl;
+        {
+            const int mu = mub + ((tid >> 3) & 3);
+            const bool ok = tid < 32 && mu >= 1 && mu <= nmu;
+            bv = buf_load1(rs_bmu, oob_unless(ok, (mu - 1) * (int)sizeof(T)), T(0));
+        }
+        const int gb = (mub + 3) * P * (int)sizeof(T);  // negative above the padded array
+#pragma unroll
+        for (int j = 0; j < NSV; ++j) {
+            const int mu = mub + (cinfo[j] & 7);
+            const bool ok = cl[j] >= 0 && mu >= -3 && mu <= nmu + 4;
+            pr[j] = __builtin_amdgcn_raw_buffer_load_b128(rs_src, oob_unless(ok, gb + (tid + j * nth) * 16), 0, 0);
+        }
+    };
+    auto commit_block = [&](int bl, const u32x4 (&pr)[NSV], T bv) {
+        if (bl > K) return;
+        const int sl = slot_of(bl);
+        if (tid < 32) ab_r[sl * 32 + tid] = row_weight(bv);
+        T *bp = ring + sl * 4 * W;
+#pragma unroll
+        for (int j = 0; j < NSV; ++j)
+            if (cl[j] >= 0) *reinterpret_cast<u32x4 *>(bp + cl[j]) = pr[j];
+    };
+    // rows [a, aend) of block bw (final) to dst, with the nu = 1 and nu = nni
+    // values extrapolated (sormcsor.F90:352-383; zero for odd symmetry)
+    auto write_out = [&](int bw) {
+        const bool wact = bw >= 4 && bw <= K - 5;
+        const int mub = r0 + 4 * bw;
+        const int gb = (mub + 3) * P * (int)sizeof(T);
+        const T *wbase = ring + slot_of(bw) * 4 * W;
+#pragma unroll
+        for (int j = 0; j < NSV; ++j) {
+            const int rr = cinfo[j] & 7;
+            const int mu = mub + rr;
+            const bool ok = wact && cl[j] >= 0 && mu >= a && mu < aend;
+            u32x4 v = *reinterpret_cast<const u32x4 *>(wbase + (cl[j] >= 0 ? cl[j] : 0));
+            if ((cinfo[j] >> 3) != 63) {
+                Chunk<T> c = __builtin_bit_cast(Chunk<T>, v);
+                const T *row = wbase + rr * W;
+                const int e1 = (cinfo[j] >> 3) & 7, e2 = (cinfo[j] >> 6) & 7;
+                const int c0 = nni + 3;
+                const T v1 = even ? d.ex[0] * row[5] + d.ex[1] * row[6] + d.ex[2] * row[7] + d.ex[3] * row[8] +
+                                        d.ex[4] * row[9]
+                                  : T(0);
+                const T v2 = even ? d.ex[0] * row[c0 - 1] + d.ex[1] * row[c0 - 2] + d.ex[2] * row[c0 - 3] +
+                                        d.ex[3] * row[c0 - 4] + d.ex[4] * row[c0 - 5]
+                                  : T(0);
+#pragma unroll
+                for (int k = 0; k < VEC; ++k) c.v[k] = k == e1 ? v1 : (k == e2 ? v2 : c.v[k]);
+                v = __builtin_bit_cast(u32x4, c);
+            }
+            __builtin_amdgcn_raw_buffer_store_b128(v, rs_dst, oob_unless(ok, gb + (tid + j * nth) * 16), 0, 0);
+        }
+    };
+
     // prologue: coefficients, d(nu), blocks -1, 0, 1 and their row weights
     if (tid < 8) coef_s[tid] = d.c[8 + tid];
     if (tid == 8) coef_s[8] = d.omega1;
-    {
-        const gmem<T> *dnu_g = as_global(d.dnu);
-        for (int e = tid; e < nni; e += nth) dnu_s[e] = dnu_g[e];
-        for (int b = -1; b <= 1; ++b) {
-            if (tid < 32) {
-                const int mu = r0 + 4 * b + (tid >> 3);
-                const T bv = (mu >= 1 && mu <= nmu) ? bmu_g[mu - 1] : T(0);
-                ab_r[((b + RB) % RB) * 32 + tid] = row_weight(bv);
-            }
-        }
-        for (int b = -1; b <= 1; ++b) {
-            T *base = block_row(b, 0);
-            for (int e = tid; e < 4 * N8; e += nth) {
-                const int rr = e / N8, col = e - rr * N8;
-                const int mu = r0 + 4 * b + rr;
-                T v = T(0);
-                if (mu >= -3 && mu <= nmu + 4) v = src[(long)(mu + 3) * N8 + col];
-                base[rr * W + col] = v;
-            }
-        }
-    }
-
-    // per-thread slots of the block copies (element e = tid + j*nth of the
-    // 4 x N8 block) and of the write-out (element e of the 4 x nni rows):
-    // the row of slot j in bits 3j..3j+2 (4 = no element), computed once
-    unsigned lrows = 0, wrows = 0;
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-        const int e = tid + j * nth;
-        lrows |= (unsigned)(e < 4 * N8 ? e / N8 : 4) << (3 * j);
-        wrows |= (unsigned)(e < 4 * nni ? e / nni : 4) << (3 * j);
+    for (int e = tid; e < nni; e += nth) dnu_s[e] = d.dnu[e];
+    for (int b = -1; b <= 1; ++b) {
+        u32x4 pr[NSV];
+        T bv;
+        load_block(b, pr, bv);
+        commit_block(b, pr, bv);
     }
 
     // lane role
@@ -494,40 +596,30 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
     const bool worker = p < 5;
     int sb = (RB - 2 * p % RB) % RB;                  // ring slot of block s - 2p
     int off0 = mod5(p - cA * (r0 - 8 * p - 2));        // (nu-2) mod 5 of pass p in row r0 + 4(s - 2p)
-    const long M5 = 5L * M;
+    const int M5 = 5 * M;
 
     auto rows_active = [&](int s) {
         const int b = s - SKEW * p;
         return worker && b >= p && b <= K - 1 - p;
     };
-    // lhs / rhs of the points of step s (off0s: colour offset of its first
-    // row).  PIPE 2: every lane issues every load (inactive ones read element
-    // 0), so each wave issues the same memory operations in every step and
-    // the compiler's waits count exactly the loads they need.  An inactive
-    // point's values are never used, so they are not masked.
+    // (omega/lhs, rhs) of the points of step s (off0s: colour offset of its
+    // first row); inactive points read zeros and are never used
     auto load_lr = [&](int s, int off0s, T (&lv)[4][NT], T (&rv)[4][NT]) {
         const bool act = rows_active(s);
         const int mu0 = r0 + 4 * (s - SKEW * p);
-        const long i0 = (long)(mu0 - 2) * M5 + tl;
         int off = off0s;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
             if (rr) off = off - cA < 0 ? off - cA + 5 : off - cA;
             const int mu = mu0 + rr;
             const bool rowok = act && mu >= 2 && mu <= nmu - 4;
-            const long ib = i0 + rr * M5 + (long)off * M;
+            const int ib = (mu - 2) * M5 + off * M + tl;
 #pragma unroll
             for (int j = 0; j < NT; ++j) {
-                const int t = tl + j * TG;
-                const bool ok = rowok && 2 + off + 5 * t <= nni - 1;
-                if (PIPE == 2) {
-                    const long idx = ok ? ib + j * TG : 0;
-                    lv[rr][j] = lhs[idx];
-                    rv[rr][j] = rhs[idx];
-                } else if (ok) {
-                    lv[rr][j] = lhs[ib + j * TG];
-                    rv[rr][j] = rhs[ib + j * TG];
-                }
+                const bool ok = rowok && 2 + off + 5 * (tl + j * TG) <= nni - 1;
+                const Pair<T> q = buf_load2(rs_lr, oob_unless(ok, (ib + j * TG) * (int)(2 * sizeof(T))), T(0));
+                lv[rr][j] = q.a;
+                rv[rr][j] = q.b;
             }
         }
     };
@@ -540,85 +632,12 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
     for (int q = 0; q < 8; ++q) cf[q] = coef_s[q];
     const T om1 = coef_s[8];
 
-    // Block copies and write-out.  PIPE 2: every lane issues every load and
-    // store (out-of-range loads read a clamped address and are discarded,
-    // stores of inactive lanes go to the context's sink), for the same reason.
-    gmem<T> *__restrict__ trash = as_global(d.trash) + (long)(wg % kTrashRegions) * 512;
-    // the block's values, plus b(mu) of row tid >> 3 of the block for the
-    // row weights (threads tid < 32 use it)
-    auto load_block = [&](int bl, T (&pr)[NS], T &bv) {
-        const int mub = r0 + 4 * bl;
-        const long base = (long)(mub + 3) * N8;
-        {
-            const int mu = mub + ((tid >> 3) & 3);
-            const bool ok = tid < 32 && bl <= K && mu >= 1 && mu <= nmu;
-            if (PIPE == 2) {
-                const T v = bmu_g[ok ? mu - 1 : 0];
-                bv = v;  // rows outside the grid have b = 0: selected at commit
-            } else {
-                bv = ok ? bmu_g[mu - 1] : T(0);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const int rr = (lrows >> (3 * j)) & 7;
-            const int mu = mub + rr;
-            // rows outside the padded array are never read from the ring and
-            // slots without an element are not committed: no select needed
-            const bool ok = rr < 4 && bl <= K && mu >= -3 && mu <= nmu + 4;
-            if (PIPE == 2) pr[j] = src[ok ? base + tid + j * nth : 0];
-            else pr[j] = ok ? src[base + tid + j * nth] : T(0);
-        }
-    };
-    auto commit_block = [&](int bl, const T (&pr)[NS], T bv) {
-        if (bl > K) return;
-        if (tid < 32) {
-            const int mu = r0 + 4 * bl + (tid >> 3);
-            ab_r[((bl + RB) % RB) * 32 + tid] = row_weight(mu >= 1 && mu <= nmu ? bv : T(0));
-        }
-        T *bp = block_row(bl, 0);
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const int rr = (lrows >> (3 * j)) & 7;
-            if (rr < 4) bp[tid + j * nth + rr * (W - N8)] = pr[j];
-        }
-    };
-    auto write_out = [&](int bw) {
-        const bool wact = bw >= 4 && bw <= K - 5;
-        const T *wbase = block_row(bw, 0);
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const int rr = (wrows >> (3 * j)) & 7;
-            const int mu = r0 + 4 * bw + rr;
-            const bool ok = wact && rr < 4 && mu >= a && mu < aend;
-            const int nu = ok ? 1 + tid + j * nth - rr * nni : 2;
-            const T *row = wbase + (ok ? rr : 0) * W;
-            T v;
-            if (nu == 1 || nu == nni) {
-                const int dir = nu == 1 ? 1 : -1;
-                const int c0 = nu + 3;
-                v = even ? d.ex[0] * row[c0 + dir] + d.ex[1] * row[c0 + 2 * dir] + d.ex[2] * row[c0 + 3 * dir] +
-                               d.ex[3] * row[c0 + 4 * dir] + d.ex[4] * row[c0 + 5 * dir]
-                         : T(0);
-            } else {
-                v = row[nu + 3];
-            }
-            if (PIPE == 2) {
-                gmem<T> *addr = ok ? dst + ((long)(mu + 3) * N8 + nu + 3) : trash + tid;
-                *addr = v;
-            } else if (ok) {
-                dst[(long)(mu + 3) * N8 + nu + 3] = v;
-            }
-        }
-    };
-    // pass p on block s - 2p with the lhs / rhs of step s; within an active
-    // pass the points are processed without branches (points outside the grid
-    // compute on in-bounds values and write to the sink).
-    // The points of one pass in the four rows of a block do not read each
-    // other (a point's neighbours have other colours), so the stencil values
-    // of GR rows are gathered first and their arithmetic follows, which keeps
-    // the LDS reads in flight together (GR bounded by the register budget).
-    constexpr int GR = 1;
+    // pass p on block s - 2p with the (omega/lhs, rhs) of step s; within an
+    // active pass the points are processed without branches (points outside
+    // the grid compute on in-bounds values and write to the sink).  The points
+    // of one pass in a row do not read each other (a point's neighbours have
+    // other colours), so a row's stencil values are gathered first and its
+    // arithmetic follows, which keeps the LDS reads in flight together.
     auto compute = [&](int s, const T (&lv)[4][NT], const T (&rv)[4][NT]) {
         const int b = s - SKEW * p;
         const bool act = worker && b >= p && b <= K - 1 - p;
@@ -629,57 +648,47 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
         const T *const bp = ring + sp * 4 * W;   // block b+1
         const int mu0 = r0 + 4 * b;
         const T *ab = ab_r + sb * 32;           // row weights of block b
-        int offs[4];
-        offs[0] = off0;
+        int off = off0;
 #pragma unroll
-        for (int rr = 1; rr < 4; ++rr) offs[rr] = offs[rr - 1] - cA < 0 ? offs[rr - 1] - cA + 5 : offs[rr - 1] - cA;
+        for (int rr = 0; rr < 4; ++rr) {
+            if (rr) off = off - cA < 0 ? off - cA + 5 : off - cA;
+            const int mu = mu0 + rr;
+            const bool rowok = mu >= 2 && mu <= nmu - 4;
+            int cbv[NT];
+            bool okv[NT];
+            T x[NT][17];  // 0..8: own row at nu-4..nu+4; 9..12: rows mu-1..mu-4; 13..16: rows mu+1..mu+4
 #pragma unroll
-        for (int g = 0; g < 4; g += GR) {
-            int cbv[GR][NT];
-            bool okv[GR][NT];
-            T x[GR][NT][17];  // 0..8: own row at nu-4..nu+4; 9..12: rows mu-1..mu-4; 13..16: rows mu+1..mu+4
+            for (int j = 0; j < NT; ++j) {
+                const int cb = 1 + off + 5 * (tl + j * TG);  // = nu - 1 = (column of nu) - 4
+                cbv[j] = cb;
+                okv[j] = rowok && cb <= nni - 2;
+                const T *qm = bm + cb, *qp = bp + cb, *q0 = b0 + cb;
 #pragma unroll
-            for (int q = 0; q < GR; ++q) {
-                const int rr = g + q;
-                const int mu = mu0 + rr;
-                const bool rowok = act && mu >= 2 && mu <= nmu - 4;
+                for (int k = 0; k < 9; ++k) x[j][k] = q0[rr * W + k];
 #pragma unroll
-                for (int j = 0; j < NT; ++j) {
-                    const int cb = 1 + offs[rr] + 5 * (tl + j * TG);  // = nu - 1 = (column of nu) - 4
-                    cbv[q][j] = cb;
-                    okv[q][j] = rowok && cb <= nni - 2;
-                    const T *qm = bm + cb, *qp = bp + cb, *q0 = b0 + cb;
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) x[q][j][k] = q0[rr * W + k];
-#pragma unroll
-                    for (int k = 1; k <= 4; ++k) {
-                        const int rdn = rr - k, rup = rr + k;
-                        x[q][j][8 + k] = rdn < 0 ? qm[(rdn + 4) * W + 4] : q0[rdn * W + 4];
-                        x[q][j][12 + k] = rup > 3 ? qp[(rup - 4) * W + 4] : q0[rup * W + 4];
-                    }
+                for (int k = 1; k <= 4; ++k) {
+                    const int rdn = rr - k, rup = rr + k;
+                    x[j][8 + k] = rdn < 0 ? qm[(rdn + 4) * W + 4] : q0[rdn * W + 4];
+                    x[j][12 + k] = rup > 3 ? qp[(rup - 4) * W + 4] : q0[rup * W + 4];
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
+            const T am4 = ab[8 * rr + 0], am3 = ab[8 * rr + 1], am2 = ab[8 * rr + 2], am1 = ab[8 * rr + 3];
+            const T ap4 = ab[8 * rr + 4], ap3 = ab[8 * rr + 5], ap2 = ab[8 * rr + 6], ap1 = ab[8 * rr + 7];
 #pragma unroll
-            for (int q = 0; q < GR; ++q) {
-                const int rr = g + q;
-                const T am4 = ab[8 * rr + 0], am3 = ab[8 * rr + 1], am2 = ab[8 * rr + 2], am1 = ab[8 * rr + 3];
-                const T ap4 = ab[8 * rr + 4], ap3 = ab[8 * rr + 5], ap2 = ab[8 * rr + 6], ap1 = ab[8 * rr + 7];
-#pragma unroll
-                for (int j = 0; j < NT; ++j) {
-                    const T *xv = x[q][j];
-                    const T dmu = (am4 * xv[12] + ap4 * xv[16]) + (am3 * xv[11] + ap3 * xv[15]) +
-                                  ((am2 * xv[10] + ap2 * xv[14]) + (am1 * xv[9] + ap1 * xv[13]));
-                    const T ddni2 = (cf[4] * (xv[0] + xv[8]) + cf[5] * (xv[1] + xv[7])) +
-                                    (cf[6] * (xv[2] + xv[6]) + cf[7] * (xv[3] + xv[5]));
-                    const T ddni1 = (cf[0] * (xv[0] - xv[8]) + cf[1] * (xv[1] - xv[7])) +
-                                    (cf[2] * (xv[2] - xv[6]) + cf[3] * (xv[3] - xv[5]));
-                    const int cb = cbv[q][j];
-                    const T cc = (dmu + ddni2) + dnu_s[cb] * ddni1;
-                    const T v = (rv[rr][j] - cc) * lv[rr][j] + om1 * xv[4];
-                    T *w = okv[q][j] ? b0 + cb + rr * W + 4 : sink + tid;
-                    *w = v;
-                }
+            for (int j = 0; j < NT; ++j) {
+                const T *xv = x[j];
+                const T dmu = (am4 * xv[12] + ap4 * xv[16]) + (am3 * xv[11] + ap3 * xv[15]) +
+                              ((am2 * xv[10] + ap2 * xv[14]) + (am1 * xv[9] + ap1 * xv[13]));
+                const T ddni2 = (cf[4] * (xv[0] + xv[8]) + cf[5] * (xv[1] + xv[7])) +
+                                (cf[6] * (xv[2] + xv[6]) + cf[7] * (xv[3] + xv[5]));
+                const T ddni1 = (cf[0] * (xv[0] - xv[8]) + cf[1] * (xv[1] - xv[7])) +
+                                (cf[2] * (xv[2] - xv[6]) + cf[3] * (xv[3] - xv[5]));
+                const int cb = cbv[j];
+                const T cc = (dmu + ddni2) + dnu_s[cb] * ddni1;
+                const T v = (rv[rr][j] - cc) * lv[rr][j] + om1 * xv[4];
+                T *w = okv[j] ? b0 + cb + rr * W + 4 : sink + tid;
+                *w = v;
             }
         }
     };
@@ -687,9 +696,9 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
     // rows 2 and 3 sit in the slot that the commit of the same step reuses)
     auto row_one = [&]() {
         const int q2 = 2 - r0;  // rows 2..6 sit at strip offsets 16..20
-        const T *rw[5] = {block_row(q2 >> 2, q2 & 3), block_row((q2 + 1) >> 2, (q2 + 1) & 3),
-                          block_row((q2 + 2) >> 2, (q2 + 2) & 3), block_row((q2 + 3) >> 2, (q2 + 3) & 3),
-                          block_row((q2 + 4) >> 2, (q2 + 4) & 3)};
+        const T *rw[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rw[k] = ring + (slot_of((q2 + k) >> 2) * 4 + ((q2 + k) & 3)) * W;
         auto r1 = [&](int nu) -> T {
             const int c0 = nu + 3;
             return d.ex[0] * rw[0][c0] + d.ex[1] * rw[1][c0] + d.ex[2] * rw[2][c0] + d.ex[3] * rw[3][c0] +
@@ -706,15 +715,16 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
                     v = r1(nu);
                 }
             }
-            dst[(long)4 * N8 + nu + 3] = v;
+            d.dst[4 * P + nu + 3] = v;
         }
     };
 
     if (PIPE == 2) {
         // Block s+2 arrives in LDS at the end of step s from the register set
-        // loaded in step s-1; the lhs / rhs of step s were loaded in step s-2.
-        // Two register sets of each alternate between even and odd steps.
-        auto step = [&](int s, const T (&pc)[NS], T (&pn)[NS], const T &bc, T &bn, T (&lc)[4][NT],
+        // loaded in step s-1; the (omega/lhs, rhs) of step s were loaded in
+        // step s-2.  Two register sets of each alternate between even and odd
+        // steps.
+        auto step = [&](int s, const u32x4 (&pc)[NSV], u32x4 (&pn)[NSV], const T &bc, T &bn, T (&lc)[4][NT],
                         T (&rcv)[4][NT]) {
             load_block(s + 3, pn, bn);
             compute(s, lc, rcv);
@@ -731,7 +741,8 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
             off0 = off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA;
             __syncthreads();
         };
-        T pa[NS], pb[NS], ba, bb;
+        u32x4 pa[NSV], pb[NSV];
+        T ba, bb;
         T la[4][NT], ra[4][NT], lb[4][NT], rb[4][NT];
         load_block(2, pa, ba);
         load_lr(0, off0, la, ra);
@@ -741,14 +752,15 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
             if (s + 1 < S) step(s + 1, pb, pa, bb, ba, lb, rb);
         }
     } else {
-        // One step ahead: block s+2 and the lhs / rhs of step s+1 are loaded at
-        // the start of step s and have landed before its commit, so the next
-        // compute phase never waits on them.
+        // One step ahead: block s+2 and the (omega/lhs, rhs) of step s+1 are
+        // loaded at the start of step s and have landed before its commit, so
+        // the next compute phase never waits on them.
         T lv[4][NT], rv[4][NT];
         load_lr(0, off0, lv, rv);
         __builtin_amdgcn_s_waitcnt(0);
         for (int s = 0; s < S; ++s) {
-            T ln[4][NT], rn[4][NT], pre[NS], bpre;
+            T ln[4][NT], rn[4][NT], bpre;
+            u32x4 pre[NSV];
             load_lr(s + 1, off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA, ln, rn);
             load_block(s + 2, pre, bpre);
             compute(s, lv, rv);
@@ -781,17 +793,18 @@ struct PadDesc {
     double *w1;
     double *w2;  // optional second copy (ping-pong partner), may be null
     int nni, nmu, isym;
+    int ps;      // row stride of w1 / w2 (>= nni+8; columns beyond nni+8 are zero)
 };
 
 __global__ void k_putin(const PadDesc *__restrict__ pd)
 {
     const PadDesc &d = pd[blockIdx.y];
-    const int nni = d.nni, nmu = d.nmu, N8 = nni + 8;
-    const long tot = (long)N8 * (nmu + 8);
+    const int nni = d.nni, nmu = d.nmu, PS = d.ps;
+    const long tot = (long)PS * (nmu + 8);
     const double s = d.isym == 1 ? 1.0 : -1.0;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
          e += (long)gridDim.x * blockDim.x) {
-        const int P = (int)(e / N8), Q = (int)(e - (long)P * N8);
+        const int P = (int)(e / PS), Q = (int)(e - (long)P * PS);
         const int mu = P - 3, nu = Q - 3;   // 1-based grid indices, halo range -3..n+4
         double v = 0.0;
         const bool nu_in = nu >= 1 && nu <= nni, mu_in = mu >= 1 && mu <= nmu;
@@ -812,13 +825,34 @@ __global__ void k_putin(const PadDesc *__restrict__ pd)
 __global__ void k_putout(const PadDesc *__restrict__ pd)
 {
     const PadDesc &d = pd[blockIdx.y];
-    const int nni = d.nni, nmu = d.nmu, N8 = nni + 8;
+    const int nni = d.nni, nmu = d.nmu, PS = d.ps;
     const long tot = (long)nni * nmu;
     double *fun = const_cast<double *>(d.fun);
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
          e += (long)gridDim.x * blockDim.x) {
         const int mu = 1 + (int)(e / nni), nu = 1 + (int)(e % nni);
-        fun[e] = d.w1[(long)(mu + 3) * N8 + nu + 3];
+        fun[e] = d.w1[(long)(mu + 3) * PS + nu + 3];
+    }
+}
+
+// A padded array copied between row strides (destination columns beyond
+// the source's are zero), into one or two destinations.
+struct RestrideDesc {
+    const double *src;
+    double *dst1, *dst2;  // dst2 may be null
+    int cols, rows;       // nni+8, nmu+8
+    int ss, ds;           // source / destination row strides (ds >= cols)
+};
+
+__global__ void k_restride(const RestrideDesc *__restrict__ rd)
+{
+    const RestrideDesc &d = rd[blockIdx.y];
+    const long tot = (long)d.ds * d.rows;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(e / d.ds), c = (int)(e - (long)r * d.ds);
+        const double v = c < d.cols ? d.src[(long)r * d.ss + c] : 0.0;
+        d.dst1[e] = v;
+        if (d.dst2) d.dst2[e] = v;
     }
 }
 
@@ -893,7 +927,7 @@ struct ResDesc {
     const double *bmu;
     const double *dnu;
     float *res_f;          // FP32 residual (may be null)
-    int res_split;         // res_f in the residue-split layout (else unpadded nni*nmu)
+    int res_split;         // res_f: component 1 of the residue-split pairs (else unpadded nni*nmu)
     double *acc;           // [2] per problem: sum res^2, sum rhs^2 (may be null)
     double c[16];
     int nni, nmu, isym;
@@ -925,7 +959,7 @@ __global__ void k_residual(const ResDesc *__restrict__ rd)
             r = d.rhs[e] - (cc + d.lhs[e] * f[e]);
             sr += r * r;
             sb += d.rhs[e] * d.rhs[e];
-            if (d.res_f && d.res_split) d.res_f[split_index(nu, mu, split_m(nni))] = (float)r;
+            if (d.res_f && d.res_split) d.res_f[2 * split_index(nu, mu, split_m(nni)) + 1] = (float)r;
         }
         if (d.res_f && !d.res_split) d.res_f[e] = (float)r;
     }
@@ -946,6 +980,7 @@ struct CorrDesc {
     double *fun;        // unpadded FP64 u, corrected in place
     const float *e;     // padded FP32 correction
     int nni, nmu;
+    int ps;             // row stride of e
 };
 
 // u += e on the relaxed interior.
@@ -953,7 +988,7 @@ __global__ void k_correct(const CorrDesc *__restrict__ cd)
 {
     const CorrDesc &d = cd[blockIdx.y];
     const int nni = d.nni, nmu = d.nmu;
-    const long s1 = nni + 8;
+    const long s1 = d.ps;
     const long tot = (long)(nni - 2) * (nmu - 5);
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
          k += (long)gridDim.x * blockDim.x) {
@@ -1042,7 +1077,7 @@ static int stage_descs(x2g_ctx *ctx, const std::vector<D> &v, size_t & /*unused*
 
 int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w2)
 {
-    std::vector<PadDesc> v{{fun, w1, w2, g->nni, g->nmu, isym}};
+    std::vector<PadDesc> v{{fun, w1, w2, g->nni, g->nmu, isym, g->nni + 8}};
     size_t off = 0;
     const PadDesc *dd;
     int rc = stage_descs(g->ctx, v, off, &dd);
@@ -1058,7 +1093,7 @@ int launch_putin_items(x2g_ctx *ctx, const std::vector<PadItem> &items)
     std::vector<PadDesc> v;
     long maxpad = 0;
     for (auto &it : items) {
-        v.push_back({it.fun, it.w, nullptr, it.nni, it.nmu, it.isym});
+        v.push_back({it.fun, it.w, nullptr, it.nni, it.nmu, it.isym, it.nni + 8});
         maxpad = std::max<long>(maxpad, (long)(it.nni + 8) * (it.nmu + 8));
     }
     const PadDesc *dd;
@@ -1071,7 +1106,7 @@ int launch_putin_items(x2g_ctx *ctx, const std::vector<PadItem> &items)
 
 int launch_putout(x2g_grid *g, double *fun, const double *work)
 {
-    std::vector<PadDesc> v{{fun, const_cast<double *>(work), nullptr, g->nni, g->nmu, 1}};
+    std::vector<PadDesc> v{{fun, const_cast<double *>(work), nullptr, g->nni, g->nmu, 1, g->nni + 8}};
     size_t off = 0;
     const PadDesc *dd;
     int rc = stage_descs(g->ctx, v, off, &dd);
@@ -1201,8 +1236,9 @@ static bool sweep_fits(x2g_ctx *ctx, int maxnni)
     sweep_lanes(maxnni, TG, NT, nth);
     const int W = sweep_stride<T>(maxnni);
     if (W == 0 || NT > 2 || (NT == 2) != (W == 768)) return false;
-    const int NS = NT == 1 ? 5 : 8;
-    if (4 * (maxnni + 8) > NS * nth) return false;
+    // a block's 16-byte chunks must fit the threads' copy slots
+    const int P = sweep_row_stride(maxnni, sweep_vec<T>());
+    if (4 * P / sweep_vec<T>() > sweep_nsv<T>() * nth || P > W) return false;
     return sweep_lds_elems(W, nth) * sizeof(T) <= ctx->max_lds;
 }
 
@@ -1335,14 +1371,21 @@ static void fill_desc(SweepDesc<T> &d, x2g_grid *g, const T *lhs, const T *rhs, 
     d.nni = g->nni;
     d.nmu = g->nmu;
     d.isym = isym;
-    d.trash = (T *)g->ctx->trash;
     d.cA = colour_mult(g->nni);
     d.wg_begin = 0;
     d.L = 0;
 }
 
+// Row stride of the padded arrays that a sweep of this kind works on.
+template <typename T>
+static int pad_row_stride(int nni, int kind)
+{
+    return kind == 2 ? sweep_row_stride(nni, sweep_vec<T>()) : nni + 8;
+}
+
 // Build the even/odd descriptor lists of a batch; a_bufs/b_bufs are the
-// ping-pong padded arrays.  Strip engine when every problem fits the ring.
+// ping-pong padded arrays, with the row stride pad_row_stride<T>(nni, kind)
+// of the batch's kind sweep_kind<T>(ctx, widest nni).
 template <typename T>
 static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vector<T *> &a_bufs,
                        std::vector<T *> &b_bufs, SweepPlan<T> &plan)
@@ -1377,6 +1420,8 @@ static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vecto
         d.L = L;
         d.wg_begin = wg;
         wg += (d.nmu - 5 + L - 1) / L;
+        d.P = pad_row_stride<T>(d.nni, plan.kind);
+        d.pad_bytes = (unsigned)((size_t)d.P * (d.nmu + 8) * sizeof(T));
     }
     plan.total_wg = wg;
     plan.even = base;
@@ -1411,6 +1456,18 @@ static int launch_pack(x2g_ctx *ctx, const std::vector<PackDesc<TO>> &v)
 }
 
 static inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+static int launch_restride(x2g_ctx *ctx, const std::vector<RestrideDesc> &v)
+{
+    const RestrideDesc *dd;
+    int rc = stage_n(ctx, v.data(), v.size(), &dd);
+    if (rc) return rc;
+    long maxe = 0;
+    for (auto &d : v) maxe = std::max<long>(maxe, (long)d.ds * d.rows);
+    hipLaunchKernelGGL(k_restride, dim3(grid_x_for(maxe, 256), (int)v.size()), dim3(256), 0, ctx->stream, dd);
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
 
 }  // namespace x2g
 
@@ -1461,27 +1518,37 @@ extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const
     x2g_ctx *ctx = g->ctx;
     rc = set_lds_attr(ctx);
     if (rc) return rc;
-    // second padded buffer for the ping-pong (a copy of work), then the
-    // residue-split lhs / rhs when k_sweep runs
-    const size_t bytes = g->npad() * sizeof(double);
-    const size_t sbytes = align256(split_size(g->nni, g->nmu) * sizeof(double));
-    rc = ws_reserve(ctx, 0, align256(bytes) + 2 * sbytes);
+    const int kind = sweep_kind<double>(ctx, g->nni);
+    const int P = pad_row_stride<double>(g->nni, kind);
+    const size_t ubytes = g->npad() * sizeof(double);                 // the caller's array (stride nni+8)
+    const size_t pbytes = (size_t)P * (g->nmu + 8) * sizeof(double);  // a sweep buffer (stride P)
+    const size_t lrbytes = 2 * split_size(g->nni, g->nmu) * sizeof(double);
+    // k_sweep: two buffers of stride P and the (omega/lhs, rhs) pairs; the
+    // other kinds: a ping-pong partner of the caller's array
+    const bool restride = kind == 2;
+    rc = ws_reserve(ctx, 0, restride ? 2 * align256(pbytes) + align256(lrbytes) : align256(ubytes));
     if (rc) return rc;
     char *w0 = (char *)ctx->ws[0].ptr;
-    double *alt = (double *)w0;
-    X2G_HIP(hipMemcpyAsync(alt, work_d, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    double *A = work_d, *B = (double *)w0;
+    if (restride) {
+        A = (double *)w0;
+        B = (double *)(w0 + align256(pbytes));
+        if ((rc = launch_restride(ctx, {{work_d, A, B, g->nni + 8, g->nmu + 8, g->nni + 8, P}}))) return rc;
+    } else {
+        X2G_HIP(hipMemcpyAsync(B, work_d, ubytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     std::vector<SweepDesc<double>> base(1);
     fill_desc<double>(base[0], g, lhs_d, rhs_d, bkind == 0 ? g->bmu_orb : g->bmu_pot, g->dnu, isym, omega);
-    std::vector<double *> A{work_d}, B{alt};
+    std::vector<double *> VA{A}, VB{B};
     SweepPlan<double> plan;
-    plan_sweeps(ctx, base, A, B, plan);
+    plan_sweeps(ctx, base, VA, VB, plan);
     if (plan.kind == 2) {
-        double *lp = (double *)(w0 + align256(bytes)), *rp = (double *)(w0 + align256(bytes) + sbytes);
-        std::vector<PackDesc<double>> pk{{lhs_d, lp, omega, g->nni, g->nmu}, {rhs_d, rp, 0.0, g->nni, g->nmu}};
+        double *lr = (double *)(w0 + 2 * align256(pbytes));
+        std::vector<PackDesc<double>> pk{{lhs_d, lr, omega, g->nni, g->nmu, 0}, {rhs_d, lr, 0.0, g->nni, g->nmu, 1}};
         if ((rc = launch_pack(ctx, pk))) return rc;
         for (auto *v : {&plan.even, &plan.odd}) {
-            (*v)[0].lhs = lp;
-            (*v)[0].rhs = rp;
+            (*v)[0].lr = lr;
+            (*v)[0].lr_bytes = (unsigned)lrbytes;
         }
     }
     size_t off = 0;
@@ -1494,8 +1561,12 @@ extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const
         rc = launch_sweep<double>(ctx, ev ? de : dodd, ev ? plan.even : plan.odd, plan);
         if (rc) return rc;
     }
-    if (!in_place && (nsweeps % 2) == 1)
-        X2G_HIP(hipMemcpyAsync(work_d, alt, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    double *fin = (!in_place && (nsweeps % 2) == 1) ? B : A;
+    if (restride) {
+        if ((rc = launch_restride(ctx, {{fin, work_d, nullptr, g->nni + 8, g->nmu + 8, P, g->nni + 8}}))) return rc;
+    } else if (fin != work_d) {
+        X2G_HIP(hipMemcpyAsync(work_d, fin, ubytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     return X2G_OK;
 }
 
@@ -1529,10 +1600,11 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     if (n == 0 || maxsor1 == 0) return X2G_OK;
     int rc = set_lds_attr(ctx);
     if (rc) return rc;
-    // validate and size the workspace: two padded FP64 arrays per item, plus
-    // for FP32 items two padded FP32 arrays and FP32 lhs/rhs.
-    size_t off64 = 0, off32 = 0;
-    std::vector<size_t> o64(n), o32(n);
+    // validate, pick the sweep kind of each precision's batch (it fixes the
+    // row stride of the padded arrays), and size the workspace: per FP64
+    // item two padded arrays and the (omega/lhs, rhs) pairs; per FP32 item
+    // two padded correction arrays and the FP32 lhs / residual.
+    int maxnni64 = 0, maxnni32 = 0;
     for (int i = 0; i < n; ++i) {
         const x2g_relax_item &it = items[i];
         X2G_CHECK(it.grid_index >= 0 && grids[it.grid_index], "x2g_relax_batch: bad grid index %d", it.grid_index);
@@ -1544,12 +1616,32 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         X2G_CHECK(it.prec == 0 || it.prec == 1, "x2g_relax_batch: item %d prec", i);
         X2G_CHECK(it.omega > 0.0 && it.omega < 2.0, "x2g_relax_batch: item %d omega must lie in (0, 2)", i);
         if ((rc = check_grid_sweepable(g))) return rc;
-        o64[i] = off64;  // FP64 items: two padded buffers and the split lhs / rhs
-        if (it.prec == 0)
-            off64 += 2 * align256(g->npad() * sizeof(double)) + 2 * align256(split_size(g->nni, g->nmu) * sizeof(double));
+        (it.prec ? maxnni32 : maxnni64) = std::max(it.prec ? maxnni32 : maxnni64, g->nni);
+    }
+    const int kind64 = maxnni64 ? sweep_kind<double>(ctx, maxnni64) : 0;
+    const int kind32 = maxnni32 ? sweep_kind<float>(ctx, maxnni32) : 0;
+    auto pstride = [&](int i) {
+        x2g_grid *g = grids[items[i].grid_index];
+        return items[i].prec ? pad_row_stride<float>(g->nni, kind32) : pad_row_stride<double>(g->nni, kind64);
+    };
+    auto padbytes = [&](int i) {  // one padded array of item i
+        x2g_grid *g = grids[items[i].grid_index];
+        return (size_t)pstride(i) * (g->nmu + 8) * (items[i].prec ? sizeof(float) : sizeof(double));
+    };
+    auto lrbytes = [&](int i) {  // (omega/lhs, rhs) pairs of item i
+        x2g_grid *g = grids[items[i].grid_index];
+        return (size_t)2 * split_size(g->nni, g->nmu) * (items[i].prec ? sizeof(float) : sizeof(double));
+    };
+    size_t off64 = 0, off32 = 0;
+    std::vector<size_t> o64(n), o32(n);
+    for (int i = 0; i < n; ++i) {
+        x2g_grid *g = grids[items[i].grid_index];
+        o64[i] = off64;
         o32[i] = off32;
-        if (it.prec == 1)
-            off32 += 2 * align256(g->npad() * sizeof(float)) +
+        if (items[i].prec == 0)
+            off64 += 2 * align256(padbytes(i)) + align256(lrbytes(i));
+        else
+            off32 += 2 * align256(padbytes(i)) +
                      2 * align256(std::max<size_t>(g->n(), split_size(g->nni, g->nmu)) * sizeof(float));
     }
     if (off64 && (rc = ws_reserve(ctx, 0, off64))) return rc;
@@ -1561,19 +1653,13 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     for (int i = 0; i < n; ++i) (items[i].prec ? i32 : i64).push_back(i);
 
     auto bufA = [&](int i) { return (double *)(w64 + o64[i]); };
-    auto bufB = [&](int i) {
+    auto bufB = [&](int i) { return (double *)(w64 + o64[i] + align256(padbytes(i))); };
+    auto lrp = [&](int i) { return (double *)(w64 + o64[i] + 2 * align256(padbytes(i))); };  // FP64 pairs
+    // FP32 item: k = 0 eA, 1 eB, 2 lhs_f (k_sweep: the (omega/lhs, residual)
+    // pairs, which extend over the region of k = 3), 3 res_f
+    auto f32p = [&](int i, int k) {
         x2g_grid *g = grids[items[i].grid_index];
-        return (double *)(w64 + o64[i] + ((g->npad() * sizeof(double) + 255) & ~size_t(255)));
-    };
-    auto splitp = [&](int i, int k) {  // FP64 residue-split lhs (k=0) / rhs (k=1)
-        x2g_grid *g = grids[items[i].grid_index];
-        const size_t pp = align256(g->npad() * sizeof(double));
-        const size_t ss = align256(split_size(g->nni, g->nmu) * sizeof(double));
-        return (double *)(w64 + o64[i] + 2 * pp + k * ss);
-    };
-    auto f32p = [&](int i, int k) {  // k: 0 eA, 1 eB, 2 lhs_f, 3 res_f
-        x2g_grid *g = grids[items[i].grid_index];
-        const size_t pp = align256(g->npad() * sizeof(float));
+        const size_t pp = align256(padbytes(i));
         const size_t nn = align256(std::max<size_t>(g->n(), split_size(g->nni, g->nmu)) * sizeof(float));
         size_t o = o32[i] + (k <= 1 ? k * pp : 2 * pp + (k - 2) * nn);
         return (float *)(w32 + o);
@@ -1588,7 +1674,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     for (int i = 0; i < n; ++i) {
         const x2g_relax_item &it = items[i];
         x2g_grid *g = grids[it.grid_index];
-        if (it.prec == 0) pin.push_back({it.fun_d, bufA(i), bufB(i), g->nni, g->nmu, it.isym});
+        if (it.prec == 0) pin.push_back({it.fun_d, bufA(i), bufB(i), g->nni, g->nmu, it.isym, pstride(i)});
         if (it.tail_fock1_d)
             tails.push_back({it.fun_d, it.tail_fock1_d, g->suppl + 5 * g->n(), g->vxisq, g->vetasq, g->nni, g->nmu});
     }
@@ -1619,11 +1705,11 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             for (size_t k = 0; k < i64.size(); ++k) {
                 const x2g_relax_item &it = items[i64[k]];
                 x2g_grid *g = grids[it.grid_index];
-                pk.push_back({it.lhs_d, splitp(i64[k], 0), it.omega, g->nni, g->nmu});
-                pk.push_back({it.rhs_d, splitp(i64[k], 1), 0.0, g->nni, g->nmu});
+                pk.push_back({it.lhs_d, lrp(i64[k]), it.omega, g->nni, g->nmu, 0});
+                pk.push_back({it.rhs_d, lrp(i64[k]), 0.0, g->nni, g->nmu, 1});
                 for (auto *v : {&p64.even, &p64.odd}) {
-                    (*v)[k].lhs = splitp(i64[k], 0);
-                    (*v)[k].rhs = splitp(i64[k], 1);
+                    (*v)[k].lr = lrp(i64[k]);
+                    (*v)[k].lr_bytes = (unsigned)lrbytes(i64[k]);
                 }
             }
             if ((rc = launch_pack(ctx, pk))) return rc;
@@ -1634,7 +1720,8 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         for (size_t k = 0; k < i64.size(); ++k) {
             const x2g_relax_item &it = items[i64[k]];
             x2g_grid *g = grids[it.grid_index];
-            pout64.push_back({it.fun_d, final_in_b ? bufB(i64[k]) : bufA(i64[k]), nullptr, g->nni, g->nmu, 1});
+            pout64.push_back({it.fun_d, final_in_b ? bufB(i64[k]) : bufA(i64[k]), nullptr, g->nni, g->nmu, 1,
+                              pstride(i64[k])});
         }
     }
     const PadDesc *d_pout64 = nullptr;
@@ -1671,16 +1758,16 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             r.rhs = it.rhs_d;
             r.bmu = it.bkind == 0 ? g->bmu_orb : g->bmu_pot;
             r.dnu = g->dnu;
-            r.res_f = f32p(i, 3);
-            r.res_split = 0;
+            r.res_f = kind32 == 2 ? f32p(i, 2) : f32p(i, 3);
+            r.res_split = kind32 == 2;
             r.acc = nullptr;
             for (int q = 0; q < 16; ++q) r.c[q] = g->coef[q];
             r.nni = g->nni;
             r.nmu = g->nmu;
             res.push_back(r);
-            corr.push_back({it.fun_d, f32p(i, 0), g->nni, g->nmu});
+            corr.push_back({it.fun_d, f32p(i, 0), g->nni, g->nmu, pstride(i)});
             cvt.push_back({it.lhs_d, f32p(i, 2), (long)g->n()});
-            zero.push_back({f32p(i, 0), (long)(2 * ((g->npad() * sizeof(float) + 255) & ~size_t(255)) / sizeof(float))});
+            zero.push_back({f32p(i, 0), (long)(2 * align256(padbytes(i)) / sizeof(float))});
             BndDesc bd{};
             bd.fun = it.fun_d;
             for (int q = 0; q < 5; ++q) bd.ex[q] = g->coef[16 + q];
@@ -1695,7 +1782,11 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             for (size_t k = 0; k < i32.size(); ++k) corr[k].e = f32p(i32[k], 1);
         const bool split32 = p32.kind == 2;
         if (split32)
-            for (auto &r : res) r.res_split = 1;
+            for (size_t k = 0; k < i32.size(); ++k)
+                for (auto *v : {&p32.even, &p32.odd}) {
+                    (*v)[k].lr = f32p(i32[k], 2);
+                    (*v)[k].lr_bytes = (unsigned)lrbytes(i32[k]);
+                }
         if ((rc = stage_descs(ctx, p32.even, doff, &d32e))) return rc;
         if ((rc = stage_descs(ctx, p32.odd, doff, &d32o))) return rc;
         if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
@@ -1707,7 +1798,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             std::vector<PackDesc<float>> pk;
             for (int i : i32) {
                 x2g_grid *g = grids[items[i].grid_index];
-                pk.push_back({items[i].lhs_d, f32p(i, 2), items[i].omega, g->nni, g->nmu});
+                pk.push_back({items[i].lhs_d, f32p(i, 2), items[i].omega, g->nni, g->nmu, 0});
             }
             if ((rc = launch_pack(ctx, pk))) return rc;
         } else {
@@ -1722,7 +1813,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     long maxpad = 0, maxn = 0, maxnni = 0;
     for (int i = 0; i < n; ++i) {
         x2g_grid *g = grids[items[i].grid_index];
-        maxpad = std::max<long>(maxpad, (long)g->npad());
+        maxpad = std::max<long>(maxpad, (long)pstride(i) * (g->nmu + 8));
         maxn = std::max<long>(maxn, (long)g->n());
         maxnni = std::max<long>(maxnni, g->nni);
     }
