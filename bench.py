@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--cpu-iters", default="61,81", help="SCF iterations of the two timed reference runs")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run a copy kernel of known size and a few sweeps (for rocprofv3 --pmc)")
+    ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn the counter_collection.csv files of the FETCH_SIZE and WRITE_SIZE runs of "
+                         "--traffic-probe into the per-launch HBM traffic file read by the benchmark")
     return ap.parse_args()
 
 
@@ -141,8 +144,47 @@ def traffic_probe(batch):
     print("traffic probe done", flush=True)
 
 
+def traffic_summary(fetch_csv, write_csv, batch):
+    """HBM bytes per sweep launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs of --traffic-probe.  Each counter (KiB per dispatch) is scaled by
+    what it reads for the probe's copy kernel (x2g_lincomb, bytes known), and
+    averaged over the probe's FP32 (first five) and FP64 (last five) sweep
+    launches."""
+    import csv
+    n_copy = 1201 * 1201 * 8  # bytes read and written by one copy launch
+
+    def per_kernel(path):
+        rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
+        copy = [float(r["Counter_Value"]) * 1024 for r in rows if r["Kernel_Name"].startswith("k_lincomb")]
+        sweep = [float(r["Counter_Value"]) * 1024 for r in rows if r["Kernel_Name"].startswith("k_sweep")]
+        if len(copy) != 3 or len(sweep) != 10:
+            raise SystemExit(f"{path}: expected 3 copy and 10 sweep dispatches, found {len(copy)} and {len(sweep)}")
+        return sum(copy) / len(copy), sweep[:5], sweep[5:]
+
+    fc, f32, f64 = per_kernel(fetch_csv)
+    wc, w32, w64 = per_kernel(write_csv)
+    kf, kw = n_copy / fc, n_copy / wc  # counter -> bytes factors from the copy kernel
+    out = {"workload": f"{batch} x 22 potentials, N2 331x601, one sweep launch", "fetch_factor": kf,
+           "write_factor": kw}
+    algo = {4: 0.0, 8: 0.0}
+    for t in (4, 8):
+        algo[t] = batch * 22 * 4.0 * t * (331 - 2) * (601 - 5)
+    for key, fv, wv, t in (("fp32", f32, w32, 4), ("fp64", f64, w64, 8)):
+        rd = kf * sum(fv) / len(fv)
+        wr = kw * sum(wv) / len(wv)
+        out[key] = {"hbm_read_bytes": rd, "hbm_write_bytes": wr, "hbm_bytes_per_launch": rd + wr,
+                    "algorithmic_bytes_per_launch": algo[t], "traffic_over_algorithmic": (rd + wr) / algo[t]}
+    os.makedirs(os.path.dirname(TRAFFIC), exist_ok=True)
+    with open(TRAFFIC, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     args = parse()
+    if args.traffic_summary:
+        traffic_summary(*args.traffic_summary, args.batch)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (use --gpus 1)",
@@ -197,10 +239,13 @@ def main():
     avg_us = ms / launches * 1e3 if launches else None
     per_launch = nbytes / launches if launches else None
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms else None
-    traffic = None
-    if os.path.exists(TRAFFIC):
-        tr = json.load(open(TRAFFIC))
-        traffic = tr.get("fp32" if k == 1 else "fp64", {}).get("hbm_bytes_per_launch")
+    # HBM traffic per launch: the ratio of measured to algorithmic bytes of the
+    # 352-potential launch (rocprofv3 counters on --traffic-probe, stored by
+    # --traffic-summary) applied to this run's average launch
+    traffic, ratio = None, None
+    if os.path.exists(TRAFFIC) and per_launch:
+        ratio = json.load(open(TRAFFIC)).get("fp32" if k == 1 else "fp64", {}).get("traffic_over_algorithmic")
+        traffic = ratio * per_launch if ratio else None
     cpu = None if args.no_cpu else cpu_baseline(ref_iters, [int(x) for x in args.cpu_iters.split(",")])
     line = {
         "metric": METRIC,
@@ -229,7 +274,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "kernel": "k_sweep<float>" if k == 1 else "k_sweep<double>",
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": avg_us,
-                     "launches": launches},
+                     "launches": launches, "traffic_over_algorithmic": ratio},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
