@@ -343,16 +343,20 @@ __global__ void __launch_bounds__(512) k_strip(const SweepDesc<T> *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Residue-split layout of lhs / rhs for k_sweep.  Relaxed point (nu, mu),
-// nu = 2..nni-1, mu = 2..nmu-4, is stored at
-//   (mu-2)*5*M + ((nu-2) mod 5)*M + (nu-2) div 5,   M = ceil((nni-2)/5),
-// so that the points of one colour pass of one row are contiguous.
+// Colour-ordered layout of lhs / rhs for k_sweep.  Relaxed point (nu, mu),
+// nu = 2..nni-1, mu = 2..nmu-4, of colour c = (nu-2 + cA*(mu-2)) mod 5 is
+// stored at
+//   (mu-2)*5*M + c*M + (nu-2) div 5,   M = ceil((nni-2)/5),
+// so the points of one colour pass of one row are contiguous and the t-th
+// point of pass c in any row sits at position t of that row's c-block (a row
+// holds M or M-1 points of each colour; a short block leaves its last slot
+// unused).
 __host__ __device__ inline int split_m(int nni) { return (nni - 2 + 4) / 5; }
 __host__ __device__ inline long split_size(int nni, int nmu) { return (long)(nmu - 5) * 5 * split_m(nni); }
-__device__ __forceinline__ long split_index(int nu, int mu, int M)
+__device__ __forceinline__ long split_index(int nu, int mu, int M, int cA)
 {
     const int q = nu - 2;
-    return (long)(mu - 2) * 5 * M + (long)(q % 5) * M + q / 5;
+    return (long)(mu - 2) * 5 * M + (long)((q + cA * (mu - 2)) % 5) * M + q / 5;
 }
 
 template <typename TO>
@@ -367,20 +371,24 @@ template <typename TO>
 __global__ void k_pack(const PackDesc<TO> *__restrict__ pd)
 {
     const PackDesc<TO> &d = pd[blockIdx.y];
-    const int nni = d.nni, nrow = nni - 2, M = split_m(nni);
+    const int nni = d.nni, nrow = nni - 2, M = split_m(nni), cA = colour_mult(nni);
     const long npts = (long)nrow * (d.nmu - 5);
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < npts; k += (long)gridDim.x * blockDim.x) {
         const int mu = 2 + (int)(k / nrow), nu = 2 + (int)(k % nrow);
         const double v = d.src[(long)(mu - 1) * nni + (nu - 1)];
-        d.dst[2 * split_index(nu, mu, M) + d.comp] = (TO)(d.omega > 0.0 ? d.omega / v : v);
+        d.dst[2 * split_index(nu, mu, M, cA) + d.comp] = (TO)(d.omega > 0.0 ? d.omega / v : v);
     }
 }
 
 // ---------------------------------------------------------------------------
 // k_sweep: the pipelined strip sweep (same block schedule as k_strip with
-// SKEW = 2) with a fixed lane -> work map.  Lane tid < 5*TG serves colour
-// pass p = tid / TG and the colour points t = tid % TG + j*TG (j < NT) of each
-// of the four rows of its current block.  The ring rows have a compile-time
+// SKEW = 2) with a fixed lane -> work map.  Wave p < 5 serves colour pass p:
+// its lane l the colour points t = l + 64 j (j < NT) of each of the four rows
+// of the pass's current block.  With the pass uniform across the wave, all
+// that depends on it (block, ring slot, rows, colour offset, lhs / rhs row
+// base) is scalar work.  The points t >= 64 NT of each pass (MS of them per
+// row at most) go to spill waves, one point per lane, pass (tid-320) / MS.
+// The ring rows have a compile-time
 // stride W >= nni+16, so a point reads its 17 stencil values with immediate
 // offsets from three block bases, and every point of a step is processed
 // without branches: points outside the pass's block range or the grid are
@@ -397,7 +405,8 @@ __global__ void k_pack(const PackDesc<TO> *__restrict__ pd)
 // written back the same way, pad columns included (src and dst hold the same
 // pads, which the sweep never changes).  lhs / rhs come as one array of
 // (omega/lhs, rhs) pairs in the residue-split order (contiguous per pass and
-// row), one pair load per point.  A point update is
+// row, the t-th point of a pass at position t), one pair load per point.
+// A point update is
 //   f = (rhs - L f) * (omega/lhs) + (1-omega) f,
 // the d/dmu part of L f with per-row weights  c2_k +- b(mu) c1_k  (k = 1..4).
 //
@@ -422,7 +431,11 @@ __host__ __device__ inline int sweep_row_stride(int nni, int vec) { return (nni 
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-constexpr unsigned kOOB = 0x80000000u;  // a byte offset outside every array (arrays are < 2 GiB)
+// Byte offsets outside every array (arrays are < 1 GiB): kOOB for a row or
+// lane without data, kOOBLane added to a row offset for an absent point (the
+// sum of both stays below 2^32).
+constexpr unsigned kOOB = 0x80000000u;
+constexpr unsigned kOOBLane = 0x40000000u;
 constexpr int kRsrcWord3 = 0x00020000;  // buffer resource word 3 for raw buffers on gfx9-family GPUs
 
 // Byte offset `off` where ok, else one outside every array (an offset with
@@ -461,7 +474,7 @@ struct Chunk {
 
 template <typename T, int NT, int PIPE, int W>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && W <= 352 ? 4 : 1)))
-k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
+k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int MS)
 {
     extern __shared__ __align__(16) unsigned char ks_smem[];
     constexpr int SKEW = 2, RB = 4 * SKEW + 4;
@@ -590,36 +603,46 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
         commit_block(b, pr, bv);
     }
 
-    // lane role
-    const int p = tid / TG;  // >= 5: lane only loads and stores
-    const int tl = tid - p * TG;
-    const bool worker = p < 5;
-    int sb = (RB - 2 * p % RB) % RB;                  // ring slot of block s - 2p
-    int off0 = mod5(p - cA * (r0 - 8 * p - 2));        // (nu-2) mod 5 of pass p in row r0 + 4(s - 2p)
+    // Lane roles: pass waves (wave < 5, pass = wave, uniform) and spill lanes
+    // (pass q / MS, point 64 NT + q % MS; pass 5 = idle).  Per point j:
+    // tq = t clamped to M-1 (keeps the LDS reads of absent points inside the
+    // row), lim = 5t for a point of the grid's colour blocks, else a value no
+    // row accepts, and toff = its byte offset in a row of (omega/lhs, rhs)
+    // pairs, or kOOBLane for an absent point.
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool pass_wave = wave < 5;
+    const int q = tid - 320;
+    const int p_spill = pass_wave || MS == 0 ? 5 : min(q / MS, 5);
+    constexpr int kNoPoint = 1 << 20;
+    int tq[NT], lim[NT];
+    unsigned toff[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int t = pass_wave ? (tid & 63) + 64 * j : (j == 0 && p_spill < 5 ? 64 * NT + q % MS : M);
+        const bool ex = t < M;
+        tq[j] = min(t, M - 1);
+        lim[j] = ex ? 5 * t : kNoPoint;
+        toff[j] = ex ? (unsigned)(t * (int)(2 * sizeof(T))) : kOOBLane;
+    }
     const int M5 = 5 * M;
 
-    auto rows_active = [&](int s) {
+    // (omega/lhs, rhs) of pass p's points in step s (rows of block s - 2p);
+    // rows outside the pass's range or the grid and absent points read
+    // zeros, which are never used
+    auto load_lr = [&](int s, int p, T (&lv)[4][NT], T (&rv)[4][NT]) {
         const int b = s - SKEW * p;
-        return worker && b >= p && b <= K - 1 - p;
-    };
-    // (omega/lhs, rhs) of the points of step s (off0s: colour offset of its
-    // first row); inactive points read zeros and are never used
-    auto load_lr = [&](int s, int off0s, T (&lv)[4][NT], T (&rv)[4][NT]) {
-        const bool act = rows_active(s);
-        const int mu0 = r0 + 4 * (s - SKEW * p);
-        int off = off0s;
+        const bool act = p < 5 && b >= p && b <= K - 1 - p;
+        const int mu0 = r0 + 4 * b;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-            if (rr) off = off - cA < 0 ? off - cA + 5 : off - cA;
             const int mu = mu0 + rr;
             const bool rowok = act && mu >= 2 && mu <= nmu - 4;
-            const int ib = (mu - 2) * M5 + off * M + tl;
+            const unsigned rb = rowok ? (unsigned)(((mu - 2) * M5 + p * M) * (int)(2 * sizeof(T))) : kOOB;
 #pragma unroll
             for (int j = 0; j < NT; ++j) {
-                const bool ok = rowok && 2 + off + 5 * (tl + j * TG) <= nni - 1;
-                const Pair<T> q = buf_load2(rs_lr, oob_unless(ok, (ib + j * TG) * (int)(2 * sizeof(T))), T(0));
-                lv[rr][j] = q.a;
-                rv[rr][j] = q.b;
+                const Pair<T> pr = buf_load2(rs_lr, rb + toff[j], T(0));
+                lv[rr][j] = pr.a;
+                rv[rr][j] = pr.b;
             }
         }
     };
@@ -629,7 +652,7 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
     // live in vector registers rather than being re-fetched as scalars)
     T cf[8];  // c[8..15]: d/dnu (c[8..11]) and d2/dnu2 (c[12..15]) weights
 #pragma unroll
-    for (int q = 0; q < 8; ++q) cf[q] = coef_s[q];
+    for (int k = 0; k < 8; ++k) cf[k] = coef_s[k];
     const T om1 = coef_s[8];
 
     // pass p on block s - 2p with the (omega/lhs, rhs) of step s; within an
@@ -638,30 +661,30 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
     // of one pass in a row do not read each other (a point's neighbours have
     // other colours), so a row's stencil values are gathered first and its
     // arithmetic follows, which keeps the LDS reads in flight together.
-    auto compute = [&](int s, const T (&lv)[4][NT], const T (&rv)[4][NT]) {
+    auto compute = [&](int s, int p, const T (&lv)[4][NT], const T (&rv)[4][NT]) {
         const int b = s - SKEW * p;
-        const bool act = worker && b >= p && b <= K - 1 - p;
-        if (!act) return;  // idle lanes and passes outside their block range
+        if (p >= 5 || b < p || b > K - 1 - p) return;  // idle lanes and passes outside their block range
+        const int sb = slot_of(b);
         const int sm = sb == 0 ? RB - 1 : sb - 1, sp = sb == RB - 1 ? 0 : sb + 1;
         const T *const bm = ring + sm * 4 * W;   // block b-1
         T *const b0 = ring + sb * 4 * W;         // block b
         const T *const bp = ring + sp * 4 * W;   // block b+1
         const int mu0 = r0 + 4 * b;
         const T *ab = ab_r + sb * 32;           // row weights of block b
-        int off = off0;
+        int off = mod5(p - cA * (mu0 - 2));     // (nu-2) mod 5 of the pass's points in row mu0
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
             if (rr) off = off - cA < 0 ? off - cA + 5 : off - cA;
             const int mu = mu0 + rr;
-            const bool rowok = mu >= 2 && mu <= nmu - 4;
+            const int lim_r = mu >= 2 && mu <= nmu - 4 ? nni - 3 - off : -1;  // point ok iff 5t <= lim_r
             int cbv[NT];
             bool okv[NT];
             T x[NT][17];  // 0..8: own row at nu-4..nu+4; 9..12: rows mu-1..mu-4; 13..16: rows mu+1..mu+4
 #pragma unroll
             for (int j = 0; j < NT; ++j) {
-                const int cb = 1 + off + 5 * (tl + j * TG);  // = nu - 1 = (column of nu) - 4
+                const int cb = 1 + off + 5 * tq[j];  // = nu - 1 = (column of nu) - 4
                 cbv[j] = cb;
-                okv[j] = rowok && cb <= nni - 2;
+                okv[j] = lim[j] <= lim_r;
                 const T *qm = bm + cb, *qp = bp + cb, *q0 = b0 + cb;
 #pragma unroll
                 for (int k = 0; k < 9; ++k) x[j][k] = q0[rr * W + k];
@@ -691,6 +714,15 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
                 *w = v;
             }
         }
+    };
+    // the two lane roles: the pass of a pass wave is a scalar
+    auto load_lr_any = [&](int s, T (&lv)[4][NT], T (&rv)[4][NT]) {
+        if (pass_wave) load_lr(s, wave, lv, rv);
+        else load_lr(s, p_spill, lv, rv);
+    };
+    auto compute_any = [&](int s, const T (&lv)[4][NT], const T (&rv)[4][NT]) {
+        if (pass_wave) compute(s, wave, lv, rv);
+        else compute(s, p_spill, lv, rv);
     };
     // the mu=1 row once rows 2..6 are final (first strip only, at bw == 5;
     // rows 2 and 3 sit in the slot that the commit of the same step reuses)
@@ -727,9 +759,8 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
         auto step = [&](int s, const u32x4 (&pc)[NSV], u32x4 (&pn)[NSV], const T &bc, T &bn, T (&lc)[4][NT],
                         T (&rcv)[4][NT]) {
             load_block(s + 3, pn, bn);
-            compute(s, lc, rcv);
-            const int off2 = (off0 + 2 * cA) % 5;
-            load_lr(s + 2, off2, lc, rcv);
+            compute_any(s, lc, rcv);
+            load_lr_any(s + 2, lc, rcv);
             const int bw = s - 4 * SKEW - 1;
             if (a == 2 && bw == 5) {
                 row_one();
@@ -737,16 +768,14 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
             }
             commit_block(s + 2, pc, bc);
             write_out(bw);
-            sb = sb == RB - 1 ? 0 : sb + 1;
-            off0 = off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA;
             __syncthreads();
         };
         u32x4 pa[NSV], pb[NSV];
         T ba, bb;
         T la[4][NT], ra[4][NT], lb[4][NT], rb[4][NT];
         load_block(2, pa, ba);
-        load_lr(0, off0, la, ra);
-        load_lr(1, off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA, lb, rb);
+        load_lr_any(0, la, ra);
+        load_lr_any(1, lb, rb);
         for (int s = 0; s < S; s += 2) {
             step(s, pa, pb, ba, bb, la, ra);
             if (s + 1 < S) step(s + 1, pb, pa, bb, ba, lb, rb);
@@ -756,14 +785,14 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
         // loaded at the start of step s and have landed before its commit, so
         // the next compute phase never waits on them.
         T lv[4][NT], rv[4][NT];
-        load_lr(0, off0, lv, rv);
+        load_lr_any(0, lv, rv);
         __builtin_amdgcn_s_waitcnt(0);
         for (int s = 0; s < S; ++s) {
             T ln[4][NT], rn[4][NT], bpre;
             u32x4 pre[NSV];
-            load_lr(s + 1, off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA, ln, rn);
+            load_lr_any(s + 1, ln, rn);
             load_block(s + 2, pre, bpre);
-            compute(s, lv, rv);
+            compute_any(s, lv, rv);
             const int bw = s - 4 * SKEW - 1;
             if (a == 2 && bw == 5) {
                 row_one();
@@ -779,8 +808,6 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int TG)
                     rv[rr][j] = rn[rr][j];
                 }
             write_out(bw);
-            sb = sb == RB - 1 ? 0 : sb + 1;
-            off0 = off0 + cA >= 5 ? off0 + cA - 5 : off0 + cA;
             __syncthreads();
         }
     }
@@ -959,7 +986,8 @@ __global__ void k_residual(const ResDesc *__restrict__ rd)
             r = d.rhs[e] - (cc + d.lhs[e] * f[e]);
             sr += r * r;
             sb += d.rhs[e] * d.rhs[e];
-            if (d.res_f && d.res_split) d.res_f[2 * split_index(nu, mu, split_m(nni)) + 1] = (float)r;
+            if (d.res_f && d.res_split)
+                d.res_f[2 * split_index(nu, mu, split_m(nni), colour_mult(nni)) + 1] = (float)r;
         }
         if (d.res_f && !d.res_split) d.res_f[e] = (float)r;
     }
@@ -1199,7 +1227,7 @@ struct SweepPlan {
     int kind = 0;
     int skew = 0;
     int total_wg = 0;
-    int nth = 0, TG = 0, NT = 0, W = 0;
+    int nth = 0, MS = 0, NT = 0, W = 0;
     size_t lds = 0;
 };
 
@@ -1220,22 +1248,23 @@ static int sweep_stride(int maxnni)
 }
 
 // Lanes of a k_sweep launch for a batch whose widest grid has nni points:
-// TG lanes per colour pass, NT colour points per lane and row.
-static void sweep_lanes(int maxnni, int &TG, int &NT, int &nth)
+// five pass waves with NT colour points per lane and row (64 NT per pass),
+// plus spill waves for the MS remaining points of each pass, one per lane.
+static void sweep_lanes(int maxnni, int &MS, int &NT, int &nth)
 {
     const int M = split_m(maxnni);
-    NT = (M + 101) / 102;
-    TG = (M + NT - 1) / NT;
-    nth = ((5 * TG + 63) / 64) * 64;
+    NT = M <= 102 ? 1 : 2;
+    MS = std::max(0, M - 64 * NT);
+    nth = 320 + 64 * ((5 * MS + 63) / 64);
 }
 
 template <typename T>
 static bool sweep_fits(x2g_ctx *ctx, int maxnni)
 {
-    int TG, NT, nth;
-    sweep_lanes(maxnni, TG, NT, nth);
+    int MS, NT, nth;
+    sweep_lanes(maxnni, MS, NT, nth);
     const int W = sweep_stride<T>(maxnni);
-    if (W == 0 || NT > 2 || (NT == 2) != (W == 768)) return false;
+    if (W == 0 || nth > 512 || (NT == 2) != (W == 768)) return false;
     // a block's 16-byte chunks must fit the threads' copy slots
     const int P = sweep_row_stride(maxnni, sweep_vec<T>());
     if (4 * P / sweep_vec<T>() > sweep_nsv<T>() * nth || P > W) return false;
@@ -1260,7 +1289,7 @@ template <typename T, int NT, int PIPE, int W>
 static void launch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, int n, const SweepPlan<T> &plan)
 {
     hipLaunchKernelGGL((k_sweep<T, NT, PIPE, W>), dim3(plan.total_wg), dim3(plan.nth), plan.lds, ctx->stream, dd, n,
-                       plan.TG);
+                       plan.MS);
 }
 
 template <typename T, int PIPE>
@@ -1401,7 +1430,7 @@ static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vecto
     plan.kind = sweep_kind<T>(ctx, maxnni);
     int L = 0;
     if (plan.kind == 2) {
-        sweep_lanes(maxnni, plan.TG, plan.NT, plan.nth);
+        sweep_lanes(maxnni, plan.MS, plan.NT, plan.nth);
         plan.W = sweep_stride<T>(maxnni);
         plan.skew = 2;
         L = choose_sweep_L(ctx, prow);
@@ -1501,6 +1530,9 @@ extern "C" int x2g_putout(x2g_grid *g, double *fun_d, const double *work_d)
 static int check_grid_sweepable(x2g_grid *g)
 {
     X2G_CHECK(g->nni >= 7 && g->nmu >= 11, "grid %dx%d too small for the 17-point stencil", g->nni, g->nmu);
+    // the sweep kernels address each array with 32-bit byte offsets below 1 GiB
+    X2G_CHECK((size_t)(g->nni + 16) * (g->nmu + 8) * 2 * sizeof(double) < ((size_t)1 << 30),
+              "grid %dx%d too large for the sweep kernels", g->nni, g->nmu);
     return X2G_OK;
 }
 
