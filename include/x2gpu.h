@@ -77,6 +77,10 @@ int x2g_stats_enable(x2g_ctx *ctx, int on);
 int x2g_stats_get(x2g_ctx *ctx, double *out);
 
 /* ---- device memory ------------------------------------------------------ */
+/* Blocks freed with x2g_free stay with the context and are handed out again
+ * by x2g_malloc for the same size (no synchronisation: all work of a
+ * context is ordered by its stream); x2g_ctx_destroy releases them, and
+ * x2g_mem_info counts them as free. */
 int x2g_malloc(x2g_ctx *ctx, size_t bytes, void **dptr);
 int x2g_free(x2g_ctx *ctx, void *dptr);
 int x2g_h2d(x2g_ctx *ctx, void *dst_d, const void *src_h, size_t bytes);  /* synchronous */

@@ -68,14 +68,27 @@ int stage_bytes(x2g_ctx *ctx, const void *src, size_t bytes, const void **dev_ou
     size_t off = (ctx->stage_off + 255) & ~size_t(255);
     if (off + bytes > cap) {
         // wrap: every copy and launch that used the ring has to be done
+        const int rc = stage_flush(ctx);
+        if (rc) return rc;
         X2G_HIP(hipStreamSynchronize(ctx->stream));
         off = 0;
     }
     std::memcpy((char *)ctx->pinned + off, src, bytes);
-    X2G_HIP(hipMemcpyAsync((char *)ctx->ws[2].ptr + off, (char *)ctx->pinned + off, bytes,
-                           hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->stage_hi == ctx->stage_lo) ctx->stage_lo = off;
+    ctx->stage_hi = off + bytes;
     *dev_out = (const char *)ctx->ws[2].ptr + off;
     ctx->stage_off = off + bytes;
+    return X2G_OK;
+}
+
+int stage_flush(x2g_ctx *ctx)
+{
+    if (ctx->stage_hi > ctx->stage_lo) {
+        const size_t lo = ctx->stage_lo;
+        X2G_HIP(hipMemcpyAsync((char *)ctx->ws[2].ptr + lo, (char *)ctx->pinned + lo, ctx->stage_hi - lo,
+                               hipMemcpyHostToDevice, ctx->stream));
+    }
+    ctx->stage_lo = ctx->stage_hi = 0;
     return X2G_OK;
 }
 
@@ -212,6 +225,7 @@ extern "C" int x2g_ctx_destroy(x2g_ctx *ctx)
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     for (auto &w : ctx->ws)
         if (w.ptr) (void)hipFree(w.ptr);
+    for (auto &c : ctx->cache) (void)hipFree(c.second);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->pinned_out) (void)hipHostFree(ctx->pinned_out);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -233,7 +247,25 @@ extern "C" int x2g_mem_info(x2g_ctx *ctx, size_t *free_bytes, size_t *total_byte
     X2G_CHECK(ctx && free_bytes && total_bytes, "x2g_mem_info: null argument");
     X2G_HIP(hipSetDevice(ctx->device));
     X2G_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    *free_bytes += ctx->cached_bytes;  // blocks this context keeps for reuse
     return X2G_OK;
+}
+
+// Freed blocks stay with the context for reuse (a job that ends and the next
+// one that starts on the same grid exchange the same sizes), up to this many
+// bytes; beyond it the largest cached blocks go back to the runtime.
+static constexpr size_t kCacheLimit = (size_t)32 << 30;
+
+static void cache_trim(x2g_ctx *ctx, size_t limit)
+{
+    if (ctx->cached_bytes <= limit) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    while (ctx->cached_bytes > limit && !ctx->cache.empty()) {
+        auto it = std::prev(ctx->cache.end());
+        (void)hipFree(it->second);
+        ctx->cached_bytes -= it->first;
+        ctx->cache.erase(it);
+    }
 }
 
 extern "C" int x2g_malloc(x2g_ctx *ctx, size_t bytes, void **dptr)
@@ -241,12 +273,28 @@ extern "C" int x2g_malloc(x2g_ctx *ctx, size_t bytes, void **dptr)
     X2G_CHECK(ctx && dptr, "x2g_malloc: null argument");
     *dptr = nullptr;
     if (bytes == 0) return X2G_OK;
-    if (hipMalloc(dptr, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        *dptr = nullptr;
-        set_error("hipMalloc(%zu) failed", bytes);
-        return X2G_ENOMEM;
+    const size_t want = (bytes + 255) & ~size_t(255);
+    // a cached block of the same size or up to 1/8 larger
+    auto it = ctx->cache.lower_bound(want);
+    if (it != ctx->cache.end() && it->first <= want + want / 8) {
+        *dptr = it->second;
+        ctx->live[*dptr] = it->first;
+        ctx->cached_bytes -= it->first;
+        ctx->cache.erase(it);
+        return X2G_OK;
     }
+    if (hipMalloc(dptr, want) != hipSuccess) {
+        (void)hipGetLastError();
+        // give the cache back and try once more
+        cache_trim(ctx, 0);
+        if (hipMalloc(dptr, want) != hipSuccess) {
+            (void)hipGetLastError();
+            *dptr = nullptr;
+            set_error("hipMalloc(%zu) failed", bytes);
+            return X2G_ENOMEM;
+        }
+    }
+    ctx->live[*dptr] = want;
     return X2G_OK;
 }
 
@@ -254,8 +302,14 @@ extern "C" int x2g_free(x2g_ctx *ctx, void *dptr)
 {
     X2G_CHECK(ctx, "x2g_free: null ctx");
     if (!dptr) return X2G_OK;
-    X2G_HIP(hipStreamSynchronize(ctx->stream));
-    X2G_HIP(hipFree(dptr));
+    auto it = ctx->live.find(dptr);
+    X2G_CHECK(it != ctx->live.end(), "x2g_free: %p was not allocated by x2g_malloc of this context", dptr);
+    // kept for reuse: later work on the block is ordered after earlier work
+    // by the context's stream, so no synchronisation is needed here
+    ctx->cache.emplace(it->second, dptr);
+    ctx->cached_bytes += it->second;
+    ctx->live.erase(it);
+    cache_trim(ctx, kCacheLimit);
     return X2G_OK;
 }
 

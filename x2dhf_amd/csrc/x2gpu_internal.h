@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 #include "../../include/x2gpu.h"
 
@@ -52,6 +54,7 @@ struct x2g_ctx {
     void *pinned_out = nullptr;    // pinned host buffer for gathered results
     size_t pinned_out_bytes = 0;
     size_t stage_off = 0;          // ring offset into ws[2] / pinned
+    size_t stage_lo = 0, stage_hi = 0;  // staged bytes not yet copied to ws[2]: [lo, hi)
     // sweep-kernel timing (x2g_stats_enable): event pairs around every sweep
     // launch, resolved at x2g_stats_get or when the pending list is long
     bool stats_on = false;
@@ -59,6 +62,11 @@ struct x2g_ctx {
     struct Pending { hipEvent_t a, b; double bytes; int prec; };
     std::vector<Pending> pending;
     double st_launch[2] = {0, 0}, st_ms[2] = {0, 0}, st_bytes[2] = {0, 0};
+    // device blocks of x2g_malloc: sizes of the live ones, and freed ones
+    // kept for reuse by size (all work on them is ordered by `stream`)
+    std::unordered_map<void *, size_t> live;
+    std::multimap<size_t, void *> cache;
+    size_t cached_bytes = 0;
 };
 
 struct x2g_grid {
@@ -80,9 +88,21 @@ namespace x2g {
 int ws_reserve(x2g_ctx *ctx, int k, size_t bytes);
 int pinned_reserve(x2g_ctx *ctx, size_t bytes);
 // Copy `bytes` of launch descriptors to device memory through the context's
-// staging ring (pinned host buffer -> stream-ordered copy).  A region is
-// reused only after the stream has been synchronised past its last use.
+// staging ring (pinned host buffer -> stream-ordered copy, issued by the next
+// stage_flush).  A region is reused only after the stream has been
+// synchronised past its last use.
 int stage_bytes(x2g_ctx *ctx, const void *src, size_t bytes, const void **dev_out);
+// Issue one stream-ordered copy of everything staged since the last flush.
+int stage_flush(x2g_ctx *ctx);
+// Launch a kernel on the context's stream once the staged descriptors are on
+// their way (every launch goes through here, so a call's descriptors cost one
+// copy however many arrays it stages).
+#define X2G_LAUNCH(ctx, kernel, grid, block, lds, ...)                                  \
+    do {                                                                                \
+        const int rc_ = ::x2g::stage_flush(ctx);                                        \
+        if (rc_) return rc_;                                                            \
+        hipLaunchKernelGGL(kernel, grid, block, lds, (ctx)->stream, __VA_ARGS__);       \
+    } while (0)
 template <typename D>
 inline int stage_n(x2g_ctx *ctx, const D *v, size_t n, const D **out)
 {

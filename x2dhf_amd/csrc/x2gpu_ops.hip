@@ -503,7 +503,7 @@ extern "C" int x2g_fill(x2g_ctx *ctx, double *dst_d, size_t n, double value)
 {
     X2G_CHECK(ctx && (dst_d || n == 0), "x2g_fill: null argument");
     if (n == 0) return X2G_OK;
-    hipLaunchKernelGGL(k_fill, dim3(gx((long)n, 256)), dim3(256), 0, ctx->stream, dst_d, n, value);
+    X2G_LAUNCH(ctx, k_fill, dim3(gx((long)n, 256)), dim3(256), 0, dst_d, n, value);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -520,7 +520,7 @@ extern "C" int x2g_lincomb(x2g_grid *g, double *out_d, const double *base_d, dou
         int rc = stage(ctx, terms, (size_t)nterms, off, &dt);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_lincomb, dim3(gx((long)g->n(), 256)), dim3(256), 0, ctx->stream, out_d, base_d, base_scale,
+    X2G_LAUNCH(ctx, k_lincomb, dim3(gx((long)g->n(), 256)), dim3(256), 0, out_d, base_d, base_scale,
                        dt, nterms, (long)g->n());
     X2G_HIP(hipGetLastError());
     return X2G_OK;
@@ -529,7 +529,7 @@ extern "C" int x2g_lincomb(x2g_grid *g, double *out_d, const double *base_d, dou
 static int reduce_finish(x2g_ctx *ctx, double *partial, int nv, double *out_h)
 {
     double *dout = partial + (size_t)nv * kRedBlocks;
-    hipLaunchKernelGGL(k_finish, dim3(nv), dim3(kRedBlock), 0, ctx->stream, partial, kRedBlocks, nv, dout);
+    X2G_LAUNCH(ctx, k_finish, dim3(nv), dim3(kRedBlock), 0, partial, kRedBlocks, nv, dout);
     X2G_HIP(hipGetLastError());
     X2G_HIP(hipMemcpyAsync(out_h, dout, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     X2G_HIP(hipStreamSynchronize(ctx->stream));
@@ -547,7 +547,7 @@ extern "C" int x2g_wdot(x2g_grid *g, const x2g_dot *terms, int nterms, double *o
     const x2g_dot *dt;
     if ((rc = stage(ctx, terms, (size_t)nterms, off, &dt))) return rc;
     double *partial = (double *)ctx->ws[1].ptr;
-    hipLaunchKernelGGL(k_wdot, dim3(kRedBlocks, nterms), dim3(kRedBlock), 0, ctx->stream, dt, (long)g->n(), partial);
+    X2G_LAUNCH(ctx, k_wdot, dim3(kRedBlocks, nterms), dim3(kRedBlock), 0, dt, (long)g->n(), partial);
     X2G_HIP(hipGetLastError());
     return reduce_finish(ctx, partial, nterms, out_h);
 }
@@ -578,7 +578,7 @@ extern "C" int x2g_oneel(x2g_grid *g, const double *psi1_d, const double *psi2_d
     A.nni = g->nni;
     A.nmu = g->nmu;
     double *partial = (double *)ctx->ws[1].ptr;
-    hipLaunchKernelGGL(k_oneel, dim3(kRedBlocks), dim3(kRedBlock), 0, ctx->stream, A, partial);
+    X2G_LAUNCH(ctx, k_oneel, dim3(kRedBlocks), dim3(kRedBlock), 0, A, partial);
     X2G_HIP(hipGetLastError());
     double r[2];
     if ((rc = reduce_finish(ctx, partial, 2, r))) return rc;
@@ -611,7 +611,7 @@ extern "C" int x2g_moments(x2g_grid *g, const double *psi1_d, const double *psi2
     A.dm = dm;
     A.mpole = mpole;
     double *partial = (double *)ctx->ws[1].ptr;
-    hipLaunchKernelGGL(k_moments, dim3(kRedBlocks), dim3(kRedBlock), 0, ctx->stream, A, partial);
+    X2G_LAUNCH(ctx, k_moments, dim3(kRedBlocks), dim3(kRedBlock), 0, A, partial);
     X2G_HIP(hipGetLastError());
     double r[8];
     if ((rc = reduce_finish(ctx, partial, 8, r))) return rc;
@@ -638,7 +638,7 @@ extern "C" int x2g_pot_tail(x2g_grid *g, double *pot_d, int kind, int dm, int mp
     A.kind = kind;
     A.dm = dm;
     A.mpole = mpole;
-    hipLaunchKernelGGL(k_pot_tail, dim3(gx(4L * g->nni, 128)), dim3(128), 0, g->ctx->stream, A);
+    X2G_LAUNCH(g->ctx, k_pot_tail, dim3(gx(4L * g->nni, 128)), dim3(128), 0, A);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -648,7 +648,7 @@ extern "C" int x2g_setci(x2g_grid *g, double *psi_d, int ihsym)
     X2G_CHECK(g && psi_d, "x2g_setci: null argument");
     X2G_CHECK(ihsym == 1 || ihsym == -1, "x2g_setci: ihsym");
     const long tot = (long)(g->nni / 2) * g->nmu;
-    hipLaunchKernelGGL(k_setci, dim3(gx(tot, 256)), dim3(256), 0, g->ctx->stream, psi_d, g->nni, g->nmu, ihsym);
+    X2G_LAUNCH(g->ctx, k_setci, dim3(gx(tot, 256)), dim3(256), 0, psi_d, g->nni, g->nmu, ihsym);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -687,7 +687,7 @@ extern "C" int x2g_lincomb_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g
     int rc = stage(ctx, v.data(), v.size(), off, &dops);
     if (rc) return rc;
     if (nterms_total && (rc = stage(ctx, terms, (size_t)nterms_total, off, &dt))) return rc;
-    hipLaunchKernelGGL(k_lincomb_batch, dim3(gx(maxn, 256), nops), dim3(256), 0, ctx->stream, dops, dt);
+    X2G_LAUNCH(ctx, k_lincomb_batch, dim3(gx(maxn, 256), nops), dim3(256), 0, dops, dt);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -732,17 +732,17 @@ extern "C" int x2g_normortho_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x
     size_t off = 0;
     const NoDesc *dd;
     if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
-    hipLaunchKernelGGL(k_no_dot, dim3(kRedBlocks, n, 1), dim3(kRedBlock), 0, ctx->stream, dd, partial, 0);
+    X2G_LAUNCH(ctx, k_no_dot, dim3(kRedBlocks, n, 1), dim3(kRedBlock), 0, dd, partial, 0);
     X2G_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_no_setci, dim3(gx(maxhalf, 256), n), dim3(256), 0, ctx->stream, dd);
+    X2G_LAUNCH(ctx, k_no_setci, dim3(gx(maxhalf, 256), n), dim3(256), 0, dd);
     X2G_HIP(hipGetLastError());
     if (maxoth > 0) {
-        hipLaunchKernelGGL(k_no_dot, dim3(kRedBlocks, n, maxoth), dim3(kRedBlock), 0, ctx->stream, dd, partial, 1);
+        X2G_LAUNCH(ctx, k_no_dot, dim3(kRedBlocks, n, maxoth), dim3(kRedBlock), 0, dd, partial, 1);
         X2G_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_no_finish, dim3(n), dim3(kRedBlock), 0, ctx->stream, dd, partial, kRedBlocks, coef);
+    X2G_LAUNCH(ctx, k_no_finish, dim3(n), dim3(kRedBlock), 0, dd, partial, kRedBlocks, coef);
     X2G_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_no_apply, dim3(gx(maxn, 256), n), dim3(256), 0, ctx->stream, dd, coef);
+    X2G_LAUNCH(ctx, k_no_apply, dim3(gx(maxn, 256), n), dim3(256), 0, dd, coef);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -799,9 +799,9 @@ extern "C" int x2g_eahf_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_ea
     size_t off = 0;
     const EaDesc *dd;
     if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
-    hipLaunchKernelGGL(k_eahf, dim3(kRedBlocks, n), dim3(kRedBlock), 0, ctx->stream, dd, partial);
+    X2G_LAUNCH(ctx, k_eahf, dim3(kRedBlocks, n), dim3(kRedBlock), 0, dd, partial);
     X2G_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_eahf_finish, dim3(n), dim3(kRedBlock), 0, ctx->stream, dd, partial, kRedBlocks);
+    X2G_LAUNCH(ctx, k_eahf_finish, dim3(n), dim3(kRedBlock), 0, dd, partial, kRedBlocks);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -845,9 +845,9 @@ extern "C" int x2g_moments_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g
     size_t off = 0;
     const MomArgs *dd;
     if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
-    hipLaunchKernelGGL(k_moments_batch, dim3(kRedBlocks, n), dim3(kRedBlock), 0, ctx->stream, dd, partial);
+    X2G_LAUNCH(ctx, k_moments_batch, dim3(kRedBlocks, n), dim3(kRedBlock), 0, dd, partial);
     X2G_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finish, dim3(nv), dim3(kRedBlock), 0, ctx->stream, partial, kRedBlocks, nv, dout);
+    X2G_LAUNCH(ctx, k_finish, dim3(nv), dim3(kRedBlock), 0, partial, kRedBlocks, nv, dout);
     X2G_HIP(hipGetLastError());
     X2G_HIP(hipMemcpyAsync(ctx->pinned_out, dout, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     X2G_HIP(hipStreamSynchronize(ctx->stream));
@@ -892,7 +892,7 @@ extern "C" int x2g_pot_tail_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2
     const TailArgs *dd;
     int rc = stage(ctx, v.data(), v.size(), off, &dd);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pot_tail_batch, dim3(gx(4L * maxnni, 128), n), dim3(128), 0, ctx->stream, dd);
+    X2G_LAUNCH(ctx, k_pot_tail_batch, dim3(gx(4L * maxnni, 128), n), dim3(128), 0, dd);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -915,7 +915,7 @@ extern "C" int x2g_gather(x2g_ctx *ctx, const double *const *src_d, const int *c
     size_t off = 0;
     const GatherDesc *dd;
     if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
-    hipLaunchKernelGGL(k_gather, dim3(n), dim3(64), 0, ctx->stream, dd, dbuf);
+    X2G_LAUNCH(ctx, k_gather, dim3(n), dim3(64), 0, dd, dbuf);
     X2G_HIP(hipGetLastError());
     X2G_HIP(hipMemcpyAsync(ctx->pinned_out, dbuf, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     X2G_HIP(hipStreamSynchronize(ctx->stream));

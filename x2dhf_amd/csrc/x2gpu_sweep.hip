@@ -954,42 +954,79 @@ struct ResDesc {
     const double *bmu;
     const double *dnu;
     float *res_f;          // FP32 residual (may be null)
-    int res_split;         // res_f: component 1 of the residue-split pairs (else unpadded nni*nmu)
+    int res_split;         // res_f holds colour-ordered (omega/lhs, r) pairs (else r, unpadded nni*nmu)
+    double omega;          // for the pairs
     double *acc;           // [2] per problem: sum res^2, sum rhs^2 (may be null)
     double c[16];
     int nni, nmu, isym;
 };
 
-__global__ void k_residual(const ResDesc *__restrict__ rd)
+// Workgroup index remapped so that each of the 8 XCDs (which take the
+// workgroups of a launch round-robin, each with its own L2) gets one
+// contiguous range of the launch's blocks: a point's neighbour rows are then
+// read by workgroups of the same XCD and come from its L2.
+__device__ __forceinline__ int xcd_block(int b, int total)
 {
-    const ResDesc &d = rd[blockIdx.y];
+    const int x = b & 7, slot = b >> 3, per = total >> 3, rem = total & 7;
+    return x * per + min(x, rem) + slot;
+}
+
+// One point per thread; blocks_per_problem blocks of 256 points per problem
+// (a 1-D grid of blocks_per_problem * problems workgroups).
+__global__ void __launch_bounds__(256) k_residual(const ResDesc *__restrict__ rd, int blocks_per_problem)
+{
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const ResDesc &d = rd[lb / blocks_per_problem];
     const int nni = d.nni, nmu = d.nmu;
-    const double s = d.isym == 1 ? 1.0 : -1.0;
-    const long tot = (long)nni * nmu;
+    const long e = (long)(lb % blocks_per_problem) * 256 + threadIdx.x;
     double sr = 0.0, sb = 0.0;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
-         e += (long)gridDim.x * blockDim.x) {
-        const int mu = 1 + (int)(e / nni), nu = 1 + (int)(e % nni);
-        double r = 0.0;
+    if (e < (long)nni * nmu) {
+        const int mu = 1 + (int)(e / nni), nu = 1 + (int)(e - (long)(mu - 1) * nni);
         if (mu >= 2 && mu <= nmu - 4 && nu >= 2 && nu <= nni - 1) {
+            const double s = d.isym == 1 ? 1.0 : -1.0;
             const double *f = d.fun, *c = d.c;
-            auto F = [&](int dn, int dm) { return fmirror(f, nu + dn, mu + dm, nni, s); };
-            const double ddmi2 = c[4] * (F(0, -4) + F(0, 4)) + c[5] * (F(0, -3) + F(0, 3)) +
-                                 c[6] * (F(0, -2) + F(0, 2)) + c[7] * (F(0, -1) + F(0, 1));
-            const double ddmi1 = c[0] * (F(0, -4) - F(0, 4)) + c[1] * (F(0, -3) - F(0, 3)) +
-                                 c[2] * (F(0, -2) - F(0, 2)) + c[3] * (F(0, -1) - F(0, 1));
-            const double ddni2 = c[12] * (F(-4, 0) + F(4, 0)) + c[13] * (F(-3, 0) + F(3, 0)) +
-                                 c[14] * (F(-2, 0) + F(2, 0)) + c[15] * (F(-1, 0) + F(1, 0));
-            const double ddni1 = c[8] * (F(-4, 0) - F(4, 0)) + c[9] * (F(-3, 0) - F(3, 0)) +
-                                 c[10] * (F(-2, 0) - F(2, 0)) + c[11] * (F(-1, 0) - F(1, 0));
-            const double cc = ddmi2 + d.bmu[mu - 1] * ddmi1 + ddni2 + d.dnu[nu - 1] * ddni1;
-            r = d.rhs[e] - (cc + d.lhs[e] * f[e]);
-            sr += r * r;
-            sb += d.rhs[e] * d.rhs[e];
-            if (d.res_f && d.res_split)
-                d.res_f[2 * split_index(nu, mu, split_m(nni), colour_mult(nni)) + 1] = (float)r;
+            double cc;
+            const double bm = d.bmu[mu - 1], dn = d.dnu[nu - 1];
+            if (nu >= 5 && nu <= nni - 4 && mu >= 5) {
+                // no mirror images: neighbours straight from u
+                const double *q = f + e;
+                const long s1 = nni, s2 = 2L * nni, s3 = 3L * nni, s4 = 4L * nni;
+                const double ddmi2 = c[4] * (q[-s4] + q[s4]) + c[5] * (q[-s3] + q[s3]) + c[6] * (q[-s2] + q[s2]) +
+                                     c[7] * (q[-s1] + q[s1]);
+                const double ddmi1 = c[0] * (q[-s4] - q[s4]) + c[1] * (q[-s3] - q[s3]) + c[2] * (q[-s2] - q[s2]) +
+                                     c[3] * (q[-s1] - q[s1]);
+                const double ddni2 = c[12] * (q[-4] + q[4]) + c[13] * (q[-3] + q[3]) + c[14] * (q[-2] + q[2]) +
+                                     c[15] * (q[-1] + q[1]);
+                const double ddni1 = c[8] * (q[-4] - q[4]) + c[9] * (q[-3] - q[3]) + c[10] * (q[-2] - q[2]) +
+                                     c[11] * (q[-1] - q[1]);
+                cc = ddmi2 + bm * ddmi1 + ddni2 + dn * ddni1;
+            } else {
+                auto F = [&](int dnu, int dmu) { return fmirror(f, nu + dnu, mu + dmu, nni, s); };
+                const double ddmi2 = c[4] * (F(0, -4) + F(0, 4)) + c[5] * (F(0, -3) + F(0, 3)) +
+                                     c[6] * (F(0, -2) + F(0, 2)) + c[7] * (F(0, -1) + F(0, 1));
+                const double ddmi1 = c[0] * (F(0, -4) - F(0, 4)) + c[1] * (F(0, -3) - F(0, 3)) +
+                                     c[2] * (F(0, -2) - F(0, 2)) + c[3] * (F(0, -1) - F(0, 1));
+                const double ddni2 = c[12] * (F(-4, 0) + F(4, 0)) + c[13] * (F(-3, 0) + F(3, 0)) +
+                                     c[14] * (F(-2, 0) + F(2, 0)) + c[15] * (F(-1, 0) + F(1, 0));
+                const double ddni1 = c[8] * (F(-4, 0) - F(4, 0)) + c[9] * (F(-3, 0) - F(3, 0)) +
+                                     c[10] * (F(-2, 0) - F(2, 0)) + c[11] * (F(-1, 0) - F(1, 0));
+                cc = ddmi2 + bm * ddmi1 + ddni2 + dn * ddni1;
+            }
+            const double l = d.lhs[e], b = d.rhs[e];
+            const double r = b - (cc + l * f[e]);
+            sr = r * r;
+            sb = b * b;
+            if (d.res_f) {
+                if (d.res_split) {
+                    float2 pr;
+                    pr.x = (float)(d.omega / l);
+                    pr.y = (float)r;
+                    reinterpret_cast<float2 *>(d.res_f)[split_index(nu, mu, split_m(nni), colour_mult(nni))] = pr;
+                } else {
+                    d.res_f[e] = (float)r;
+                }
+            }
         }
-        if (d.res_f && !d.res_split) d.res_f[e] = (float)r;
     }
     if (d.acc) {
         // wavefront reduction then one atomic per wavefront (diagnostic only)
@@ -997,7 +1034,7 @@ __global__ void k_residual(const ResDesc *__restrict__ rd)
             sr += __shfl_down(sr, off, 64);
             sb += __shfl_down(sb, off, 64);
         }
-        if ((threadIdx.x & 63) == 0) {
+        if ((threadIdx.x & 63) == 0 && (sr != 0.0 || sb != 0.0)) {
             atomicAdd(&d.acc[0], sr);
             atomicAdd(&d.acc[1], sb);
         }
@@ -1110,7 +1147,7 @@ int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w
     const PadDesc *dd;
     int rc = stage_descs(g->ctx, v, off, &dd);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_putin, dim3(grid_x_for((long)g->npad(), 256), 1), dim3(256), 0, g->ctx->stream, dd);
+    X2G_LAUNCH(g->ctx, k_putin, dim3(grid_x_for((long)g->npad(), 256), 1), dim3(256), 0, dd);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -1127,7 +1164,7 @@ int launch_putin_items(x2g_ctx *ctx, const std::vector<PadItem> &items)
     const PadDesc *dd;
     int rc = stage_n(ctx, v.data(), v.size(), &dd);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_putin, dim3(grid_x_for(maxpad, 256), (int)v.size()), dim3(256), 0, ctx->stream, dd);
+    X2G_LAUNCH(ctx, k_putin, dim3(grid_x_for(maxpad, 256), (int)v.size()), dim3(256), 0, dd);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -1139,7 +1176,7 @@ int launch_putout(x2g_grid *g, double *fun, const double *work)
     const PadDesc *dd;
     int rc = stage_descs(g->ctx, v, off, &dd);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_putout, dim3(grid_x_for((long)g->n(), 256), 1), dim3(256), 0, g->ctx->stream, dd);
+    X2G_LAUNCH(g->ctx, k_putout, dim3(grid_x_for((long)g->n(), 256), 1), dim3(256), 0, dd);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -1288,6 +1325,7 @@ static int sweep_kind(x2g_ctx *ctx, int maxnni)
 template <typename T, int NT, int PIPE, int W>
 static void launch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, int n, const SweepPlan<T> &plan)
 {
+    // the caller has flushed the staged descriptors
     hipLaunchKernelGGL((k_sweep<T, NT, PIPE, W>), dim3(plan.total_wg), dim3(plan.nth), plan.lds, ctx->stream, dd, n,
                        plan.MS);
 }
@@ -1322,8 +1360,11 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
     double bytes = 0.0;
     for (auto &d : hv) bytes += 4.0 * sizeof(T) * (double)(d.nni - 2) * (double)(d.nmu - 5);
     const int prec = sizeof(T) == 8 ? 0 : 1;
+    // descriptors go out before the timed region
+    int rc = stage_flush(ctx);
+    if (rc) return rc;
     hipEvent_t ea, eb;
-    int rc = stats_begin(ctx, &ea, &eb);
+    rc = stats_begin(ctx, &ea, &eb);
     if (rc) return rc;
     if (plan.kind == 2) {
         // two-deep pipeline for FP32 batches that place several workgroups on
@@ -1340,9 +1381,9 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
     if (!in_place && skew > 0) {
         const int nth = 512;
         if (skew == 2) {
-            hipLaunchKernelGGL((k_strip<T, 2>), dim3(total_wg), dim3(nth), lds, ctx->stream, dd, n);
+            X2G_LAUNCH(ctx, (k_strip<T, 2>), dim3(total_wg), dim3(nth), lds, dd, n);
         } else {
-            hipLaunchKernelGGL((k_strip<T, 1>), dim3(total_wg), dim3(nth), lds, ctx->stream, dd, n);
+            X2G_LAUNCH(ctx, (k_strip<T, 1>), dim3(total_wg), dim3(nth), lds, dd, n);
         }
         X2G_HIP(hipGetLastError());
         return stats_end(ctx, ea, eb, bytes, prec);
@@ -1350,12 +1391,12 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
     long maxpts = 0;
     for (auto &d : hv) maxpts = std::max<long>(maxpts, (long)(d.nni - 2) * (d.nmu - 5));
     for (int pass = 0; pass < 5; ++pass) {
-        hipLaunchKernelGGL((k_phase<T>), dim3(grid_x_for(maxpts, 256), n), dim3(256), 0, ctx->stream, dd, pass);
+        X2G_LAUNCH(ctx, (k_phase<T>), dim3(grid_x_for(maxpts, 256), n), dim3(256), 0, dd, pass);
         X2G_HIP(hipGetLastError());
     }
     long maxb = 0;
     for (auto &d : hv) maxb = std::max<long>(maxb, (long)(d.nni - 2) + 2L * (d.nmu - 4));
-    hipLaunchKernelGGL((k_boundary<T>), dim3(grid_x_for(maxb, 256), n), dim3(256), 0, ctx->stream, dd);
+    X2G_LAUNCH(ctx, (k_boundary<T>), dim3(grid_x_for(maxb, 256), n), dim3(256), 0, dd);
     X2G_HIP(hipGetLastError());
     return stats_end(ctx, ea, eb, bytes, prec);
 }
@@ -1479,7 +1520,7 @@ static int launch_pack(x2g_ctx *ctx, const std::vector<PackDesc<TO>> &v)
     if (rc) return rc;
     long maxpts = 0;
     for (auto &d : v) maxpts = std::max<long>(maxpts, (long)(d.nni - 2) * (d.nmu - 5));
-    hipLaunchKernelGGL((k_pack<TO>), dim3(grid_x_for(maxpts, 256), (int)v.size()), dim3(256), 0, ctx->stream, dd);
+    X2G_LAUNCH(ctx, (k_pack<TO>), dim3(grid_x_for(maxpts, 256), (int)v.size()), dim3(256), 0, dd);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -1493,7 +1534,7 @@ static int launch_restride(x2g_ctx *ctx, const std::vector<RestrideDesc> &v)
     if (rc) return rc;
     long maxe = 0;
     for (auto &d : v) maxe = std::max<long>(maxe, (long)d.ds * d.rows);
-    hipLaunchKernelGGL(k_restride, dim3(grid_x_for(maxe, 256), (int)v.size()), dim3(256), 0, ctx->stream, dd);
+    X2G_LAUNCH(ctx, k_restride, dim3(grid_x_for(maxe, 256), (int)v.size()), dim3(256), 0, dd);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
@@ -1766,6 +1807,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     const CorrDesc *d_corr = nullptr;
     const CvtDesc *d_cvt = nullptr;
     const FillFDesc *d_zero = nullptr;
+    int nzero = 0;
     const BndDesc *d_bnd = nullptr;
     std::vector<BndDesc> bnd;
     if (!i32.empty()) {
@@ -1791,7 +1833,8 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             r.bmu = it.bkind == 0 ? g->bmu_orb : g->bmu_pot;
             r.dnu = g->dnu;
             r.res_f = kind32 == 2 ? f32p(i, 2) : f32p(i, 3);
-            r.res_split = kind32 == 2;
+            r.res_split = kind32 == 2;  // k_sweep: (omega/lhs, r) pairs, formed here each macro-iteration
+            r.omega = it.omega;
             r.acc = nullptr;
             for (int q = 0; q < 16; ++q) r.c[q] = g->coef[q];
             r.nni = g->nni;
@@ -1799,7 +1842,16 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             res.push_back(r);
             corr.push_back({it.fun_d, f32p(i, 0), g->nni, g->nmu, pstride(i)});
             cvt.push_back({it.lhs_d, f32p(i, 2), (long)g->n()});
-            zero.push_back({f32p(i, 0), (long)(2 * align256(padbytes(i)) / sizeof(float))});
+            if (kind32 == 2) {
+                // the correction starts from zero; k_sweep writes every row of
+                // its destination except the mu <= 1 and mu >= nmu-3 rows
+                const long P = pstride(i);
+                zero.push_back({f32p(i, 0), P * (g->nmu + 8)});
+                zero.push_back({f32p(i, 1), 5 * P});
+                zero.push_back({f32p(i, 1) + (long)g->nmu * P, 8 * P});
+            } else {
+                zero.push_back({f32p(i, 0), (long)(2 * align256(padbytes(i)) / sizeof(float))});
+            }
             BndDesc bd{};
             bd.fun = it.fun_d;
             for (int q = 0; q < 5; ++q) bd.ex[q] = g->coef[16 + q];
@@ -1825,34 +1877,29 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         if ((rc = stage_descs(ctx, corr, doff, &d_corr))) return rc;
         if ((rc = stage_descs(ctx, cvt, doff, &d_cvt))) return rc;
         if ((rc = stage_descs(ctx, zero, doff, &d_zero))) return rc;
+        nzero = (int)zero.size();
         if ((rc = stage_descs(ctx, bnd, doff, &d_bnd))) return rc;
-        if (split32) {
-            std::vector<PackDesc<float>> pk;
-            for (int i : i32) {
-                x2g_grid *g = grids[items[i].grid_index];
-                pk.push_back({items[i].lhs_d, f32p(i, 2), items[i].omega, g->nni, g->nmu, 0});
-            }
-            if ((rc = launch_pack(ctx, pk))) return rc;
-        } else {
+        if (!split32) {
             long maxn = 0;
             for (int i : i32) maxn = std::max<long>(maxn, (long)grids[items[i].grid_index]->n());
-            hipLaunchKernelGGL(k_to_f32, dim3(grid_x_for(maxn, 256), (int)i32.size()), dim3(256), 0, ctx->stream,
+            X2G_LAUNCH(ctx, k_to_f32, dim3(grid_x_for(maxn, 256), (int)i32.size()), dim3(256), 0,
                                d_cvt);
             X2G_HIP(hipGetLastError());
         }
     }
 
-    long maxpad = 0, maxn = 0, maxnni = 0;
+    long maxpad = 0, maxn = 0, maxnni = 0, maxnmu = 0;
     for (int i = 0; i < n; ++i) {
         x2g_grid *g = grids[items[i].grid_index];
         maxpad = std::max<long>(maxpad, (long)pstride(i) * (g->nmu + 8));
         maxn = std::max<long>(maxn, (long)g->n());
         maxnni = std::max<long>(maxnni, g->nni);
+        maxnmu = std::max<long>(maxnmu, g->nmu);
     }
     for (int itr1 = 0; itr1 < maxsor1; ++itr1) {
         // FP64 items: putin into both ping-pong buffers
         if (d_pin) {
-            hipLaunchKernelGGL(k_putin, dim3(grid_x_for(maxpad, 256), (int)pin.size()), dim3(256), 0, ctx->stream,
+            X2G_LAUNCH(ctx, k_putin, dim3(grid_x_for(maxpad, 256), (int)pin.size()), dim3(256), 0,
                                d_pin);
             X2G_HIP(hipGetLastError());
         }
@@ -1862,29 +1909,30 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
                 if ((rc = launch_sweep<double>(ctx, ev ? d64e : d64o, ev ? p64.even : p64.odd, p64)))
                     return rc;
             }
-            hipLaunchKernelGGL(k_putout, dim3(grid_x_for(maxn, 256), (int)i64.size()), dim3(256), 0, ctx->stream,
+            X2G_LAUNCH(ctx, k_putout, dim3(grid_x_for(maxn, 256), (int)i64.size()), dim3(256), 0,
                                d_pout64);
             X2G_HIP(hipGetLastError());
         }
         if (!i32.empty()) {
             const int n32 = (int)i32.size();
-            hipLaunchKernelGGL(k_residual, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, ctx->stream, d_res);
+            const int bpp = (int)((maxn + 255) / 256);
+            X2G_LAUNCH(ctx, k_residual, dim3(bpp * n32), dim3(256), 0, d_res, bpp);
             X2G_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_zero_f32, dim3(grid_x_for(2 * maxpad, 256), n32), dim3(256), 0, ctx->stream, d_zero);
+            X2G_LAUNCH(ctx, k_zero_f32, dim3(std::min(grid_x_for(2 * maxpad, 256), 128), nzero), dim3(256), 0, d_zero);
             X2G_HIP(hipGetLastError());
             for (int it2 = 0; it2 < maxsor2; ++it2) {
                 const bool ev = (it2 % 2) == 0;
                 if ((rc = launch_sweep<float>(ctx, ev ? d32e : d32o, ev ? p32.even : p32.odd, p32)))
                     return rc;
             }
-            hipLaunchKernelGGL(k_correct, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, ctx->stream, d_corr);
+            X2G_LAUNCH(ctx, k_correct, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, d_corr);
             X2G_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_boundary_fun, dim3(grid_x_for(2 * maxn / std::max<long>(maxnni, 1) + maxnni, 256), n32),
-                               dim3(256), 0, ctx->stream, d_bnd);
+            X2G_LAUNCH(ctx, k_boundary_fun, dim3(grid_x_for(2 * maxn / std::max<long>(maxnni, 1) + maxnni, 256), n32),
+                               dim3(256), 0, d_bnd);
             X2G_HIP(hipGetLastError());
         }
         if (d_tail) {
-            hipLaunchKernelGGL(k_orb_tail, dim3(grid_x_for(maxnni, 128), (int)tails.size()), dim3(128), 0, ctx->stream,
+            X2G_LAUNCH(ctx, k_orb_tail, dim3(grid_x_for(maxnni, 128), (int)tails.size()), dim3(128), 0,
                                d_tail);
             X2G_HIP(hipGetLastError());
         }
@@ -1926,7 +1974,8 @@ extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2
     const ResDesc *d_res;
     if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
     if ((rc = x2g_fill(ctx, acc, 2 * (size_t)n, 0.0))) return rc;
-    hipLaunchKernelGGL(k_residual, dim3(grid_x_for(maxn, 256), n), dim3(256), 0, ctx->stream, d_res);
+    const int bpp = (int)((maxn + 255) / 256);
+    X2G_LAUNCH(ctx, k_residual, dim3(bpp * n), dim3(256), 0, d_res, bpp);
     X2G_HIP(hipGetLastError());
     X2G_HIP(hipMemcpyAsync(out_h, acc, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     X2G_HIP(hipStreamSynchronize(ctx->stream));

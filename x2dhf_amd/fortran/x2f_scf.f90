@@ -71,6 +71,11 @@ module x2f_scf
      real(dp) :: ee(MAXO,MAXO) = 0, eeold(MAXO) = 0, deltaee(MAXO) = 0, orbnorm(MAXO) = 1
      real(dp) :: cmulti(8,MAXO) = 0, exc(8,MAXO*(MAXO+1)) = 0
      integer :: ifixorb(MAXO) = 0, itouch(MAXO) = 0, iasympt = 0
+     ! a multipole-moment refresh decided by convergence_control, with the
+     ! orbitals touched in that iteration (done by the caller, several jobs
+     ! in one launch in the batched schedule)
+     logical :: mm_due = .false.
+     integer :: mm_touch(MAXO) = 0
      logical :: lfixorb = .false., lfixorb4init = .false.
      real(dp) :: omega_orb = 0, omega_pot = 0
      ! control state of the SCF loop
@@ -494,32 +499,53 @@ contains
   ! mpoleMoments: Coulomb moments of touched orbitals (odd ones vanish by
   ! symmetry for homonuclear molecules) and the moments of every exchange pair,
   ! all in one batched reduction
-  subroutine mpole_moments(j)
+  subroutine mpole_moments(j, touch)
     type(scf_job), intent(inout) :: j
+    integer, intent(in) :: touch(:)
     type(x2g_moment_item), allocatable :: it(:)
     real(dp), allocatable :: out(:)
     integer, allocatable :: who(:)
     type(c_ptr) :: grids(1)
-    integer :: i, k, n, mp
+    integer :: n
     if (j%c%nel == 1) return
     allocate(it(j%c%norb + j%c%nmm), who(j%c%norb + j%c%nmm))
     n = 0
+    call mm_items(j, touch, 0, it, who, n)
+    if (n == 0) return
+    allocate(out(8*n))
+    grids(1) = j%gd
+    call x2check(x2g_moments_batch(j%ctx, grids, it, int(n, c_int), int(j%o%mpole, c_int), out), 'mpoleMoments')
+    call mm_store(j, who, out, n)
+  end subroutine mpole_moments
+
+  ! the moment items of job j (coulMom / exchMom): the densities of the
+  ! touched orbitals and the exchange pairs with their Delta m, on grid gidx
+  subroutine mm_items(j, touch, gidx, it, who, n)
+    type(scf_job), intent(in) :: j
+    integer, intent(in) :: touch(:), gidx
+    type(x2g_moment_item), intent(inout) :: it(:)
+    integer, intent(inout) :: who(:), n
+    integer :: i, k
     do i = 1, j%c%norb
-       if (j%itouch(i) == 0) cycle
+       if (touch(i) == 0) cycle
        n = n + 1
-       it(n) = x2g_moment_item(j%orb(i), j%orb(i), 0_c_int, 0_c_int)
+       it(n) = x2g_moment_item(j%orb(i), j%orb(i), 0_c_int, int(gidx, c_int))
        who(n) = i
     enddo
     do k = 1, j%c%nmm
        n = n + 1
-       it(n) = x2g_moment_item(j%orb(j%c%mm1(k)), j%orb(j%c%mm2(k)), int(j%c%mmdel(k), c_int), 0_c_int)
+       it(n) = x2g_moment_item(j%orb(j%c%mm1(k)), j%orb(j%c%mm2(k)), int(j%c%mmdel(k), c_int), int(gidx, c_int))
        who(n) = -k
     enddo
-    if (n == 0) return
-    allocate(out(8*n))
-    grids(1) = j%gd
+  end subroutine mm_items
+
+  ! moments out(8*(k-1)+1..) of the items who(1..n) of job j into its state
+  subroutine mm_store(j, who, out, n)
+    type(scf_job), intent(inout) :: j
+    integer, intent(in) :: who(:), n
+    real(dp), intent(in) :: out(:)
+    integer :: i, k, mp
     mp = j%o%mpole
-    call x2check(x2g_moments_batch(j%ctx, grids, it, int(n, c_int), int(mp, c_int), out), 'mpoleMoments')
     do k = 1, n
        if (who(k) > 0) then
           i = who(k)
@@ -532,7 +558,52 @@ contains
           j%exc(1:mp, j%c%mmslot(-who(k))) = out(8*(k-1)+1:8*(k-1)+mp)
        endif
     enddo
-  end subroutine mpole_moments
+  end subroutine mm_store
+
+  ! the moment refreshes that convergence_control scheduled for jobs
+  ! js(idx(q)), q = 1..nj, in one launch and one transfer
+  subroutine mm_refresh(js, idx, nj)
+    type(scf_job), intent(inout), target :: js(:)
+    integer, intent(in) :: idx(:), nj
+    type(x2g_moment_item), allocatable :: it(:)
+    type(c_ptr), allocatable :: grids(:)
+    integer, allocatable :: who(:), first(:)
+    real(dp), allocatable :: out(:)
+    integer :: q, h, n, nmax, mp
+    nmax = 0; mp = 2
+    allocate(grids(nj), first(nj + 1))
+    do q = 1, nj
+       h = idx(q)
+       grids(q) = js(h)%gd
+       if (.not. js(h)%mm_due .or. js(h)%c%nel == 1) cycle
+       nmax = nmax + js(h)%c%norb + js(h)%c%nmm
+       mp = max(mp, js(h)%o%mpole)
+    enddo
+    if (nmax > 0) then
+       allocate(it(nmax), who(nmax), out(8*nmax))
+       n = 0
+       do q = 1, nj
+          h = idx(q)
+          first(q) = n
+          if (.not. js(h)%mm_due .or. js(h)%c%nel == 1) cycle
+          call mm_items(js(h), js(h)%mm_touch, q-1, it, who, n)
+       enddo
+       first(nj + 1) = n
+       if (n > 0) then
+          call x2check(x2g_moments_batch(js(idx(1))%ctx, grids, it, int(n, c_int), int(mp, c_int), out), &
+               'batched mpoleMoments')
+          do q = 1, nj
+             h = idx(q)
+             if (first(q + 1) > first(q)) &
+                  call mm_store(js(h), who(first(q)+1:first(q+1)), out(8*first(q)+1:8*first(q+1)), &
+                  first(q+1) - first(q))
+          enddo
+       endif
+    endif
+    do q = 1, nj
+       js(idx(q))%mm_due = .false.
+    enddo
+  end subroutine mm_refresh
 
   ! potAsympt: the multipole-tail items of the potentials of orbital i1; a
   ! slot already queued (done) is not queued again
@@ -676,7 +747,7 @@ contains
        call ortho(j, i)
     enddo
     j%itouch = 1
-    call mpole_moments(j)
+    call mpole_moments(j, j%itouch)
     do i = j%c%norb, 1, -1
        call ea_hf(j, i)
        j%eeold(i) = j%ee(i,i)
@@ -739,6 +810,10 @@ contains
        call orbital_step(j, j%c%norb - ii + 1)
     enddo
     stop_now = convergence_control(j)
+    if (j%mm_due) then
+       call mpole_moments(j, j%mm_touch)
+       j%mm_due = .false.
+    endif
   end function scf_iteration
 
   ! the end-of-iteration bookkeeping of scf (doSCF.F90:546-744)
@@ -863,7 +938,8 @@ contains
     ! refresh the multipole moments when the energy change dropped enough
     if (abs(j%eeconv/j%deltaeemax) > j%o%recalc_mm) then
        j%eeconv = j%deltaeemax
-       call mpole_moments(j)
+       j%mm_due = .true.
+       j%mm_touch = j%itouch
     endif
     j%iasympt = j%c%norb
     j%itouch = 0
@@ -886,7 +962,7 @@ contains
     do i = j%c%norb, 1, -1
        call ea_hf(j, i)
     enddo
-    call mpole_moments(j)
+    call mpole_moments(j, j%itouch)
     call etotal_hf(j)
     j%done = .true.
   end subroutine scf_finish
@@ -1090,6 +1166,7 @@ contains
        enddo
        stopped(q) = convergence_control(js(h))
     enddo
+    call mm_refresh(js, idx, nj)
     deallocate(items, tails, grids, no, ea, recs, cnt, recq, rech)
   end subroutine batched_iteration
 end module x2f_scf
