@@ -408,7 +408,13 @@ __global__ void k_pack(const PackDesc<TO> *__restrict__ pd)
 // row, the t-th point of a pass at position t), one pair load per point.
 // A point update is
 //   f = (rhs - L f) * (omega/lhs) + (1-omega) f,
-// the d/dmu part of L f with per-row weights  c2_k +- b(mu) c1_k  (k = 1..4).
+// the d/dmu part of L f with per-row weights  c2_k +- b(mu) c1_k  (k = 1..4);
+// FP32 sums L f in one fused multiply-add chain (28 operations per point).
+// The rows of a block are taken two at a time: the 2 x 17 stencil values are
+// read from LDS before the arithmetic of either row.  The next block and the
+// next step's (omega/lhs, rhs) are loaded one step ahead.  The mu = 1 row of
+// the first strip is formed after the last step from the written rows 2..6,
+// so the steps issue a fixed number of memory operations.
 //
 // LDS (elements of T): d(nu) at index nu-1 [W] (so a point's column base also
 // addresses its d(nu)), the row weights of the ring's rows [48 x 8, written
@@ -455,6 +461,15 @@ __device__ __forceinline__ double buf_load1(__amdgpu_buffer_rsrc_t r, unsigned o
 {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
+__device__ __forceinline__ void buf_store1(__amdgpu_buffer_rsrc_t r, unsigned off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_store1(__amdgpu_buffer_rsrc_t r, unsigned off, double v)
+{
+    typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, off, 0, 0);
+}
 template <typename T>
 struct Pair {
     T a, b;
@@ -472,7 +487,7 @@ struct Chunk {
     T v[16 / sizeof(T)];
 };
 
-template <typename T, int NT, int PIPE, int W>
+template <typename T, int NT, int W>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && W <= 352 ? 4 : 1)))
 k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int MS)
 {
@@ -672,46 +687,93 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int MS)
         const int mu0 = r0 + 4 * b;
         const T *ab = ab_r + sb * 32;           // row weights of block b
         int off = mod5(p - cA * (mu0 - 2));     // (nu-2) mod 5 of the pass's points in row mu0
+        // Rows are processed in groups of G: the stencil values of a group are
+        // gathered before its arithmetic (the points of one pass never read
+        // each other, in one row or in neighbouring rows).
+        constexpr int G = 2;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            if (rr) off = off - cA < 0 ? off - cA + 5 : off - cA;
-            const int mu = mu0 + rr;
-            const int lim_r = mu >= 2 && mu <= nmu - 4 ? nni - 3 - off : -1;  // point ok iff 5t <= lim_r
-            int cbv[NT];
-            bool okv[NT];
-            T x[NT][17];  // 0..8: own row at nu-4..nu+4; 9..12: rows mu-1..mu-4; 13..16: rows mu+1..mu+4
+        for (int rg = 0; rg < 4; rg += G) {
+            int cbv[G][NT], ov[G][NT];
+            bool okv[G][NT];
+            T x[G][NT][17];  // 0..8: own row at nu-4..nu+4; 9..12: rows mu-1..mu-4; 13..16: rows mu+1..mu+4
 #pragma unroll
-            for (int j = 0; j < NT; ++j) {
-                const int cb = 1 + off + 5 * tq[j];  // = nu - 1 = (column of nu) - 4
-                cbv[j] = cb;
-                okv[j] = lim[j] <= lim_r;
-                const T *qm = bm + cb, *qp = bp + cb, *q0 = b0 + cb;
+            for (int g = 0; g < G; ++g) {
+                const int rr = rg + g;
+                if (rr) off = off - cA < 0 ? off - cA + 5 : off - cA;
+                const int mu = mu0 + rr;
+                const int lim_r = mu >= 2 && mu <= nmu - 4 ? nni - 3 - off : -1;  // point ok iff 5t <= lim_r
 #pragma unroll
-                for (int k = 0; k < 9; ++k) x[j][k] = q0[rr * W + k];
+                for (int j = 0; j < NT; ++j) {
+                    const int cb = 1 + off + 5 * tq[j];  // = nu - 1 = (column of nu) - 4
+                    cbv[g][j] = cb;
+                    okv[g][j] = lim[j] <= lim_r;
+                    const T *qm = bm + cb, *qp = bp + cb, *q0 = b0 + cb;
+                    if constexpr (sizeof(T) == 4) {
+                        // the own row's nine values from one opaque base, so that
+                        // they are read with immediate offsets of that base
+                        int o = (int)(b0 - dnu_s) + rr * W + cb;  // from the start of the LDS
+                        asm("" : "+v"(o));
+                        ov[g][j] = o;
 #pragma unroll
-                for (int k = 1; k <= 4; ++k) {
-                    const int rdn = rr - k, rup = rr + k;
-                    x[j][8 + k] = rdn < 0 ? qm[(rdn + 4) * W + 4] : q0[rdn * W + 4];
-                    x[j][12 + k] = rup > 3 ? qp[(rup - 4) * W + 4] : q0[rup * W + 4];
+                        for (int k = 0; k < 9; ++k) x[g][j][k] = dnu_s[o + k];
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) x[g][j][k] = q0[rr * W + k];
+                    }
+#pragma unroll
+                    for (int k = 1; k <= 4; ++k) {
+                        const int rdn = rr - k, rup = rr + k;
+                        x[g][j][8 + k] = rdn < 0 ? qm[(rdn + 4) * W + 4] : q0[rdn * W + 4];
+                        x[g][j][12 + k] = rup > 3 ? qp[(rup - 4) * W + 4] : q0[rup * W + 4];
+                    }
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
-            const T am4 = ab[8 * rr + 0], am3 = ab[8 * rr + 1], am2 = ab[8 * rr + 2], am1 = ab[8 * rr + 3];
-            const T ap4 = ab[8 * rr + 4], ap3 = ab[8 * rr + 5], ap2 = ab[8 * rr + 6], ap1 = ab[8 * rr + 7];
 #pragma unroll
-            for (int j = 0; j < NT; ++j) {
-                const T *xv = x[j];
-                const T dmu = (am4 * xv[12] + ap4 * xv[16]) + (am3 * xv[11] + ap3 * xv[15]) +
-                              ((am2 * xv[10] + ap2 * xv[14]) + (am1 * xv[9] + ap1 * xv[13]));
-                const T ddni2 = (cf[4] * (xv[0] + xv[8]) + cf[5] * (xv[1] + xv[7])) +
-                                (cf[6] * (xv[2] + xv[6]) + cf[7] * (xv[3] + xv[5]));
-                const T ddni1 = (cf[0] * (xv[0] - xv[8]) + cf[1] * (xv[1] - xv[7])) +
-                                (cf[2] * (xv[2] - xv[6]) + cf[3] * (xv[3] - xv[5]));
-                const int cb = cbv[j];
-                const T cc = (dmu + ddni2) + dnu_s[cb] * ddni1;
-                const T v = (rv[rr][j] - cc) * lv[rr][j] + om1 * xv[4];
-                T *w = okv[j] ? b0 + cb + rr * W + 4 : sink + tid;
-                *w = v;
+            for (int g = 0; g < G; ++g) {
+                const int rr = rg + g;
+                const T am4 = ab[8 * rr + 0], am3 = ab[8 * rr + 1], am2 = ab[8 * rr + 2], am1 = ab[8 * rr + 3];
+                const T ap4 = ab[8 * rr + 4], ap3 = ab[8 * rr + 5], ap2 = ab[8 * rr + 6], ap1 = ab[8 * rr + 7];
+#pragma unroll
+                for (int j = 0; j < NT; ++j) {
+                    const T *xv = x[g][j];
+                    const int cb = cbv[g][j];
+                    if constexpr (sizeof(T) == 4) {
+                        // one accumulator: the 8 mu terms, the 4 symmetric nu terms,
+                        // then d(nu) times the antisymmetric nu sum (28 operations)
+                        float acc = am4 * xv[12];
+                        acc = __builtin_fmaf(ap4, xv[16], acc);
+                        acc = __builtin_fmaf(am3, xv[11], acc);
+                        acc = __builtin_fmaf(ap3, xv[15], acc);
+                        acc = __builtin_fmaf(am2, xv[10], acc);
+                        acc = __builtin_fmaf(ap2, xv[14], acc);
+                        acc = __builtin_fmaf(am1, xv[9], acc);
+                        acc = __builtin_fmaf(ap1, xv[13], acc);
+                        acc = __builtin_fmaf(cf[4], xv[0] + xv[8], acc);
+                        acc = __builtin_fmaf(cf[5], xv[1] + xv[7], acc);
+                        acc = __builtin_fmaf(cf[6], xv[2] + xv[6], acc);
+                        acc = __builtin_fmaf(cf[7], xv[3] + xv[5], acc);
+                        float t = cf[0] * (xv[0] - xv[8]);
+                        t = __builtin_fmaf(cf[1], xv[1] - xv[7], t);
+                        t = __builtin_fmaf(cf[2], xv[2] - xv[6], t);
+                        t = __builtin_fmaf(cf[3], xv[3] - xv[5], t);
+                        acc = __builtin_fmaf(dnu_s[cb], t, acc);
+                        const float v = __builtin_fmaf(rv[rr][j] - acc, lv[rr][j], om1 * xv[4]);
+                        T *w = okv[g][j] ? dnu_s + ov[g][j] + 4 : sink + tid;
+                        *w = v;
+                    } else {
+                        const T dmu = (am4 * xv[12] + ap4 * xv[16]) + (am3 * xv[11] + ap3 * xv[15]) +
+                                      ((am2 * xv[10] + ap2 * xv[14]) + (am1 * xv[9] + ap1 * xv[13]));
+                        const T ddni2 = (cf[4] * (xv[0] + xv[8]) + cf[5] * (xv[1] + xv[7])) +
+                                        (cf[6] * (xv[2] + xv[6]) + cf[7] * (xv[3] + xv[5]));
+                        const T ddni1 = (cf[0] * (xv[0] - xv[8]) + cf[1] * (xv[1] - xv[7])) +
+                                        (cf[2] * (xv[2] - xv[6]) + cf[3] * (xv[3] - xv[5]));
+                        const T cc = (dmu + ddni2) + dnu_s[cb] * ddni1;
+                        const T v = (rv[rr][j] - cc) * lv[rr][j] + om1 * xv[4];
+                        T *w = okv[g][j] ? b0 + cb + rr * W + 4 : sink + tid;
+                        *w = v;
+                    }
+                }
             }
         }
     };
@@ -724,19 +786,25 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int MS)
         if (pass_wave) compute(s, wave, lv, rv);
         else compute(s, p_spill, lv, rv);
     };
-    // the mu=1 row once rows 2..6 are final (first strip only, at bw == 5;
-    // rows 2 and 3 sit in the slot that the commit of the same step reuses)
+    // The mu=1 row (sormcsor.F90:333-350) of the first strip, after its last
+    // step: rows 2..6 are final in dst, written by this workgroup (the
+    // barrier makes its stores visible to all of its threads).  A fixed number
+    // of loads and stores per thread (nni < W <= RT * 320 <= RT * nth; lanes
+    // beyond nni store out of range).
     auto row_one = [&]() {
-        const int q2 = 2 - r0;  // rows 2..6 sit at strip offsets 16..20
-        const T *rw[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) rw[k] = ring + (slot_of((q2 + k) >> 2) * 4 + ((q2 + k) & 3)) * W;
+        constexpr int RT = (W + 319) / 320;
         auto r1 = [&](int nu) -> T {
-            const int c0 = nu + 3;
-            return d.ex[0] * rw[0][c0] + d.ex[1] * rw[1][c0] + d.ex[2] * rw[2][c0] + d.ex[3] * rw[3][c0] +
-                   d.ex[4] * rw[4][c0];
+            T acc = T(0);
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                acc += d.ex[k] * buf_load1(rs_dst, (unsigned)(((5 + k) * P + nu + 3) * (int)sizeof(T)), T(0));
+            return acc;
         };
-        for (int nu = 1 + tid; nu <= nni; nu += nth) {
+#pragma unroll
+        for (int k = 0; k < RT; ++k) {
+            const int nu0 = 1 + tid + k * nth;
+            const bool ok = nu0 <= nni;
+            const int nu = ok ? nu0 : 2;
             T v = T(0);
             if (even) {
                 if (nu == 1 || nu == nni) {
@@ -747,69 +815,43 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int MS)
                     v = r1(nu);
                 }
             }
-            d.dst[4 * P + nu + 3] = v;
+            buf_store1(rs_dst, oob_unless(ok, (4 * P + nu + 3) * (int)sizeof(T)), v);
         }
     };
 
-    if (PIPE == 2) {
-        // Block s+2 arrives in LDS at the end of step s from the register set
-        // loaded in step s-1; the (omega/lhs, rhs) of step s were loaded in
-        // step s-2.  Two register sets of each alternate between even and odd
-        // steps.
-        auto step = [&](int s, const u32x4 (&pc)[NSV], u32x4 (&pn)[NSV], const T &bc, T &bn, T (&lc)[4][NT],
-                        T (&rcv)[4][NT]) {
-            load_block(s + 3, pn, bn);
-            compute_any(s, lc, rcv);
-            load_lr_any(s + 2, lc, rcv);
-            const int bw = s - 4 * SKEW - 1;
-            if (a == 2 && bw == 5) {
-                row_one();
-                __syncthreads();
-            }
-            commit_block(s + 2, pc, bc);
-            write_out(bw);
-            __syncthreads();
-        };
-        u32x4 pa[NSV], pb[NSV];
-        T ba, bb;
-        T la[4][NT], ra[4][NT], lb[4][NT], rb[4][NT];
-        load_block(2, pa, ba);
-        load_lr_any(0, la, ra);
-        load_lr_any(1, lb, rb);
-        for (int s = 0; s < S; s += 2) {
-            step(s, pa, pb, ba, bb, la, ra);
-            if (s + 1 < S) step(s + 1, pb, pa, bb, ba, lb, rb);
-        }
-    } else {
-        // One step ahead: block s+2 and the (omega/lhs, rhs) of step s+1 are
-        // loaded at the start of step s and have landed before its commit, so
-        // the next compute phase never waits on them.
-        T lv[4][NT], rv[4][NT];
-        load_lr_any(0, lv, rv);
+    // Block s+2 and the (omega/lhs, rhs) of step s+1 are loaded at the start
+    // of step s and land during its compute phase.  Block bw = s - 9 is final
+    // from the start of step s (pass 4 finished it in step s-1).  FP64 writes
+    // it out right after issuing the loads, so its stores overlap the compute
+    // phase; FP32 writes it out at the end of the step (each measured faster
+    // that way).  The commit of block s+2 reuses the slot of block s-10,
+    // which was written out in step s-1.
+    constexpr bool early_out = sizeof(T) == 8;
+    T lv[4][NT], rv[4][NT];
+    load_lr_any(0, lv, rv);
+    __builtin_amdgcn_s_waitcnt(0);
+    for (int s = 0; s < S; ++s) {
+        T ln[4][NT], rn[4][NT], bpre;
+        u32x4 pre[NSV];
+        load_lr_any(s + 1, ln, rn);
+        load_block(s + 2, pre, bpre);
+        if (early_out) write_out(s - 4 * SKEW - 1);
+        compute_any(s, lv, rv);
         __builtin_amdgcn_s_waitcnt(0);
-        for (int s = 0; s < S; ++s) {
-            T ln[4][NT], rn[4][NT], bpre;
-            u32x4 pre[NSV];
-            load_lr_any(s + 1, ln, rn);
-            load_block(s + 2, pre, bpre);
-            compute_any(s, lv, rv);
-            const int bw = s - 4 * SKEW - 1;
-            if (a == 2 && bw == 5) {
-                row_one();
-                __syncthreads();
+        commit_block(s + 2, pre, bpre);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                lv[rr][j] = ln[rr][j];
+                rv[rr][j] = rn[rr][j];
             }
-            __builtin_amdgcn_s_waitcnt(0);
-            commit_block(s + 2, pre, bpre);
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr)
-#pragma unroll
-                for (int j = 0; j < NT; ++j) {
-                    lv[rr][j] = ln[rr][j];
-                    rv[rr][j] = rn[rr][j];
-                }
-            write_out(bw);
-            __syncthreads();
-        }
+        if (!early_out) write_out(s - 4 * SKEW - 1);
+        __syncthreads();
+    }
+    if (a == 2) {
+        __syncthreads();
+        row_one();
     }
 }
 
@@ -1322,27 +1364,27 @@ static int sweep_kind(x2g_ctx *ctx, int maxnni)
 // Launch one sweep of every problem in `descs` (device copy dd, host copy hv).
 // k_sweep for the plan's stride class and depth (the instantiations are
 // listed once here and in set_lds_attr).
-template <typename T, int NT, int PIPE, int W>
+template <typename T, int NT, int W>
 static void launch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, int n, const SweepPlan<T> &plan)
 {
     // the caller has flushed the staged descriptors
-    hipLaunchKernelGGL((k_sweep<T, NT, PIPE, W>), dim3(plan.total_wg), dim3(plan.nth), plan.lds, ctx->stream, dd, n,
+    hipLaunchKernelGGL((k_sweep<T, NT, W>), dim3(plan.total_wg), dim3(plan.nth), plan.lds, ctx->stream, dd, n,
                        plan.MS);
 }
 
-template <typename T, int PIPE>
+template <typename T>
 static bool dispatch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, int n, const SweepPlan<T> &plan)
 {
     switch (plan.W) {
-    case 128: launch_ks<T, 1, PIPE, 128>(ctx, dd, n, plan); return true;
-    case 192: launch_ks<T, 1, PIPE, 192>(ctx, dd, n, plan); return true;
-    case 256: launch_ks<T, 1, PIPE, 256>(ctx, dd, n, plan); return true;
-    case 352: launch_ks<T, 1, PIPE, 352>(ctx, dd, n, plan); return true;
+    case 128: launch_ks<T, 1, 128>(ctx, dd, n, plan); return true;
+    case 192: launch_ks<T, 1, 192>(ctx, dd, n, plan); return true;
+    case 256: launch_ks<T, 1, 256>(ctx, dd, n, plan); return true;
+    case 352: launch_ks<T, 1, 352>(ctx, dd, n, plan); return true;
     default: break;
     }
     if constexpr (sizeof(T) == 4) {
-        if (plan.W == 528) { launch_ks<T, 1, PIPE, 528>(ctx, dd, n, plan); return true; }
-        if (plan.W == 768) { launch_ks<T, 2, PIPE, 768>(ctx, dd, n, plan); return true; }
+        if (plan.W == 528) { launch_ks<T, 1, 528>(ctx, dd, n, plan); return true; }
+        if (plan.W == 768) { launch_ks<T, 2, 768>(ctx, dd, n, plan); return true; }
     }
     return false;
 }
@@ -1367,13 +1409,7 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
     rc = stats_begin(ctx, &ea, &eb);
     if (rc) return rc;
     if (plan.kind == 2) {
-        // two-deep pipeline for FP32 batches that place several workgroups on
-        // each CU (measured faster there); one-deep elsewhere
-        bool ok;
-        if constexpr (sizeof(T) == 4)
-            ok = total_wg >= 2 * ctx->num_cu ? dispatch_ks<T, 2>(ctx, dd, n, plan) : dispatch_ks<T, 1>(ctx, dd, n, plan);
-        else
-            ok = dispatch_ks<T, 1>(ctx, dd, n, plan);
+        const bool ok = dispatch_ks<T>(ctx, dd, n, plan);
         X2G_CHECK(ok, "no sweep kernel for ring stride %d", plan.W);
         X2G_HIP(hipGetLastError());
         return stats_end(ctx, ea, eb, bytes, prec);
@@ -1413,14 +1449,11 @@ static int set_lds_attr(x2g_ctx *ctx)
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     const void *ks[] = {
-        (const void *)k_sweep<double, 1, 1, 128>, (const void *)k_sweep<double, 1, 1, 192>,
-        (const void *)k_sweep<double, 1, 1, 256>, (const void *)k_sweep<double, 1, 1, 352>,
-        (const void *)k_sweep<float, 1, 1, 128>,  (const void *)k_sweep<float, 1, 1, 192>,
-        (const void *)k_sweep<float, 1, 1, 256>,  (const void *)k_sweep<float, 1, 1, 352>,
-        (const void *)k_sweep<float, 1, 1, 528>,  (const void *)k_sweep<float, 2, 1, 768>,
-        (const void *)k_sweep<float, 1, 2, 128>,  (const void *)k_sweep<float, 1, 2, 192>,
-        (const void *)k_sweep<float, 1, 2, 256>,  (const void *)k_sweep<float, 1, 2, 352>,
-        (const void *)k_sweep<float, 1, 2, 528>,  (const void *)k_sweep<float, 2, 2, 768>};
+        (const void *)k_sweep<double, 1, 128>, (const void *)k_sweep<double, 1, 192>,
+        (const void *)k_sweep<double, 1, 256>, (const void *)k_sweep<double, 1, 352>,
+        (const void *)k_sweep<float, 1, 128>,  (const void *)k_sweep<float, 1, 192>,
+        (const void *)k_sweep<float, 1, 256>,  (const void *)k_sweep<float, 1, 352>,
+        (const void *)k_sweep<float, 1, 528>,  (const void *)k_sweep<float, 2, 768>};
     for (const void *k : ks) X2G_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     done = true;
     return X2G_OK;
