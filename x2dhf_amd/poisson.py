@@ -127,18 +127,19 @@ class Batch:
             self.ctx.close()
 
 
-def run(count=256, seed=0, prec=1, tol=1e-12):
+def run(count=256, seed=0, prec=1, tol=1e-12, stall_checks=40):
     """Build, solve and time a batch; returns a summary dict."""
     probs = make_problems(count, seed)
     t0 = time.perf_counter()
     b = Batch(probs, prec=prec)
+    gpu.check(b.lib.x2g_ctx_sync(b.ctx.h))
     t1 = time.perf_counter()
     gpu.check(b.lib.x2g_stats_enable(b.ctx.h, 1))
-    calls, res = b.solve(tol=tol)
+    calls, res = b.solve(tol=tol, stall_checks=stall_checks)
     gpu.check(b.lib.x2g_ctx_sync(b.ctx.h))
     t2 = time.perf_counter()
     st = (C.c_double * 6)()
     gpu.check(b.lib.x2g_stats_get(b.ctx.h, st))
     b.close()
     return dict(problems=count, setup_s=t1 - t0, solve_s=t2 - t1, max_residual=float(res.max()),
-                calls=calls.tolist(), stats=list(st))
+                residuals=res.tolist(), calls=calls.tolist(), stats=list(st))

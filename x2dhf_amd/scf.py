@@ -156,6 +156,23 @@ def fh_spec(nni=481, nmu=901, rinf=50.0, r=1.733, **kw):
     return make_spec(9.0, 1.0, r, nni, nmu, rinf, orbs, **kw)
 
 
+def lih_spec(nni=169, nmu=241, rinf=40.0, r=3.015, **kw):
+    """LiH: 2 sigma, 1 sigma with the hydrogenic LCAO guess of the reference's
+    LiH deck (test-sets/lih/set-01/input.data)."""
+    orbs = [Orbital(0, 0, 1.0, 1, 0, 3.0, -1.0, 1, 0, 1.0),
+            Orbital(0, 0, 1.0, 1, 0, 3.0, 1.0, 1, 0, 1.0)]
+    return make_spec(3.0, 1.0, r, nni, nmu, rinf, orbs, **kw)
+
+
+def bh_spec(nni=169, nmu=241, rinf=40.0, r=2.3289, **kw):
+    """BH: 3 sigma, 2 sigma, 1 sigma with the hydrogenic LCAO guess of the
+    reference's BH deck (test-sets/bh-elprop/set-01/input.data)."""
+    orbs = [Orbital(0, 0, 1.0, 2, 1, 4.0, -1.0, 1, 0, 1.0),
+            Orbital(0, 0, 1.0, 2, 0, 4.0, 0.0, 1, 0, 1.0),
+            Orbital(0, 0, 1.0, 1, 0, 5.0, 0.0, 1, 0, 1.0)]
+    return make_spec(5.0, 1.0, r, nni, nmu, rinf, orbs, **kw)
+
+
 def spec_bytes(spec):
     """Device bytes a job of this specification holds (its own arrays), without
     creating it; raises ValueError for a specification the driver rejects."""

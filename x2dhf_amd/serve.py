@@ -144,6 +144,32 @@ class Server:
         self.close()
 
 
+# closed-shell diatomics of a served stream: (name, spec builder, equilibrium R)
+STREAM_MOLECULES = (("H2", "h2_spec", 1.4), ("LiH", "lih_spec", 3.015), ("BH", "bh_spec", 2.3289),
+                    ("FH", "fh_spec", 1.733))
+STREAM_GRIDS = ((169, 241), (337, 601), (505, 901))
+
+
+def poisson_stream(count, rate, seed=0, prec=1, rinf=40.0):
+    """A seeded stream of requests: exponential inter-arrival times of mean
+    1/rate seconds; each request a molecule of STREAM_MOLECULES with R drawn
+    uniformly within 10% of its equilibrium value and a grid of STREAM_GRIDS,
+    converged to orbital-energy and norm changes below 1e-12.
+    Returns [(arrival_s, name, spec)]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for _ in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        name, builder, req = STREAM_MOLECULES[int(rng.integers(len(STREAM_MOLECULES)))]
+        nni, nmu = STREAM_GRIDS[int(rng.integers(len(STREAM_GRIDS)))]
+        r = req * float(rng.uniform(0.9, 1.1))
+        spec = getattr(scf, builder)(nni=nni, nmu=nmu, rinf=rinf, r=r, prec=prec, schedule=1, ienterm=12,
+                                     inoterm=12, maxscf=8000)
+        out.append((t, name, spec))
+    return out
+
+
 def percentile(values, q):
     """q-th percentile (0..100) by linear interpolation between order statistics."""
     v = sorted(values)
