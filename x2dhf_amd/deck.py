@@ -1,7 +1,7 @@
 """Input decks in the reference's card grammar -> job specifications.
 
 x2dhf reads a deck of cards (inputInterface.F90:681-818): one label per card,
-then its items, blank-separated; lines starting with '!' or '#' are comments; labels and
+then its items, blank-separated; '!' starts a comment (a whole line starting with '#' too); labels and
 keywords count by their first 8 characters, as the reference reads them.
 This module reads the cards that define a closed-shell Hartree-Fock job on the
 relaxation path and builds the scf.JobSpec the GPU driver runs:
@@ -15,7 +15,8 @@ relaxation path and builds the scf.JobSpec the GPU driver runs:
   grid nnu [nmu] Rinf                nnu and nmu rounded down to 1 mod 30; without nmu
                                      it follows from Rinf (read_grid 1491-1556,
                                      nmucalc 2961-3037, nnucalc 3043-3095)
-  orbpot hydrogen [ienterm4init]     hydrogenic LCAO start only (read_orbpot, 2257-2360)
+  orbpot hydrogen [ienterm4init]     hydrogenic LCAO start, or
+  orbpot old                         a start from a checkpoint (read_orbpot, 2257-2360)
   lcao [1|2]                         then one card per orbital:
      c1 n1 l1 Z1' c2 n2 l2 Z2' [..]  with '2' the exponents are Z - value (read_lcao, 1749-1881)
   scf maxscf [save ienterm inoterm verbose]   (read_scf, 2735-2783)
@@ -93,10 +94,16 @@ class Deck:
     recalc_mm: float = 1.15
     maxsororb: tuple = (1, 10)
     maxsorpot: tuple = (1, 10)
+    start: str = "hydrogen"                        # orbpot: hydrogen | old
     notes: list = field(default_factory=list)
 
     def spec(self, prec=0, schedule=0, device=0):
-        """The scf.JobSpec of this deck."""
+        """The scf.JobSpec of this deck.  For orbpot old the LCAO guess is a
+        placeholder (one hydrogenic function per orbital), replaced by the
+        checkpoint's functions (x2dhf_amd.checkpoint.resume)."""
+        if self.start == "old" and not self.lcao:
+            zc = self.z1 if self.z1 > 0 else self.z2
+            self.lcao = [(1.0, m + 1, m, zc, 0.0, m + 1, m, zc) for m, _ in self.orbitals]
         if len(self.lcao) != len(self.orbitals):
             raise DeckError("the deck needs one lcao card per orbital")
         orbs = [scf.Orbital(m, gu, *lc) for (m, gu), lc in zip(self.orbitals, self.lcao)]
@@ -109,8 +116,8 @@ class Deck:
 
 def _cards(text):
     for raw in text.splitlines():
-        line = raw.strip()
-        if not line or line[0] in "!#":
+        line = raw.split("!")[0].strip()  # '!' starts a comment, also after items
+        if not line or line[0] == "#":
             continue
         yield line.split()
 
@@ -200,11 +207,19 @@ def parse(text):
             else:
                 raise DeckError("grid: nnu [nmu] Rinf expected")
         elif label == "orbpot":
-            if not items or _word(items[0]) != "hydrogen":
-                raise DeckError(f"orbpot {' '.join(items)}: only the hydrogenic LCAO start runs on this path")
-            n = _ints(items[1:], label)
-            if n:
-                d.ienterm4init = n[0]
+            kind = _word(items[0]) if items else ""
+            if kind == "old":
+                # functions from a checkpoint (x2dhf_amd.checkpoint); no
+                # frozen-orbital start (read_orbpot leaves lfixorb4init off)
+                d.start, d.ienterm4init = "old", 0
+            elif kind == "hydrogen":
+                d.start = "hydrogen"
+                n = _ints(items[1:], label)
+                if n:
+                    d.ienterm4init = n[0]
+            else:
+                raise DeckError(f"orbpot {' '.join(items)}: only the hydrogenic LCAO start and checkpoints "
+                                "(orbpot old) run on this path")
         elif label == "lcao":
             incl = _ints(items, label)
             mode = incl[0] if incl else 1
@@ -274,7 +289,7 @@ def parse(text):
             d.notes.append(f"{label} {' '.join(items)}".strip())
         else:
             raise DeckError(f"card '{label}' is outside the relaxation path of this build")
-    for need in ("nuclei", "config", "grid", "orbpot", "lcao"):
+    for need in ("nuclei", "config", "grid", "orbpot") + (("lcao",) if d.start == "hydrogen" else ()):
         if need not in seen:
             raise DeckError(f"the deck has no '{need}' card")
     d.nni = nnu_calc(d.nnu_card)

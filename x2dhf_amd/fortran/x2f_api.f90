@@ -187,6 +187,21 @@ contains
     rc = 0
   end function x2f_job_prep
 
+  ! Set-up pass after a checkpoint was loaded with x2f_job_put: the Lagrange
+  ! multipliers ee(ld, norb) come from the checkpoint, the orbitals are taken
+  ! as they are (doSCF.F90:47-174 with initAddData false).
+  integer(c_int) function x2f_job_prep_restart(handle, ee, ld) bind(c, name='x2f_job_prep_restart') result(rc)
+    integer(c_int), value :: handle, ld
+    real(c_double), intent(in) :: ee(ld, *)
+    integer :: n
+    rc = 1
+    if (.not. valid(handle)) return
+    n = jobs(handle)%c%norb
+    if (ld < n) return
+    call prep_scf_restart(jobs(handle), ee(1:n, 1:n))
+    rc = 0
+  end function x2f_job_prep_restart
+
   ! Up to niter iterations of the job's own schedule; 1 when the SCF stopped.
   integer(c_int) function x2f_job_iterate(handle, niter) bind(c, name='x2f_job_iterate') result(rc)
     integer(c_int), value :: handle, niter
@@ -306,6 +321,35 @@ contains
       rc = x2g_d2h(j%ctx, c_loc(out), p, 8_c_size_t*j%n)
     end associate
   end function x2f_job_get
+
+  ! Copy the host array in(nni*nmu) to orbital i (kind 0), Coulomb potential i
+  ! (kind 1) or exchange slot i (kind 2): loading a checkpoint (readFuncs,
+  ! diskInterface.F90:247-779).  Call before x2f_job_prep, which forms the
+  ! Fock operators, norms, moments and energies from what is loaded.
+  integer(c_int) function x2f_job_put(handle, kind, i, in) bind(c, name='x2f_job_put') result(rc)
+    integer(c_int), value :: handle, kind, i
+    real(c_double), intent(in), target :: in(*)
+    type(c_ptr) :: p
+    rc = 1
+    if (.not. valid(handle)) return
+    associate (j => jobs(handle))
+      select case (kind)
+      case (0)
+         if (i < 1 .or. i > j%c%norb) return
+         p = j%orb(i)
+      case (1)
+         if (i < 1 .or. i > j%c%norb) return
+         p = j%coul(i)
+      case (2)
+         if (i < 1 .or. i > j%c%nslots) return
+         p = j%exch(i)
+      case default
+         return
+      end select
+      rc = x2g_h2d(j%ctx, p, c_loc(in), 8_c_size_t*j%n)
+      if (rc == 0) rc = x2g_ctx_sync(j%ctx)
+    end associate
+  end function x2f_job_put
 
   ! Device context of a job (x2g_ctx*, for x2g_stats_* and x2g_ctx_sync);
   ! null for an invalid handle.

@@ -31,7 +31,7 @@ module x2f_scf
   use x2f_guess
   implicit none
   private
-  public :: scf_job, scf_opts, job_init, job_free, prep_scf, scf_iteration, scf_finish, &
+  public :: scf_job, scf_opts, job_init, job_free, prep_scf, prep_scf_restart, scf_iteration, scf_finish, &
        batched_iteration, MAXPOT, job_bytes
 
   integer, parameter :: dp = c_double
@@ -753,7 +753,33 @@ contains
        j%eeold(i) = j%ee(i,i)
     enddo
     call etotal_hf(j)
-    ! scf's own initialisation (doSCF.F90:265-334)
+    call scf_init_state(j)
+  end subroutine prep_scf
+
+  ! prepSCF for a start from a checkpoint that carries the Lagrange
+  ! multipliers (readFuncs leaves initAddData false, doSCF.F90:47-174): the
+  ! Fock operators from the loaded orbitals, potentials and multipliers ee,
+  ! no normalisation or orthogonalisation, then the total energy.  The
+  ! multipole moments are formed from the loaded orbitals.
+  subroutine prep_scf_restart(j, ee)
+    type(scf_job), intent(inout) :: j
+    real(dp), intent(in) :: ee(:,:)
+    integer :: i, n
+    n = j%c%norb
+    j%ee(1:n, 1:n) = ee(1:n, 1:n)
+    do i = 1, n
+       call fock_hf(j, i, j%f1o(i), j%f2o(i))
+    enddo
+    j%itouch = 1
+    call mpole_moments(j, j%itouch)
+    call etotal_hf(j)
+    call scf_init_state(j)
+  end subroutine prep_scf_restart
+
+  ! scf's own initialisation (doSCF.F90:265-334)
+  subroutine scf_init_state(j)
+    type(scf_job), intent(inout) :: j
+    integer :: i
     j%iscf = 0
     j%nresets = 0; j%nresets4no = 0; j%norbenergythld = 0; j%norbnormthld = 0
     j%eeconv = 10.0_dp
@@ -769,7 +795,7 @@ contains
     j%noenergydec = 0; j%nonormdec = 0
     j%done = .false.; j%stopcode = 0
     call setci(j, 0)
-  end subroutine prep_scf
+  end subroutine scf_init_state
 
   ! one orbital of the plain schedule (doSCF.F90:368-533)
   subroutine orbital_step(j, i)

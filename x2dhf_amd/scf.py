@@ -72,6 +72,10 @@ def load():
     L.x2f_job_result.restype = _i
     L.x2f_job_get.argtypes = [_i, _i, _i, vp]
     L.x2f_job_get.restype = _i
+    L.x2f_job_put.argtypes = [_i, _i, _i, vp]
+    L.x2f_job_put.restype = _i
+    L.x2f_job_prep_restart.argtypes = [_i, vp, _i]
+    L.x2f_job_prep_restart.restype = _i
     L.x2f_job_bytes.argtypes = [_i]
     L.x2f_job_bytes.restype = C.c_longlong
     L.x2f_job_ctx.argtypes = [_i]
@@ -209,6 +213,12 @@ class Job:
     def prep(self):
         self._chk(self.L.x2f_job_prep(self.h))
 
+    def prep_restart(self, ee):
+        """Set-up pass after loading a checkpoint with put(): ee is the
+        checkpoint's Lagrange-multiplier matrix (at least norb x norb)."""
+        a = np.asfortranarray(np.asarray(ee, dtype=np.float64))
+        self._chk(self.L.x2f_job_prep_restart(self.h, a.ctypes.data, a.shape[0]))
+
     def iterate(self, n=1):
         """Run up to n SCF iterations; True when the SCF stopped."""
         rc = self.L.x2f_job_iterate(self.h, n)
@@ -241,6 +251,14 @@ class Job:
         out = np.empty(self.spec.nni * self.spec.nmu)
         self._chk(self.L.x2f_job_get(self.h, kind, i, out.ctypes.data))
         return out
+
+    def put(self, kind, i, values):
+        """Load orbital i (kind 0), Coulomb potential i (1) or exchange slot i
+        (2, 1-based) from a host array of nni*nmu values (before prep())."""
+        a = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        if a.size != self.spec.nni * self.spec.nmu:
+            raise ValueError(f"put: {a.size} values for a {self.spec.nni}x{self.spec.nmu} grid")
+        self._chk(self.L.x2f_job_put(self.h, kind, i, a.ctypes.data))
 
     def bytes(self):
         return int(self.L.x2f_job_bytes(self.h))
