@@ -163,8 +163,13 @@ def traffic_summary(fetch_csv, write_csv, batch):
 
     def per_kernel(path):
         rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
-        copy = [float(r["Counter_Value"]) * 1024 for r in rows if r["Kernel_Name"].startswith("k_lincomb")]
-        sweep = [float(r["Counter_Value"]) * 1024 for r in rows if r["Kernel_Name"].startswith("k_sweep")]
+        # kernel names come demangled ("x2g::k_lincomb(double*, ...)") or bare ("k_lincomb")
+        def named(r, k):
+            n = r["Kernel_Name"]
+            return n.startswith(k) or f"::{k}(" in n or f"::{k}<" in n
+
+        copy = [float(r["Counter_Value"]) * 1024 for r in rows if named(r, "k_lincomb")]
+        sweep = [float(r["Counter_Value"]) * 1024 for r in rows if named(r, "k_sweep")]
         if len(copy) != 3 or len(sweep) != 10:
             raise SystemExit(f"{path}: expected 3 copy and 10 sweep dispatches, found {len(copy)} and {len(sweep)}")
         return sum(copy) / len(copy), sweep[:5], sweep[5:]
