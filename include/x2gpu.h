@@ -28,6 +28,9 @@
  *   x2g_moments        <- coulMom / exchMom                    src/scfUtils.F90:490,615
  *   x2g_pot_tail       <- coulAsympt / exchAsympt (vcoul/vexch) src/scfUtils.F90:275,310
  *   x2g_setci          <- setCiOrb                             src/scfUtils.F90:380
+ *   x2g_guess_orbital  <- initHyd (hydrogenic LCAO start)      src/initOrbitalsPotentials.F90:714-880
+ *   x2g_guess_coulomb  <- initCoulomb (Thomas-Fermi start)     src/coulombExchangePotentials.f90:14-71
+ *   x2g_guess_exchange <- initExchange (1/r start)             src/coulombExchangePotentials.f90:74-147
  *
  * Batched forms (many problems, possibly on different grids, per launch; the
  * scalars stay on the device until x2g_gather):
@@ -98,6 +101,28 @@ int x2g_fill(x2g_ctx *ctx, double *dst_d, size_t n, double value);
 int x2g_grid_create(x2g_ctx *ctx, int nni, int nmu, const double *coef, const double *suppl,
                     const double *vec, x2g_grid **out);
 int x2g_grid_destroy(x2g_grid *g);
+
+/* ---- starting functions of a job ---------------------------------------- */
+/* LCAO start of one orbital, c1 phi_A(n1,l1,m; za) + c2 phi_B(n2,l2,m; zb),
+ * with phi the hydrogenic function in the reference's normalisation and sign
+ * (unnormalised sum, as initHyd leaves it); c1 or c2 = 0 omits that centre.
+ * Replaces initHyd / hydrogenOrbA / hydrogenOrbB
+ * src/initOrbitalsPotentials.F90:714-880. */
+typedef struct {
+    int m;               /* magnetic quantum number of the orbital */
+    int n1, l1, n2, l2;  /* hydrogenic quantum numbers on centres A and B */
+    double c1, za;       /* mixing coefficient and exponent on centre A */
+    double c2, zb;       /* the same on centre B */
+} x2g_lcao;
+int x2g_guess_orbital(x2g_grid *g, double *orb_d, const x2g_lcao *lc);
+/* Thomas-Fermi start of a Coulomb potential (times f4): -|c1| V_TF(r_A)
+ * [- |c2| V_TF(r_B) when two_centres != 0].  Replaces initCoulomb + pottf
+ * src/coulombExchangePotentials.f90:14-71,222-239. */
+int x2g_guess_coulomb(x2g_grid *g, double *pot_d, double c1, double c2, int two_centres);
+/* 1/r start of an exchange potential (times f4): c1/r_A + c2/r_B on the
+ * interior nu = 2..nni-1, mu >= 2.  Replaces initExchange
+ * src/coulombExchangePotentials.f90:74-147. */
+int x2g_guess_exchange(x2g_grid *g, double *pot_d, double c1, double c2);
 /* Device pointer of supplementary array k (0..13, i4b order) of grid g. */
 const double *x2g_grid_suppl(x2g_grid *g, int k);
 

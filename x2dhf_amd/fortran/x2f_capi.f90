@@ -35,6 +35,12 @@ module x2f_capi
 
   integer, parameter :: X2G_MAX_ORTHO = 16
 
+  type, bind(c) :: x2g_lcao
+     integer(c_int) :: m = 0, n1 = 1, l1 = 0, n2 = 1, l2 = 0
+     real(c_double) :: c1 = 0.0_c_double, za = 0.0_c_double
+     real(c_double) :: c2 = 0.0_c_double, zb = 0.0_c_double
+  end type x2g_lcao
+
   type, bind(c) :: x2g_lincomb_op
      type(c_ptr) :: out_d = c_null_ptr
      type(c_ptr) :: base_d = c_null_ptr
@@ -137,6 +143,25 @@ module x2f_capi
        integer(c_size_t), value :: n
        real(c_double), value :: value
      end function x2g_fill
+
+     integer(c_int) function x2g_guess_orbital(g, orb, lc) bind(c, name='x2g_guess_orbital')
+       import :: c_int, c_ptr, x2g_lcao
+       type(c_ptr), value :: g, orb
+       type(x2g_lcao), intent(in) :: lc
+     end function x2g_guess_orbital
+
+     integer(c_int) function x2g_guess_coulomb(g, pot, c1, c2, two_centres) bind(c, name='x2g_guess_coulomb')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g, pot
+       real(c_double), value :: c1, c2
+       integer(c_int), value :: two_centres
+     end function x2g_guess_coulomb
+
+     integer(c_int) function x2g_guess_exchange(g, pot, c1, c2) bind(c, name='x2g_guess_exchange')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g, pot
+       real(c_double), value :: c1, c2
+     end function x2g_guess_exchange
 
      integer(c_int) function x2g_grid_create(ctx, nni, nmu, coef, suppl, vec, out) &
           bind(c, name='x2g_grid_create')

@@ -28,7 +28,6 @@ module x2f_scf
   use x2f_capi
   use x2f_grid
   use x2f_case
-  use x2f_guess
   implicit none
   private
   public :: scf_job, scf_opts, job_init, job_free, prep_scf, prep_scf_restart, scf_iteration, scf_finish, &
@@ -106,7 +105,8 @@ contains
     integer, intent(in) :: device
     type(c_ptr), intent(in) :: ctx_in
     real(dp) :: coef(25)
-    real(dp), allocatable, target :: vec(:), orbs(:,:), tmp(:)
+    real(dp), allocatable, target :: vec(:)
+    type(x2g_lcao) :: lc
     integer :: i, k, ip, ii, slot
     integer(c_size_t) :: nb
 
@@ -159,24 +159,26 @@ contains
        enddo
     enddo
 
-    ! starting orbitals and potentials (initHyd + initCoulomb + initExchange)
-    allocate(orbs(j%n, j%c%norb), tmp(j%n))
-    call guess_orbitals(j%g, j%c, orbs)
+    ! starting orbitals and potentials, formed on the device (initHyd +
+    ! initCoulomb + initExchange); the exchange start of the pair (i, i') is
+    ! that of its first orbital i
     do i = 1, j%c%norb
-       call x2check(x2g_h2d(j%ctx, j%orb(i), c_loc(orbs(1,i)), nb), 'x2g_h2d')
-       call guess_coulomb(j%g, j%c, i, tmp)
-       call x2check(x2g_h2d(j%ctx, j%coul(i), c_loc(tmp), nb), 'x2g_h2d')
+       lc = x2g_lcao(int(j%c%m(i), c_int), int(j%c%n1(i), c_int), int(j%c%l1(i), c_int), &
+            int(j%c%n2(i), c_int), int(j%c%l2(i), c_int), j%c%c1(i), j%c%za(i), j%c%c2(i), j%c%zb(i))
+       call x2check(x2g_guess_orbital(j%gd, j%orb(i), lc), 'x2g_guess_orbital')
+       call x2check(x2g_guess_coulomb(j%gd, j%coul(i), j%c%c1(i), j%c%c2(i), &
+            merge(1_c_int, 0_c_int, nint(j%c%z1) /= 0 .and. nint(j%c%z2) /= 0)), 'x2g_guess_coulomb')
     enddo
     do i = 1, j%c%norb
        do ii = i, j%c%norb
           k = pair_index(i, ii)
           if (j%c%ilc(k) == 0) cycle
-          call guess_exchange(j%g, j%c, i, tmp)
-          call x2check(x2g_h2d(j%ctx, j%exch(j%c%slot(k)), c_loc(tmp), nb), 'x2g_h2d')
-          if (j%c%ilc(k) == 2) call x2check(x2g_h2d(j%ctx, j%exch(j%c%slot2(k)), c_loc(tmp), nb), 'x2g_h2d')
+          call x2check(x2g_guess_exchange(j%gd, j%exch(j%c%slot(k)), j%c%c1(i), j%c%c2(i)), 'x2g_guess_exchange')
+          if (j%c%ilc(k) == 2) call x2check(x2g_guess_exchange(j%gd, j%exch(j%c%slot2(k)), j%c%c1(i), &
+               j%c%c2(i)), 'x2g_guess_exchange')
        enddo
     enddo
-    deallocate(orbs, tmp, vec)
+    deallocate(vec)
 
     ! relaxation parameters
     call omega_defaults(j%g%nni, j%g%nmu, j%c%z1, j%c%z2, j%omega_orb, j%omega_pot)

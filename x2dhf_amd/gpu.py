@@ -69,6 +69,12 @@ class TailItem(C.Structure):
     _fields_ = [("pot_d", _vp), ("kind", _int), ("dm", _int), ("grid_index", _int), ("mom", _dbl * 8)]
 
 
+class Lcao(C.Structure):
+    """x2g_lcao"""
+    _fields_ = [("m", _int), ("n1", _int), ("l1", _int), ("n2", _int), ("l2", _int),
+                ("c1", _dbl), ("za", _dbl), ("c2", _dbl), ("zb", _dbl)]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "x2g_version": (_int, []),
@@ -88,6 +94,9 @@ _PROTOS = {
     "x2g_fill": (_int, [_vp, _vp, _szt, _dbl]),
     "x2g_grid_create": (_int, [_vp, _int, _int, _vp, _vp, _vp, C.POINTER(_vp)]),
     "x2g_grid_destroy": (_int, [_vp]),
+    "x2g_guess_orbital": (_int, [_vp, _vp, C.POINTER(Lcao)]),
+    "x2g_guess_coulomb": (_int, [_vp, _vp, _dbl, _dbl, _int]),
+    "x2g_guess_exchange": (_int, [_vp, _vp, _dbl, _dbl]),
     "x2g_grid_suppl": (_vp, [_vp, _int]),
     "x2g_set_sweep_engine": (_int, [_int, _int]),
     "x2g_putin": (_int, [_vp, _int, _vp, _vp]),
