@@ -408,3 +408,26 @@ def test_mixed_precision_refines_to_fp64(ctx):
     assert out[0][1] < 1e-10 and out[1][1] < 1e-10, (out[0][1], out[1][1])
     assert relerr(out[1][0], out[0][0]) < 1e-10
     g.destroy()
+
+
+@pytest.mark.parametrize("nni,nmu,isym", [(91, 121, 1), (169, 241, -1), (331, 601, 1), (505, 901, -1), (523, 151, 1)])
+def test_residual_norms_vs_oracle(ctx, nni, nmu, isym):
+    """The FP64 residual rhs - (L + lhs) u of the correction equations (both
+    residual kernels: the row kernel for nni <= 512, the per-point kernel for
+    wider grids): its squared norm against the oracle's residual."""
+    from x2dhf_amd import gpu
+    from oracle import pyoracle
+    P = synthetic_problem(nni, nmu, seed=nni + nmu)
+    print(f"residual {nni}x{nmu} isym={isym}", flush=True)
+    g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
+    gh = (gpu._vp * 1)(g.h.value)
+    f, l, r = ctx.upload(P["fun"]), ctx.upload(P["lhs"]), ctx.upload(P["rhs"])
+    it = (gpu.RelaxItem * 1)(gpu.RelaxItem(f.ptr, l.ptr, r.ptr, None, 1.9, isym, 1, 1, 0))
+    out = np.zeros(2)
+    gpu.check(ctx.lib.x2g_residual_norms(ctx.h, gh, it, 1, out.ctypes.data))
+    w = pyoracle.putin(nni, nmu, isym, P["fun"])
+    res = pyoracle.residual(nni, nmu, w, P["lhs"], P["rhs"], P["suppl"][1], P["suppl"][2], P["coef"][:16])
+    rhs2 = P["rhs"].reshape(nmu, nni)[1:nmu - 4, 1:nni - 1]
+    assert abs(out[0] - np.sum(res * res)) <= 1e-12 * np.sum(res * res)
+    assert abs(out[1] - np.sum(rhs2 * rhs2)) <= 1e-12 * np.sum(rhs2 * rhs2)
+    g.destroy()

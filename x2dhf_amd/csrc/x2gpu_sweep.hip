@@ -1083,6 +1083,121 @@ __global__ void __launch_bounds__(256) k_residual(const ResDesc *__restrict__ rd
     }
 }
 
+// The same residual with u read once: a workgroup owns the nu columns of a
+// strip of kResRows relaxed rows of one problem (thread t <-> nu = t+1), keeps
+// u(nu, mu-4..mu+4) in registers as it walks down the strip, and reads the nu
+// neighbours from the current row staged in LDS (two row buffers, one barrier
+// per row).  The mirror images of the mu < 1 rows and of the nu < 1, nu > nni
+// columns are placed as they are read, so every point sees the values of the
+// per-point kernel and combines them in the same order.  Needs nni <= blockDim.
+constexpr int kResRows = 64;
+constexpr int kResThreads = 512;
+
+__global__ void __launch_bounds__(kResThreads) k_residual_rows(const ResDesc *__restrict__ rd, int strips)
+{
+    __shared__ double row[2][kResThreads + 8];
+    const ResDesc &d = rd[blockIdx.x / strips];
+    const int nni = d.nni, nmu = d.nmu;
+    const int m0 = 2 + (blockIdx.x % strips) * kResRows;
+    if (m0 > nmu - 4) return;                      // uniform over the workgroup
+    const int m1 = min(m0 + kResRows - 1, nmu - 4);
+    const int nu = threadIdx.x + 1;
+    const bool col = nu <= nni;
+    const bool interior = nu >= 2 && nu <= nni - 1;
+    const double s = d.isym == 1 ? 1.0 : -1.0;
+    const double *f = d.fun, *c = d.c;
+    auto load_u = [&](int mu) -> double {         // u(nu, mu) for mu >= -3, mirrored below mu = 1
+        if (!col) return 0.0;
+        return mu < 1 ? s * f[(long)(1 - mu) * nni + nu - 1] : f[(long)(mu - 1) * nni + nu - 1];
+    };
+    double w[9];                                   // u(nu, mu-4 .. mu+4)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = load_u(m0 - 4 + k);
+    const double dn = col ? d.dnu[nu - 1] : 0.0;
+    const int M = split_m(nni), cA = colour_mult(nni);
+    double sr = 0.0, sb = 0.0;
+    // row mu+4 of u and row mu of lhs / rhs arrive one row ahead
+    auto load_lb = [&](int mu, double &l, double &b) {
+        const long e = (long)(mu - 1) * nni + nu - 1;
+        const bool ok = interior && mu <= m1;
+        l = ok ? d.lhs[e] : 0.0;
+        b = ok ? d.rhs[e] : 0.0;
+    };
+    double un = load_u(m0 + 4), ln, bn;
+    load_lb(m0, ln, bn);
+    for (int mu = m0; mu <= m1; ++mu) {
+        w[8] = un;
+        const double l = ln, b = bn;
+        if (mu < m1) {
+            un = load_u(mu + 5);
+            load_lb(mu + 1, ln, bn);
+        }
+        const long e = (long)(mu - 1) * nni + nu - 1;
+        // stage row mu with its mirrored columns: slot nu+3 for nu = -3 .. nni+4
+        double *r = row[mu & 1];
+        if (col) {
+            r[nu + 3] = w[4];
+            if (nu >= 2 && nu <= 5) r[5 - nu] = s * w[4];                        // nu' = 2 - nu
+            if (nu >= nni - 4 && nu <= nni - 1) r[2 * nni - nu + 3] = s * w[4];  // nu' = 2 nni - nu
+        }
+        __syncthreads();
+        if (interior) {
+            const double *q = r + nu + 3;
+            const double ddmi2 = c[4] * (w[0] + w[8]) + c[5] * (w[1] + w[7]) + c[6] * (w[2] + w[6]) +
+                                 c[7] * (w[3] + w[5]);
+            const double ddmi1 = c[0] * (w[0] - w[8]) + c[1] * (w[1] - w[7]) + c[2] * (w[2] - w[6]) +
+                                 c[3] * (w[3] - w[5]);
+            const double ddni2 = c[12] * (q[-4] + q[4]) + c[13] * (q[-3] + q[3]) + c[14] * (q[-2] + q[2]) +
+                                 c[15] * (q[-1] + q[1]);
+            const double ddni1 = c[8] * (q[-4] - q[4]) + c[9] * (q[-3] - q[3]) + c[10] * (q[-2] - q[2]) +
+                                 c[11] * (q[-1] - q[1]);
+            const double cc = ddmi2 + d.bmu[mu - 1] * ddmi1 + ddni2 + dn * ddni1;
+            const double res = b - (cc + l * w[4]);
+            sr += res * res;
+            sb += b * b;
+            if (d.res_f) {
+                if (d.res_split) {
+                    float2 pr;
+                    pr.x = (float)(d.omega / l);
+                    pr.y = (float)res;
+                    reinterpret_cast<float2 *>(d.res_f)[split_index(nu, mu, M, cA)] = pr;
+                } else {
+                    d.res_f[e] = (float)res;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = w[k + 1];
+    }
+    if (d.acc) {
+        for (int off = 32; off > 0; off >>= 1) {
+            sr += __shfl_down(sr, off, 64);
+            sb += __shfl_down(sb, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0 && (sr != 0.0 || sb != 0.0)) {
+            atomicAdd(&d.acc[0], sr);
+            atomicAdd(&d.acc[1], sb);
+        }
+    }
+}
+
+// Launch the residual of n problems (descriptors d_res) on grids of at most
+// maxnni x maxnmu points: the row kernel when a row fits one workgroup, else
+// the per-point kernel.
+static int launch_residual(x2g_ctx *ctx, const ResDesc *d_res, int n, long maxnni, long maxnmu, long maxn)
+{
+    if (maxnni <= kResThreads) {
+        const int strips = (int)((maxnmu - 5 + kResRows - 1) / kResRows);
+        const int nth = (int)((maxnni + 63) / 64 * 64);
+        X2G_LAUNCH(ctx, k_residual_rows, dim3(strips * n), dim3(nth), 0, d_res, strips);
+    } else {
+        const int bpp = (int)((maxn + 255) / 256);
+        X2G_LAUNCH(ctx, k_residual, dim3(bpp * n), dim3(256), 0, d_res, bpp);
+    }
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
 struct CorrDesc {
     double *fun;        // unpadded FP64 u, corrected in place
     const float *e;     // padded FP32 correction
@@ -1948,9 +2063,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         }
         if (!i32.empty()) {
             const int n32 = (int)i32.size();
-            const int bpp = (int)((maxn + 255) / 256);
-            X2G_LAUNCH(ctx, k_residual, dim3(bpp * n32), dim3(256), 0, d_res, bpp);
-            X2G_HIP(hipGetLastError());
+            if ((rc = launch_residual(ctx, d_res, n32, maxnni, maxnmu, maxn))) return rc;
             X2G_LAUNCH(ctx, k_zero_f32, dim3(std::min(grid_x_for(2 * maxpad, 256), 128), nzero), dim3(256), 0, d_zero);
             X2G_HIP(hipGetLastError());
             for (int it2 = 0; it2 < maxsor2; ++it2) {
@@ -1982,7 +2095,7 @@ extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2
     if (rc) return rc;
     double *acc = (double *)ctx->ws[1].ptr;
     std::vector<ResDesc> res;
-    long maxn = 0;
+    long maxn = 0, maxnni = 0, maxnmu = 0;
     for (int i = 0; i < n; ++i) {
         const x2g_relax_item &it = items[i];
         X2G_CHECK(it.grid_index >= 0 && grids[it.grid_index], "x2g_residual_norms: bad grid index");
@@ -2002,14 +2115,14 @@ extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2
         r.nmu = g->nmu;
         res.push_back(r);
         maxn = std::max<long>(maxn, (long)g->n());
+        maxnni = std::max<long>(maxnni, g->nni);
+        maxnmu = std::max<long>(maxnmu, g->nmu);
     }
     size_t doff = 0;
     const ResDesc *d_res;
     if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
     if ((rc = x2g_fill(ctx, acc, 2 * (size_t)n, 0.0))) return rc;
-    const int bpp = (int)((maxn + 255) / 256);
-    X2G_LAUNCH(ctx, k_residual, dim3(bpp * n), dim3(256), 0, d_res, bpp);
-    X2G_HIP(hipGetLastError());
+    if ((rc = launch_residual(ctx, d_res, n, maxnni, maxnmu, maxn))) return rc;
     X2G_HIP(hipMemcpyAsync(out_h, acc, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     X2G_HIP(hipStreamSynchronize(ctx->stream));
     return X2G_OK;
