@@ -1,0 +1,82 @@
+"""Continuous batching (x2dhf_amd.serve): requests that arrive while others are
+on the GPU join the running batch as they arrive, their iterations overlap
+with the others', and each result is bit for bit the one the request gets
+when it is served alone.  A clock advanced by one unit per server step makes
+the arrivals deterministic."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+class StepClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+
+def _specs():
+    from x2dhf_amd import scf
+    return [scf.n2_spec(91, 121, 40.0, schedule=1, prec=1, ienterm=11, inoterm=11, maxscf=2000),
+            scf.lih_spec(61, 91, 40.0, schedule=1, prec=1, ienterm=11, inoterm=11, maxscf=2000),
+            scf.h2_spec(61, 91, 40.0, schedule=1, prec=0, ienterm=11, inoterm=11, maxscf=2000),
+            scf.bh_spec(91, 121, 40.0, schedule=1, prec=1, ienterm=11, inoterm=11, maxscf=2000)]
+
+
+def _serve(specs, arrivals):
+    from x2dhf_amd import serve
+    clk = StepClock()
+    srv = serve.Server(max_active=8, clock=clk)
+    try:
+        reqs = [srv.submit(s, arrival=a) for s, a in zip(specs, arrivals)]
+        steps = 0
+        while srv.pending or srv.active:
+            srv.step()
+            clk.t += 1.0
+            steps += 1
+            assert steps < 20000
+        return reqs
+    finally:
+        srv.close()
+
+
+def test_arrivals_join_the_running_batch_and_results_do_not_depend_on_the_mix():
+    specs = _specs()
+    print("served alone", flush=True)
+    alone = [_serve([s], [0.0])[0].result for s in specs]
+    print("served with staggered arrivals", flush=True)
+    reqs = _serve(specs, [0.0, 15.0, 40.0, 41.0])
+    # the later requests were admitted while the first was still iterating
+    assert reqs[1].admitted < reqs[0].finished and reqs[2].admitted < reqs[0].finished
+    assert all(r.admitted >= r.arrival for r in reqs)
+    for a, r in zip(alone, reqs):
+        m = r.result
+        print(f"alone E={a['etot']:.15f} it={a['iterations']}   served E={m['etot']:.15f} it={m['iterations']} "
+              f"(arrival {r.arrival:.0f}, admitted {r.admitted:.0f}, finished {r.finished:.0f})")
+        assert m["iterations"] == a["iterations"]
+        assert m["etot"] == a["etot"]
+        assert np.array_equal(m["eps"], a["eps"])
+
+
+def test_admission_respects_the_memory_budget():
+    """A budget that holds the N2 job alone (but the two small jobs together):
+    the small requests wait for the N2 job, in arrival order, and the
+    admitted footprint never exceeds the budget."""
+    from x2dhf_amd import scf, serve
+    specs = _specs()[:3]                       # N2 91x121, LiH 61x91, H2 61x91
+    fp = [int(serve.WORKSPACE_FACTOR * scf.spec_bytes(s)) for s in specs]
+    assert fp[0] + fp[1] > fp[0] and fp[1] + fp[2] <= fp[0]
+    clk = StepClock()
+    srv = serve.Server(max_active=8, hbm_budget=fp[0], clock=clk)
+    try:
+        reqs = [srv.submit(s, arrival=0.0) for s in specs]
+        while srv.pending or srv.active:
+            srv.step()
+            assert srv.held <= srv.budget
+            clk.t += 1.0
+    finally:
+        srv.close()
+    assert reqs[1].admitted >= reqs[0].finished and reqs[2].admitted >= reqs[0].finished
+    assert reqs[1].admitted <= reqs[2].admitted
