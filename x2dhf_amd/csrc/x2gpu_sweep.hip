@@ -57,7 +57,7 @@ struct SweepDesc {
     unsigned lr_bytes;    // bytes of lr
     int P;                // row stride of src / dst (nni+8 except for k_sweep)
     int nni, nmu, isym, cA;
-    int wg_begin, L;
+    int wg_begin, L;      // k_strip: first workgroup and strip height
 };
 
 // ---------------------------------------------------------------------------
@@ -487,25 +487,40 @@ struct Chunk {
     T v[16 / sizeof(T)];
 };
 
+// A k_sweep launch runs a list of row segments per workgroup: wg_seg[w] ..
+// wg_seg[w+1]-1 index segs, each strip [a, a+L) of problem prob.
+struct SweepSeg {
+    int prob, a, L;
+};
+template <typename T, int NT, int W>
+__device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a, const int L, const int MS,
+                                              unsigned char *ks_smem);
+
 template <typename T, int NT, int W>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && W <= 352 ? 4 : 1)))
-k_sweep(const SweepDesc<T> *__restrict__ descs, int nprob, int MS)
+k_sweep(const SweepDesc<T> *__restrict__ descs, const SweepSeg *__restrict__ segs, const int *__restrict__ wg_seg,
+        int MS)
 {
     extern __shared__ __align__(16) unsigned char ks_smem[];
+    const int s0 = wg_seg[blockIdx.x], s1 = wg_seg[blockIdx.x + 1];
+    for (int si = s0; si < s1; ++si) {
+        const SweepSeg sg = segs[si];
+        if (si > s0) __syncthreads();  // the previous segment is done with the LDS
+        sweep_segment<T, NT, W>(descs[sg.prob], sg.a, sg.L, MS, ks_smem);
+    }
+}
+
+// One strip of rows [a, a+L) of problem d (L a multiple of 4, or the rest of
+// the problem): the pipelined sweep described above.
+template <typename T, int NT, int W>
+__device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a, const int L, const int MS,
+                                              unsigned char *ks_smem)
+{
     constexpr int SKEW = 2, RB = 4 * SKEW + 4;
     constexpr int VEC = sweep_vec<T>(), NSV = sweep_nsv<T>();
     T *const dnu_s = reinterpret_cast<T *>(ks_smem);
 
-    const int wg = blockIdx.x;
-    int lo = 0, hi = nprob - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (descs[mid].wg_begin <= wg) lo = mid; else hi = mid - 1;
-    }
-    const SweepDesc<T> &d = descs[lo];
-    const int nni = d.nni, nmu = d.nmu, P = d.P, L = d.L;
-    const int chunk = wg - d.wg_begin;
-    const int a = 2 + chunk * L;
+    const int nni = d.nni, nmu = d.nmu, P = d.P;
     const int aend = min(a + L, nmu - 3);
     const int r0 = a - 16;
     const int K = L / 4 + 8;
@@ -1297,6 +1312,11 @@ static int stage_descs(x2g_ctx *ctx, const std::vector<D> &v, size_t & /*unused*
     return stage_n(ctx, v.data(), v.size(), out);
 }
 
+template <typename T>
+struct SweepPlan;
+template <typename T>
+static int stage_plan(x2g_ctx *ctx, SweepPlan<T> &plan, const SweepDesc<T> **de, const SweepDesc<T> **dodd);
+
 int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w2)
 {
     std::vector<PadDesc> v{{fun, w1, w2, g->nni, g->nmu, isym, g->nni + 8}};
@@ -1343,6 +1363,9 @@ int launch_putout(x2g_grid *g, double *fun, const double *work)
 // LDS ring fits.  strip_rows > 0 fixes the strip height.  Initial values come
 // from X2G_SWEEP=phase|strip and X2G_STRIP_L.
 static int g_engine = -1, g_strip_rows = -1;
+// k_sweep workgroups per CU for the balanced schedule (0: as many as the LDS
+// holds); from X2G_SWEEP_WPC.
+static int g_sweep_wpc = 0;
 static void engine_defaults()
 {
     if (g_engine >= 0) return;
@@ -1352,6 +1375,8 @@ static void engine_defaults()
     if (e && std::strcmp(e, "strip") == 0) g_engine = 2;
     const char *l = getenv("X2G_STRIP_L");
     g_strip_rows = l ? atoi(l) : -1;
+    const char *w = getenv("X2G_SWEEP_WPC");
+    g_sweep_wpc = w ? atoi(w) : 0;
 }
 static int env_engine()
 {
@@ -1389,29 +1414,79 @@ static int choose_L(x2g_ctx *ctx, long total_rows)
     return (int)((L + 3) & ~3L);
 }
 
-// Strip height for k_sweep from a cost model: a workgroup of strip height L
-// runs L/4 + 13 pipeline steps (the 16-row halos and the pipeline fill); the
-// W = sum over problems of ceil(rows / L) workgroups spread over the CUs, and
-// workgroups sharing a CU share its throughput, so the launch takes
-// ceil(W / CUs) workgroup times.
-static int choose_sweep_L(x2g_ctx *ctx, const std::vector<int> &rows)
+// Row segments of a k_sweep launch.  A segment of L rows costs L/4 + 13
+// pipeline steps (its halos and the pipeline fill), whatever problem it
+// belongs to, so the launch time is set by the busiest workgroup's sum of
+// segment costs.  With a forced strip height every workgroup runs one strip.
+// Otherwise the workgroups take consecutive rows of the batch, each up to a
+// common budget of B steps, cut into a multiple of 4 rows where a problem
+// straddles two workgroups; B is the smallest budget for which the rows fit
+// in nwg workgroups (those the chip holds at once: wpc per CU, at most one
+// per 8 rows).  Results do not depend on the cut: a strip's rows come out
+// the same whatever lies beyond its halos.
+static void segments_for_budget(const std::vector<int> &rows, long B, std::vector<SweepSeg> &segs,
+                                std::vector<int> &wgseg)
 {
-    const int forced = env_strip_l();
-    if (forced > 0) return (forced + 3) & ~3;
-    int maxrows = 0;
-    for (int r : rows) maxrows = std::max(maxrows, r);
-    double best = 1e300;
-    int bestL = 32;
-    for (int L = 24; L <= std::max(24, ((maxrows + 3) & ~3)); L += 4) {
-        long nwg = 0;
-        for (int r : rows) nwg += (r + L - 1) / L;
-        const double cost = (double)((nwg + ctx->num_cu - 1) / ctx->num_cu) * (L / 4.0 + 13.0);
-        if (cost < best - 1e-9) {
-            best = cost;
-            bestL = L;
+    constexpr int kMinRows = 8, kFill = 13;
+    segs.clear();
+    wgseg.assign(1, 0);
+    long left_budget = B;
+    auto close_wg = [&]() {
+        wgseg.push_back((int)segs.size());
+        left_budget = B;
+    };
+    for (int p = 0; p < (int)rows.size(); ++p) {
+        int a = 2, left = rows[p];
+        while (left > 0) {
+            const long fit = 4 * (left_budget - kFill);
+            if (left <= fit) {
+                const int L = (left + 3) & ~3;
+                segs.push_back({p, a, L});
+                left_budget -= L / 4 + kFill;
+                left = 0;
+                if (left_budget - kFill < kMinRows / 4) close_wg();
+            } else if (fit >= kMinRows) {
+                segs.push_back({p, a, (int)fit});
+                a += (int)fit;
+                left -= (int)fit;
+                close_wg();
+            } else {
+                close_wg();
+            }
         }
     }
-    return bestL;
+    if (wgseg.back() != (int)segs.size()) wgseg.push_back((int)segs.size());
+}
+
+static void plan_segments(x2g_ctx *ctx, const std::vector<int> &rows, int wpc, std::vector<SweepSeg> &segs,
+                          std::vector<int> &wgseg)
+{
+    segs.clear();
+    wgseg.clear();
+    const int forced = env_strip_l();
+    if (forced > 0) {
+        const int L = (forced + 3) & ~3;
+        for (int p = 0; p < (int)rows.size(); ++p)
+            for (int a = 2; a < 2 + rows[p]; a += L) {
+                wgseg.push_back((int)segs.size());
+                segs.push_back({p, a, L});
+            }
+        wgseg.push_back((int)segs.size());
+        return;
+    }
+    long R = 0;
+    for (int r : rows) R += r;
+    // small batches: down to 8-row segments (a shorter pipeline per
+    // workgroup pays more than the halo re-reads cost while CUs are idle)
+    const long nwg = std::max<long>(1, std::min<long>((long)wpc * ctx->num_cu, (R + 7) / 8));
+    // a budget of all rows plus a fill per problem fits in one workgroup
+    long lo = 15, hi = R / 4 + 14 * (long)rows.size() + 15;
+    while (lo < hi) {
+        const long mid = (lo + hi) / 2;
+        segments_for_budget(rows, mid, segs, wgseg);
+        if ((long)wgseg.size() - 1 <= nwg) hi = mid; else lo = mid + 1;
+    }
+    segments_for_budget(rows, lo, segs, wgseg);
 }
 
 // kind: 0 per-pass kernels (in place), 1 k_strip, 2 k_sweep (residue-split lhs/rhs)
@@ -1423,6 +1498,12 @@ struct SweepPlan {
     int total_wg = 0;
     int nth = 0, MS = 0, NT = 0, W = 0;
     size_t lds = 0;
+    // k_sweep: the row segments and each workgroup's first segment
+    // (wgseg[w] .. wgseg[w+1]-1), with their staged device copies
+    std::vector<SweepSeg> segs;
+    std::vector<int> wgseg;
+    const SweepSeg *d_segs = nullptr;
+    const int *d_wgseg = nullptr;
 };
 
 // Does k_sweep apply to a batch whose widest grid has nni points?
@@ -1483,8 +1564,9 @@ template <typename T, int NT, int W>
 static void launch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, int n, const SweepPlan<T> &plan)
 {
     // the caller has flushed the staged descriptors
-    hipLaunchKernelGGL((k_sweep<T, NT, W>), dim3(plan.total_wg), dim3(plan.nth), plan.lds, ctx->stream, dd, n,
-                       plan.MS);
+    (void)n;
+    hipLaunchKernelGGL((k_sweep<T, NT, W>), dim3(plan.total_wg), dim3(plan.nth), plan.lds, ctx->stream, dd,
+                       plan.d_segs, plan.d_wgseg, plan.MS);
 }
 
 template <typename T>
@@ -1622,7 +1704,7 @@ static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vecto
         sweep_lanes(maxnni, plan.MS, plan.NT, plan.nth);
         plan.W = sweep_stride<T>(maxnni);
         plan.skew = 2;
-        L = choose_sweep_L(ctx, prow);
+        L = 0;  // the row segments below
         plan.lds = sweep_lds_elems(plan.W, plan.nth) * sizeof(T);
     } else if (plan.kind == 1) {
         plan.skew = choose_skew<T>(ctx, maxnni);
@@ -1637,11 +1719,17 @@ static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vecto
     for (auto &d : base) {
         d.L = L;
         d.wg_begin = wg;
-        wg += (d.nmu - 5 + L - 1) / L;
+        if (L > 0) wg += (d.nmu - 5 + L - 1) / L;
         d.P = pad_row_stride<T>(d.nni, plan.kind);
         d.pad_bytes = (unsigned)((size_t)d.P * (d.nmu + 8) * sizeof(T));
     }
     plan.total_wg = wg;
+    if (plan.kind == 2) {
+        int wpc = g_sweep_wpc;
+        if (wpc <= 0) wpc = std::max<int>(1, (int)(ctx->max_lds / std::max<size_t>(plan.lds, 1)));
+        plan_segments(ctx, prow, wpc, plan.segs, plan.wgseg);
+        plan.total_wg = (int)plan.wgseg.size() - 1;
+    }
     plan.even = base;
     plan.odd = base;
     for (size_t i = 0; i < base.size(); ++i) {
@@ -1654,6 +1742,22 @@ static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vecto
             plan.even[i].src = plan.even[i].dst = a_bufs[i];
             plan.odd[i].src = plan.odd[i].dst = a_bufs[i];
         }
+    }
+    return X2G_OK;
+}
+
+// Stage a plan's even/odd descriptors and, for k_sweep, its segment lists.
+template <typename T>
+static int stage_plan(x2g_ctx *ctx, SweepPlan<T> &plan, const SweepDesc<T> **de, const SweepDesc<T> **dodd)
+{
+    size_t off = 0;
+    int rc;
+    if ((rc = stage_descs(ctx, plan.even, off, de))) return rc;
+    if ((rc = stage_descs(ctx, plan.odd, off, dodd))) return rc;
+    if (plan.kind == 2) {
+        X2G_CHECK(!plan.segs.empty() && plan.total_wg > 0, "sweep plan without segments");
+        if ((rc = stage_descs(ctx, plan.segs, off, &plan.d_segs))) return rc;
+        if ((rc = stage_descs(ctx, plan.wgseg, off, &plan.d_wgseg))) return rc;
     }
     return X2G_OK;
 }
@@ -1772,10 +1876,8 @@ extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const
             (*v)[0].lr_bytes = (unsigned)lrbytes;
         }
     }
-    size_t off = 0;
     const SweepDesc<double> *de, *dodd;
-    if ((rc = stage_descs(ctx, plan.even, off, &de))) return rc;
-    if ((rc = stage_descs(ctx, plan.odd, off, &dodd))) return rc;
+    if ((rc = stage_plan(ctx, plan, &de, &dodd))) return rc;
     const bool in_place = plan.kind == 0;
     for (int it = 0; it < nsweeps; ++it) {
         const bool ev = (it % 2) == 0;
@@ -1935,8 +2037,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             }
             if ((rc = launch_pack(ctx, pk))) return rc;
         }
-        if ((rc = stage_descs(ctx, p64.even, doff, &d64e))) return rc;
-        if ((rc = stage_descs(ctx, p64.odd, doff, &d64o))) return rc;
+        if ((rc = stage_plan(ctx, p64, &d64e, &d64o))) return rc;
         const bool final_in_b = p64.kind != 0 && (maxsor2 % 2) == 1;
         for (size_t k = 0; k < i64.size(); ++k) {
             const x2g_relax_item &it = items[i64[k]];
@@ -2019,8 +2120,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
                     (*v)[k].lr = f32p(i32[k], 2);
                     (*v)[k].lr_bytes = (unsigned)lrbytes(i32[k]);
                 }
-        if ((rc = stage_descs(ctx, p32.even, doff, &d32e))) return rc;
-        if ((rc = stage_descs(ctx, p32.odd, doff, &d32o))) return rc;
+        if ((rc = stage_plan(ctx, p32, &d32e, &d32o))) return rc;
         if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
         if ((rc = stage_descs(ctx, corr, doff, &d_corr))) return rc;
         if ((rc = stage_descs(ctx, cvt, doff, &d_cvt))) return rc;
