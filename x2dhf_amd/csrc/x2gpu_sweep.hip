@@ -22,6 +22,10 @@
 //    sweep from `src` and writes the final rows (plus the nu=1/nu=nni boundary
 //    values, and the mu=1 row for the first strip) to `dst`.  Every problem
 //    of a batch, on any grid, is handled by the same launch.
+//  * k_sweep: the same strip pipeline with the lhs/rhs pairs pre-split by
+//    colour, compile-time ring strides and one 16-byte chunk per block copy;
+//    each workgroup walks a balanced list of strips (row segments) across
+//    the batch's problems.
 #include "x2gpu_internal.h"
 #include <algorithm>
 #include <cstring>
