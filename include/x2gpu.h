@@ -71,8 +71,10 @@ int x2g_ctx_sync(x2g_ctx *ctx);
 void *x2g_ctx_stream(x2g_ctx *ctx);
 /* Free and total device memory of the context's GPU (bytes). */
 int x2g_mem_info(x2g_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
-/* Time every sweep launch of this context with HIP events on its stream
- * (on != 0 starts and clears the counters, 0 stops). */
+/* Time the sweep launches of this context with HIP events on its stream: one
+ * event pair around each run of back-to-back sweeps (a relaxation's maxsor2
+ * sweeps), counted as that many launches, so the average launch time includes
+ * the launch-to-launch gaps (on != 0 starts and clears the counters, 0 stops). */
 int x2g_stats_enable(x2g_ctx *ctx, int on);
 /* out[6] = {launches, milliseconds, algorithmic bytes} of the FP64 sweeps,
  * then of the FP32 sweeps.  Algorithmic bytes per relaxed point and sweep:

@@ -118,11 +118,11 @@ int stats_begin(x2g_ctx *ctx, hipEvent_t *a, hipEvent_t *b)
     return X2G_OK;
 }
 
-int stats_end(x2g_ctx *ctx, hipEvent_t a, hipEvent_t b, double bytes, int prec)
+int stats_end(x2g_ctx *ctx, hipEvent_t a, hipEvent_t b, double bytes, int prec, int launches)
 {
     if (!ctx->stats_on || !a) return X2G_OK;
     X2G_HIP(hipEventRecord(b, ctx->stream));
-    ctx->pending.push_back({a, b, bytes, prec});
+    ctx->pending.push_back({a, b, bytes, prec, launches});
     if (ctx->pending.size() >= 4096) return stats_flush(ctx);
     return X2G_OK;
 }
@@ -134,7 +134,7 @@ int stats_flush(x2g_ctx *ctx)
     for (auto &p : ctx->pending) {
         float ms = 0.0f;
         X2G_HIP(hipEventElapsedTime(&ms, p.a, p.b));
-        ctx->st_launch[p.prec] += 1;
+        ctx->st_launch[p.prec] += p.launches;
         ctx->st_ms[p.prec] += ms;
         ctx->st_bytes[p.prec] += p.bytes;
         ctx->ev_pool.push_back(p.a);

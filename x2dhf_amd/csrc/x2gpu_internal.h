@@ -55,11 +55,12 @@ struct x2g_ctx {
     size_t pinned_out_bytes = 0;
     size_t stage_off = 0;          // ring offset into ws[2] / pinned
     size_t stage_lo = 0, stage_hi = 0;  // staged bytes not yet copied to ws[2]: [lo, hi)
-    // sweep-kernel timing (x2g_stats_enable): event pairs around every sweep
-    // launch, resolved at x2g_stats_get or when the pending list is long
+    // sweep-kernel timing (x2g_stats_enable): an event pair around each run of
+    // back-to-back sweep launches, resolved at x2g_stats_get or when the
+    // pending list is long
     bool stats_on = false;
     std::vector<hipEvent_t> ev_pool;
-    struct Pending { hipEvent_t a, b; double bytes; int prec; };
+    struct Pending { hipEvent_t a, b; double bytes; int prec, launches; };
     std::vector<Pending> pending;
     double st_launch[2] = {0, 0}, st_ms[2] = {0, 0}, st_bytes[2] = {0, 0};
     // device blocks of x2g_malloc: sizes of the live ones, and freed ones
@@ -126,6 +127,6 @@ int launch_putin_items(x2g_ctx *ctx, const std::vector<PadItem> &items);
 int pinned_out_reserve(x2g_ctx *ctx, size_t bytes);
 // Sweep timing hooks (no-ops unless statistics are enabled on the context).
 int stats_begin(x2g_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
-int stats_end(x2g_ctx *ctx, hipEvent_t a, hipEvent_t b, double bytes, int prec);
+int stats_end(x2g_ctx *ctx, hipEvent_t a, hipEvent_t b, double bytes, int prec, int launches = 1);
 int stats_flush(x2g_ctx *ctx);
 }  // namespace x2g

@@ -1109,8 +1109,18 @@ __global__ void __launch_bounds__(256) k_residual(const ResDesc *__restrict__ rd
 // per row).  The mirror images of the mu < 1 rows and of the nu < 1, nu > nni
 // columns are placed as they are read, so every point sees the values of the
 // per-point kernel and combines them in the same order.  Needs nni <= blockDim.
-constexpr int kResRows = 64;
+//
+// The loads of the next kResAhead rows are in flight while a group of rows
+// is computed.  Every lane issues every load (rows clamped to the strip,
+// columns to the grid; buffer loads, which only the vector-memory counter
+// tracks) and every store (through a buffer resource, at an out-of-range
+// offset for lanes without a relaxed point), so each wave issues the same
+// memory operations in every row and the compiler's waits count exactly.
+// One row ahead and 32-row strips measured fastest: deeper prefetch costs
+// more in occupancy (registers) than it gains in latency hiding.
+constexpr int kResRows = 32;
 constexpr int kResThreads = 512;
+constexpr int kResAhead = 1;
 
 __global__ void __launch_bounds__(kResThreads) k_residual_rows(const ResDesc *__restrict__ rd, int strips)
 {
@@ -1123,70 +1133,100 @@ __global__ void __launch_bounds__(kResThreads) k_residual_rows(const ResDesc *__
     const int nu = threadIdx.x + 1;
     const bool col = nu <= nni;
     const bool interior = nu >= 2 && nu <= nni - 1;
+    const int cl = min(nu, nni) - 1;               // column read by this lane
     const double s = d.isym == 1 ? 1.0 : -1.0;
-    const double *f = d.fun, *c = d.c;
-    auto load_u = [&](int mu) -> double {         // u(nu, mu) for mu >= -3, mirrored below mu = 1
-        if (!col) return 0.0;
-        return mu < 1 ? s * f[(long)(1 - mu) * nni + nu - 1] : f[(long)(mu - 1) * nni + nu - 1];
+    const double *c = d.c;
+    // u, lhs and rhs through buffer resources: vector-memory loads, waited
+    // for by their own counter only
+    const unsigned nbytes = (unsigned)((long)nni * nmu * 8);
+    const __amdgpu_buffer_rsrc_t rs_u = buf_rsrc(d.fun, nbytes), rs_l = buf_rsrc(d.lhs, nbytes),
+                                 rs_b = buf_rsrc(d.rhs, nbytes);
+    auto load_u = [&](int mu) -> double {         // u(nu, mu) for -3 <= mu <= nmu, mirrored below mu = 1
+        const double v = buf_load1(rs_u, (unsigned)(((mu < 1 ? 1 - mu : mu - 1) * nni + cl) * 8), 0.0);
+        return mu < 1 ? s * v : v;
     };
-    double w[9];                                   // u(nu, mu-4 .. mu+4)
+    const int M = split_m(nni), cA = colour_mult(nni);
+    const bool out = d.res_f != nullptr;
+    const __amdgpu_buffer_rsrc_t rs_out =
+        buf_rsrc(d.res_f, out ? (unsigned)(d.res_split ? split_size(nni, nmu) * 8 : (long)nni * nmu * 4) : 0u);
+    double w[8];                                   // u(nu, mu-4 .. mu+3)
 #pragma unroll
     for (int k = 0; k < 8; ++k) w[k] = load_u(m0 - 4 + k);
-    const double dn = col ? d.dnu[nu - 1] : 0.0;
-    const int M = split_m(nni), cA = colour_mult(nni);
+    const __amdgpu_buffer_rsrc_t rs_bm = buf_rsrc(d.bmu, (unsigned)(nmu * 8));
+    const double dn = buf_load1(buf_rsrc(d.dnu, (unsigned)(nni * 8)), (unsigned)(cl * 8), 0.0);
     double sr = 0.0, sb = 0.0;
-    // row mu+4 of u and row mu of lhs / rhs arrive one row ahead
-    auto load_lb = [&](int mu, double &l, double &b) {
-        const long e = (long)(mu - 1) * nni + nu - 1;
-        const bool ok = interior && mu <= m1;
-        l = ok ? d.lhs[e] : 0.0;
-        b = ok ? d.rhs[e] : 0.0;
+    // the rows are taken in groups of kResAhead: while a group is computed,
+    // the next group's u(nu, mu+4), lhs(nu, mu), rhs(nu, mu) and b(mu) are in
+    // flight (rows past m1 read m1); the group's values are then moved into
+    // place once all have arrived, so no load in flight is ever moved
+    double up[kResAhead], lp[kResAhead], bp[kResAhead], mp[kResAhead];
+    auto issue = [&](double (&u_)[kResAhead], double (&l_)[kResAhead], double (&b_)[kResAhead],
+                     double (&m_)[kResAhead], int mb) {
+#pragma unroll
+        for (int k = 0; k < kResAhead; ++k) {
+            const int mr = min(mb + k, m1);
+            const unsigned e8 = (unsigned)(((mr - 1) * nni + cl) * 8);
+            u_[k] = load_u(mr + 4);
+            l_[k] = buf_load1(rs_l, e8, 0.0);
+            b_[k] = buf_load1(rs_b, e8, 0.0);
+            m_[k] = buf_load1(rs_bm, (unsigned)((mr - 1) * 8), 0.0);
+        }
     };
-    double un = load_u(m0 + 4), ln, bn;
-    load_lb(m0, ln, bn);
-    for (int mu = m0; mu <= m1; ++mu) {
-        w[8] = un;
-        const double l = ln, b = bn;
-        if (mu < m1) {
-            un = load_u(mu + 5);
-            load_lb(mu + 1, ln, bn);
-        }
-        const long e = (long)(mu - 1) * nni + nu - 1;
-        // stage row mu with its mirrored columns: slot nu+3 for nu = -3 .. nni+4
-        double *r = row[mu & 1];
-        if (col) {
-            r[nu + 3] = w[4];
-            if (nu >= 2 && nu <= 5) r[5 - nu] = s * w[4];                        // nu' = 2 - nu
-            if (nu >= nni - 4 && nu <= nni - 1) r[2 * nni - nu + 3] = s * w[4];  // nu' = 2 nni - nu
-        }
-        __syncthreads();
-        if (interior) {
+    issue(up, lp, bp, mp, m0);
+    // rows past m1 of the last group run with clamped loads and neither
+    // store nor count (no early exit: every wave issues the same operations)
+    for (int mb = m0; mb <= m1; mb += kResAhead) {
+        double un[kResAhead], ln[kResAhead], bn[kResAhead], mn[kResAhead];
+        issue(un, ln, bn, mn, mb + kResAhead);
+#pragma unroll
+        for (int k = 0; k < kResAhead; ++k) {
+            const int mu = mb + k;
+            const bool live = interior && mu <= m1;
+            // stage row mu with its mirrored columns: slot nu+3 for nu = -3 .. nni+4
+            double *r = row[mu & 1];
+            if (col) {
+                r[nu + 3] = w[4];
+                if (nu >= 2 && nu <= 5) r[5 - nu] = s * w[4];                        // nu' = 2 - nu
+                if (nu >= nni - 4 && nu <= nni - 1) r[2 * nni - nu + 3] = s * w[4];  // nu' = 2 nni - nu
+            }
+            __syncthreads();
+            // every lane computes (slots nu-1 .. nu+7 lie inside the row
+            // buffer); only the relaxed points of rows <= m1 count and store
             const double *q = r + nu + 3;
-            const double ddmi2 = c[4] * (w[0] + w[8]) + c[5] * (w[1] + w[7]) + c[6] * (w[2] + w[6]) +
+            const double w8 = up[k], l = lp[k], b = bp[k];
+            const double ddmi2 = c[4] * (w[0] + w8) + c[5] * (w[1] + w[7]) + c[6] * (w[2] + w[6]) +
                                  c[7] * (w[3] + w[5]);
-            const double ddmi1 = c[0] * (w[0] - w[8]) + c[1] * (w[1] - w[7]) + c[2] * (w[2] - w[6]) +
+            const double ddmi1 = c[0] * (w[0] - w8) + c[1] * (w[1] - w[7]) + c[2] * (w[2] - w[6]) +
                                  c[3] * (w[3] - w[5]);
             const double ddni2 = c[12] * (q[-4] + q[4]) + c[13] * (q[-3] + q[3]) + c[14] * (q[-2] + q[2]) +
                                  c[15] * (q[-1] + q[1]);
             const double ddni1 = c[8] * (q[-4] - q[4]) + c[9] * (q[-3] - q[3]) + c[10] * (q[-2] - q[2]) +
                                  c[11] * (q[-1] - q[1]);
-            const double cc = ddmi2 + d.bmu[mu - 1] * ddmi1 + ddni2 + dn * ddni1;
+            const double cc = ddmi2 + mp[k] * ddmi1 + ddni2 + dn * ddni1;
             const double res = b - (cc + l * w[4]);
-            sr += res * res;
-            sb += b * b;
-            if (d.res_f) {
+            sr += live ? res * res : 0.0;
+            sb += live ? b * b : 0.0;
+            if (out) {                             // uniform over the launch
                 if (d.res_split) {
-                    float2 pr;
-                    pr.x = (float)(d.omega / l);
-                    pr.y = (float)res;
-                    reinterpret_cast<float2 *>(d.res_f)[split_index(nu, mu, M, cA)] = pr;
+                    const long o = live ? split_index(nu, mu, M, cA) * 8 : 0;
+                    const float2 pr = make_float2((float)(d.omega / l), (float)res);
+                    buf_store1(rs_out, oob_unless(live, (int)o), __builtin_bit_cast(double, pr));
                 } else {
-                    d.res_f[e] = (float)res;
+                    const long o = ((long)(mu - 1) * nni + nu - 1) * 4;
+                    buf_store1(rs_out, oob_unless(live, (int)o), (float)res);
                 }
             }
+#pragma unroll
+            for (int k2 = 0; k2 < 7; ++k2) w[k2] = w[k2 + 1];
+            w[7] = w8;
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) w[k] = w[k + 1];
+        for (int k = 0; k < kResAhead; ++k) {
+            up[k] = un[k];
+            lp[k] = ln[k];
+            bp[k] = bn[k];
+            mp[k] = mn[k];
+        }
     }
     if (d.acc) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -1590,30 +1630,20 @@ static bool dispatch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, int n, const Sweep
     return false;
 }
 
+// One sweep of every problem in the plan (device descriptors dd, host copy hv).
 template <typename T>
-static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<SweepDesc<T>> &hv,
-                        const SweepPlan<T> &plan)
+static int launch_sweep_once(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<SweepDesc<T>> &hv,
+                             const SweepPlan<T> &plan)
 {
     const int skew = plan.skew, total_wg = plan.total_wg;
     const size_t lds = plan.lds;
     const bool in_place = plan.kind == 0;
     const int n = (int)hv.size();
-    // algorithmic bytes of one sweep: read u, write u, read lhs and rhs per
-    // relaxed point (b and d are 1-D and neighbour reads are reuse)
-    double bytes = 0.0;
-    for (auto &d : hv) bytes += 4.0 * sizeof(T) * (double)(d.nni - 2) * (double)(d.nmu - 5);
-    const int prec = sizeof(T) == 8 ? 0 : 1;
-    // descriptors go out before the timed region
-    int rc = stage_flush(ctx);
-    if (rc) return rc;
-    hipEvent_t ea, eb;
-    rc = stats_begin(ctx, &ea, &eb);
-    if (rc) return rc;
     if (plan.kind == 2) {
         const bool ok = dispatch_ks<T>(ctx, dd, n, plan);
         X2G_CHECK(ok, "no sweep kernel for ring stride %d", plan.W);
         X2G_HIP(hipGetLastError());
-        return stats_end(ctx, ea, eb, bytes, prec);
+        return X2G_OK;
     }
     if (!in_place && skew > 0) {
         const int nth = 512;
@@ -1623,7 +1653,7 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
             X2G_LAUNCH(ctx, (k_strip<T, 1>), dim3(total_wg), dim3(nth), lds, dd, n);
         }
         X2G_HIP(hipGetLastError());
-        return stats_end(ctx, ea, eb, bytes, prec);
+        return X2G_OK;
     }
     long maxpts = 0;
     for (auto &d : hv) maxpts = std::max<long>(maxpts, (long)(d.nni - 2) * (d.nmu - 5));
@@ -1635,7 +1665,33 @@ static int launch_sweep(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<
     for (auto &d : hv) maxb = std::max<long>(maxb, (long)(d.nni - 2) + 2L * (d.nmu - 4));
     X2G_LAUNCH(ctx, (k_boundary<T>), dim3(grid_x_for(maxb, 256), n), dim3(256), 0, dd);
     X2G_HIP(hipGetLastError());
-    return stats_end(ctx, ea, eb, bytes, prec);
+    return X2G_OK;
+}
+
+// nsweeps back-to-back sweeps, alternating the even (src = A) and odd
+// (src = B) descriptor lists.  With statistics on, one pair of HIP events
+// brackets the whole run, so the per-launch average includes the short
+// launch-to-launch gaps and no event sits between two sweeps.
+template <typename T>
+static int launch_sweeps(x2g_ctx *ctx, const SweepDesc<T> *de, const SweepDesc<T> *dodd, const SweepPlan<T> &plan,
+                         int nsweeps)
+{
+    if (nsweeps <= 0) return X2G_OK;
+    // algorithmic bytes of one sweep: read u, write u, read lhs and rhs per
+    // relaxed point (b and d are 1-D and neighbour reads are reuse)
+    double bytes = 0.0;
+    for (auto &d : plan.even) bytes += 4.0 * sizeof(T) * (double)(d.nni - 2) * (double)(d.nmu - 5);
+    const int prec = sizeof(T) == 8 ? 0 : 1;
+    // descriptors go out before the timed region
+    int rc = stage_flush(ctx);
+    if (rc) return rc;
+    hipEvent_t ea, eb;
+    if ((rc = stats_begin(ctx, &ea, &eb))) return rc;
+    for (int it = 0; it < nsweeps; ++it) {
+        const bool ev = (it % 2) == 0;
+        if ((rc = launch_sweep_once<T>(ctx, ev ? de : dodd, ev ? plan.even : plan.odd, plan))) return rc;
+    }
+    return stats_end(ctx, ea, eb, bytes * nsweeps, prec, nsweeps);
 }
 
 static std::mutex g_attr_mutex;
@@ -1883,11 +1939,7 @@ extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const
     const SweepDesc<double> *de, *dodd;
     if ((rc = stage_plan(ctx, plan, &de, &dodd))) return rc;
     const bool in_place = plan.kind == 0;
-    for (int it = 0; it < nsweeps; ++it) {
-        const bool ev = (it % 2) == 0;
-        rc = launch_sweep<double>(ctx, ev ? de : dodd, ev ? plan.even : plan.odd, plan);
-        if (rc) return rc;
-    }
+    if ((rc = launch_sweeps<double>(ctx, de, dodd, plan, nsweeps))) return rc;
     double *fin = (!in_place && (nsweeps % 2) == 1) ? B : A;
     if (restride) {
         if ((rc = launch_restride(ctx, {{fin, work_d, nullptr, g->nni + 8, g->nmu + 8, P, g->nni + 8}}))) return rc;
@@ -2156,11 +2208,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             X2G_HIP(hipGetLastError());
         }
         if (!i64.empty()) {
-            for (int it2 = 0; it2 < maxsor2; ++it2) {
-                const bool ev = (it2 % 2) == 0;
-                if ((rc = launch_sweep<double>(ctx, ev ? d64e : d64o, ev ? p64.even : p64.odd, p64)))
-                    return rc;
-            }
+            if ((rc = launch_sweeps<double>(ctx, d64e, d64o, p64, maxsor2))) return rc;
             X2G_LAUNCH(ctx, k_putout, dim3(grid_x_for(maxn, 256), (int)i64.size()), dim3(256), 0,
                                d_pout64);
             X2G_HIP(hipGetLastError());
@@ -2170,11 +2218,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             if ((rc = launch_residual(ctx, d_res, n32, maxnni, maxnmu, maxn))) return rc;
             X2G_LAUNCH(ctx, k_zero_f32, dim3(std::min(grid_x_for(2 * maxpad, 256), 128), nzero), dim3(256), 0, d_zero);
             X2G_HIP(hipGetLastError());
-            for (int it2 = 0; it2 < maxsor2; ++it2) {
-                const bool ev = (it2 % 2) == 0;
-                if ((rc = launch_sweep<float>(ctx, ev ? d32e : d32o, ev ? p32.even : p32.odd, p32)))
-                    return rc;
-            }
+            if ((rc = launch_sweeps<float>(ctx, d32e, d32o, p32, maxsor2))) return rc;
             X2G_LAUNCH(ctx, k_correct, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, d_corr);
             X2G_HIP(hipGetLastError());
             X2G_LAUNCH(ctx, k_boundary_fun, dim3(grid_x_for(2 * maxn / std::max<long>(maxnni, 1) + maxnni, 256), n32),
