@@ -172,6 +172,15 @@ typedef struct {
     int bkind;
     int prec;
     int grid_index; /* index into the grids[] array of x2g_relax_batch */
+    /* Optional right-hand side as a product of up to three arrays (the
+     * potentials' psi_i psi_j g of relaxPots.F90:97-104): when rhs_f1_d is
+     * not NULL, rhs = rhs_f1 * rhs_f2 * rhs_f3 (a NULL rhs_f2 / rhs_f3 counts
+     * as 1), formed where the relaxation reads it (the FP64 residual of a
+     * prec = 1 item), and rhs_d is scratch that the call may overwrite with
+     * the product.  Zero-initialise all three to pass rhs_d alone. */
+    const double *rhs_f1_d;
+    const double *rhs_f2_d;
+    const double *rhs_f3_d;
 } x2g_relax_item;
 
 /* Relax n problems, possibly on different grids, together: every sweep of

@@ -5,6 +5,8 @@ Tolerances: FP64 kernels agree with the reference to 1e-12 relative to the
 array's largest magnitude (the device forms fused multiply-adds, the
 reference does not); FP32 sweeps agree with the FP32 oracle to 2e-5.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -430,4 +432,44 @@ def test_residual_norms_vs_oracle(ctx, nni, nmu, isym):
     rhs2 = P["rhs"].reshape(nmu, nni)[1:nmu - 4, 1:nni - 1]
     assert abs(out[0] - np.sum(res * res)) <= 1e-12 * np.sum(res * res)
     assert abs(out[1] - np.sum(rhs2 * rhs2)) <= 1e-12 * np.sum(rhs2 * rhs2)
+    g.destroy()
+
+
+@pytest.mark.parametrize("nni,nmu,nf", [(169, 241, 3), (331, 601, 2), (523, 151, 3)])
+def test_rhs_as_product_of_factors(ctx, nni, nmu, nf):
+    """An item may give its right-hand side as rhs_f1*rhs_f2*rhs_f3 (the
+    potentials' psi_i psi_j g): FP32-correction relaxation (the residual forms
+    the product; both residual kernels) and FP64 relaxation (the product is
+    formed into rhs_d first) agree with the same item given rhs = the product
+    computed by x2g_lincomb, bit for bit, and so do the residual norms."""
+    from x2dhf_amd import gpu
+    gpu.check(ctx.lib.x2g_set_sweep_engine(0, -1))
+    P = synthetic_problem(nni, nmu, seed=7 * nni + nf)
+    rng = np.random.default_rng(nni)
+    print(f"rhs factors {nni}x{nmu} nf={nf}", flush=True)
+    g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
+    gh = (gpu._vp * 1)(g.h.value)
+    fac = [ctx.upload(1.0 + 0.1 * rng.standard_normal(nni * nmu)) for _ in range(nf)]
+    prod = ctx.upload(np.zeros(nni * nmu))
+    t = (gpu.Term * 1)(gpu.Term(1.0, fac[0].ptr, fac[1].ptr, fac[2].ptr if nf == 3 else None))
+    gpu.check(ctx.lib.x2g_lincomb(g.h, prod.ptr, None, C.c_double(0.0), t, 1))
+    l = ctx.upload(P["lhs"])
+    for prec in (1, 0):
+        out = {}
+        for mode in ("rhs", "factors"):
+            f = ctx.upload(P["fun"])
+            scratch = ctx.upload(np.zeros(nni * nmu))
+            it = gpu.RelaxItem(f.ptr, l.ptr, prod.ptr if mode == "rhs" else scratch.ptr, None, 1.9, 1, 1, prec, 0)
+            if mode == "factors":
+                it.rhs_f1_d, it.rhs_f2_d = fac[0].ptr, fac[1].ptr
+                it.rhs_f3_d = fac[2].ptr if nf == 3 else None
+            its = (gpu.RelaxItem * 1)(it)
+            norms = np.zeros(2)
+            gpu.check(ctx.lib.x2g_residual_norms(ctx.h, gh, its, 1, norms.ctypes.data))
+            gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, its, 1, 2, 10))
+            out[mode] = (f.get(), norms)
+        print(f"prec {prec}: max |du| {np.max(np.abs(out['rhs'][0] - out['factors'][0])):.3e}")
+        # the norms are summed with atomics (order varies between calls)
+        assert np.allclose(out["rhs"][1], out["factors"][1], rtol=1e-13, atol=0.0)
+        assert np.array_equal(out["rhs"][0], out["factors"][0])
     g.destroy()

@@ -56,19 +56,63 @@ __global__ void k_fill(double *dst, size_t n, double v)
         dst[e] = v;
 }
 
-__global__ void k_lincomb(double *out, const double *base, double bscale, const x2g_term *__restrict__ t, int nt,
-                          long n)
+// Global-memory view of a device pointer: loads through it are global (not
+// flat) loads, which only the vector-memory counter tracks, so waiting for a
+// scalar load (a term's pointers) does not also wait for them.
+typedef const double __attribute__((address_space(1))) *gdptr;
+typedef double __attribute__((address_space(1))) *gdptr_w;
+__device__ __forceinline__ gdptr gmem(const double *p) { return (gdptr)p; }
+__device__ __forceinline__ gdptr_w gmem_w(double *p) { return (gdptr_w)p; }
+
+// out = bscale*base + sum_k c_k a_k b_k cc_k over n points, for the block's
+// share of a grid-stride loop with blocks of 256 threads.  Each thread takes
+// kLcIlp points 256 apart per pass, so a term's loads for all of them are in
+// flight together (lanes past n read point n-1 and store nothing).  Per point
+// the operations and their order are those of the plain loop.
+constexpr int kLcIlp = 4;
+__device__ __forceinline__ void lincomb_points(double *out, const double *base, double bscale,
+                                               const x2g_term *__restrict__ t, int nt, long n, long first, long stride)
 {
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        double v = base ? bscale * base[e] : 0.0;
-        for (int k = 0; k < nt; ++k) {
-            double x = t[k].c * t[k].a_d[e];
-            if (t[k].b_d) x *= t[k].b_d[e];
-            if (t[k].cc_d) x *= t[k].cc_d[e];
-            v += x;
+    for (long e0 = first; e0 < n; e0 += stride) {
+        long e[kLcIlp];
+        double v[kLcIlp];
+#pragma unroll
+        for (int j = 0; j < kLcIlp; ++j) e[j] = min(e0 + 256L * j, n - 1);
+        if (base) {
+#pragma unroll
+            for (int j = 0; j < kLcIlp; ++j) v[j] = bscale * gmem(base)[e[j]];
+        } else {
+#pragma unroll
+            for (int j = 0; j < kLcIlp; ++j) v[j] = 0.0;
         }
-        out[e] = v;
+        for (int k = 0; k < nt; ++k) {
+            const double c = t[k].c;
+            const double *a = t[k].a_d, *b = t[k].b_d, *cc = t[k].cc_d;
+            double x[kLcIlp];
+#pragma unroll
+            for (int j = 0; j < kLcIlp; ++j) x[j] = c * gmem(a)[e[j]];
+            if (b) {
+#pragma unroll
+                for (int j = 0; j < kLcIlp; ++j) x[j] *= gmem(b)[e[j]];
+            }
+            if (cc) {
+#pragma unroll
+                for (int j = 0; j < kLcIlp; ++j) x[j] *= gmem(cc)[e[j]];
+            }
+#pragma unroll
+            for (int j = 0; j < kLcIlp; ++j) v[j] += x[j];
+        }
+#pragma unroll
+        for (int j = 0; j < kLcIlp; ++j)
+            if (e0 + 256L * j < n) gmem_w(out)[e0 + 256L * j] = v[j];
     }
+}
+
+__global__ void __launch_bounds__(256) k_lincomb(double *out, const double *base, double bscale,
+                                                 const x2g_term *__restrict__ t, int nt, long n)
+{
+    lincomb_points(out, base, bscale, t, nt, n, (long)blockIdx.x * 256 * kLcIlp + threadIdx.x,
+                   (long)gridDim.x * 256 * kLcIlp);
 }
 
 // partial[k*kRedBlocks + blk] = sum over the block's slice of w*a*b*c
@@ -302,20 +346,11 @@ struct LcOp {
     long n;
 };
 
-__global__ void k_lincomb_batch(const LcOp *__restrict__ ops, const x2g_term *__restrict__ terms)
+__global__ void __launch_bounds__(256) k_lincomb_batch(const LcOp *__restrict__ ops, const x2g_term *__restrict__ terms)
 {
     const LcOp &o = ops[blockIdx.y];
-    const x2g_term *t = terms + o.first;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < o.n; e += (long)gridDim.x * blockDim.x) {
-        double v = o.base ? o.bscale * o.base[e] : 0.0;
-        for (int k = 0; k < o.nterms; ++k) {
-            double x = t[k].c * t[k].a_d[e];
-            if (t[k].b_d) x *= t[k].b_d[e];
-            if (t[k].cc_d) x *= t[k].cc_d[e];
-            v += x;
-        }
-        o.out[e] = v;
-    }
+    lincomb_points(o.out, o.base, o.bscale, terms + o.first, o.nterms, o.n,
+                   (long)blockIdx.x * 256 * kLcIlp + threadIdx.x, (long)gridDim.x * 256 * kLcIlp);
 }
 
 // Sum of the nb partials at p (fixed order), valid in thread 0.
@@ -520,7 +555,7 @@ extern "C" int x2g_lincomb(x2g_grid *g, double *out_d, const double *base_d, dou
         int rc = stage(ctx, terms, (size_t)nterms, off, &dt);
         if (rc) return rc;
     }
-    X2G_LAUNCH(ctx, k_lincomb, dim3(gx((long)g->n(), 256)), dim3(256), 0, out_d, base_d, base_scale,
+    X2G_LAUNCH(ctx, k_lincomb, dim3(gx((long)g->n(), 256 * kLcIlp)), dim3(256), 0, out_d, base_d, base_scale,
                        dt, nterms, (long)g->n());
     X2G_HIP(hipGetLastError());
     return X2G_OK;
@@ -687,7 +722,7 @@ extern "C" int x2g_lincomb_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g
     int rc = stage(ctx, v.data(), v.size(), off, &dops);
     if (rc) return rc;
     if (nterms_total && (rc = stage(ctx, terms, (size_t)nterms_total, off, &dt))) return rc;
-    X2G_LAUNCH(ctx, k_lincomb_batch, dim3(gx(maxn, 256), nops), dim3(256), 0, dops, dt);
+    X2G_LAUNCH(ctx, k_lincomb_batch, dim3(gx(maxn, 256 * kLcIlp), nops), dim3(256), 0, dops, dt);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }

@@ -384,6 +384,24 @@ __global__ void k_pack(const PackDesc<TO> *__restrict__ pd)
     }
 }
 
+// rhs = f[0] * f[1] * f[2] (null f[1] / f[2] count as 1) for the items that
+// give their right-hand side as a product and relax in FP64.
+struct ProdDesc {
+    double *out;
+    const double *f[3];
+    long n;
+};
+__global__ void k_rhs_product(const ProdDesc *__restrict__ pd)
+{
+    const ProdDesc &d = pd[blockIdx.y];
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
+        double v = d.f[0][e];
+        if (d.f[1]) v *= d.f[1][e];
+        if (d.f[2]) v *= d.f[2][e];
+        d.out[e] = v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_sweep: the pipelined strip sweep (same block schedule as k_strip with
 // SKEW = 2) with a fixed lane -> work map.  Wave p < 5 serves colour pass p:
@@ -1012,6 +1030,7 @@ struct ResDesc {
     const double *fun;     // unpadded u
     const double *lhs;
     const double *rhs;
+    const double *rf[3];   // rf[0] != null: rhs = rf[0] * rf[1] * rf[2] (null factors count as 1)
     const double *bmu;
     const double *dnu;
     float *res_f;          // FP32 residual (may be null)
@@ -1073,7 +1092,15 @@ __global__ void __launch_bounds__(256) k_residual(const ResDesc *__restrict__ rd
                                      c[10] * (F(-2, 0) - F(2, 0)) + c[11] * (F(-1, 0) - F(1, 0));
                 cc = ddmi2 + bm * ddmi1 + ddni2 + dn * ddni1;
             }
-            const double l = d.lhs[e], b = d.rhs[e];
+            const double l = d.lhs[e];
+            double b;
+            if (d.rf[0]) {
+                b = d.rf[0][e];
+                if (d.rf[1]) b *= d.rf[1][e];
+                if (d.rf[2]) b *= d.rf[2][e];
+            } else {
+                b = d.rhs[e];
+            }
             const double r = b - (cc + l * f[e]);
             sr = r * r;
             sb = b * b;
@@ -1122,6 +1149,9 @@ constexpr int kResRows = 32;
 constexpr int kResThreads = 512;
 constexpr int kResAhead = 1;
 
+// PROD: some problem of the launch gives its rhs as a product of factors
+// (those that do not read rhs alone, with the same operation count).
+template <bool PROD>
 __global__ void __launch_bounds__(kResThreads) k_residual_rows(const ResDesc *__restrict__ rd, int strips)
 {
     __shared__ double row[2][kResThreads + 8];
@@ -1139,8 +1169,13 @@ __global__ void __launch_bounds__(kResThreads) k_residual_rows(const ResDesc *__
     // u, lhs and rhs through buffer resources: vector-memory loads, waited
     // for by their own counter only
     const unsigned nbytes = (unsigned)((long)nni * nmu * 8);
+    // the right-hand side is rhs, or the product of up to three factors
+    // (loads of absent factors fall outside a zero-size buffer; unused)
+    const bool prod = PROD && d.rf[0] != nullptr, has1 = prod && d.rf[1], has2 = prod && d.rf[2];
     const __amdgpu_buffer_rsrc_t rs_u = buf_rsrc(d.fun, nbytes), rs_l = buf_rsrc(d.lhs, nbytes),
-                                 rs_b = buf_rsrc(d.rhs, nbytes);
+                                 rs_b = buf_rsrc(prod ? d.rf[0] : d.rhs, nbytes),
+                                 rs_b1 = buf_rsrc(d.rf[1], has1 ? nbytes : 0u),
+                                 rs_b2 = buf_rsrc(d.rf[2], has2 ? nbytes : 0u);
     auto load_u = [&](int mu) -> double {         // u(nu, mu) for -3 <= mu <= nmu, mirrored below mu = 1
         const double v = buf_load1(rs_u, (unsigned)(((mu < 1 ? 1 - mu : mu - 1) * nni + cl) * 8), 0.0);
         return mu < 1 ? s * v : v;
@@ -1169,6 +1204,13 @@ __global__ void __launch_bounds__(kResThreads) k_residual_rows(const ResDesc *__
             u_[k] = load_u(mr + 4);
             l_[k] = buf_load1(rs_l, e8, 0.0);
             b_[k] = buf_load1(rs_b, e8, 0.0);
+            if constexpr (PROD) {
+                // issued by every problem of the launch (same operation count
+                // in every row)
+                const double f1 = buf_load1(rs_b1, e8, 0.0), f2 = buf_load1(rs_b2, e8, 0.0);
+                b_[k] = has1 ? b_[k] * f1 : b_[k];
+                b_[k] = has2 ? b_[k] * f2 : b_[k];
+            }
             m_[k] = buf_load1(rs_bm, (unsigned)((mr - 1) * 8), 0.0);
         }
     };
@@ -1242,13 +1284,17 @@ __global__ void __launch_bounds__(kResThreads) k_residual_rows(const ResDesc *__
 
 // Launch the residual of n problems (descriptors d_res) on grids of at most
 // maxnni x maxnmu points: the row kernel when a row fits one workgroup, else
-// the per-point kernel.
-static int launch_residual(x2g_ctx *ctx, const ResDesc *d_res, int n, long maxnni, long maxnmu, long maxn)
+// the per-point kernel; prod: some problem's rhs is a product of factors.
+static int launch_residual(x2g_ctx *ctx, const ResDesc *d_res, int n, long maxnni, long maxnmu, long maxn,
+                           bool prod)
 {
     if (maxnni <= kResThreads) {
         const int strips = (int)((maxnmu - 5 + kResRows - 1) / kResRows);
         const int nth = (int)((maxnni + 63) / 64 * 64);
-        X2G_LAUNCH(ctx, k_residual_rows, dim3(strips * n), dim3(nth), 0, d_res, strips);
+        if (prod)
+            X2G_LAUNCH(ctx, k_residual_rows<true>, dim3(strips * n), dim3(nth), 0, d_res, strips);
+        else
+            X2G_LAUNCH(ctx, k_residual_rows<false>, dim3(strips * n), dim3(nth), 0, d_res, strips);
     } else {
         const int bpp = (int)((maxn + 255) / 256);
         X2G_LAUNCH(ctx, k_residual, dim3(bpp * n), dim3(256), 0, d_res, bpp);
@@ -2062,6 +2108,25 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     if (!pin.empty() && (rc = stage_descs(ctx, pin, doff, &d_pin))) return rc;
     if (!tails.empty() && (rc = stage_descs(ctx, tails, doff, &d_tail))) return rc;
 
+    // FP64 items that give their rhs as a product: form it in rhs_d first
+    {
+        std::vector<ProdDesc> pv;
+        long maxn = 0;
+        for (int i : i64) {
+            const x2g_relax_item &it = items[i];
+            if (!it.rhs_f1_d) continue;
+            const long np = grids[it.grid_index]->n();
+            pv.push_back({const_cast<double *>(it.rhs_d), {it.rhs_f1_d, it.rhs_f2_d, it.rhs_f3_d}, np});
+            maxn = std::max(maxn, np);
+        }
+        if (!pv.empty()) {
+            const ProdDesc *dp;
+            if ((rc = stage_descs(ctx, pv, doff, &dp))) return rc;
+            X2G_LAUNCH(ctx, k_rhs_product, dim3(grid_x_for(maxn, 256), (int)pv.size()), dim3(256), 0, dp);
+            X2G_HIP(hipGetLastError());
+        }
+    }
+
     // FP64 items: sweep plan
     SweepPlan<double> p64;
     const SweepDesc<double> *d64e = nullptr, *d64o = nullptr;
@@ -2113,6 +2178,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     const CvtDesc *d_cvt = nullptr;
     const FillFDesc *d_zero = nullptr;
     int nzero = 0;
+    bool res_prod = false;  // some FP32 item's rhs is a product of factors
     const BndDesc *d_bnd = nullptr;
     std::vector<BndDesc> bnd;
     if (!i32.empty()) {
@@ -2135,6 +2201,9 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             r.isym = it.isym;
             r.lhs = it.lhs_d;
             r.rhs = it.rhs_d;
+            r.rf[0] = it.rhs_f1_d;
+            r.rf[1] = it.rhs_f1_d ? it.rhs_f2_d : nullptr;
+            r.rf[2] = it.rhs_f1_d ? it.rhs_f3_d : nullptr;
             r.bmu = it.bkind == 0 ? g->bmu_orb : g->bmu_pot;
             r.dnu = g->dnu;
             r.res_f = kind32 == 2 ? f32p(i, 2) : f32p(i, 3);
@@ -2182,6 +2251,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         if ((rc = stage_descs(ctx, cvt, doff, &d_cvt))) return rc;
         if ((rc = stage_descs(ctx, zero, doff, &d_zero))) return rc;
         nzero = (int)zero.size();
+        for (auto &r : res) res_prod = res_prod || r.rf[0];
         if ((rc = stage_descs(ctx, bnd, doff, &d_bnd))) return rc;
         if (!split32) {
             long maxn = 0;
@@ -2215,7 +2285,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         }
         if (!i32.empty()) {
             const int n32 = (int)i32.size();
-            if ((rc = launch_residual(ctx, d_res, n32, maxnni, maxnmu, maxn))) return rc;
+            if ((rc = launch_residual(ctx, d_res, n32, maxnni, maxnmu, maxn, res_prod))) return rc;
             X2G_LAUNCH(ctx, k_zero_f32, dim3(std::min(grid_x_for(2 * maxpad, 256), 128), nzero), dim3(256), 0, d_zero);
             X2G_HIP(hipGetLastError());
             if ((rc = launch_sweeps<float>(ctx, d32e, d32o, p32, maxsor2))) return rc;
@@ -2254,6 +2324,9 @@ extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2
         r.isym = it.isym;
         r.lhs = it.lhs_d;
         r.rhs = it.rhs_d;
+        r.rf[0] = it.rhs_f1_d;
+        r.rf[1] = it.rhs_f1_d ? it.rhs_f2_d : nullptr;
+        r.rf[2] = it.rhs_f1_d ? it.rhs_f3_d : nullptr;
         r.bmu = it.bkind == 0 ? g->bmu_orb : g->bmu_pot;
         r.dnu = g->dnu;
         r.res_f = nullptr;
@@ -2270,7 +2343,9 @@ extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2
     const ResDesc *d_res;
     if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
     if ((rc = x2g_fill(ctx, acc, 2 * (size_t)n, 0.0))) return rc;
-    if ((rc = launch_residual(ctx, d_res, n, maxnni, maxnmu, maxn))) return rc;
+    bool prod = false;
+    for (auto &r : res) prod = prod || r.rf[0];
+    if ((rc = launch_residual(ctx, d_res, n, maxnni, maxnmu, maxn, prod))) return rc;
     X2G_HIP(hipMemcpyAsync(out_h, acc, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     X2G_HIP(hipStreamSynchronize(ctx->stream));
     return X2G_OK;

@@ -17,6 +17,9 @@ module x2f_capi
      integer(c_int) :: bkind = 1
      integer(c_int) :: prec = 0
      integer(c_int) :: grid_index = 0
+     type(c_ptr) :: rhs_f1_d = c_null_ptr   ! rhs = f1*f2*f3 formed by the residual (optional)
+     type(c_ptr) :: rhs_f2_d = c_null_ptr
+     type(c_ptr) :: rhs_f3_d = c_null_ptr
   end type x2g_relax_item
 
   type, bind(c) :: x2g_term

@@ -362,7 +362,9 @@ contains
     type(x2g_term) :: t(1)
     do k = 1, j%c%npot(i)
        ip = j%pidx(i,k)
-       if (present(lb)) then
+       if (present(lb) .and. j%o%prec == 1) then
+          ! the FP64 residual of the correction forms psi1 psi2 g itself (below)
+       else if (present(lb)) then
           t(1) = x2g_term(1.0_dp, j%orb(j%c%pin1(i,k)), j%orb(j%c%pin2(i,k)), j%S(10))
           call lc_add(lb, j%prhs(ip), c_null_ptr, 0.0_dp, t, 1, gidx)
        else
@@ -377,6 +379,11 @@ contains
        n = n + 1
        items(n) = x2g_relax_item(fun, j%plhs(ip), j%prhs(ip), c_null_ptr, j%omega_pot, &
             int(j%c%psym(i,k), c_int), 1_c_int, int(j%o%prec, c_int), int(gidx, c_int))
+       if (present(lb) .and. j%o%prec == 1) then
+          items(n)%rhs_f1_d = j%orb(j%c%pin1(i,k))
+          items(n)%rhs_f2_d = j%orb(j%c%pin2(i,k))
+          items(n)%rhs_f3_d = j%S(10)
+       endif
     enddo
     j%nsweep_pot = j%nsweep_pot + int(j%c%npot(i), c_long_long)*j%o%maxsorpot(1)*j%o%maxsorpot(2)
   end subroutine pot_items

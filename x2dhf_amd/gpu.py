@@ -25,7 +25,7 @@ class RelaxItem(C.Structure):
     """x2g_relax_item"""
     _fields_ = [("fun_d", _vp), ("lhs_d", _vp), ("rhs_d", _vp), ("tail_fock1_d", _vp),
                 ("omega", _dbl), ("isym", _int), ("bkind", _int), ("prec", _int),
-                ("grid_index", _int)]
+                ("grid_index", _int), ("rhs_f1_d", _vp), ("rhs_f2_d", _vp), ("rhs_f3_d", _vp)]
 
 
 class Term(C.Structure):
