@@ -373,7 +373,7 @@ struct NoDesc {
     int nni, nmu, noth, ihomo, normalise;
 };
 // partial layout: ((item*(kMaxOrtho+1) + k)*kRedBlocks + block), k = 0 norm,
-// k = 1+q overlap with other q; coef layout: item*(kMaxOrtho+1) + k
+// k = 1+q overlap with other q
 
 // stage 0: s = sum w*(psi*psi*f); stage 1 (blockIdx.z = q): sum w*(psi*oth_q*f)
 __global__ void __launch_bounds__(kRedBlock) k_no_dot(const NoDesc *__restrict__ nd, double *partial, int stage)
@@ -386,10 +386,10 @@ __global__ void __launch_bounds__(kRedBlock) k_no_dot(const NoDesc *__restrict__
     const double *b = stage == 0 ? d.psi : d.oth[q];
     double s[1] = {0.0};
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
-        double x = d.psi[e];
-        x *= b[e];
-        x *= d.f[e];
-        s[0] += d.w[e] * x;
+        double x = gmem(d.psi)[e];
+        x *= gmem(b)[e];
+        x *= gmem(d.f)[e];
+        s[0] += gmem(d.w)[e] * x;
     }
     block_sum<1>(s, sh);
     const int k = stage == 0 ? 0 : 1 + q;
@@ -410,45 +410,39 @@ __global__ void k_no_setci(const NoDesc *__restrict__ nd)
     }
 }
 
-// one block per item: the update coefficients of k_no_apply
-__global__ void __launch_bounds__(kRedBlock) k_no_finish(const NoDesc *__restrict__ nd, const double *partial, int nb,
-                                                         double *coef)
+// The update psi <- c0 psi + sum_k c_k other_k.  Each workgroup forms the
+// coefficients itself from the dot partials (the same fixed-order sums in
+// every workgroup, so all agree bit for bit); workgroup 0 writes the norm.
+__global__ void __launch_bounds__(kRedBlock) k_no_apply(const NoDesc *__restrict__ nd, const double *partial, int nb)
 {
     __shared__ double sh[4];
-    __shared__ double ov[kMaxOrtho + 1];
-    const NoDesc &d = nd[blockIdx.x];
+    __shared__ double c[kMaxOrtho + 1];
+    const NoDesc &d = nd[blockIdx.y];
     if (!d.normalise) return;
-    const double *p = partial + (long)blockIdx.x * (kMaxOrtho + 1) * nb;
+    const double *p = partial + (long)blockIdx.y * (kMaxOrtho + 1) * nb;
     for (int k = 0; k <= d.noth; ++k) {
         const double v = sum_partials(p + (long)k * nb, nb, sh);
-        if (threadIdx.x == 0) ov[k] = v;
+        if (threadIdx.x == 0) c[k] = v;
     }
     if (threadIdx.x == 0) {
-        const double s = ov[0];
-        if (d.norm_out) *d.norm_out = s;
+        const double s = c[0];
+        if (d.norm_out && blockIdx.x == 0) *d.norm_out = s;
         const double rs = 1.0 / sqrt(s);
         double a = 0.0;
         for (int k = 1; k <= d.noth; ++k) {
-            ov[k] = ov[k] * rs;
-            a += ov[k] * ov[k];
+            c[k] = c[k] * rs;
+            a += c[k] * c[k];
         }
         const double ano = d.noth > 0 ? sqrt(1.0 + a) / (1.0 - a * a) : 1.0;
-        double *c = coef + (long)blockIdx.x * (kMaxOrtho + 1);
         c[0] = ano * rs;
-        for (int k = 1; k <= d.noth; ++k) c[k] = -ano * ov[k];
+        for (int k = 1; k <= d.noth; ++k) c[k] = -ano * c[k];
     }
-}
-
-__global__ void k_no_apply(const NoDesc *__restrict__ nd, const double *coef)
-{
-    const NoDesc &d = nd[blockIdx.y];
-    if (!d.normalise) return;
-    const double *c = coef + (long)blockIdx.y * (kMaxOrtho + 1);
+    __syncthreads();
     const int no = d.noth;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
-        double v = c[0] * d.psi[e];
-        for (int k = 0; k < no; ++k) v += c[1 + k] * d.oth[k][e];
-        d.psi[e] = v;
+        double v = c[0] * gmem(d.psi)[e];
+        for (int k = 0; k < no; ++k) v += c[1 + k] * gmem(d.oth[k])[e];
+        gmem_w(d.psi)[e] = v;
     }
 }
 
@@ -759,11 +753,10 @@ extern "C" int x2g_normortho_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x
         maxhalf = std::max<long>(maxhalf, (long)(g->nni / 2) * g->nmu);
         maxoth = std::max(maxoth, d.noth);
     }
-    const size_t per = (size_t)(kMaxOrtho + 1) * (kRedBlocks + 1);
+    const size_t per = (size_t)(kMaxOrtho + 1) * kRedBlocks;
     int rc = ws_reserve(ctx, 5, (size_t)n * per * sizeof(double));
     if (rc) return rc;
     double *partial = (double *)ctx->ws[5].ptr;
-    double *coef = partial + (size_t)n * (kMaxOrtho + 1) * kRedBlocks;
     size_t off = 0;
     const NoDesc *dd;
     if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
@@ -775,9 +768,7 @@ extern "C" int x2g_normortho_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x
         X2G_LAUNCH(ctx, k_no_dot, dim3(kRedBlocks, n, maxoth), dim3(kRedBlock), 0, dd, partial, 1);
         X2G_HIP(hipGetLastError());
     }
-    X2G_LAUNCH(ctx, k_no_finish, dim3(n), dim3(kRedBlock), 0, dd, partial, kRedBlocks, coef);
-    X2G_HIP(hipGetLastError());
-    X2G_LAUNCH(ctx, k_no_apply, dim3(gx(maxn, 256), n), dim3(256), 0, dd, coef);
+    X2G_LAUNCH(ctx, k_no_apply, dim3(gx(maxn, kRedBlock), n), dim3(kRedBlock), 0, dd, partial, kRedBlocks);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }

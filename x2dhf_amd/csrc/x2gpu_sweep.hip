@@ -516,27 +516,28 @@ struct SweepSeg {
 };
 template <typename T, int NT, int W>
 __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a, const int L, const int MS,
-                                              unsigned char *ks_smem);
+                                              const bool zsrc, unsigned char *ks_smem);
 
 template <typename T, int NT, int W>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && W <= 352 ? 4 : 1)))
 k_sweep(const SweepDesc<T> *__restrict__ descs, const SweepSeg *__restrict__ segs, const int *__restrict__ wg_seg,
-        int MS)
+        int MS, int zero_src)
 {
     extern __shared__ __align__(16) unsigned char ks_smem[];
     const int s0 = wg_seg[blockIdx.x], s1 = wg_seg[blockIdx.x + 1];
     for (int si = s0; si < s1; ++si) {
         const SweepSeg sg = segs[si];
         if (si > s0) __syncthreads();  // the previous segment is done with the LDS
-        sweep_segment<T, NT, W>(descs[sg.prob], sg.a, sg.L, MS, ks_smem);
+        sweep_segment<T, NT, W>(descs[sg.prob], sg.a, sg.L, MS, zero_src != 0, ks_smem);
     }
 }
 
 // One strip of rows [a, a+L) of problem d (L a multiple of 4, or the rest of
-// the problem): the pipelined sweep described above.
+// the problem): the pipelined sweep described above.  zsrc: src is known to
+// be zero everywhere (the first sweep of a correction), so it is not read.
 template <typename T, int NT, int W>
 __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a, const int L, const int MS,
-                                              unsigned char *ks_smem)
+                                              const bool zsrc, unsigned char *ks_smem)
 {
     constexpr int SKEW = 2, RB = 4 * SKEW + 4;
     constexpr int VEC = sweep_vec<T>(), NSV = sweep_nsv<T>();
@@ -599,7 +600,7 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
 #pragma unroll
         for (int j = 0; j < NSV; ++j) {
             const int mu = mub + (cinfo[j] & 7);
-            const bool ok = cl[j] >= 0 && mu >= -3 && mu <= nmu + 4;
+            const bool ok = !zsrc && cl[j] >= 0 && mu >= -3 && mu <= nmu + 4;
             pr[j] = __builtin_amdgcn_raw_buffer_load_b128(rs_src, oob_unless(ok, gb + (tid + j * nth) * 16), 0, 0);
         }
     };
@@ -1651,42 +1652,42 @@ static int sweep_kind(x2g_ctx *ctx, int maxnni)
 // k_sweep for the plan's stride class and depth (the instantiations are
 // listed once here and in set_lds_attr).
 template <typename T, int NT, int W>
-static void launch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, int n, const SweepPlan<T> &plan)
+static void launch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, const SweepPlan<T> &plan, int zero_src)
 {
     // the caller has flushed the staged descriptors
-    (void)n;
     hipLaunchKernelGGL((k_sweep<T, NT, W>), dim3(plan.total_wg), dim3(plan.nth), plan.lds, ctx->stream, dd,
-                       plan.d_segs, plan.d_wgseg, plan.MS);
+                       plan.d_segs, plan.d_wgseg, plan.MS, zero_src);
 }
 
 template <typename T>
-static bool dispatch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, int n, const SweepPlan<T> &plan)
+static bool dispatch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, const SweepPlan<T> &plan, int zero_src)
 {
     switch (plan.W) {
-    case 128: launch_ks<T, 1, 128>(ctx, dd, n, plan); return true;
-    case 192: launch_ks<T, 1, 192>(ctx, dd, n, plan); return true;
-    case 256: launch_ks<T, 1, 256>(ctx, dd, n, plan); return true;
-    case 352: launch_ks<T, 1, 352>(ctx, dd, n, plan); return true;
+    case 128: launch_ks<T, 1, 128>(ctx, dd, plan, zero_src); return true;
+    case 192: launch_ks<T, 1, 192>(ctx, dd, plan, zero_src); return true;
+    case 256: launch_ks<T, 1, 256>(ctx, dd, plan, zero_src); return true;
+    case 352: launch_ks<T, 1, 352>(ctx, dd, plan, zero_src); return true;
     default: break;
     }
     if constexpr (sizeof(T) == 4) {
-        if (plan.W == 528) { launch_ks<T, 1, 528>(ctx, dd, n, plan); return true; }
-        if (plan.W == 768) { launch_ks<T, 2, 768>(ctx, dd, n, plan); return true; }
+        if (plan.W == 528) { launch_ks<T, 1, 528>(ctx, dd, plan, zero_src); return true; }
+        if (plan.W == 768) { launch_ks<T, 2, 768>(ctx, dd, plan, zero_src); return true; }
     }
     return false;
 }
 
-// One sweep of every problem in the plan (device descriptors dd, host copy hv).
+// One sweep of every problem in the plan (device descriptors dd, host copy
+// hv); zero_src (k_sweep only): every src array is zero, so none is read.
 template <typename T>
 static int launch_sweep_once(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::vector<SweepDesc<T>> &hv,
-                             const SweepPlan<T> &plan)
+                             const SweepPlan<T> &plan, bool zero_src)
 {
     const int skew = plan.skew, total_wg = plan.total_wg;
     const size_t lds = plan.lds;
     const bool in_place = plan.kind == 0;
     const int n = (int)hv.size();
     if (plan.kind == 2) {
-        const bool ok = dispatch_ks<T>(ctx, dd, n, plan);
+        const bool ok = dispatch_ks<T>(ctx, dd, plan, zero_src ? 1 : 0);
         X2G_CHECK(ok, "no sweep kernel for ring stride %d", plan.W);
         X2G_HIP(hipGetLastError());
         return X2G_OK;
@@ -1717,10 +1718,12 @@ static int launch_sweep_once(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::ve
 // nsweeps back-to-back sweeps, alternating the even (src = A) and odd
 // (src = B) descriptor lists.  With statistics on, one pair of HIP events
 // brackets the whole run, so the per-launch average includes the short
-// launch-to-launch gaps and no event sits between two sweeps.
+// launch-to-launch gaps and no event sits between two sweeps.  a_zero: the
+// A arrays are zero (a correction's start), which the first k_sweep launch
+// takes without reading them.
 template <typename T>
 static int launch_sweeps(x2g_ctx *ctx, const SweepDesc<T> *de, const SweepDesc<T> *dodd, const SweepPlan<T> &plan,
-                         int nsweeps)
+                         int nsweeps, bool a_zero = false)
 {
     if (nsweeps <= 0) return X2G_OK;
     // algorithmic bytes of one sweep: read u, write u, read lhs and rhs per
@@ -1735,7 +1738,8 @@ static int launch_sweeps(x2g_ctx *ctx, const SweepDesc<T> *de, const SweepDesc<T
     if ((rc = stats_begin(ctx, &ea, &eb))) return rc;
     for (int it = 0; it < nsweeps; ++it) {
         const bool ev = (it % 2) == 0;
-        if ((rc = launch_sweep_once<T>(ctx, ev ? de : dodd, ev ? plan.even : plan.odd, plan))) return rc;
+        const bool zsrc = it == 0 && a_zero && plan.kind == 2;
+        if ((rc = launch_sweep_once<T>(ctx, ev ? de : dodd, ev ? plan.even : plan.odd, plan, zsrc))) return rc;
     }
     return stats_end(ctx, ea, eb, bytes * nsweeps, prec, nsweeps);
 }
@@ -2178,6 +2182,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     const CvtDesc *d_cvt = nullptr;
     const FillFDesc *d_zero = nullptr;
     int nzero = 0;
+    long maxzero = 0;  // longest zero fill
     bool res_prod = false;  // some FP32 item's rhs is a product of factors
     const BndDesc *d_bnd = nullptr;
     std::vector<BndDesc> bnd;
@@ -2217,12 +2222,15 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             corr.push_back({it.fun_d, f32p(i, 0), g->nni, g->nmu, pstride(i)});
             cvt.push_back({it.lhs_d, f32p(i, 2), (long)g->n()});
             if (kind32 == 2) {
-                // the correction starts from zero; k_sweep writes every row of
-                // its destination except the mu <= 1 and mu >= nmu-3 rows
+                // the correction starts from zero: the first sweep takes A as
+                // zero without reading it, and k_sweep writes every row of
+                // its destination except the mu <= 1 and mu >= nmu-3 rows,
+                // which stay zero in A and B
                 const long P = pstride(i);
-                zero.push_back({f32p(i, 0), P * (g->nmu + 8)});
-                zero.push_back({f32p(i, 1), 5 * P});
-                zero.push_back({f32p(i, 1) + (long)g->nmu * P, 8 * P});
+                for (int k = 0; k < 2; ++k) {
+                    zero.push_back({f32p(i, k), 5 * P});
+                    zero.push_back({f32p(i, k) + (long)g->nmu * P, 8 * P});
+                }
             } else {
                 zero.push_back({f32p(i, 0), (long)(2 * align256(padbytes(i)) / sizeof(float))});
             }
@@ -2251,6 +2259,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         if ((rc = stage_descs(ctx, cvt, doff, &d_cvt))) return rc;
         if ((rc = stage_descs(ctx, zero, doff, &d_zero))) return rc;
         nzero = (int)zero.size();
+        for (auto &z : zero) maxzero = std::max(maxzero, z.n);
         for (auto &r : res) res_prod = res_prod || r.rf[0];
         if ((rc = stage_descs(ctx, bnd, doff, &d_bnd))) return rc;
         if (!split32) {
@@ -2286,9 +2295,9 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         if (!i32.empty()) {
             const int n32 = (int)i32.size();
             if ((rc = launch_residual(ctx, d_res, n32, maxnni, maxnmu, maxn, res_prod))) return rc;
-            X2G_LAUNCH(ctx, k_zero_f32, dim3(std::min(grid_x_for(2 * maxpad, 256), 128), nzero), dim3(256), 0, d_zero);
+            X2G_LAUNCH(ctx, k_zero_f32, dim3(std::min(grid_x_for(maxzero, 256), 128), nzero), dim3(256), 0, d_zero);
             X2G_HIP(hipGetLastError());
-            if ((rc = launch_sweeps<float>(ctx, d32e, d32o, p32, maxsor2))) return rc;
+            if ((rc = launch_sweeps<float>(ctx, d32e, d32o, p32, maxsor2, /*a_zero=*/true))) return rc;
             X2G_LAUNCH(ctx, k_correct, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, d_corr);
             X2G_HIP(hipGetLastError());
             X2G_LAUNCH(ctx, k_boundary_fun, dim3(grid_x_for(2 * maxn / std::max<long>(maxnni, 1) + maxnni, 256), n32),
