@@ -70,42 +70,46 @@ __device__ __forceinline__ gdptr_w gmem_w(double *p) { return (gdptr_w)p; }
 // flight together (lanes past n read point n-1 and store nothing).  Per point
 // the operations and their order are those of the plain loop.
 constexpr int kLcIlp = 4;
+__device__ __forceinline__ void lincomb_at(double *out, const double *base, double bscale,
+                                           const x2g_term *__restrict__ t, int nt, long n, long e0)
+{
+    long e[kLcIlp];
+    double v[kLcIlp];
+#pragma unroll
+    for (int j = 0; j < kLcIlp; ++j) e[j] = min(e0 + 256L * j, n - 1);
+    if (base) {
+#pragma unroll
+        for (int j = 0; j < kLcIlp; ++j) v[j] = bscale * gmem(base)[e[j]];
+    } else {
+#pragma unroll
+        for (int j = 0; j < kLcIlp; ++j) v[j] = 0.0;
+    }
+    for (int k = 0; k < nt; ++k) {
+        const double c = t[k].c;
+        const double *a = t[k].a_d, *b = t[k].b_d, *cc = t[k].cc_d;
+        double x[kLcIlp];
+#pragma unroll
+        for (int j = 0; j < kLcIlp; ++j) x[j] = c * gmem(a)[e[j]];
+        if (b) {
+#pragma unroll
+            for (int j = 0; j < kLcIlp; ++j) x[j] *= gmem(b)[e[j]];
+        }
+        if (cc) {
+#pragma unroll
+            for (int j = 0; j < kLcIlp; ++j) x[j] *= gmem(cc)[e[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < kLcIlp; ++j) v[j] += x[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kLcIlp; ++j)
+        if (e0 + 256L * j < n) gmem_w(out)[e0 + 256L * j] = v[j];
+}
+
 __device__ __forceinline__ void lincomb_points(double *out, const double *base, double bscale,
                                                const x2g_term *__restrict__ t, int nt, long n, long first, long stride)
 {
-    for (long e0 = first; e0 < n; e0 += stride) {
-        long e[kLcIlp];
-        double v[kLcIlp];
-#pragma unroll
-        for (int j = 0; j < kLcIlp; ++j) e[j] = min(e0 + 256L * j, n - 1);
-        if (base) {
-#pragma unroll
-            for (int j = 0; j < kLcIlp; ++j) v[j] = bscale * gmem(base)[e[j]];
-        } else {
-#pragma unroll
-            for (int j = 0; j < kLcIlp; ++j) v[j] = 0.0;
-        }
-        for (int k = 0; k < nt; ++k) {
-            const double c = t[k].c;
-            const double *a = t[k].a_d, *b = t[k].b_d, *cc = t[k].cc_d;
-            double x[kLcIlp];
-#pragma unroll
-            for (int j = 0; j < kLcIlp; ++j) x[j] = c * gmem(a)[e[j]];
-            if (b) {
-#pragma unroll
-                for (int j = 0; j < kLcIlp; ++j) x[j] *= gmem(b)[e[j]];
-            }
-            if (cc) {
-#pragma unroll
-                for (int j = 0; j < kLcIlp; ++j) x[j] *= gmem(cc)[e[j]];
-            }
-#pragma unroll
-            for (int j = 0; j < kLcIlp; ++j) v[j] += x[j];
-        }
-#pragma unroll
-        for (int j = 0; j < kLcIlp; ++j)
-            if (e0 + 256L * j < n) gmem_w(out)[e0 + 256L * j] = v[j];
-    }
+    for (long e0 = first; e0 < n; e0 += stride) lincomb_at(out, base, bscale, t, nt, n, e0);
 }
 
 __global__ void __launch_bounds__(256) k_lincomb(double *out, const double *base, double bscale,
@@ -346,11 +350,22 @@ struct LcOp {
     long n;
 };
 
-__global__ void __launch_bounds__(256) k_lincomb_batch(const LcOp *__restrict__ ops, const x2g_term *__restrict__ terms)
+// One workgroup per (op, chunk of 256*kLcIlp points).  The 8 XCDs take a
+// launch's workgroups round-robin, each with its own L2: XCD x gets chunks
+// [x*slice, (x+1)*slice) and walks them chunk by chunk with the op index
+// fastest, so the ops that share input arrays (a job's Coulomb and exchange
+// potentials and orbitals) read a chunk's segment of each from that XCD's L2
+// after the first of them.
+__global__ void __launch_bounds__(256) k_lincomb_batch(const LcOp *__restrict__ ops, const x2g_term *__restrict__ terms,
+                                                       int nops, int nchunk, int slice)
 {
-    const LcOp &o = ops[blockIdx.y];
-    lincomb_points(o.out, o.base, o.bscale, terms + o.first, o.nterms, o.n,
-                   (long)blockIdx.x * 256 * kLcIlp + threadIdx.x, (long)gridDim.x * 256 * kLcIlp);
+    const int b = blockIdx.x, x = b & 7, j = b >> 3;
+    const int chunk = x * slice + j / nops, q = j % nops;
+    if (chunk >= nchunk) return;
+    const LcOp &o = ops[q];
+    const long e0 = (long)chunk * 256 * kLcIlp + threadIdx.x;
+    if ((long)chunk * 256 * kLcIlp >= o.n) return;
+    lincomb_at(o.out, o.base, o.bscale, terms + o.first, o.nterms, o.n, e0);
 }
 
 // Sum of the nb partials at p (fixed order), valid in thread 0.
@@ -716,7 +731,8 @@ extern "C" int x2g_lincomb_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g
     int rc = stage(ctx, v.data(), v.size(), off, &dops);
     if (rc) return rc;
     if (nterms_total && (rc = stage(ctx, terms, (size_t)nterms_total, off, &dt))) return rc;
-    X2G_LAUNCH(ctx, k_lincomb_batch, dim3(gx(maxn, 256 * kLcIlp), nops), dim3(256), 0, dops, dt);
+    const int nchunk = (int)((maxn + 256 * kLcIlp - 1) / (256 * kLcIlp)), slice = (nchunk + 7) / 8;
+    X2G_LAUNCH(ctx, k_lincomb_batch, dim3(8 * nops * slice), dim3(256), 0, dops, dt, nops, nchunk, slice);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
