@@ -107,7 +107,8 @@ contains
     real(dp) :: coef(25)
     real(dp), allocatable, target :: vec(:)
     type(x2g_lcao) :: lc
-    integer :: i, k, ip, ii, slot
+    integer :: i, k, ip, ip2, ii, slot
+    integer :: pdm_of(MAXPOT)
     integer(c_size_t) :: nb
 
     ierr = 0
@@ -149,8 +150,20 @@ contains
        do k = 1, j%c%npot(i)
           ip = ip + 1
           j%pidx(i,k) = ip
-          j%plhs(ip) = dalloc(j); j%prhs(ip) = dalloc(j)
-          ! lhs = f3 + dm^2 e (relaxPots.F90:87-95), fixed for the job
+          j%prhs(ip) = dalloc(j)
+          ! lhs = f3 + dm^2 e (relaxPots.F90:87-95), fixed for the job: one
+          ! array per distinct dm, shared by the potentials with that dm
+          ! (their relaxations then read it from cache)
+          j%plhs(ip) = c_null_ptr
+          do ip2 = 1, ip - 1
+             if (pdm_of(ip2) == j%c%pdm(i,k)) then
+                j%plhs(ip) = j%plhs(ip2)
+                exit
+             endif
+          enddo
+          pdm_of(ip) = j%c%pdm(i,k)
+          if (c_associated(j%plhs(ip))) cycle
+          j%plhs(ip) = dalloc(j)
           if (j%c%pdm(i,k) == 0) then
              call lincomb0(j, j%plhs(ip), j%S(8), 1.0_dp)
           else
