@@ -156,8 +156,9 @@ def traffic_summary(fetch_csv, write_csv, batch):
     """HBM bytes per sweep launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
     runs of --traffic-probe.  Each counter (KiB per dispatch) is scaled by
     what it reads for the probe's copy kernel (x2g_lincomb, bytes known), and
-    averaged over the probe's FP32 (first five) and FP64 (last five) sweep
-    launches."""
+    averaged over the probe's FP32 sweep launches that read their source
+    (launches 2-5: the first sweep of a correction takes its zero start
+    without reading it) and its FP64 launches (the last five)."""
     import csv
     n_copy = 1201 * 1201 * 8  # bytes read and written by one copy launch
 
@@ -172,7 +173,7 @@ def traffic_summary(fetch_csv, write_csv, batch):
         sweep = [float(r["Counter_Value"]) * 1024 for r in rows if named(r, "k_sweep")]
         if len(copy) != 3 or len(sweep) != 10:
             raise SystemExit(f"{path}: expected 3 copy and 10 sweep dispatches, found {len(copy)} and {len(sweep)}")
-        return sum(copy) / len(copy), sweep[:5], sweep[5:]
+        return sum(copy) / len(copy), sweep[1:5], sweep[5:]
 
     fc, f32, f64 = per_kernel(fetch_csv)
     wc, w32, w64 = per_kernel(write_csv)

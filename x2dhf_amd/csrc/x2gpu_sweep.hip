@@ -1736,12 +1736,14 @@ static int launch_sweeps(x2g_ctx *ctx, const SweepDesc<T> *de, const SweepDesc<T
     if (rc) return rc;
     hipEvent_t ea, eb;
     if ((rc = stats_begin(ctx, &ea, &eb))) return rc;
+    double total = 0.0;
     for (int it = 0; it < nsweeps; ++it) {
         const bool ev = (it % 2) == 0;
         const bool zsrc = it == 0 && a_zero && plan.kind == 2;
         if ((rc = launch_sweep_once<T>(ctx, ev ? de : dodd, ev ? plan.even : plan.odd, plan, zsrc))) return rc;
+        total += zsrc ? 0.75 * bytes : bytes;  // a zero source is not read: 3 words per point
     }
-    return stats_end(ctx, ea, eb, bytes * nsweeps, prec, nsweeps);
+    return stats_end(ctx, ea, eb, total, prec, nsweeps);
 }
 
 static std::mutex g_attr_mutex;
