@@ -113,6 +113,16 @@ inline int stage_n(x2g_ctx *ctx, const D *v, size_t n, const D **out)
     return rc;
 }
 
+// Global-memory view of a device pointer: loads through it are global (not
+// flat) loads, which only the vector-memory counter tracks, so waiting for a
+// scalar load (a term's pointers) or an LDS result does not also wait for them.
+typedef const double __attribute__((address_space(1))) *gdptr;
+typedef double __attribute__((address_space(1))) *gdptr_w;
+typedef const float __attribute__((address_space(1))) *gfptr;
+__device__ __forceinline__ gdptr gmem(const double *p) { return (gdptr)p; }
+__device__ __forceinline__ gdptr_w gmem_w(double *p) { return (gdptr_w)p; }
+__device__ __forceinline__ gfptr gmem(const float *p) { return (gfptr)p; }
+
 // Kernel-level entry points shared between translation units.
 int launch_putin(x2g_grid *g, int isym, const double *fun, double *w1, double *w2);
 int launch_putout(x2g_grid *g, double *fun, const double *work);

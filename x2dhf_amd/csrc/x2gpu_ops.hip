@@ -56,13 +56,6 @@ __global__ void k_fill(double *dst, size_t n, double v)
         dst[e] = v;
 }
 
-// Global-memory view of a device pointer: loads through it are global (not
-// flat) loads, which only the vector-memory counter tracks, so waiting for a
-// scalar load (a term's pointers) does not also wait for them.
-typedef const double __attribute__((address_space(1))) *gdptr;
-typedef double __attribute__((address_space(1))) *gdptr_w;
-__device__ __forceinline__ gdptr gmem(const double *p) { return (gdptr)p; }
-__device__ __forceinline__ gdptr_w gmem_w(double *p) { return (gdptr_w)p; }
 
 // out = bscale*base + sum_k c_k a_k b_k cc_k over n points, for the block's
 // share of a grid-stride loop with blocks of 256 threads.  Each thread takes

@@ -1312,16 +1312,18 @@ struct CorrDesc {
 };
 
 // u += e on the relaxed interior.
-__global__ void k_correct(const CorrDesc *__restrict__ cd)
+// u += e over the relaxed points (32-bit index arithmetic: a grid has fewer
+// than 2^31 points; global loads and stores).
+__global__ void __launch_bounds__(256) k_correct(const CorrDesc *__restrict__ cd)
 {
     const CorrDesc &d = cd[blockIdx.y];
-    const int nni = d.nni, nmu = d.nmu;
-    const long s1 = d.ps;
-    const long tot = (long)(nni - 2) * (nmu - 5);
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
-         k += (long)gridDim.x * blockDim.x) {
-        const int mu = 2 + (int)(k / (nni - 2)), nu = 2 + (int)(k % (nni - 2));
-        d.fun[(long)(mu - 1) * nni + nu - 1] += (double)d.e[(long)(mu + 3) * s1 + nu + 3];
+    const int nni = d.nni, w = nni - 2, s1 = d.ps;
+    const int tot = w * (d.nmu - 5);
+    const gfptr e = gmem(d.e);
+    const gdptr_w u = gmem_w(d.fun);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += gridDim.x * blockDim.x) {
+        const int r = k / w, c = k - r * w;   // mu = 2 + r, nu = 2 + c
+        u[(1 + r) * nni + 1 + c] += (double)e[(5 + r) * s1 + 5 + c];
     }
 }
 
