@@ -383,74 +383,118 @@ struct NoDesc {
 // partial layout: ((item*(kMaxOrtho+1) + k)*kRedBlocks + block), k = 0 norm,
 // k = 1+q overlap with other q
 
-// stage 0: s = sum w*(psi*psi*f); stage 1 (blockIdx.z = q): sum w*(psi*oth_q*f)
-__global__ void __launch_bounds__(kRedBlock) k_no_dot(const NoDesc *__restrict__ nd, double *partial, int stage)
+// The g/u symmetry (setCiOrb, scfUtils.F90:380-406) as a view: for ihomo != 0
+// the nu > n2 = (nni+1)/2 half of a row is the mirror image of the nu < n2
+// half (times ihomo), and for ihomo = -1 the middle column is zero.  Grids
+// have odd nni.  psi_sym(e) reads only the nu <= n2 half.
+__device__ __forceinline__ double psi_sym(const NoDesc &d, long e)
+{
+    const double *psi = d.psi;
+    if (d.ihomo == 0) return gmem(psi)[e];
+    const int nni = d.nni, n2 = nni / 2 + 1;
+    const int mi = (int)(e / nni), nu = 1 + (int)(e - (long)mi * nni);
+    if (nu < n2) return gmem(psi)[e];
+    if (nu == n2) return d.ihomo == 1 ? gmem(psi)[e] : 0.0;
+    const double v = gmem(psi)[(long)mi * nni + (nni - nu)];  // nu' = nni + 1 - nu
+    return d.ihomo == 1 ? v : -v;
+}
+
+// blockIdx.z = 0: s = sum w*(psi*psi*f) over the orbital as it arrives (the
+// reference normalises before imposing the symmetry); blockIdx.z = 1+q:
+// sum w*(psi_sym*oth_q*f) with the symmetry imposed.
+__global__ void __launch_bounds__(kRedBlock) k_no_dot(const NoDesc *__restrict__ nd, double *partial)
 {
     __shared__ double sh[4];
     const NoDesc &d = nd[blockIdx.y];
     if (!d.normalise) return;
-    const int q = blockIdx.z;
-    if (stage == 1 && q >= d.noth) return;
-    const double *b = stage == 0 ? d.psi : d.oth[q];
+    const int k = blockIdx.z;
+    if (k > d.noth) return;
     double s[1] = {0.0};
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
-        double x = gmem(d.psi)[e];
-        x *= gmem(b)[e];
-        x *= gmem(d.f)[e];
-        s[0] += gmem(d.w)[e] * x;
+    if (k == 0) {
+        for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
+            double x = gmem(d.psi)[e];
+            x *= gmem(d.psi)[e];
+            x *= gmem(d.f)[e];
+            s[0] += gmem(d.w)[e] * x;
+        }
+    } else {
+        const double *b = d.oth[k - 1];
+        for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
+            double x = psi_sym(d, e);
+            x *= gmem(b)[e];
+            x *= gmem(d.f)[e];
+            s[0] += gmem(d.w)[e] * x;
+        }
     }
     block_sum<1>(s, sh);
-    const int k = stage == 0 ? 0 : 1 + q;
     if (threadIdx.x == 0) partial[((long)blockIdx.y * (kMaxOrtho + 1) + k) * gridDim.x + blockIdx.x] = s[0];
 }
 
-__global__ void k_no_setci(const NoDesc *__restrict__ nd)
-{
-    const NoDesc &d = nd[blockIdx.y];
-    if (d.ihomo == 0) return;
-    const int nni = d.nni, n2 = nni / 2 + 1;
-    const long tot = (long)(n2 - 1) * d.nmu;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
-        const int mi = (int)(e / (n2 - 1)), ni = 1 + (int)(e % (n2 - 1));
-        const double v = d.psi[(long)mi * nni + ni - 1];
-        d.psi[(long)mi * nni + (nni - ni)] = d.ihomo == 1 ? v : -v;
-        if (d.ihomo != 1 && ni == 1) d.psi[(long)mi * nni + n2 - 1] = 0.0;
-    }
-}
-
-// The update psi <- c0 psi + sum_k c_k other_k.  Each workgroup forms the
-// coefficients itself from the dot partials (the same fixed-order sums in
-// every workgroup, so all agree bit for bit); workgroup 0 writes the norm.
+// The update psi <- c0 psi_sym + sum_k c_k other_k (with normalise = 0: psi
+// <- psi_sym).  Each workgroup forms the coefficients itself from the dot
+// partials (the same fixed-order sums in every workgroup, so all agree bit
+// for bit); workgroup 0 writes the norm.  With the symmetry, a thread owns a
+// point nu <= n2 and its mirror image, so the half it reads is written only
+// by itself.
 __global__ void __launch_bounds__(kRedBlock) k_no_apply(const NoDesc *__restrict__ nd, const double *partial, int nb)
 {
     __shared__ double sh[4];
     __shared__ double c[kMaxOrtho + 1];
     const NoDesc &d = nd[blockIdx.y];
-    if (!d.normalise) return;
-    const double *p = partial + (long)blockIdx.y * (kMaxOrtho + 1) * nb;
-    for (int k = 0; k <= d.noth; ++k) {
-        const double v = sum_partials(p + (long)k * nb, nb, sh);
-        if (threadIdx.x == 0) c[k] = v;
-    }
-    if (threadIdx.x == 0) {
-        const double s = c[0];
-        if (d.norm_out && blockIdx.x == 0) *d.norm_out = s;
-        const double rs = 1.0 / sqrt(s);
-        double a = 0.0;
-        for (int k = 1; k <= d.noth; ++k) {
-            c[k] = c[k] * rs;
-            a += c[k] * c[k];
+    if (!d.normalise && d.ihomo == 0) return;
+    const int no = d.normalise ? d.noth : 0;
+    if (d.normalise) {
+        const double *p = partial + (long)blockIdx.y * (kMaxOrtho + 1) * nb;
+        for (int k = 0; k <= no; ++k) {
+            const double v = sum_partials(p + (long)k * nb, nb, sh);
+            if (threadIdx.x == 0) c[k] = v;
         }
-        const double ano = d.noth > 0 ? sqrt(1.0 + a) / (1.0 - a * a) : 1.0;
-        c[0] = ano * rs;
-        for (int k = 1; k <= d.noth; ++k) c[k] = -ano * c[k];
+        if (threadIdx.x == 0) {
+            const double s = c[0];
+            if (d.norm_out && blockIdx.x == 0) *d.norm_out = s;
+            const double rs = 1.0 / sqrt(s);
+            double a = 0.0;
+            for (int k = 1; k <= no; ++k) {
+                c[k] = c[k] * rs;
+                a += c[k] * c[k];
+            }
+            const double ano = no > 0 ? sqrt(1.0 + a) / (1.0 - a * a) : 1.0;
+            c[0] = ano * rs;
+            for (int k = 1; k <= no; ++k) c[k] = -ano * c[k];
+        }
+    } else if (threadIdx.x == 0) {
+        c[0] = 1.0;
     }
     __syncthreads();
-    const int no = d.noth;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
-        double v = c[0] * gmem(d.psi)[e];
-        for (int k = 0; k < no; ++k) v += c[1 + k] * gmem(d.oth[k])[e];
-        gmem_w(d.psi)[e] = v;
+    const double *psi = d.psi;
+    if (d.ihomo == 0) {
+        for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
+            double v = c[0] * gmem(psi)[e];
+            for (int k = 0; k < no; ++k) v += c[1 + k] * gmem(d.oth[k])[e];
+            gmem_w(d.psi)[e] = v;
+        }
+        return;
+    }
+    const int nni = d.nni, n2 = nni / 2 + 1;
+    const long tot = (long)n2 * d.nmu;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+        const int mi = (int)(t / n2), nu = 1 + (int)(t - (long)mi * n2);
+        const long e = (long)mi * nni + nu - 1;
+        const double v1 = gmem(psi)[e];
+        if (nu < n2) {
+            const long e2 = (long)mi * nni + (nni - nu);  // nu' = nni + 1 - nu
+            double a = c[0] * v1, b = c[0] * (d.ihomo == 1 ? v1 : -v1);
+            for (int k = 0; k < no; ++k) {
+                a += c[1 + k] * gmem(d.oth[k])[e];
+                b += c[1 + k] * gmem(d.oth[k])[e2];
+            }
+            gmem_w(d.psi)[e] = a;
+            gmem_w(d.psi)[e2] = b;
+        } else {
+            double a = c[0] * (d.ihomo == 1 ? v1 : 0.0);
+            for (int k = 0; k < no; ++k) a += c[1 + k] * gmem(d.oth[k])[e];
+            gmem_w(d.psi)[e] = a;
+        }
     }
 }
 
@@ -735,7 +779,7 @@ extern "C" int x2g_normortho_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x
     X2G_CHECK(ctx && grids && (items || n == 0) && n >= 0, "x2g_normortho_batch: null argument");
     if (n == 0) return X2G_OK;
     std::vector<NoDesc> v(n);
-    long maxn = 0, maxhalf = 0;
+    long maxn = 0;
     int maxoth = 0;
     for (int i = 0; i < n; ++i) {
         const x2g_normortho_item &it = items[i];
@@ -745,6 +789,7 @@ extern "C" int x2g_normortho_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x
         X2G_CHECK(it.psi_d, "x2g_normortho_batch: item %d has no orbital", i);
         X2G_CHECK(it.nothers >= 0 && it.nothers <= kMaxOrtho, "x2g_normortho_batch: item %d nothers", i);
         X2G_CHECK(it.ihomo >= -1 && it.ihomo <= 1, "x2g_normortho_batch: item %d ihomo", i);
+        X2G_CHECK(it.ihomo == 0 || g->nni % 2 == 1, "x2g_normortho_batch: item %d: symmetry needs odd nni", i);
         NoDesc &d = v[i];
         d.psi = it.psi_d;
         d.w = g->suppl + 13 * g->n();  // wgt2
@@ -759,7 +804,6 @@ extern "C" int x2g_normortho_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x
         d.ihomo = it.ihomo;
         d.normalise = it.normalise ? 1 : 0;
         maxn = std::max(maxn, d.n);
-        maxhalf = std::max<long>(maxhalf, (long)(g->nni / 2) * g->nmu);
         maxoth = std::max(maxoth, d.noth);
     }
     const size_t per = (size_t)(kMaxOrtho + 1) * kRedBlocks;
@@ -769,14 +813,8 @@ extern "C" int x2g_normortho_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x
     size_t off = 0;
     const NoDesc *dd;
     if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
-    X2G_LAUNCH(ctx, k_no_dot, dim3(kRedBlocks, n, 1), dim3(kRedBlock), 0, dd, partial, 0);
+    X2G_LAUNCH(ctx, k_no_dot, dim3(kRedBlocks, n, 1 + maxoth), dim3(kRedBlock), 0, dd, partial);
     X2G_HIP(hipGetLastError());
-    X2G_LAUNCH(ctx, k_no_setci, dim3(gx(maxhalf, 256), n), dim3(256), 0, dd);
-    X2G_HIP(hipGetLastError());
-    if (maxoth > 0) {
-        X2G_LAUNCH(ctx, k_no_dot, dim3(kRedBlocks, n, maxoth), dim3(kRedBlock), 0, dd, partial, 1);
-        X2G_HIP(hipGetLastError());
-    }
     X2G_LAUNCH(ctx, k_no_apply, dim3(gx(maxn, kRedBlock), n), dim3(kRedBlock), 0, dd, partial, kRedBlocks);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
