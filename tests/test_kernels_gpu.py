@@ -473,3 +473,22 @@ def test_rhs_as_product_of_factors(ctx, nni, nmu, nf):
         assert np.allclose(out["rhs"][1], out["factors"][1], rtol=1e-13, atol=0.0)
         assert np.array_equal(out["rhs"][0], out["factors"][0])
     g.destroy()
+
+
+def test_zero_sweeps_leave_the_function_unchanged(ctx):
+    """maxsor2 = 0: no sweep runs, so the function comes back bit for bit,
+    in FP64 and with the FP32 correction (whose start array is then never
+    cleared or read)."""
+    from x2dhf_amd import gpu
+    gpu.check(ctx.lib.x2g_set_sweep_engine(0, -1))
+    nni, nmu = 169, 241
+    P = synthetic_problem(nni, nmu, seed=3)
+    g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
+    gh = (gpu._vp * 1)(g.h.value)
+    l, r = ctx.upload(P["lhs"]), ctx.upload(P["rhs"])
+    for prec in (0, 1):
+        f = ctx.upload(P["fun"])
+        it = (gpu.RelaxItem * 1)(gpu.RelaxItem(f.ptr, l.ptr, r.ptr, None, 1.9, 1, 1, prec, 0))
+        gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, it, 1, 2, 0))
+        assert np.array_equal(f.get(), P["fun"]), prec
+    g.destroy()

@@ -2296,7 +2296,9 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
                                d_pout64);
             X2G_HIP(hipGetLastError());
         }
-        if (!i32.empty()) {
+        // (no sweeps: u stays as it is, as in the FP64 path; the correction
+        // arrays are not even cleared then)
+        if (!i32.empty() && maxsor2 > 0) {
             const int n32 = (int)i32.size();
             if ((rc = launch_residual(ctx, d_res, n32, maxnni, maxnmu, maxn, res_prod))) return rc;
             X2G_LAUNCH(ctx, k_zero_f32, dim3(std::min(grid_x_for(maxzero, 256), 128), nzero), dim3(256), 0, d_zero);
