@@ -1117,14 +1117,22 @@ contains
           else
              n1 = n1 + 1; t1(n1) = x2g_term(1.0_dp, js(h)%S(7), js(h)%cpot(i), c_null_ptr)
              call lc_add(lb, js(h)%f1o(i), js(h)%S(5), 1.0_dp, t1, n1, q-1)
-             tc(1) = x2g_term(1.0_dp, js(h)%S(7), js(h)%xpot(i), c_null_ptr)
-             call lc_add(lb, js(h)%f2o(i), c_null_ptr, 0.0_dp, tc, 1, q-1)
+             ! fock2 = S7 xpot: with the FP32 correction the residual forms it
+             ! from the two factors (below)
+             if (js(h)%o%prec /= 1) then
+                tc(1) = x2g_term(1.0_dp, js(h)%S(7), js(h)%xpot(i), c_null_ptr)
+                call lc_add(lb, js(h)%f2o(i), c_null_ptr, 0.0_dp, tc, 1, q-1)
+             endif
           endif
           n1 = n1 + 1; t1(n1) = x2g_term(js(h)%g%diag, js(h)%ones, c_null_ptr, c_null_ptr)
           call lc_add(lb, js(h)%lo(i), js(h)%S(5), 1.0_dp, t1, n1, q-1)
           n = n + 1
           items(n) = x2g_relax_item(js(h)%orb(i), js(h)%lo(i), js(h)%f2o(i), js(h)%f1o(i), js(h)%omega_orb, &
                int(js(h)%c%isym(i), c_int), 0_c_int, int(js(h)%o%prec, c_int), int(q-1, c_int))
+          if (js(h)%o%prec == 1 .and. js(h)%c%nel /= 1) then
+             items(n)%rhs_f1_d = js(h)%S(7)
+             items(n)%rhs_f2_d = js(h)%xpot(i)
+          endif
           js(h)%nsweep_orb = js(h)%nsweep_orb + int(js(h)%o%maxsororb(1)*js(h)%o%maxsororb(2), c_long_long)
        enddo
        ms1 = max(ms1, js(h)%o%maxsororb(1)); ms2 = max(ms2, js(h)%o%maxsororb(2))
