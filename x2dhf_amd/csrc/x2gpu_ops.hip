@@ -237,8 +237,9 @@ __device__ __forceinline__ void moments_body(const MomArgs &A, double *partial)
     const long n = (long)nni * A.nmu;
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const double eps = 2.220446049250313e-16;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        const int mu = (int)(e / nni), nu = (int)(e % nni);
+    // 32-bit index arithmetic (a grid has fewer than 2^31 points)
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < (int)n; e += gridDim.x * blockDim.x) {
+        const int mu = e / nni, nu = e - mu * nni;
         const double rr = sqrt(A.vxisq[mu] + A.vetasq[nu] - 1.0);
         double c = 0.0;
         if (A.dm == 0) { if (!(fabs(rr) < eps)) c = A.vxi[mu] * A.veta[nu] / rr; }
@@ -515,9 +516,10 @@ __global__ void __launch_bounds__(kRedBlock) k_eahf(const EaDesc *__restrict__ e
     const long s1 = nni + 8;
     const long n = (long)nni * nmu;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        const int mu = 1 + (int)(e / nni), nu = 1 + (int)(e % nni);
-        const long p = (long)(mu + 3) * s1 + nu + 3;
+    // 32-bit index arithmetic (a grid has fewer than 2^31 points)
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < (int)n; e += gridDim.x * blockDim.x) {
+        const int mu0 = e / nni, mu = 1 + mu0, nu = 1 + (e - mu0 * nni);
+        const int p = (mu + 3) * (int)s1 + nu + 3;
         const double dn = A.dnu[nu - 1], bm = A.bmu[mu - 1];
         double tn = 0.0, tm = 0.0;
 #pragma unroll

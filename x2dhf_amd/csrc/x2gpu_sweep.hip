@@ -909,9 +909,9 @@ __global__ void k_putin(const PadDesc *__restrict__ pd)
     const int nni = d.nni, nmu = d.nmu, PS = d.ps;
     const long tot = (long)PS * (nmu + 8);
     const double s = d.isym == 1 ? 1.0 : -1.0;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
-         e += (long)gridDim.x * blockDim.x) {
-        const int P = (int)(e / PS), Q = (int)(e - (long)P * PS);
+    // 32-bit index arithmetic (a padded grid has fewer than 2^31 points)
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < (int)tot; e += gridDim.x * blockDim.x) {
+        const int P = e / PS, Q = e - P * PS;
         const int mu = P - 3, nu = Q - 3;   // 1-based grid indices, halo range -3..n+4
         double v = 0.0;
         const bool nu_in = nu >= 1 && nu <= nni, mu_in = mu >= 1 && mu <= nmu;
