@@ -188,6 +188,15 @@ typedef struct {
 int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
                     int maxsor1, int maxsor2);
 
+/* Device workspace bytes (an upper bound) that one relaxation problem on an
+ * nni x nmu grid of precision prec adds to the context's workspace while
+ * x2g_relax_batch runs (padded sweep buffers and the colour-ordered
+ * lhs/rhs pairs); the workspace is shared by the items of a call and kept
+ * between calls.  No device call; -1 for bad sizes.  No reference
+ * counterpart (the reference's per-thread temporaries,
+ * c/relax_single_pot_pthread.c:65-67, are host mallocs). */
+long long x2g_relax_workspace_bytes(int nni, int nmu, int prec);
+
 /* FP64 residual norm^2 sum (rhs - A u)^2 over the relaxed interior of each
  * item, and sum rhs^2 (for relative residuals).  out[2*i], out[2*i+1]. */
 int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,

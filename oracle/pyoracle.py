@@ -29,8 +29,6 @@ def lib():
         L.x2o_residual.argtypes = [_int, _int] + [_vp] * 7
         L.x2o_orb_asympt_get.argtypes = [_int, _int, _vp, _vp, _vp, _vp]
         L.x2o_orb_asympt_set.argtypes = [_int, _int, _vp, _vp]
-        L.x2o_wdot3.argtypes = [C.c_long, _vp, _vp, _vp, _vp]
-        L.x2o_wdot3.restype = _dbl
         _lib = L
     return _lib
 

@@ -256,17 +256,3 @@ void x2o_orb_asympt_set(int nni, int nmu, double *psi, const double *edecay)
             psi[UIDX(i, j, nni)] = psi[UIDX(i, j - 1, nni)] * edecay[(size_t)jj * nni + (i - 1)];
     }
 }
-
-/* Weighted quadrature sum_k w[k]*a[k]*b[k]*c[k] in the order of the
- * reference's ddot over prod2/prod temporaries (utils.f90:29-55, blas.F90:16),
- * used by norm, overlaps and the moments. c may be NULL. */
-double x2o_wdot3(long n, const double *w, const double *a, const double *b, const double *c)
-{
-    double s = 0.0;
-    for (long k = 0; k < n; ++k) {
-        double t = a[k] * b[k];
-        if (c) t = t * c[k];
-        s += w[k] * t;
-    }
-    return s;
-}

@@ -19,7 +19,6 @@ void x2o_residual(int nni, int nmu, const double *work, const double *lhs, const
 void x2o_orb_asympt_get(int nni, int nmu, const double *vxisq, const double *vetasq,
                         const double *fa, double *edecay);
 void x2o_orb_asympt_set(int nni, int nmu, double *psi, const double *edecay);
-double x2o_wdot3(long n, const double *w, const double *a, const double *b, const double *c);
 #ifdef __cplusplus
 }
 #endif

@@ -492,3 +492,25 @@ def test_zero_sweeps_leave_the_function_unchanged(ctx):
         gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, it, 1, 2, 0))
         assert np.array_equal(f.get(), P["fun"]), prec
     g.destroy()
+
+
+@pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
+def test_mcsor_host_dropin_vs_reference(ctx, case):
+    """x2g_mcsor_host -- the call the reference's orbMCSOR makes in place of
+    mcsor (host arrays in and out, relaxOrbs.F90:393-395) -- on the reference's
+    own padded input reproduces its mcsor output."""
+    z = load_golden(case)
+    g = make_grid(ctx, z)
+    isym = int(z["k1_isym"][0])
+    omega, nsw = float(z["k1_scal"][0]), int(z["k1_scal"][1])
+    for name, ns in (("wout_mcsor", nsw), ("wout_mcsor1", 1)):
+        work = np.ascontiguousarray(z["k1_win"], dtype=np.float64).copy()
+        lhs = np.ascontiguousarray(z["k1_lhs"], dtype=np.float64)
+        rhs = np.ascontiguousarray(z["k1_rhs"], dtype=np.float64)
+        rc = ctx.lib.x2g_mcsor_host(g.h, isym, 1, work.ctypes.data, lhs.ctypes.data, rhs.ctypes.data,
+                                    omega, ns)
+        assert rc == 0, ctx.lib.x2g_last_error()
+        e = relerr(work, z["k1_" + name])
+        print(f"{case} x2g_mcsor_host {ns} sweeps: rel err {e:.2e}")
+        assert e < REL64, (name, e)
+    g.destroy()

@@ -131,3 +131,65 @@ def test_oracle_boundary_lines_even_odd():
     pyoracle.lib().x2o_boundary(nni, nmu, -1, pyoracle._p(h), pyoracle._p(ex))
     V = h.reshape(nmu + 8, nni + 8)
     assert np.all(V[4, 5:nni + 3] == 0) and np.all(V[4:nmu, 4] == 0) and np.all(V[4:nmu, nni + 3] == 0)
+
+
+def _f32_vs_f64(z, pfx, nsweeps, bsign=1.0):
+    """Relative max difference between the FP32 twin and the FP64 oracle (itself
+    bit-exact against the reference's mcsor) after nsweeps sweeps from the same
+    start; bsign = -1 plants a sign error in the FP32 twin's b(mu) d/dmu term."""
+    nni, nmu = int(z["sizes"][0]), int(z["sizes"][1])
+    isym = int(z[pfx + "isym"][0])
+    omega = float(z[pfx + "scal"][0])
+    b, d, coef, ex = z["suppl"][1], z["suppl"][2], z["coef"][:16], z["coef"][16:21]
+    w64 = pyoracle.mcsor(nni, nmu, isym, z[pfx + "win"].copy(), z[pfx + "lhs"], z[pfx + "rhs"], b, d, coef, ex,
+                         omega, nsweeps, 0)
+    w32 = pyoracle.mcsor_f32(nni, nmu, isym, z[pfx + "win"].astype(np.float32), z[pfx + "lhs"], z[pfx + "rhs"],
+                             bsign * b, d, coef, ex, omega, nsweeps)
+    return np.abs(w32.astype(np.float64) - w64).max() / np.abs(w64).max()
+
+
+@pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
+@pytest.mark.parametrize("pfx", ["k1_", "k2_"])
+def test_oracle_f32_twin_tracks_pinned_f64_sweep(case, pfx):
+    """x2o_mcsor_f32 (the checker of the FP32 correction sweeps) follows the
+    FP64 oracle to FP32 rounding after 1 and 10 sweeps on the reference's own
+    inputs; the same comparison with a sign error planted in its b(mu) term
+    (the oracle on the CPU, never the product) fails by orders of magnitude."""
+    z = load_golden(case)
+    if pfx + "win" not in z:
+        pytest.skip("case has a single potential")
+    e1, e10 = _f32_vs_f64(z, pfx, 1), _f32_vs_f64(z, pfx, 10)
+    # one sweep: a few FP32 roundings of O(1) terms; ten sweeps at omega = 1.95
+    # amplify them (measured up to 4.6e-5 on the H2 Coulomb problem)
+    assert e1 < 1e-5, e1
+    assert e10 < 2e-4, e10
+    assert _f32_vs_f64(z, pfx, 1, -1.0) > 1e-2
+    assert _f32_vs_f64(z, pfx, 10, -1.0) > 1e-2
+
+
+def test_oracle_f32_twin_fixed_point():
+    """Many FP32 twin sweeps reach the FP64 oracle's fixed point to within the
+    FP32 conditioning of the operator (2.5e-4 relative measured on the H2
+    Coulomb problem, whose FP64 residual there is 4e-12 relative), and the
+    pinned FP64 residual of the FP32 iterate is at FP32 level; with a sign
+    error planted in the twin's b(mu) term the iteration does not get there."""
+    z = load_golden("h2_61x91")
+    nni, nmu = int(z["sizes"][0]), int(z["sizes"][1])
+    b, d, coef, ex = z["suppl"][1], z["suppl"][2], z["coef"][:16], z["coef"][16:21]
+    lhs, rhs = z["k1_lhs"], z["k1_rhs"]
+    w64 = pyoracle.mcsor(nni, nmu, 1, z["k1_win"].copy(), lhs, rhs, b, d, coef, ex, 1.95, 1500, 0)
+    r64 = pyoracle.residual(nni, nmu, w64, lhs, rhs, b, d, coef)
+    assert np.linalg.norm(r64) < 1e-10 * np.linalg.norm(rhs)
+
+    def run(bsign):
+        with np.errstate(all="ignore"):
+            w = pyoracle.mcsor_f32(nni, nmu, 1, z["k1_win"].astype(np.float32), lhs, rhs, bsign * b, d, coef,
+                                   ex, 1.95, 1500).astype(np.float64)
+            r = pyoracle.residual(nni, nmu, w, lhs, rhs, b, d, coef)
+            return (np.abs(w - w64).max() / np.abs(w64).max(), np.linalg.norm(r) / np.linalg.norm(rhs))
+
+    dw, res = run(1.0)
+    assert dw < 1e-3 and res < 1e-2, (dw, res)
+    dw_bad, res_bad = run(-1.0)
+    assert not (dw_bad < 1e-1), dw_bad
+    assert not (res_bad < 1e-1), res_bad

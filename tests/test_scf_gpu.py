@@ -137,3 +137,20 @@ def test_jobs_give_the_same_result_alone_and_in_a_mix():
         assert a["iterations"] == m["iterations"]
         assert a["etot"] == m["etot"]
         assert np.array_equal(a["eps"], m["eps"])
+
+
+def test_n2_config2_331x601_matches_reference_trajectory_and_energy():
+    """Configuration 2 at its stated size: the reference's N2 deck on the
+    331x601 grid (22 potentials and 6 orbitals per iteration) through the
+    plain FP64 schedule follows the reference program's orbital energies
+    iteration by iteration and stops at its total energy within 1e-10 Eh."""
+    from x2dhf_amd import deck
+    ref = golden("config2_n2_331x601")
+    spec = deck.load(os.path.join(GOLDEN, "decks", "config2_n2_331x601.data")).spec(prec=0, schedule=0)
+    res, worst = run_plain(spec, ref, check_iters=40)
+    print(f"N2 331x601: E = {res['etot']:.14f} (ref {ref['etot']:.14f}) dE = {res['etot'] - ref['etot']:.2e}, "
+          f"iterations {res['iterations']} (ref {len(ref['iterations'])}), trajectory rel err {worst:.2e}, "
+          f"stop '{res['stop']}'")
+    assert worst < 1e-9
+    assert abs(res["etot"] - ref["etot"]) < TOL_E
+    assert np.abs(res["eps"] - np.array(ref["eps"])).max() < 1e-9

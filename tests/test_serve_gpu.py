@@ -66,7 +66,7 @@ def test_admission_respects_the_memory_budget():
     admitted footprint never exceeds the budget."""
     from x2dhf_amd import scf, serve
     specs = _specs()[:3]                       # N2 91x121, LiH 61x91, H2 61x91
-    fp = [int(serve.WORKSPACE_FACTOR * scf.spec_bytes(s)) for s in specs]
+    fp = [serve.footprint(s) for s in specs]
     assert fp[0] + fp[1] > fp[0] and fp[1] + fp[2] <= fp[0]
     clk = StepClock()
     srv = serve.Server(max_active=8, hbm_budget=fp[0], clock=clk)

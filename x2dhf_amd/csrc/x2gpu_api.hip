@@ -20,18 +20,27 @@ void set_error(const char *fmt, ...)
 int ws_reserve(x2g_ctx *ctx, int k, size_t bytes)
 {
     if (ctx->ws[k].bytes >= bytes) return X2G_OK;
-    // earlier launches may still read the old buffer
-    X2G_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->ws[k].ptr) X2G_HIP(hipFree(ctx->ws[k].ptr));
-    ctx->ws[k].ptr = nullptr;
-    ctx->ws[k].bytes = 0;
-    const size_t want = std::max(bytes, ctx->ws[k].bytes + ctx->ws[k].bytes / 2);
-    if (hipMalloc(&ctx->ws[k].ptr, want) != hipSuccess) {
+    // grow geometrically (a batch that grows one admission at a time then
+    // pays few reallocations); the old buffer stays until the new one exists,
+    // so a failed growth leaves the workspace as it was
+    const size_t grown = ctx->ws[k].bytes + ctx->ws[k].bytes / 2;
+    void *p = nullptr;
+    size_t want = std::max(bytes, grown);
+    if (hipMalloc(&p, want) != hipSuccess) {
         (void)hipGetLastError();
-        set_error("workspace %d: hipMalloc(%zu) failed", k, want);
-        ctx->ws[k].ptr = nullptr;
-        return X2G_ENOMEM;
+        want = bytes;  // the exact size may still fit
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("workspace %d: hipMalloc(%zu) failed", k, bytes);
+            return X2G_ENOMEM;
+        }
     }
+    // earlier launches may still read the old buffer
+    if (ctx->ws[k].ptr) {
+        X2G_HIP(hipStreamSynchronize(ctx->stream));
+        X2G_HIP(hipFree(ctx->ws[k].ptr));
+    }
+    ctx->ws[k].ptr = p;
     ctx->ws[k].bytes = want;
     return X2G_OK;
 }

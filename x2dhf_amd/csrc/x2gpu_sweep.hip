@@ -1037,7 +1037,8 @@ struct ResDesc {
     float *res_f;          // FP32 residual (may be null)
     int res_split;         // res_f holds colour-ordered (omega/lhs, r) pairs (else r, unpadded nni*nmu)
     double omega;          // for the pairs
-    double *acc;           // [2] per problem: sum res^2, sum rhs^2 (may be null)
+    double *part;          // per-wave partial sums (res^2, rhs^2) of this problem at
+                           // [2 * (block of the problem * waves per block + wave)] (may be null)
     double c[16];
     int nni, nmu, isym;
 };
@@ -1117,15 +1118,16 @@ __global__ void __launch_bounds__(256) k_residual(const ResDesc *__restrict__ rd
             }
         }
     }
-    if (d.acc) {
-        // wavefront reduction then one atomic per wavefront (diagnostic only)
+    if (d.part) {
+        // fixed-order wavefront reduction; each wave owns one partial slot
         for (int off = 32; off > 0; off >>= 1) {
             sr += __shfl_down(sr, off, 64);
             sb += __shfl_down(sb, off, 64);
         }
-        if ((threadIdx.x & 63) == 0 && (sr != 0.0 || sb != 0.0)) {
-            atomicAdd(&d.acc[0], sr);
-            atomicAdd(&d.acc[1], sb);
+        if ((threadIdx.x & 63) == 0) {
+            const int slot = (lb % blocks_per_problem) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+            d.part[2 * slot] = sr;
+            d.part[2 * slot + 1] = sb;
         }
     }
 }
@@ -1271,15 +1273,36 @@ __global__ void __launch_bounds__(kResThreads) k_residual_rows(const ResDesc *__
             mp[k] = mn[k];
         }
     }
-    if (d.acc) {
+    if (d.part) {
         for (int off = 32; off > 0; off >>= 1) {
             sr += __shfl_down(sr, off, 64);
             sb += __shfl_down(sb, off, 64);
         }
-        if ((threadIdx.x & 63) == 0 && (sr != 0.0 || sb != 0.0)) {
-            atomicAdd(&d.acc[0], sr);
-            atomicAdd(&d.acc[1], sb);
+        if ((threadIdx.x & 63) == 0) {
+            const int slot = (blockIdx.x % strips) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+            d.part[2 * slot] = sr;
+            d.part[2 * slot + 1] = sb;
         }
+    }
+}
+
+// out[2 i], out[2 i + 1] = the sums of problem i's nslot partial pairs, in a
+// fixed order (one wave per problem: lane-strided sums, then a fixed tree).
+__global__ void __launch_bounds__(64) k_sum_parts(const double *__restrict__ part, int nslot, double *__restrict__ out)
+{
+    const double *p = part + (size_t)2 * nslot * blockIdx.x;
+    double sr = 0.0, sb = 0.0;
+    for (int k = threadIdx.x; k < nslot; k += 64) {
+        sr += p[2 * k];
+        sb += p[2 * k + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_down(sr, off, 64);
+        sb += __shfl_down(sb, off, 64);
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = sr;
+        out[2 * blockIdx.x + 1] = sb;
     }
 }
 
@@ -2218,7 +2241,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             r.res_f = kind32 == 2 ? f32p(i, 2) : f32p(i, 3);
             r.res_split = kind32 == 2;  // k_sweep: (omega/lhs, r) pairs, formed here each macro-iteration
             r.omega = it.omega;
-            r.acc = nullptr;
+            r.part = nullptr;
             for (int q = 0; q < 16; ++q) r.c[q] = g->coef[q];
             r.nni = g->nni;
             r.nmu = g->nmu;
@@ -2319,16 +2342,39 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     return X2G_OK;
 }
 
+extern "C" long long x2g_relax_workspace_bytes(int nni, int nmu, int prec)
+{
+    if (nni < 7 || nmu < 11 || (prec != 0 && prec != 1)) return -1;
+    // the sizes x2g_relax_batch reserves per item, with the widest row
+    // stride any sweep kind uses (nni+8 rounded up to 16 bytes, plus 16)
+    const size_t elem = prec ? sizeof(float) : sizeof(double);
+    const size_t stride = (size_t)(nni + 8 + 16 / elem - 1) / (16 / elem) * (16 / elem) + 16;
+    const size_t pad = align256(stride * (nmu + 8) * elem);
+    const size_t n = (size_t)nni * nmu, ns = (size_t)split_size(nni, nmu);
+    if (prec == 0) return (long long)(2 * pad + align256(2 * ns * sizeof(double)));
+    return (long long)(2 * pad + 2 * align256(std::max(n, ns) * sizeof(float)));
+}
+
 extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
                                   double *out_h)
 {
     X2G_CHECK(ctx && grids && items && out_h && n >= 0, "x2g_residual_norms: null argument");
     if (n == 0) return X2G_OK;
-    int rc = ws_reserve(ctx, 1, 2 * n * sizeof(double));
-    if (rc) return rc;
-    double *acc = (double *)ctx->ws[1].ptr;
-    std::vector<ResDesc> res;
     long maxn = 0, maxnni = 0, maxnmu = 0;
+    for (int i = 0; i < n; ++i) {
+        X2G_CHECK(items[i].grid_index >= 0 && grids[items[i].grid_index], "x2g_residual_norms: bad grid index");
+        const x2g_grid *g = grids[items[i].grid_index];
+        maxn = std::max<long>(maxn, (long)g->n());
+        maxnni = std::max<long>(maxnni, g->nni);
+        maxnmu = std::max<long>(maxnmu, g->nmu);
+    }
+    // partial-sum slots per problem: one per wave of the residual launch
+    const long nslot = maxnni <= kResThreads ? (maxnmu - 5 + kResRows - 1) / kResRows * ((maxnni + 63) / 64)
+                                             : (maxn + 255) / 256 * 4;
+    int rc = ws_reserve(ctx, 1, (size_t)(2 * nslot * n + 2 * n) * sizeof(double));
+    if (rc) return rc;
+    double *part = (double *)ctx->ws[1].ptr, *acc = part + 2 * nslot * n;
+    std::vector<ResDesc> res;
     for (int i = 0; i < n; ++i) {
         const x2g_relax_item &it = items[i];
         X2G_CHECK(it.grid_index >= 0 && grids[it.grid_index], "x2g_residual_norms: bad grid index");
@@ -2345,22 +2391,22 @@ extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2
         r.bmu = it.bkind == 0 ? g->bmu_orb : g->bmu_pot;
         r.dnu = g->dnu;
         r.res_f = nullptr;
-        r.acc = acc + 2 * i;
+        r.part = part + 2 * nslot * i;
         for (int q = 0; q < 16; ++q) r.c[q] = g->coef[q];
         r.nni = g->nni;
         r.nmu = g->nmu;
         res.push_back(r);
-        maxn = std::max<long>(maxn, (long)g->n());
-        maxnni = std::max<long>(maxnni, g->nni);
-        maxnmu = std::max<long>(maxnmu, g->nmu);
     }
     size_t doff = 0;
     const ResDesc *d_res;
     if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
-    if ((rc = x2g_fill(ctx, acc, 2 * (size_t)n, 0.0))) return rc;
+    // slots of strips beyond a smaller problem's rows are not written
+    if ((rc = x2g_fill(ctx, part, 2 * (size_t)nslot * n, 0.0))) return rc;
     bool prod = false;
     for (auto &r : res) prod = prod || r.rf[0];
     if ((rc = launch_residual(ctx, d_res, n, maxnni, maxnmu, maxn, prod))) return rc;
+    X2G_LAUNCH(ctx, k_sum_parts, dim3(n), dim3(64), 0, part, (int)nslot, acc);
+    X2G_HIP(hipGetLastError());
     X2G_HIP(hipMemcpyAsync(out_h, acc, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     X2G_HIP(hipStreamSynchronize(ctx->stream));
     return X2G_OK;

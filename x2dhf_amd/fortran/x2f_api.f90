@@ -120,6 +120,57 @@ contains
     x2f_spec_bytes = 8_c_long_long*n*int(7*c%norb + c%nslots + 2*npotall + 2 + 14, c_long_long)
   end function x2f_spec_bytes
 
+  ! Device workspace bytes a job of this specification adds to its context
+  ! while it is relaxed (x2g_relax_workspace_bytes of its largest relaxation
+  ! call, the potentials, plus the padded copies of the batched quadratures
+  ! and their partial sums), without creating it; -1 as x2f_spec_bytes.
+  integer(c_long_long) function x2f_spec_workspace_bytes(spec) bind(c, name='x2f_spec_workspace_bytes')
+    type(x2f_spec), intent(in) :: spec
+    type(case_t) :: c
+    integer :: i, ierr, npotall
+    integer(c_long_long) :: per, npad
+    x2f_spec_workspace_bytes = -1
+    if (x2f_spec_bytes(spec) < 0) return
+    c = case_t()
+    c%norb = spec%norb
+    c%z1 = spec%z1; c%z2 = spec%z2; c%r = spec%r
+    do i = 1, min(int(spec%norb), MAXO)
+       c%m(i) = spec%m(i); c%gu(i) = spec%gu(i); c%occ(i) = spec%occ(i)
+    enddo
+    ierr = case_setup(c)
+    if (ierr /= 0) return
+    npotall = sum(c%npot(1:c%norb))
+    per = x2g_relax_workspace_bytes(spec%nni, spec%nmu, spec%prec)
+    if (per < 0) return
+    npad = int(spec%nni + 8, c_long_long)*int(spec%nmu + 8, c_long_long)
+    x2f_spec_workspace_bytes = int(max(npotall, int(spec%norb)), c_long_long)*per + &
+         int(spec%norb, c_long_long)*(8_c_long_long*npad + 65536_c_long_long)
+  end function x2f_spec_workspace_bytes
+
+  ! The SCF loop's host bookkeeping (energy_change, convergence_control) fed
+  ! with given orbital energies ee(norb, nit) and norms(norb, nit), starting
+  ! from energies ee0(norb); no device call.  Returns the stop code (0 still
+  ! running after nit iterations, else as x2f_result's) in stopcode and the
+  ! iteration it stopped at in iters.
+  integer(c_int) function x2f_control_probe(norb, nit, ee, norms, ee0, maxscf, ienterm, inoterm, nscf2skip, &
+       stopcode, iters) bind(c, name='x2f_control_probe') result(rc)
+    integer(c_int), value :: norb, nit, maxscf, ienterm, inoterm, nscf2skip
+    real(c_double), intent(in) :: ee(norb, nit), norms(norb, nit), ee0(norb)
+    integer(c_int), intent(out) :: stopcode, iters
+    type(scf_opts) :: o
+    integer :: sc, it
+    rc = 1
+    stopcode = 0; iters = 0
+    if (norb < 1 .or. norb > MAXO .or. nit < 1) return
+    o = scf_opts()
+    o%maxscf = maxscf; o%ienterm = ienterm; o%inoterm = inoterm
+    o%ienterm4init = 1
+    if (nscf2skip > 0) o%nscf2skip = nscf2skip
+    call control_probe(int(norb), int(nit), ee, norms, ee0, o, sc, it)
+    stopcode = int(sc, c_int); iters = int(it, c_int)
+    rc = 0
+  end function x2f_control_probe
+
   integer(c_int) function create_job(spec, ctx_in, handle) result(rc)
     type(x2f_spec), intent(in) :: spec
     type(c_ptr), intent(in) :: ctx_in

@@ -196,6 +196,12 @@ module x2f_capi
        integer(c_int), value :: n, maxsor1, maxsor2
      end function x2g_relax_batch
 
+     integer(c_long_long) function x2g_relax_workspace_bytes(nni, nmu, prec) &
+          bind(c, name='x2g_relax_workspace_bytes')
+       import :: c_int, c_long_long
+       integer(c_int), value :: nni, nmu, prec
+     end function x2g_relax_workspace_bytes
+
      integer(c_int) function x2g_lincomb(g, out, base, base_scale, terms, nterms) &
           bind(c, name='x2g_lincomb')
        import :: c_int, c_ptr, c_double, x2g_term

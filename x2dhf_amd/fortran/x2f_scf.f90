@@ -25,13 +25,14 @@
 !     equations, a Jacobi instead of Gauss-Seidel order across orbitals.
 module x2f_scf
   use, intrinsic :: iso_c_binding
+  use, intrinsic :: ieee_arithmetic, only: ieee_is_nan
   use x2f_capi
   use x2f_grid
   use x2f_case
   implicit none
   private
   public :: scf_job, scf_opts, job_init, job_free, prep_scf, prep_scf_restart, scf_iteration, scf_finish, &
-       batched_iteration, MAXPOT, job_bytes
+       batched_iteration, MAXPOT, job_bytes, control_probe
 
   integer, parameter :: dp = c_double
   integer, parameter :: MAXPOT = MAXO*(MAXO+1) + MAXO   ! exchange slots + Coulomb
@@ -42,6 +43,9 @@ module x2f_scf
      integer :: mpole = 4, prec = 0, schedule = 0
      integer :: nscf2skip = 600, nenlast = 20, nnolast = 20, maxresets = 5
      real(dp) :: omega_orb = -1.0_dp, omega_pot = -1.0_dp, recalc_mm = 1.15_dp
+     ! orbEnergyIncThld: an orbital-energy change above it is divergence
+     ! (initVariables.F90:187-191)
+     real(dp) :: orb_energy_inc_thld = 1.0e4_dp
   end type scf_opts
 
   type :: lc_batch
@@ -83,7 +87,8 @@ module x2f_scf
      logical :: lenergythld = .false., lnormthld = .false., done = .false.
      real(dp) :: eeconv = 10, eeconvprev = 1.e35_dp, dnmaxprev = 1.e35_dp
      real(dp) :: deltaeemax = 100, deltanormmax = 100
-     integer :: stopcode = 0       ! 1 energy, 2 norm, 3 iteration limit, 4 saturation, 5 resets
+     integer :: stopcode = 0       ! 1 energy, 2 norm, 3 iteration limit, 4 saturation, 5 resets,
+                                   ! 6 divergence, 7 not a number
      integer(c_long_long) :: nsweep_orb = 0, nsweep_pot = 0
      ! device record written by the batched schedule: <psi|psi> before
      ! normalisation (1..MAXO) and orbital energies (MAXO+1..2*MAXO)
@@ -889,6 +894,22 @@ contains
     enddo
     if (abs(j%eeconv) < eps0) j%eeconv = j%deltaeemax
 
+    ! failure guards.  A NaN anywhere in an orbital or in a potential it sees
+    ! reaches that orbital's norm or energy, which the device quadratures
+    ! form over every grid point (the reference can only report one,
+    ! detectNaN detect.F90:9-25, relaxOrbs.F90:379-405); an orbital-energy
+    ! change above orbEnergyIncThld is divergence (the test of
+    ! doSCF.F90:612-616, applied to the largest change of this iteration).
+    ! Either retires the job instead of iterating to maxscf.
+    do nei = 1, j%c%norb
+       if (ieee_is_nan(j%ee(nei,nei)) .or. ieee_is_nan(j%orbnorm(nei)) .or. ieee_is_nan(j%deltaee(nei))) then
+          j%stopcode = 7; stop_now = .true.; return
+       endif
+    enddo
+    if (abs(j%deltaeemax) > j%o%orb_energy_inc_thld) then
+       j%stopcode = 6; stop_now = .true.; return
+    endif
+
     ! saturation (no decrease for nenlast/nnolast iterations after nscf2skip)
     if (j%iscf > j%o%nscf2skip) then
        if (abs(j%deltaeemax) >= abs(j%eeconvprev)) then
@@ -1225,4 +1246,40 @@ contains
     call mm_refresh(js, idx, nj)
     deallocate(items, tails, grids, no, ea, recs, cnt, recq, rech)
   end subroutine batched_iteration
+
+  ! The host bookkeeping of the SCF loop (energy_change + convergence_control)
+  ! driven by given per-iteration orbital energies ee(norb, nit) and norms
+  ! norms(norb, nit) instead of device results; no device call.  Returns the
+  ! stop code (0: still running after nit iterations) and the iteration it
+  ! stopped at.  Used to test the stop rules and the failure guards.
+  subroutine control_probe(norb, nit, ee, norms, ee0, o, stopcode, iters)
+    integer, intent(in) :: norb, nit
+    real(dp), intent(in) :: ee(norb, nit), norms(norb, nit), ee0(norb)
+    type(scf_opts), intent(in) :: o
+    integer, intent(out) :: stopcode, iters
+    type(scf_job) :: j
+    integer :: it, i
+    j%o = o
+    j%c%norb = norb
+    j%iscf = 0
+    do i = 1, norb
+       j%ee(i,i) = ee0(i)
+       j%eeold(i) = ee0(i)
+    enddo
+    stopcode = 0
+    iters = 0
+    do it = 1, nit
+       j%iscf = it
+       do i = norb, 1, -1
+          j%orbnorm(i) = norms(i, it)
+          j%ee(i,i) = ee(i, it)
+          call energy_change(j, i)
+       enddo
+       iters = it
+       if (convergence_control(j)) then
+          stopcode = j%stopcode
+          return
+       endif
+    enddo
+  end subroutine control_probe
 end module x2f_scf

@@ -105,6 +105,7 @@ _PROTOS = {
     "x2g_mcsor_host": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
     "x2g_relax_batch": (_int, [_vp, _vp, C.POINTER(RelaxItem), _int, _int, _int]),
     "x2g_residual_norms": (_int, [_vp, _vp, C.POINTER(RelaxItem), _int, _vp]),
+    "x2g_relax_workspace_bytes": (C.c_longlong, [_int, _int, _int]),
     "x2g_lincomb": (_int, [_vp, _vp, _vp, _dbl, C.POINTER(Term), _int]),
     "x2g_wdot": (_int, [_vp, C.POINTER(Dot), _int, _vp]),
     "x2g_oneel": (_int, [_vp, _vp, _vp, _int, _int, _vp]),

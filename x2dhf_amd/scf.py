@@ -42,7 +42,7 @@ class JobResult(C.Structure):
 
 
 STOPCODES = {0: "running", 1: "orbital energy threshold", 2: "orbital norm threshold", 3: "iteration limit",
-             4: "saturation", 5: "resets"}
+             4: "saturation", 5: "resets", 6: "diverged", 7: "not a number"}
 
 _lib = None
 
@@ -84,6 +84,10 @@ def load():
     L.x2f_job_create_ctx.restype = _i
     L.x2f_spec_bytes.argtypes = [C.POINTER(JobSpec)]
     L.x2f_spec_bytes.restype = C.c_longlong
+    L.x2f_spec_workspace_bytes.argtypes = [C.POINTER(JobSpec)]
+    L.x2f_spec_workspace_bytes.restype = C.c_longlong
+    L.x2f_control_probe.argtypes = [_i, _i, vp, vp, vp, _i, _i, _i, _i, C.POINTER(_i), C.POINTER(_i)]
+    L.x2f_control_probe.restype = _i
     _lib = L
     return L
 
@@ -190,6 +194,32 @@ def spec_bytes(spec):
     if b < 0:
         raise ValueError("unsupported job specification")
     return b
+
+
+def spec_workspace_bytes(spec):
+    """Device workspace bytes a job of this specification adds to its context
+    while it is relaxed (the library's relaxation and quadrature buffers)."""
+    b = int(load().x2f_spec_workspace_bytes(C.byref(spec)))
+    if b < 0:
+        raise ValueError("unsupported job specification")
+    return b
+
+
+def control_probe(ee, norms, ee0, maxscf=1000, ienterm=10, inoterm=8, nscf2skip=0):
+    """Run the SCF loop's host bookkeeping (energy_change and
+    convergence_control of the Fortran driver) on given per-iteration orbital
+    energies ee[nit, norb] and norms[nit, norb], starting from energies ee0;
+    no device work.  Returns (stop reason, iteration it stopped at)."""
+    ee = np.asfortranarray(np.asarray(ee, dtype=np.float64).T)
+    nm = np.asfortranarray(np.asarray(norms, dtype=np.float64).T)
+    e0 = np.ascontiguousarray(ee0, dtype=np.float64)
+    norb, nit = ee.shape
+    sc, it = _i(), _i()
+    rc = load().x2f_control_probe(norb, nit, ee.ctypes.data, nm.ctypes.data, e0.ctypes.data, maxscf, ienterm,
+                                  inoterm, nscf2skip, C.byref(sc), C.byref(it))
+    if rc != 0:
+        raise ValueError("x2f_control_probe: bad sizes")
+    return STOPCODES.get(sc.value, str(sc.value)), it.value
 
 
 class Job:

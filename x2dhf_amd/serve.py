@@ -12,11 +12,15 @@ memory are free, so new work joins the running batch instead of waiting for
 it to drain.
 
 Admission by footprint: a job is admitted when
-    held + footprint(job) <= budget,  footprint = WORKSPACE_FACTOR * spec_bytes,
-where spec_bytes counts the job's own arrays (x2f_spec_bytes) and the factor
-covers its share of the library's relaxation workspace (two padded copies and
-two residue-split lhs/rhs per potential); budget defaults to a fraction of
-the device memory free when the server starts.
+    held + footprint(job) <= budget,
+    footprint = spec_bytes + WORKSPACE_GROWTH * workspace_bytes,
+where spec_bytes counts the job's own arrays (x2f_spec_bytes) and
+workspace_bytes its share of the library's relaxation and quadrature
+workspace (x2f_spec_workspace_bytes, from x2g_relax_workspace_bytes), with
+room for the workspace's geometric growth; budget defaults to a fraction of
+the device memory free when the server starts.  A request whose set-up runs
+out of device memory all the same goes back to the head of the queue and
+waits for a running job to retire.
 
 Every job's result is independent of which other jobs share its batch: the
 sweeps, quadratures and linear combinations of a job read and write only that
@@ -27,7 +31,8 @@ from dataclasses import dataclass, field
 
 from . import gpu, scf
 
-WORKSPACE_FACTOR = 2.0
+# the workspace grows by 1.5x steps, so a batch may hold up to 1.5x its sum
+WORKSPACE_GROWTH = 1.5
 
 
 @dataclass
@@ -66,6 +71,7 @@ class Server:
         self.done = []
         self.held = 0
         self.iterations = 0
+        self.deferred = 0      # admissions put back for lack of device memory
         self._next_id = 0
 
     def now(self):
@@ -75,7 +81,7 @@ class Server:
         """Queue a request arriving at `arrival` seconds on the server clock
         (default: now).  Raises ValueError for an unsupported specification or
         one that could never fit the budget."""
-        fp = int(WORKSPACE_FACTOR * scf.spec_bytes(spec))
+        fp = footprint(spec)
         if fp > self.budget:
             raise ValueError(f"request needs {fp} bytes, more than the budget {self.budget}")
         r = Request(self._next_id, spec, self.now() if arrival is None else float(arrival), fp)
@@ -91,8 +97,21 @@ class Server:
             if self.held + r.footprint > self.budget:
                 break
             self.pending.pop(0)
-            r.job = scf.Job(r.spec, ctx=self.ctx)
-            r.job.prep()
+            try:
+                r.job = scf.Job(r.spec, ctx=self.ctx)
+                r.job.prep()
+            except (RuntimeError, ValueError) as e:
+                # out of device memory after all: back to the head of the
+                # queue until a running job retires (with nothing running,
+                # the request can never fit and fails)
+                if r.job is not None:
+                    r.job.close()
+                    r.job = None
+                if not self.active:
+                    raise RuntimeError(f"request {r.rid} cannot be set up on an idle device: {e}") from e
+                self.pending.insert(0, r)
+                self.deferred += 1
+                break
             r.admitted = self.now()
             self.held += r.footprint
             self.active.append(r)
@@ -142,6 +161,12 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+def footprint(spec):
+    """Device bytes reserved for a job while it is admitted: its own arrays
+    and its share of the library workspace (with room for growth)."""
+    return int(scf.spec_bytes(spec) + WORKSPACE_GROWTH * scf.spec_workspace_bytes(spec))
 
 
 # closed-shell diatomics of a served stream: (name, spec builder, equilibrium R)
