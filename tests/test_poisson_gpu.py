@@ -54,3 +54,23 @@ def test_mixed_precision_batch_reaches_fp64_solution():
             b.close()
     for a, m in zip(sols[0], sols[1]):
         assert np.abs(a - m).max() < 1e-8 * np.abs(a).max()
+
+
+def test_config4_256_problems_reach_tolerance_or_fp64_floor():
+    """Configuration 4 at its stated size: the 256 seeded problems on the four
+    grids, FP32 sweeps with FP64 refinement, relaxed together.  Every problem
+    reaches max(1e-12, FLOOR_FACTOR x its FP64 floor) -- the residual that the
+    rounding of its own FP64 solution causes (x2dhf_amd.poisson; the floor
+    exceeds 1e-12 for most of these problems, see
+    tests/test_poisson_floor.py).  A problem that stalls above that does not
+    count as solved, and fails this test."""
+    from x2dhf_amd import poisson
+    print("256 problems, prec=1", flush=True)
+    s = poisson.run(256, seed=0, prec=1, tol=1e-12)
+    res, fl = np.array(s["residuals"]), np.array(s["floors"])
+    above = res > 1e-12
+    ratio = (res[above] / fl[above]).max() if above.any() else 0.0
+    print(f"solve {s['solve_s']:.2f} s: solved {s['solved']}/256, at 1e-12 {s['at_tol']}, max residual "
+          f"{res.max():.2e}, worst residual/floor {ratio:.2f}, floors {fl[above].min() if above.any() else 0:.1e}.."
+          f"{fl.max():.1e}")
+    assert s["solved"] == 256

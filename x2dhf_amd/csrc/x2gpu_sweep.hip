@@ -1053,6 +1053,20 @@ __device__ __forceinline__ int xcd_block(int b, int total)
     return x * per + min(x, rem) + slot;
 }
 
+// The residual is formed with the second differences taken on u - u(p):
+//   L u + lhs u = sum_k c2_k ((u_-k - u0) + (u_+k - u0)) [mu and nu]
+//                 + b d1 u + d d1 u + (lhs + 2 sum_k c2_k) u0,
+// the same operator (the 9-point second-difference weights sum to minus the
+// diagonal built into lhs), but the large terms c2_k u ~ u / h^2 that cancel
+// against lhs u are no longer rounded: neighbour differences of a smooth
+// function are exact (Sterbenz) and small, and lhs + 2 sum c2_k is a small
+// exact difference.  The FP64 rounding of the residual then stays well
+// below the rounding floor of u itself.  Returns 2 sum_k c2_k over mu and nu.
+__device__ __forceinline__ double residual_diag_shift(const double *c)
+{
+    return 2.0 * ((c[4] + c[5]) + (c[6] + c[7])) + 2.0 * ((c[12] + c[13]) + (c[14] + c[15]));
+}
+
 // One point per thread; blocks_per_problem blocks of 256 points per problem
 // (a 1-D grid of blocks_per_problem * problems workgroups).
 __global__ void __launch_bounds__(256) k_residual(const ResDesc *__restrict__ rd, int blocks_per_problem)
@@ -1068,30 +1082,32 @@ __global__ void __launch_bounds__(256) k_residual(const ResDesc *__restrict__ rd
             const double s = d.isym == 1 ? 1.0 : -1.0;
             const double *f = d.fun, *c = d.c;
             double cc;
-            const double bm = d.bmu[mu - 1], dn = d.dnu[nu - 1];
+            const double bm = d.bmu[mu - 1], dn = d.dnu[nu - 1], u0 = f[e];
+            // second differences on u - u0, as in k_residual_rows
             if (nu >= 5 && nu <= nni - 4 && mu >= 5) {
                 // no mirror images: neighbours straight from u
                 const double *q = f + e;
                 const long s1 = nni, s2 = 2L * nni, s3 = 3L * nni, s4 = 4L * nni;
-                const double ddmi2 = c[4] * (q[-s4] + q[s4]) + c[5] * (q[-s3] + q[s3]) + c[6] * (q[-s2] + q[s2]) +
-                                     c[7] * (q[-s1] + q[s1]);
+                const double ddmi2 = c[4] * ((q[-s4] - u0) + (q[s4] - u0)) + c[5] * ((q[-s3] - u0) + (q[s3] - u0)) +
+                                     c[6] * ((q[-s2] - u0) + (q[s2] - u0)) + c[7] * ((q[-s1] - u0) + (q[s1] - u0));
                 const double ddmi1 = c[0] * (q[-s4] - q[s4]) + c[1] * (q[-s3] - q[s3]) + c[2] * (q[-s2] - q[s2]) +
                                      c[3] * (q[-s1] - q[s1]);
-                const double ddni2 = c[12] * (q[-4] + q[4]) + c[13] * (q[-3] + q[3]) + c[14] * (q[-2] + q[2]) +
-                                     c[15] * (q[-1] + q[1]);
+                const double ddni2 = c[12] * ((q[-4] - u0) + (q[4] - u0)) + c[13] * ((q[-3] - u0) + (q[3] - u0)) +
+                                     c[14] * ((q[-2] - u0) + (q[2] - u0)) + c[15] * ((q[-1] - u0) + (q[1] - u0));
                 const double ddni1 = c[8] * (q[-4] - q[4]) + c[9] * (q[-3] - q[3]) + c[10] * (q[-2] - q[2]) +
                                      c[11] * (q[-1] - q[1]);
                 cc = ddmi2 + bm * ddmi1 + ddni2 + dn * ddni1;
             } else {
-                auto F = [&](int dnu, int dmu) { return fmirror(f, nu + dnu, mu + dmu, nni, s); };
+                auto F = [&](int dnu, int dmu) { return fmirror(f, nu + dnu, mu + dmu, nni, s) - u0; };
+                auto G = [&](int dnu, int dmu) { return fmirror(f, nu + dnu, mu + dmu, nni, s); };
                 const double ddmi2 = c[4] * (F(0, -4) + F(0, 4)) + c[5] * (F(0, -3) + F(0, 3)) +
                                      c[6] * (F(0, -2) + F(0, 2)) + c[7] * (F(0, -1) + F(0, 1));
-                const double ddmi1 = c[0] * (F(0, -4) - F(0, 4)) + c[1] * (F(0, -3) - F(0, 3)) +
-                                     c[2] * (F(0, -2) - F(0, 2)) + c[3] * (F(0, -1) - F(0, 1));
+                const double ddmi1 = c[0] * (G(0, -4) - G(0, 4)) + c[1] * (G(0, -3) - G(0, 3)) +
+                                     c[2] * (G(0, -2) - G(0, 2)) + c[3] * (G(0, -1) - G(0, 1));
                 const double ddni2 = c[12] * (F(-4, 0) + F(4, 0)) + c[13] * (F(-3, 0) + F(3, 0)) +
                                      c[14] * (F(-2, 0) + F(2, 0)) + c[15] * (F(-1, 0) + F(1, 0));
-                const double ddni1 = c[8] * (F(-4, 0) - F(4, 0)) + c[9] * (F(-3, 0) - F(3, 0)) +
-                                     c[10] * (F(-2, 0) - F(2, 0)) + c[11] * (F(-1, 0) - F(1, 0));
+                const double ddni1 = c[8] * (G(-4, 0) - G(4, 0)) + c[9] * (G(-3, 0) - G(3, 0)) +
+                                     c[10] * (G(-2, 0) - G(2, 0)) + c[11] * (G(-1, 0) - G(1, 0));
                 cc = ddmi2 + bm * ddmi1 + ddni2 + dn * ddni1;
             }
             const double l = d.lhs[e];
@@ -1103,7 +1119,7 @@ __global__ void __launch_bounds__(256) k_residual(const ResDesc *__restrict__ rd
             } else {
                 b = d.rhs[e];
             }
-            const double r = b - (cc + l * f[e]);
+            const double r = b - (cc + (l + residual_diag_shift(c)) * u0);
             sr = r * r;
             sb = b * b;
             if (d.res_f) {
@@ -1169,6 +1185,7 @@ __global__ void __launch_bounds__(kResThreads) k_residual_rows(const ResDesc *__
     const int cl = min(nu, nni) - 1;               // column read by this lane
     const double s = d.isym == 1 ? 1.0 : -1.0;
     const double *c = d.c;
+    const double shift = residual_diag_shift(c);
     // u, lhs and rhs through buffer resources: vector-memory loads, waited
     // for by their own counter only
     const unsigned nbytes = (unsigned)((long)nni * nmu * 8);
@@ -1238,17 +1255,18 @@ __global__ void __launch_bounds__(kResThreads) k_residual_rows(const ResDesc *__
             // every lane computes (slots nu-1 .. nu+7 lie inside the row
             // buffer); only the relaxed points of rows <= m1 count and store
             const double *q = r + nu + 3;
-            const double w8 = up[k], l = lp[k], b = bp[k];
-            const double ddmi2 = c[4] * (w[0] + w8) + c[5] * (w[1] + w[7]) + c[6] * (w[2] + w[6]) +
-                                 c[7] * (w[3] + w[5]);
+            const double w8 = up[k], l = lp[k], b = bp[k], u0 = w[4];
+            // second differences on u - u0 (see residual_diag_shift)
+            const double ddmi2 = c[4] * ((w[0] - u0) + (w8 - u0)) + c[5] * ((w[1] - u0) + (w[7] - u0)) +
+                                 c[6] * ((w[2] - u0) + (w[6] - u0)) + c[7] * ((w[3] - u0) + (w[5] - u0));
             const double ddmi1 = c[0] * (w[0] - w8) + c[1] * (w[1] - w[7]) + c[2] * (w[2] - w[6]) +
                                  c[3] * (w[3] - w[5]);
-            const double ddni2 = c[12] * (q[-4] + q[4]) + c[13] * (q[-3] + q[3]) + c[14] * (q[-2] + q[2]) +
-                                 c[15] * (q[-1] + q[1]);
+            const double ddni2 = c[12] * ((q[-4] - u0) + (q[4] - u0)) + c[13] * ((q[-3] - u0) + (q[3] - u0)) +
+                                 c[14] * ((q[-2] - u0) + (q[2] - u0)) + c[15] * ((q[-1] - u0) + (q[1] - u0));
             const double ddni1 = c[8] * (q[-4] - q[4]) + c[9] * (q[-3] - q[3]) + c[10] * (q[-2] - q[2]) +
                                  c[11] * (q[-1] - q[1]);
             const double cc = ddmi2 + mp[k] * ddmi1 + ddni2 + dn * ddni1;
-            const double res = b - (cc + l * w[4]);
+            const double res = b - (cc + (l + shift) * u0);
             sr += live ? res * res : 0.0;
             sb += live ? b * b : 0.0;
             if (out) {                             // uniform over the launch

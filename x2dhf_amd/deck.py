@@ -97,8 +97,9 @@ class Deck:
     start: str = "hydrogen"                        # orbpot: hydrogen | old
     notes: list = field(default_factory=list)
 
-    def spec(self, prec=0, schedule=0, device=0):
-        """The scf.JobSpec of this deck.  For orbpot old the LCAO guess is a
+    def spec(self, prec=0, schedule=0, device=0, polish=0):
+        """The scf.JobSpec of this deck (polish: FP64 finish iterations of a
+        prec = 1 job, see scf.make_spec).  For orbpot old the LCAO guess is a
         placeholder (one hydrogenic function per orbital), replaced by the
         checkpoint's functions (x2dhf_amd.checkpoint.resume)."""
         if self.start == "old" and not self.lcao:
@@ -111,7 +112,7 @@ class Deck:
                              ienterm=self.ienterm, inoterm=self.inoterm, omega=self.omega, prec=prec,
                              schedule=schedule, device=device, maxsor=self.maxsororb, maxsorpot=self.maxsorpot,
                              mpole=self.mpole, ienterm4init=self.ienterm4init, recalc_mm=self.recalc_mm,
-                             conv=self.conv)
+                             conv=self.conv, polish=polish)
 
 
 def _cards(text):

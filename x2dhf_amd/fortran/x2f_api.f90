@@ -23,6 +23,8 @@ module x2f_api
      real(c_double) :: occ(MAXO), c1(MAXO), c2(MAXO), za(MAXO), zb(MAXO)
      ! saturation test of the conv card (inputInterface.F90:1079-1102); <= 0: default
      integer(c_int) :: nscf2skip, nenlast, nnolast
+     ! FP64 finish of a mixed-precision job (scf_opts%polish); 0 = none
+     integer(c_int) :: polish
   end type x2f_spec
 
   ! job results (mirrored by x2dhf_amd/scf.py: JobResult)
@@ -213,6 +215,7 @@ contains
       j%o%maxsororb = spec%maxsororb; j%o%maxsorpot = spec%maxsorpot
       j%o%mpole = spec%mpole; j%o%prec = spec%prec; j%o%schedule = spec%schedule
       j%o%omega_orb = spec%omega_orb; j%o%omega_pot = spec%omega_pot; j%o%recalc_mm = spec%recalc_mm
+      if (spec%polish > 0) j%o%polish = spec%polish
       if (spec%nscf2skip > 0) j%o%nscf2skip = spec%nscf2skip
       if (spec%nenlast > 0) j%o%nenlast = spec%nenlast
       if (spec%nnolast > 0) j%o%nnolast = spec%nnolast

@@ -46,6 +46,10 @@ module x2f_scf
      ! orbEnergyIncThld: an orbital-energy change above it is divergence
      ! (initVariables.F90:187-191)
      real(dp) :: orb_energy_inc_thld = 1.0e4_dp
+     ! FP64 finish of a mixed-precision job: the last `polish` iterations
+     ! relax with FP64 sweeps (the reference's update), so the job stops at
+     ! the reference's FP64 fixed point; 0 = none
+     integer :: polish = 0
   end type scf_opts
 
   type :: lc_batch
@@ -87,6 +91,7 @@ module x2f_scf
      logical :: lenergythld = .false., lnormthld = .false., done = .false.
      real(dp) :: eeconv = 10, eeconvprev = 1.e35_dp, dnmaxprev = 1.e35_dp
      real(dp) :: deltaeemax = 100, deltanormmax = 100
+     integer :: polish_end = 0     ! no threshold stop before this iteration (FP64 finish)
      integer :: stopcode = 0       ! 1 energy, 2 norm, 3 iteration limit, 4 saturation, 5 resets,
                                    ! 6 divergence, 7 not a number
      integer(c_long_long) :: nsweep_orb = 0, nsweep_pot = 0
@@ -862,12 +867,45 @@ contains
     do ii = 1, j%c%norb
        call orbital_step(j, j%c%norb - ii + 1)
     enddo
-    stop_now = convergence_control(j)
+    stop_now = scf_control(j)
     if (j%mm_due) then
        call mpole_moments(j, j%mm_touch)
        j%mm_due = .false.
     endif
   end function scf_iteration
+
+  ! The end-of-iteration decision: the reference's rules (convergence_control)
+  ! plus the FP64 finish of a mixed-precision job.  A job relaxing with FP32
+  ! correction sweeps converges to the fixed point of that iteration, which
+  ! lies within rounding of the FP64 sweeps' fixed point but not on it (the
+  ! FP64 update's own rounding shifts its fixed point by a few units in the
+  ! last place, 1e-10 Eh in the total energy of FH on 481x901).  With polish
+  ! > 0 the job switches to FP64 sweeps when it would otherwise stop (or
+  ! `polish` iterations before maxscf) and runs at least `polish` more
+  ! iterations under the same rules, so it ends where the FP64 path ends.
+  logical function scf_control(j) result(stop_now)
+    type(scf_job), intent(inout) :: j
+    logical :: threshold_stop
+    stop_now = convergence_control(j)
+    if (j%o%polish <= 0) return
+    threshold_stop = stop_now .and. j%stopcode /= 3 .and. j%stopcode < 6
+    if (j%o%prec == 1) then
+       if (threshold_stop .or. (.not. stop_now .and. j%iscf >= j%o%maxscf - j%o%polish)) then
+          j%o%prec = 0
+          j%polish_end = min(j%iscf + j%o%polish, j%o%maxscf)
+          ! the reference's rules start over for the FP64 iterations
+          j%lenergythld = .false.; j%lnormthld = .false.
+          j%norbenergythld = 0; j%norbnormthld = 0; j%nresets = 0; j%nresets4no = 0
+          j%noenergydec = 0; j%nonormdec = 0
+          j%eeconvprev = 1.e35_dp; j%dnmaxprev = 1.e35_dp
+          j%stopcode = 0
+          stop_now = .false.
+       endif
+    else if (threshold_stop .and. j%iscf < j%polish_end) then
+       j%stopcode = 0
+       stop_now = .false.
+    endif
+  end function scf_control
 
   ! the end-of-iteration bookkeeping of scf (doSCF.F90:546-744)
   logical function convergence_control(j) result(stop_now)
@@ -1241,7 +1279,7 @@ contains
           js(h)%ee(i,i) = rech(base + MAXO + i)
           call energy_change(js(h), i)
        enddo
-       stopped(q) = convergence_control(js(h))
+       stopped(q) = scf_control(js(h))
     enddo
     call mm_refresh(js, idx, nj)
     deallocate(items, tails, grids, no, ea, recs, cnt, recq, rech)
@@ -1276,7 +1314,7 @@ contains
           call energy_change(j, i)
        enddo
        iters = it
-       if (convergence_control(j)) then
+       if (scf_control(j)) then
           stopcode = j%stopcode
           return
        endif

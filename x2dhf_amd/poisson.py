@@ -12,8 +12,20 @@ density of N electrons gives N times the potential of the unit one.
 
 The problems are relaxed with x2g_relax_batch (all active problems in one
 launch per sweep), FP32 correction sweeps with FP64 residuals by default,
-until ||rhs - A u|| / ||rhs|| <= tol for each; converged problems leave the
-batch.
+until ||rhs - A u|| / ||rhs|| <= tol for each, or until the residual stops
+decreasing; those problems leave the batch.
+
+The FP64 floor.  A solution stored in FP64 cannot have a smaller residual
+than the one its own rounding causes: with u* the exact discrete solution and
+u = fl(u*), rhs - A u = A (u* - u), of size ||A (u * delta)|| with delta the
+relative rounding (uniform in +-eps/2).  floors() measures exactly that on
+the device for each solved problem, ||A (u * delta)|| / ||rhs||, and a
+problem counts as solved when its residual is at most
+max(tol, FLOOR_FACTOR * floor).  On the 256 seeded problems of the batched
+benchmark the floor lies between 4e-14 and 3e-11, above 1e-12 for 174 of
+them (the 17-point operator's h^-2 weights times |u| ~ the electron count,
+against a density confined to a few bohr), so a tolerance of 1e-12 relative
+is below what FP64 storage allows for most of them.
 """
 import ctypes as C
 import time
@@ -23,6 +35,8 @@ import numpy as np
 from . import gpu, scf
 
 GRIDS = [(91, 121), (169, 241), (337, 601), (505, 901)]
+# a problem is solved at max(tol, FLOOR_FACTOR * its FP64 floor)
+FLOOR_FACTOR = 4.0
 
 
 def make_problems(count=256, seed=0, rinf=40.0):
@@ -90,10 +104,42 @@ class Batch:
         gpu.check(self.lib.x2g_residual_norms(self.ctx.h, self.gh, its, len(active), out.ctypes.data))
         return np.sqrt(out[0::2] / np.maximum(out[1::2], 1e-300))
 
+    def floors(self, idx=None, seed=0):
+        """FP64 rounding floor of each problem's relative residual:
+        ||A (u * delta)|| / ||rhs|| for its current solution u and a seeded
+        relative perturbation delta uniform in +-eps/2 (one sample), formed
+        with the device residual (rhs = 0)."""
+        idx = range(len(self.items)) if idx is None else idx
+        rng = np.random.default_rng(seed)
+        eps = np.finfo(np.float64).eps
+        out = []
+        scratch = {}
+        for i in idx:
+            g = self.grids[i]
+            if g.n not in scratch:
+                scratch[g.n] = (self.ctx.alloc(g.n), self.ctx.upload(np.zeros(g.n)))
+            du, zero = scratch[g.n]
+            u = self.solution(i)
+            du.set(u * rng.uniform(-0.5 * eps, 0.5 * eps, u.size))
+            it = self.items[i]
+            pert = (gpu.RelaxItem * 1)(gpu.RelaxItem(du.ptr, it.lhs_d, zero.ptr, None, it.omega, it.isym, it.bkind,
+                                                     it.prec, it.grid_index))
+            base = (gpu.RelaxItem * 1)(it)
+            a, b = np.zeros(2), np.zeros(2)
+            gpu.check(self.lib.x2g_residual_norms(self.ctx.h, self.gh, pert, 1, a.ctypes.data))
+            gpu.check(self.lib.x2g_residual_norms(self.ctx.h, self.gh, base, 1, b.ctypes.data))
+            out.append(float(np.sqrt(a[0] / max(b[1], 1e-300))))
+        for du, zero in scratch.values():
+            du.free()
+            zero.free()
+        return np.array(out)
+
     def solve(self, tol=1e-12, sweeps_per_call=10, check_every=5, max_calls=20000, stall_checks=40):
         """Relax until every problem meets tol (or stops improving for
         stall_checks residual checks, or max_calls relaxation calls); returns
-        per-problem calls and final relative residuals."""
+        per-problem calls and final relative residuals.  A problem that left
+        the batch without reaching tol is solved only if it reached its FP64
+        floor (see floors and solved)."""
         n = len(self.items)
         active = list(range(n))
         calls = np.zeros(n, dtype=int)
@@ -116,6 +162,16 @@ class Batch:
                     stall[i] += 1
             active = [i for i in active if res[i] > tol and stall[i] < stall_checks]
         return calls, res
+
+    def solved(self, res, tol=1e-12, floors=None):
+        """Per-problem verdict: residual <= max(tol, FLOOR_FACTOR * floor)."""
+        res = np.asarray(res)
+        if floors is None:
+            floors = self.floors([i for i in range(len(res)) if res[i] > tol])
+            full = np.zeros(len(res))
+            full[res > tol] = floors
+            floors = full
+        return res <= np.maximum(tol, FLOOR_FACTOR * np.asarray(floors)), np.asarray(floors)
 
     def solution(self, i):
         return self.keep[5 * i + 2].get()
@@ -140,6 +196,9 @@ def run(count=256, seed=0, prec=1, tol=1e-12, stall_checks=40):
     t2 = time.perf_counter()
     st = (C.c_double * 6)()
     gpu.check(b.lib.x2g_stats_get(b.ctx.h, st))
+    gpu.check(b.lib.x2g_stats_enable(b.ctx.h, 0))
+    ok, floors = b.solved(res, tol)
     b.close()
     return dict(problems=count, setup_s=t1 - t0, solve_s=t2 - t1, max_residual=float(res.max()),
-                residuals=res.tolist(), calls=calls.tolist(), stats=list(st))
+                solved=int(ok.sum()), at_tol=int((res <= tol).sum()), residuals=res.tolist(),
+                floors=floors.tolist(), calls=calls.tolist(), stats=list(st))

@@ -169,28 +169,114 @@ def footprint(spec):
     return int(scf.spec_bytes(spec) + WORKSPACE_GROWTH * scf.spec_workspace_bytes(spec))
 
 
-# closed-shell diatomics of a served stream: (name, spec builder, equilibrium R)
-STREAM_MOLECULES = (("H2", "h2_spec", 1.4), ("LiH", "lih_spec", 3.015), ("BH", "bh_spec", 2.3289),
-                    ("FH", "fh_spec", 1.733))
-STREAM_GRIDS = ((169, 241), (337, 601), (505, 901))
+# Served stream: closed-shell diatomics with 1 <= Z1, Z2 <= 10, neutral, whose
+# closed-shell configuration has at most 5 orbitals (2, 4, 6, 8, 10 or 12
+# electrons), R in [1, 6] bohr, on grids from 169x241 to 505x901.
+STREAM_MAX_ORBITALS = 5
+STREAM_ZMAX = 10
+STREAM_R = (1.0, 6.0)
+# admissible grid sizes (n - 1 divisible by 6) between the two ends, in
+# steps that keep the nu:mu aspect of the end points
+STREAM_NNI = tuple(range(169, 506, 48))      # 169, 217, ..., 505
+STREAM_NMU = tuple(range(241, 902, 60))      # 241, 301, ..., 901
+
+
+def _shells(z):
+    """Atomic shells of a neutral atom of charge z up to n = 2 as
+    (energy, kind, n, l, exponent): hydrogenic energies with Slater
+    screening (the 1s exponent is z itself, as the reference's decks take
+    it); kind "s" holds 2 electrons, "p" 6 (p0 sigma + p1 pi)."""
+    out = [(-z * z / 2.0, "s", 1, 0, float(z))]
+    if z > 2:
+        zeff = z - 0.35 * (z - 3) - 0.85 * 2
+        e2 = -zeff * zeff / 8.0
+        out += [(e2 - 1e-6, "s", 2, 0, zeff), (e2, "p", 2, 1, zeff)]
+    return out
+
+
+def _p_shell(left):
+    """Orbitals taken from a p shell with `left` electrons still to place:
+    6 or more -> sigma (p0) and pi (p1); 4 -> pi; 2 -> sigma."""
+    return ["p0", "p1"] if left >= 6 else (["p1"] if left == 4 else (["p0"] if left == 2 else []))
+
+
+def closed_shell_orbitals(z1, z2):
+    """Closed-shell configuration and hydrogenic LCAO start of the neutral
+    diatomic (z1, z2), or None when it needs more than STREAM_MAX_ORBITALS
+    orbitals or has an odd electron count.  Orbitals come highest first (the
+    order of the reference's config card).  A homonuclear molecule gets g/u
+    pairs of the atomic shells (sigma-g then sigma-u from an s shell;
+    sigma-g from p0, pi-u from p1); a heteronuclear one takes the shells of
+    both centres in energy order, each on its own centre."""
+    nel = z1 + z2
+    if nel % 2:
+        return None
+    orbs, left = [], nel
+    if z1 == z2:
+        for e, kind, n, l, ex in _shells(z1):
+            if kind == "s":
+                for gu, c2 in ((1, 1.0), (-1, -1.0)):
+                    if left >= 2:
+                        orbs.append(scf.Orbital(0, gu, 1.0, n, l, ex, c2, n, l, ex))
+                        left -= 2
+            else:
+                for part in _p_shell(left):
+                    if part == "p0":
+                        orbs.append(scf.Orbital(0, 1, 1.0, n, l, ex, 1.0, n, l, ex))
+                        left -= 2
+                    else:
+                        orbs.append(scf.Orbital(1, -1, 1.0, n, l, ex, 1.0, n, l, ex))
+                        left -= 4
+    else:
+        shells = sorted([(e, 0, kind, n, l, ex) for e, kind, n, l, ex in _shells(z1)] +
+                        [(e, 1, kind, n, l, ex) for e, kind, n, l, ex in _shells(z2)])
+        for e, centre, kind, n, l, ex in shells:
+            if left <= 0:
+                break
+            c1, c2 = (1.0, 0.0) if centre == 0 else (0.0, 1.0)
+            za = ex if centre == 0 else 1.0
+            zb = ex if centre == 1 else 1.0
+            if kind == "s":
+                orbs.append(scf.Orbital(0, 0, c1, n, l, za, c2, n, l, zb))
+                left -= 2
+            else:
+                for part in _p_shell(left):
+                    orbs.append(scf.Orbital(0 if part == "p0" else 1, 0, c1, n, l, za, c2, n, l, zb))
+                    left -= 2 if part == "p0" else 4
+    if left != 0 or not 1 <= len(orbs) <= STREAM_MAX_ORBITALS:
+        return None
+    return orbs[::-1]
+
+
+STREAM_MOLECULES = tuple((z1, z2) for z1 in range(1, STREAM_ZMAX + 1) for z2 in range(1, z1 + 1)
+                         if closed_shell_orbitals(z1, z2) is not None)
+
+
+def stream_request(rng, prec=1, rinf=40.0, ienterm=12, inoterm=12, maxscf=8000):
+    """One request of the served stream: a molecule of STREAM_MOLECULES (Z1 >=
+    Z2), R uniform in STREAM_R, a grid size class drawn uniformly from the
+    sizes between 169x241 and 505x901.  Returns (name, spec)."""
+    z1, z2 = STREAM_MOLECULES[int(rng.integers(len(STREAM_MOLECULES)))]
+    r = float(rng.uniform(*STREAM_R))
+    k = int(rng.integers(len(STREAM_NNI)))
+    nni = STREAM_NNI[k]
+    nmu = STREAM_NMU[round(k * (len(STREAM_NMU) - 1) / (len(STREAM_NNI) - 1))]
+    orbs = closed_shell_orbitals(z1, z2)
+    spec = scf.make_spec(float(z1), float(z2), r, nni, nmu, rinf, orbs, prec=prec, schedule=1, ienterm=ienterm,
+                         inoterm=inoterm, maxscf=maxscf)
+    return f"Z{z1}-Z{z2} R={r:.3f} {nni}x{nmu} {len(orbs)}orb", spec
 
 
 def poisson_stream(count, rate, seed=0, prec=1, rinf=40.0):
     """A seeded stream of requests: exponential inter-arrival times of mean
-    1/rate seconds; each request a molecule of STREAM_MOLECULES with R drawn
-    uniformly within 10% of its equilibrium value and a grid of STREAM_GRIDS,
-    converged to orbital-energy and norm changes below 1e-12.
-    Returns [(arrival_s, name, spec)]."""
+    1/rate seconds, each request from stream_request.  Returns
+    [(arrival_s, name, spec)]."""
     import numpy as np
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     for _ in range(count):
         t += float(rng.exponential(1.0 / rate))
-        name, builder, req = STREAM_MOLECULES[int(rng.integers(len(STREAM_MOLECULES)))]
-        nni, nmu = STREAM_GRIDS[int(rng.integers(len(STREAM_GRIDS)))]
-        r = req * float(rng.uniform(0.9, 1.1))
-        spec = getattr(scf, builder)(nni=nni, nmu=nmu, rinf=rinf, r=r, prec=prec, schedule=1, ienterm=12,
-                                     inoterm=12, maxscf=8000)
+        name, spec = stream_request(rng, prec=prec, rinf=rinf)
         out.append((t, name, spec))
     return out
 
