@@ -80,3 +80,42 @@ def test_admission_respects_the_memory_budget():
         srv.close()
     assert reqs[1].admitted >= reqs[0].finished and reqs[2].admitted >= reqs[0].finished
     assert reqs[1].admitted <= reqs[2].admitted
+
+
+def test_config5_widened_stream_each_result_as_alone_and_within_1e8_of_fp64():
+    """Configuration 5's request mix: a seeded Poisson-arrival stream of
+    closed-shell diatomics (Z up to 9 here, R in [1, 6], 1-5 orbitals, grids
+    from 169x241 to 505x901), served with continuous batching in mixed
+    precision.  Each served result is bit for bit the one the request gets
+    served alone, and its total energy is within the stated 1e-8 Eh of the
+    same request run alone with FP64 sweeps."""
+    from x2dhf_amd import scf, serve
+    stream = serve.poisson_stream(8, 20.0, seed=2, prec=1)
+    names = [n for _, n, _ in stream]
+    grids = {(s.nni, s.nmu) for _, _, s in stream}
+    assert (169, 241) in grids and (505, 901) in grids
+    assert {s.norb for _, _, s in stream} >= {1, 2, 3, 4, 5}
+    print("served stream:", names, flush=True)
+    # arrival seconds -> server steps (one step per clock unit)
+    reqs = _serve([s for _, _, s in stream], [100.0 * a for a, _, _ in stream])
+    assert any(r.admitted < max(q.finished for q in reqs) and r.admitted > 0 for r in reqs)
+    for (a, name, spec), r in zip(stream, reqs):
+        alone = _serve([spec], [0.0])[0].result
+        s64 = type(spec).from_buffer_copy(spec)
+        s64.prec = 0
+        job = scf.Job(s64)
+        try:
+            job.prep()
+            while not scf.iterate_batched([job])[0]:
+                pass
+            job.finish()
+            fp64 = job.result()
+        finally:
+            job.close()
+        m = r.result
+        print(f"{name}: served E={m['etot']:.12f} it={m['iterations']} ({m['stop']}), alone it={alone['iterations']}, "
+              f"FP64 E={fp64['etot']:.12f} dE={m['etot'] - fp64['etot']:.1e}", flush=True)
+        assert m["iterations"] == alone["iterations"] and m["etot"] == alone["etot"]
+        assert np.array_equal(m["eps"], alone["eps"])
+        assert m["stop"] not in ("diverged", "not a number")
+        assert abs(m["etot"] - fp64["etot"]) < 1e-8
