@@ -149,6 +149,19 @@ int x2g_putout(x2g_grid *g, double *fun_d, const double *work_d);
  * 0 = borb (orbitals), 1 = bpot (potentials).  work_d is updated in place. */
 int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
               const double *rhs_d, double omega, int nsweeps);
+/* x2g_mcsor without any device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and result as x2g_mcsor, with the call's workspace in a static
+ * device array of 24 MiB per GPU (held until the process ends; enough for a
+ * grid of about 560 x 1000 points), instead of the context's growable
+ * workspace.  Larger grids are rejected with X2G_EINVAL.  No hipMalloc /
+ * hipFree, and sweep timing (x2g_stats_enable) is not recorded for it.  Every
+ * call on a GPU shares that array: calls that overlap on one GPU (two
+ * contexts or two host threads) race; calls on one context are ordered by
+ * its stream. */
+int x2g_mcsor_noalloc(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
+                      const double *rhs_d, double omega, int nsweeps);
 /* Host-pointer drop-in for mcsor/sor: work (padded), lhs, rhs on the host;
  * copies in, relaxes on the device, copies work back.  PCIe-inclusive. */
 int x2g_mcsor_host(x2g_grid *g, int isym, int bkind, double *work, const double *lhs,

@@ -215,6 +215,7 @@ extern "C" int x2g_ctx_create(int device, x2g_ctx **out)
     }
     int rc = ws_reserve(c, 2, kStageBytes);
     if (!rc) rc = pinned_reserve(c, kStageBytes);
+    if (!rc) rc = scratch_init(c);
     if (rc) {
         x2g_ctx_destroy(c);
         return rc;

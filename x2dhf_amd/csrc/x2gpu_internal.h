@@ -68,6 +68,10 @@ struct x2g_ctx {
     std::unordered_map<void *, size_t> live;
     std::multimap<size_t, void *> cache;
     size_t cached_bytes = 0;
+    // device address of the static scratch array of x2g_mcsor_noalloc on
+    // this context's device (shared by every context on that device)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
 };
 
 struct x2g_grid {
@@ -135,6 +139,9 @@ struct PadItem {
 int launch_putin_items(x2g_ctx *ctx, const std::vector<PadItem> &items);
 // pinned host buffer of at least `bytes` for device -> host results
 int pinned_out_reserve(x2g_ctx *ctx, size_t bytes);
+// Look up the device address of the no-allocation scratch array on the
+// context's device (current at the call); x2g_ctx_create calls it once.
+int scratch_init(x2g_ctx *ctx);
 // Sweep timing hooks (no-ops unless statistics are enabled on the context).
 int stats_begin(x2g_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
 int stats_end(x2g_ctx *ctx, hipEvent_t a, hipEvent_t b, double bytes, int prec, int launches = 1);

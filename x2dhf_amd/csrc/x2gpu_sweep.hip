@@ -1984,21 +1984,51 @@ static int check_grid_sweepable(x2g_grid *g)
     return X2G_OK;
 }
 
-extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
-                         const double *rhs_d, double omega, int nsweeps)
+// Static device scratch of x2g_mcsor_noalloc: the workspace of one mcsor
+// call (two padded sweep buffers and the colour-ordered lhs/rhs pairs, or a
+// ping-pong partner of the caller's array) for callers that cannot allocate.
+// Each GPU holds its own copy for the life of the process; host code uses
+// only its per-device address from hipGetSymbolAddress (x2g::scratch_init).
+constexpr size_t kScratchBytes = (size_t)24 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace x2g {
+// The device address of `scratch` on the context's device (which is
+// current): called once per context by x2g_ctx_create.
+int scratch_init(x2g_ctx *ctx)
 {
-    X2G_CHECK(g && work_d && lhs_d && rhs_d, "x2g_mcsor: null argument");
-    X2G_CHECK(isym == 1 || isym == -1, "x2g_mcsor: isym must be +1 or -1");
-    X2G_CHECK(bkind == 0 || bkind == 1, "x2g_mcsor: bkind must be 0 or 1");
-    X2G_CHECK(nsweeps >= 0, "x2g_mcsor: nsweeps < 0");
-    X2G_CHECK(omega > 0.0 && omega < 2.0, "x2g_mcsor: omega must lie in (0, 2)");
-    int rc = check_grid_sweepable(g);
-    if (rc) return rc;
-    if (nsweeps == 0) return X2G_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        set_error("hipGetSymbolAddress(scratch) on device %d: %s", ctx->device, hipGetErrorString(e));
+        return X2G_EHIP;
+    }
+    ctx->scratch = ptr;
+    ctx->scratch_bytes = kScratchBytes;
+    return X2G_OK;
+}
+}  // namespace x2g
+
+// Workspace bytes of one mcsor call of this sweep kind on grid g.
+static size_t mcsor_ws_bytes(const x2g_grid *g, int kind)
+{
+    const int P = pad_row_stride<double>(g->nni, kind);
+    const size_t ubytes = g->npad() * sizeof(double);
+    const size_t pbytes = (size_t)P * (g->nmu + 8) * sizeof(double);
+    const size_t lrbytes = 2 * split_size(g->nni, g->nmu) * sizeof(double);
+    return kind == 2 ? 2 * align256(pbytes) + align256(lrbytes) : align256(ubytes);
+}
+
+// mcsor of one padded array; the workspace is w0 (at least mcsor_ws_bytes).
+// Every workspace byte a kernel reads is written earlier in the same call
+// (the restride fills both sweep buffers, k_pack the pairs; absent points of
+// a short colour block are computed into an LDS sink), so nothing in it has
+// to start at zero.
+static int mcsor_run(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d, const double *rhs_d,
+                     double omega, int nsweeps, int kind, char *w0)
+{
     x2g_ctx *ctx = g->ctx;
-    rc = set_lds_attr(ctx);
-    if (rc) return rc;
-    const int kind = sweep_kind<double>(ctx, g->nni);
+    int rc;
     const int P = pad_row_stride<double>(g->nni, kind);
     const size_t ubytes = g->npad() * sizeof(double);                 // the caller's array (stride nni+8)
     const size_t pbytes = (size_t)P * (g->nmu + 8) * sizeof(double);  // a sweep buffer (stride P)
@@ -2006,9 +2036,6 @@ extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const
     // k_sweep: two buffers of stride P and the (omega/lhs, rhs) pairs; the
     // other kinds: a ping-pong partner of the caller's array
     const bool restride = kind == 2;
-    rc = ws_reserve(ctx, 0, restride ? 2 * align256(pbytes) + align256(lrbytes) : align256(ubytes));
-    if (rc) return rc;
-    char *w0 = (char *)ctx->ws[0].ptr;
     double *A = work_d, *B = (double *)w0;
     if (restride) {
         A = (double *)w0;
@@ -2042,6 +2069,50 @@ extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const
         X2G_HIP(hipMemcpyAsync(work_d, fin, ubytes, hipMemcpyDeviceToDevice, ctx->stream));
     }
     return X2G_OK;
+}
+
+static int mcsor_check(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d, const double *rhs_d,
+                       double omega, int nsweeps, const char *who)
+{
+    X2G_CHECK(g && work_d && lhs_d && rhs_d, "%s: null argument", who);
+    X2G_CHECK(isym == 1 || isym == -1, "%s: isym must be +1 or -1", who);
+    X2G_CHECK(bkind == 0 || bkind == 1, "%s: bkind must be 0 or 1", who);
+    X2G_CHECK(nsweeps >= 0, "%s: nsweeps < 0", who);
+    X2G_CHECK(omega > 0.0 && omega < 2.0, "%s: omega must lie in (0, 2)", who);
+    return check_grid_sweepable(g);
+}
+
+extern "C" int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
+                         const double *rhs_d, double omega, int nsweeps)
+{
+    int rc = mcsor_check(g, isym, bkind, work_d, lhs_d, rhs_d, omega, nsweeps, "x2g_mcsor");
+    if (rc || nsweeps == 0) return rc;
+    x2g_ctx *ctx = g->ctx;
+    if ((rc = set_lds_attr(ctx))) return rc;
+    const int kind = sweep_kind<double>(ctx, g->nni);
+    if ((rc = ws_reserve(ctx, 0, mcsor_ws_bytes(g, kind)))) return rc;
+    return mcsor_run(g, isym, bkind, work_d, lhs_d, rhs_d, omega, nsweeps, kind, (char *)ctx->ws[0].ptr);
+}
+
+extern "C" int x2g_mcsor_noalloc(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
+                                 const double *rhs_d, double omega, int nsweeps)
+{
+    int rc = mcsor_check(g, isym, bkind, work_d, lhs_d, rhs_d, omega, nsweeps, "x2g_mcsor_noalloc");
+    if (rc || nsweeps == 0) return rc;
+    x2g_ctx *ctx = g->ctx;
+    X2G_CHECK(ctx->scratch, "x2g_mcsor_noalloc: no scratch address for device %d", ctx->device);
+    if ((rc = set_lds_attr(ctx))) return rc;
+    const int kind = sweep_kind<double>(ctx, g->nni);
+    const size_t need = mcsor_ws_bytes(g, kind);
+    X2G_CHECK(need <= ctx->scratch_bytes,
+              "x2g_mcsor_noalloc: grid %dx%d needs %zu bytes of scratch, more than its %zu (use x2g_mcsor)", g->nni,
+              g->nmu, need, ctx->scratch_bytes);
+    // the sweep timing's events would be created on demand: not in this call
+    const bool stats = ctx->stats_on;
+    ctx->stats_on = false;
+    rc = mcsor_run(g, isym, bkind, work_d, lhs_d, rhs_d, omega, nsweeps, kind, (char *)ctx->scratch);
+    ctx->stats_on = stats;
+    return rc;
 }
 
 extern "C" int x2g_mcsor_host(x2g_grid *g, int isym, int bkind, double *work, const double *lhs,

@@ -103,6 +103,7 @@ _PROTOS = {
     "x2g_putout": (_int, [_vp, _vp, _vp]),
     "x2g_mcsor": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
     "x2g_mcsor_host": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
+    "x2g_mcsor_noalloc": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
     "x2g_relax_batch": (_int, [_vp, _vp, C.POINTER(RelaxItem), _int, _int, _int]),
     "x2g_residual_norms": (_int, [_vp, _vp, C.POINTER(RelaxItem), _int, _vp]),
     "x2g_relax_workspace_bytes": (C.c_longlong, [_int, _int, _int]),
