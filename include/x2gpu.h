@@ -15,6 +15,8 @@
  *                         and mcsor_pthread_(isym,nthreads,wk2,lhs,rhs)
  *                                                               src/c/mcsor_pthread.c:32
  *   x2g_mcsor_host     <- the same, with host arrays (copies in and out)
+ *   x2g_relax_ordered  <- sor / mcsor in any mesh ordering    src/sormcsor.F90:20,143
+ *                         (mesh, src/initCBAllocArrays.F90:2408-2540)
  *   x2g_putin/putout   <- putin/putout, putinc_/putoutc_         src/inout.F90:23,165,195,266
  *   x2g_relax_batch    <- coulExch_pthread_() + relax_single_pot_pthread_
  *                                                               src/c/coulExch_pthread.c:19
@@ -162,6 +164,20 @@ int x2g_mcsor(x2g_grid *g, int isym, int bkind, double *work_d, const double *lh
  * its stream. */
 int x2g_mcsor_noalloc(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
                       const double *rhs_d, double omega, int nsweeps);
+/* sor (sormcsor.F90:20-132, method X2G_SOR) or mcsor (sormcsor.F90:143-386,
+ * method X2G_MCSOR, one thread) in mesh ordering `ordering`
+ * (initCBAllocArrays.F90:2408-2540; the reference's default is middle) on the
+ * padded work_d, in place: nsweeps sweeps, each followed by the boundary
+ * lines.  The points are updated in exactly the reference's sequential order
+ * of dependencies: each sweep runs as a list of phases of mutually
+ * independent points (for sor: the anti-diagonals of the ordering's
+ * (position in row, row) plane), all in one workgroup. */
+#define X2G_MCSOR 0
+#define X2G_SOR 1
+#define X2G_ORDER_COLWISE 0
+#define X2G_ORDER_MIDDLE 1
+int x2g_relax_ordered(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
+                      const double *rhs_d, double omega, int nsweeps, int method, int ordering);
 /* Host-pointer drop-in for mcsor/sor: work (padded), lhs, rhs on the host;
  * copies in, relaxes on the device, copies work back.  PCIe-inclusive. */
 int x2g_mcsor_host(x2g_grid *g, int isym, int bkind, double *work, const double *lhs,
@@ -194,10 +210,18 @@ typedef struct {
     const double *rhs_f1_d;
     const double *rhs_f2_d;
     const double *rhs_f3_d;
+    /* Relaxation scheme (prec = 0 items): method X2G_MCSOR (five passes) or
+     * X2G_SOR, over the points in mesh ordering X2G_ORDER_COLWISE or
+     * X2G_ORDER_MIDDLE (initCBAllocArrays.F90:2408-2540).  Zero (MCSOR,
+     * col-wise) is the pipelined colour sweep; any other pair runs the
+     * reference's sequential point order exactly (x2g_relax_ordered). */
+    int method;
+    int ordering;
 } x2g_relax_item;
 
 /* Relax n problems, possibly on different grids, together: every sweep of
- * every problem is issued in one fused launch. */
+ * every problem is issued in one fused launch (items of another method or
+ * ordering: all their sweeps in one launch, a workgroup per item). */
 int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
                     int maxsor1, int maxsor2);
 

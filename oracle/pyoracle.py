@@ -29,6 +29,9 @@ def lib():
         L.x2o_residual.argtypes = [_int, _int] + [_vp] * 7
         L.x2o_orb_asympt_get.argtypes = [_int, _int, _vp, _vp, _vp, _vp]
         L.x2o_orb_asympt_set.argtypes = [_int, _int, _vp, _vp]
+        L.x2o_mesh.argtypes = [_int, _int, _int, _vp]
+        L.x2o_mesh.restype = C.c_long
+        L.x2o_relax_ordered.argtypes = [_int, _int, _int] + [_vp] * 7 + [_dbl, _int, _int, _int, _vp]
         _lib = L
     return _lib
 
@@ -83,3 +86,26 @@ def orb_tail(nni, nmu, vxisq, vetasq, fa, psi):
     out = np.array(psi, dtype=np.float64, copy=True)
     lib().x2o_orb_asympt_set(nni, nmu, _p(out), _p(ed))
     return out
+
+
+ORDERINGS = {"col-wise": 0, "middle": 1}
+
+
+def mesh(nni, nmu, ordering):
+    """Relaxation sequence of mesh ordering (name or code) as unpadded indices."""
+    code = ORDERINGS.get(ordering, ordering)
+    seq = np.empty((nni - 2) * (nmu - 5), dtype=np.int32)
+    lib().x2o_mesh(nni, nmu, int(code), _p(seq))
+    return seq
+
+
+def relax_ordered(nni, nmu, isym, work, lhs, rhs, b2d, d2d, coef16, exeven, omega, nsweeps, mode, ordering):
+    """In-place sor (mode 1) / mcsor (mode 0) of `work` in mesh ordering `ordering`."""
+    code = ORDERINGS.get(ordering, ordering)
+    seq = np.empty((nni - 2) * (nmu - 5), dtype=np.int32)
+    lib().x2o_relax_ordered(nni, nmu, isym, _p(work), _p(np.ascontiguousarray(lhs, np.float64)),
+                            _p(np.ascontiguousarray(rhs, np.float64)), _p(np.ascontiguousarray(b2d, np.float64)),
+                            _p(np.ascontiguousarray(d2d, np.float64)), _p(np.ascontiguousarray(coef16, np.float64)),
+                            _p(np.ascontiguousarray(exeven, np.float64)), float(omega), int(nsweeps), int(mode),
+                            int(code), _p(seq))
+    return work

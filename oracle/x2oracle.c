@@ -256,3 +256,49 @@ void x2o_orb_asympt_set(int nni, int nmu, double *psi, const double *edecay)
             psi[UIDX(i, j, nni)] = psi[UIDX(i, j - 1, nni)] * edecay[(size_t)jj * nni + (i - 1)];
     }
 }
+
+/* Sequence of relaxed points of mesh (initCBAllocArrays.F90:2408-2540) as
+ * unpadded 0-based indices ik = (mu-1)*nni + (nu-1), ordering 0 = col-wise
+ * (2467-2497: mu = 2..nmu-4 outer, nu = 2..nni-1 inner), 1 = middle
+ * (2499-2540: rows from the middle row down to mu = 2, then from the one
+ * above it up to nmu-4; in every row nu from the middle column down to 2,
+ * then from the one after it up to nni-1).  Returns the count,
+ * (nni-2)*(nmu-5). */
+long x2o_mesh(int nni, int nmu, int ordering, int *seq)
+{
+    const int nni8 = nni + 8, nmu8 = nmu + 8;
+    const int iend1 = nmu;                         /* padded rows 6..nmu <-> mu = 2..nmu-4 */
+    const int nmux = (nmu8 - 1) / 2, nnix = (nni8 - 1) / 2;
+    long k = 0;
+    for (int i = 6; i <= iend1; ++i) {
+        const int ii = (ordering == 1 && i <= nmux) ? nmux - i + 6 : i;
+        for (int j = 6; j <= nni + 3; ++j) {
+            const int jj = (ordering == 1 && j <= nnix) ? nnix - j + 6 : j;
+            const int mu = ii - 4, nu = jj - 4;
+            seq[k++] = (mu - 1) * nni + (nu - 1);
+        }
+    }
+    return k;
+}
+
+/* sor (sormcsor.F90:20-132, mode 1) or mcsor (sormcsor.F90:143-386, mode 0:
+ * five passes, pass c over the sequence positions i = c, c+5, c+10, ...)
+ * of nsweeps sweeps over the points of mesh ordering `ordering` (x2o_mesh),
+ * each sweep followed by the boundary lines. */
+void x2o_relax_ordered(int nni, int nmu, int isym, double *work, const double *lhs, const double *rhs,
+                       const double *b, const double *d, const double *coef, const double *exeven,
+                       double omega, int nsweeps, int mode, int ordering, int *seq)
+{
+    const int N8 = nni + 8;
+    const double omega1 = 1.0 - omega;
+    const long n = x2o_mesh(nni, nmu, ordering, seq);
+    for (int it = 0; it < nsweeps; ++it) {
+        for (int c = 0; c < (mode == 0 ? 5 : 1); ++c)
+            for (long k = c; k < n; k += (mode == 0 ? 5 : 1)) {
+                const int ik = seq[k];
+                const int mu = ik / nni + 1, nu = ik % nni + 1;
+                update_point(work, PIDX(nu, mu, N8), (size_t)ik, N8, lhs, rhs, b, d, coef, omega, omega1);
+            }
+        x2o_boundary(nni, nmu, isym, work, exeven);
+    }
+}

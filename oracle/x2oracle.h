@@ -19,6 +19,10 @@ void x2o_residual(int nni, int nmu, const double *work, const double *lhs, const
 void x2o_orb_asympt_get(int nni, int nmu, const double *vxisq, const double *vetasq,
                         const double *fa, double *edecay);
 void x2o_orb_asympt_set(int nni, int nmu, double *psi, const double *edecay);
+long x2o_mesh(int nni, int nmu, int ordering, int *seq);
+void x2o_relax_ordered(int nni, int nmu, int isym, double *work, const double *lhs, const double *rhs,
+                       const double *b, const double *d, const double *coef, const double *exeven,
+                       double omega, int nsweeps, int mode, int ordering, int *seq);
 #ifdef __cplusplus
 }
 #endif

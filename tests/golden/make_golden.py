@@ -8,7 +8,7 @@ the fixtures it writes are plain arrays (numpy .npz, no pickles) and travel
 with the repository.
 
     make -f oracle/ref.mk harness
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 import os
 import subprocess
@@ -27,9 +27,16 @@ CASES = {
     "h2_61x91": ("h2_61x91.data", None),
     "n2_61x91": ("n2_61x91.data", 1),
 }
+# the same decks in the reference's default (middle) ordering: only the
+# relaxation vectors (inputs and the sor / mcsor outputs) are kept
+LIGHT = {
+    "h2_61x91_middle": ("h2_61x91_middle.data", None),
+    "n2_61x91_middle": ("n2_61x91_middle.data", 1),
+}
+LIGHT_KEYS = ("sizes", "scal", "coef", "suppl", "vxi", "veta")
 
 
-def run_case(name, deck, iorb):
+def run_case(name, deck, iorb, light=False):
     with tempfile.TemporaryDirectory() as d:
         env = dict(os.environ, X2_DUMP=d, OMP_NUM_THREADS="2")
         if iorb is not None:
@@ -69,6 +76,12 @@ def run_case(name, deck, iorb):
                 out[pfx + k] = f64(pfx + k)
             out[pfx + "isym"] = i32(pfx + "isym")
             out[pfx + "scal"] = f64(pfx + "scal")
+
+        if light:
+            keep = {k: v for k, v in out.items() if k in LIGHT_KEYS or k.startswith("k")}
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **keep)
+            print(name, "->", os.path.getsize(os.path.join(HERE, name + ".npz")), "bytes")
+            return
 
         def slices(arr, offs):
             return np.stack([arr[o - 1:o - 1 + n] for o in offs])
@@ -110,5 +123,10 @@ def run_case(name, deck, iorb):
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the harness first: make -f oracle/ref.mk harness")
+    only = set(sys.argv[1:])
     for name, (deck, iorb) in CASES.items():
-        run_case(name, deck, iorb)
+        if not only or name in only:
+            run_case(name, deck, iorb)
+    for name, (deck, iorb) in LIGHT.items():
+        if not only or name in only:
+            run_case(name, deck, iorb, light=True)

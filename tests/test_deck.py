@@ -55,7 +55,9 @@ def test_configuration_decks_give_the_benchmark_specs(name, builder, kw):
     kw = dict(dict(ienterm=12, inoterm=12), **kw)
     want = getattr(scf, builder)(**kw)
     assert _fields(d.spec()) == _fields(want)
-    assert not d.notes[-1].startswith("relaxed with MCSOR")  # these decks ask for MCSOR col-wise
+    # these decks ask for MCSOR col-wise for orbitals and potentials
+    assert (d.orb_method, d.pot_method, d.ordering) == (0, 0, 0)
+    assert d.notes[-1] == "orbitals by MCSOR, potentials by MCSOR, col-wise ordering"
 
 
 def test_reference_lih_deck():
@@ -65,10 +67,11 @@ def test_reference_lih_deck():
     assert (d.nni, d.nmu, d.rinf) == (241, 331, 60.0)
     assert (d.maxscf, d.ienterm, d.inoterm) == (10000, 12, 14)
     assert d.conv == (2000, 0, 0) and d.omega == (1.85, 1.85)
+    # no mcsor / order cards: the reference's default, SOR in the middle ordering
     want = scf.lih_spec(nni=241, nmu=331, rinf=60.0, maxscf=10000, ienterm=12, inoterm=14, omega=(1.85, 1.85),
-                        conv=(2000, 0, 0))
+                        conv=(2000, 0, 0), orb_method=1, pot_method=1, ordering=1)
     assert _fields(d.spec()) == _fields(want)
-    assert any("relaxed with MCSOR" in n for n in d.notes)  # the deck asks for the default SOR
+    assert d.notes[-1] == "orbitals by SOR, potentials by SOR, middle ordering"
 
 
 def test_reference_bh_deck():
@@ -78,7 +81,7 @@ def test_reference_bh_deck():
     assert (d.nni, d.nmu, d.rinf) == (331, 601, 200.0)
     assert (d.maxscf, d.ienterm, d.inoterm, d.ienterm4init) == (4000, 14, 16, 0)
     want = scf.bh_spec(nni=331, nmu=601, rinf=200.0, maxscf=4000, ienterm=14, inoterm=16, ienterm4init=0,
-                       conv=(4000, 0, 0))
+                       conv=(4000, 0, 0), orb_method=d.orb_method, pot_method=d.pot_method, ordering=d.ordering)
     assert _fields(d.spec()) == _fields(want)
 
 
@@ -131,3 +134,22 @@ def test_h2_deck_runs_to_the_reference_energy():
         job.close()
     print(f"E = {res['etot']:.16f} (ref {ref['etot']:.16f})")
     assert abs(res["etot"] - ref["etot"]) < 1e-10
+
+
+def test_relaxation_scheme_cards():
+    """Without solver cards the reference relaxes orbitals and potentials by
+    SOR in the middle ordering (initVariables.F90:206, 253); mcsor / mcsor-o
+    switch the orbitals to MCSOR, mcsor-ce the potentials
+    (inputInterface.F90:1915-2028), order picks the mesh ordering."""
+    from x2dhf_amd import deck
+    base = "method hf\nnuclei 1.0 1.0 1.4\nconfig 0\n 1 sigma g end\ngrid 61 91 20.0\norbpot hydrogen\n" \
+           "lcao\n 1.0 1 0 1.0 1.0 1 0 1.0\n"
+    for extra, want in (("", (1, 1, 1)), ("mcsor\n", (0, 1, 1)), ("mcsor-o 1 10 1\n", (0, 1, 1)),
+                        ("mcsor\nmcsor-ce\n", (0, 0, 1)), ("order col-wise\n", (1, 1, 0)),
+                        ("mcsor 1\nmcsor-ce 1\norder middle\n", (0, 0, 1))):
+        d = deck.parse(base + extra + "stop\n")
+        assert (d.orb_method, d.pot_method, d.ordering) == want, extra
+        s = d.spec()
+        assert (s.orb_method, s.pot_method, s.ordering) == want
+    with pytest.raises(deck.DeckError):
+        deck.parse(base + "order row-wise\nstop\n")

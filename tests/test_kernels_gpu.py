@@ -514,3 +514,68 @@ def test_mcsor_host_dropin_vs_reference(ctx, case):
         print(f"{case} x2g_mcsor_host {ns} sweeps: rel err {e:.2e}")
         assert e < REL64, (name, e)
     g.destroy()
+
+
+@pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91", "h2_61x91_middle", "n2_61x91_middle"])
+@pytest.mark.parametrize("pfx", ["k1_", "k2_"])
+def test_ordered_relaxation_bitexact_vs_reference(ctx, case, pfx):
+    """x2g_relax_ordered: sor (10 sweeps) and one-thread mcsor (10 and 1
+    sweeps) in the col-wise and the reference's default middle ordering
+    reproduce the reference's own routines bit for bit (the level schedule
+    keeps its sequential order of dependencies; no fused multiply-adds)."""
+    from x2dhf_amd import gpu
+    z = load_golden(case)
+    if pfx + "win" not in z:
+        pytest.skip("single potential")
+    ordering = 1 if case.endswith("_middle") else 0
+    print(f"case={case} {pfx} ordering={ordering}", flush=True)
+    g = make_grid(ctx, z)
+    isym = int(z[pfx + "isym"][0])
+    omega, nsw = float(z[pfx + "scal"][0]), int(z[pfx + "scal"][1])
+    lhs, rhs = ctx.upload(z[pfx + "lhs"]), ctx.upload(z[pfx + "rhs"])
+    for method, name, ns in ((1, "wout_sor", nsw), (0, "wout_mcsor", nsw), (0, "wout_mcsor1", 1)):
+        w = ctx.upload(z[pfx + "win"])
+        gpu.check(ctx.lib.x2g_relax_ordered(g.h, isym, 1, w.ptr, lhs.ptr, rhs.ptr, omega, ns, method, ordering))
+        got = w.get()
+        assert np.array_equal(got, z[pfx + name]), (name, relerr(got, z[pfx + name]))
+        w.free()
+    g.destroy()
+
+
+def test_ordered_potential_batch_vs_oracle(ctx):
+    """All potentials of the N2 pi orbital in one x2g_relax_batch call with
+    SOR in the middle ordering (each a workgroup of the level kernel), two
+    macro-iterations of putin, 10 sweeps, putout: equal to the oracle's
+    sequential sor on each, bit for bit."""
+    from x2dhf_amd import gpu
+    z = load_golden("n2_61x91")
+    nni, nmu = int(z["sizes"][0]), int(z["sizes"][1])
+    g = make_grid(ctx, z)
+    f3, e, gg = z["suppl"][7], z["suppl"][3], z["suppl"][9]
+    omega = float(z["scal"][5])
+    orb = z["orb_prep"]
+    items, keep, want = [], [], []
+    for in1, in2, dm, isym, _ in z["potlist"]:
+        rhs = orb[in1 - 1] * orb[in2 - 1] * gg
+        lhs = f3 + float(dm * dm) * e if dm else f3.copy()
+        k = len(want)
+        fun0 = z["pots_asympt"][k]
+        w = fun0.copy()
+        for _ in range(2):
+            wp = pyoracle.putin(nni, nmu, int(isym), w)
+            pyoracle.relax_ordered(nni, nmu, int(isym), wp, lhs, rhs, z["suppl"][1], z["suppl"][2], z["coef"][:16],
+                                   z["coef"][16:21], omega, 10, 1, "middle")
+            w = pyoracle.putout(nni, nmu, wp)
+        want.append(w)
+        fun, l, r = ctx.upload(fun0), ctx.upload(lhs), ctx.upload(rhs)
+        keep += [fun, l, r]
+        it = gpu.RelaxItem(fun.ptr, l.ptr, r.ptr, None, omega, int(isym), 1, 0, 0)
+        it.method, it.ordering = 1, 1
+        items.append(it)
+    arr = (gpu.RelaxItem * len(items))(*items)
+    gh = (C.c_void_p * 1)(g.h.value)
+    gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, arr, len(items), 2, 10))
+    for k, w in enumerate(want):
+        got = keep[3 * k].get()
+        assert np.array_equal(got, w), (k, relerr(got, w))
+    g.destroy()

@@ -193,3 +193,42 @@ def test_oracle_f32_twin_fixed_point():
     dw_bad, res_bad = run(-1.0)
     assert not (dw_bad < 1e-1), dw_bad
     assert not (res_bad < 1e-1), res_bad
+
+
+@pytest.mark.parametrize("case", ["h2_61x91_middle", "n2_61x91_middle"])
+@pytest.mark.parametrize("pfx", ["k1_", "k2_"])
+def test_oracle_middle_ordering_bitexact_vs_reference(case, pfx):
+    """sor and mcsor in the reference's default `middle` mesh ordering
+    (initCBAllocArrays.F90:2499-2540), 10 and 1 sweeps, against the reference's
+    own routines run on the same deck (mcsor with one thread: with several,
+    the reference's OpenMP passes race on the neighbouring points that this
+    ordering puts into one pass)."""
+    z = load_golden(case)
+    if pfx + "win" not in z:
+        pytest.skip("case has a single potential")
+    nni, nmu = int(z["sizes"][0]), int(z["sizes"][1])
+    isym = int(z[pfx + "isym"][0])
+    omega, nsw = float(z[pfx + "scal"][0]), int(z[pfx + "scal"][1])
+    args = (z[pfx + "lhs"], z[pfx + "rhs"], z["suppl"][1], z["suppl"][2], z["coef"][:16], z["coef"][16:21], omega)
+    for mode, name, ns in [(0, "wout_mcsor", nsw), (0, "wout_mcsor1", 1), (1, "wout_sor", nsw)]:
+        w = pyoracle.relax_ordered(nni, nmu, isym, z[pfx + "win"].copy(), *args, ns, mode, "middle")
+        assert _ok(w, z[pfx + name]), (case, pfx, name, np.abs(w - z[pfx + name]).max())
+
+
+def test_mesh_orderings_are_permutations():
+    """Both orderings visit every relaxed point once; col-wise is mu-major
+    with nu fastest, middle starts at the middle point and ends at the corner
+    (nu, mu) = (nni-1, nmu-4)."""
+    for nni, nmu in ((61, 91), (91, 121), (151, 211)):
+        n = (nni - 2) * (nmu - 5)
+        for o in ("col-wise", "middle"):
+            s = pyoracle.mesh(nni, nmu, o)
+            mu, nu = s // nni + 1, s % nni + 1
+            assert len(set(s.tolist())) == n and mu.min() == 2 and mu.max() == nmu - 4
+            assert nu.min() == 2 and nu.max() == nni - 1
+        s = pyoracle.mesh(nni, nmu, "middle")
+        assert (s[-1] // nni + 1, s[-1] % nni + 1) == (nmu - 4, nni - 1)
+        assert s[0] // nni + 1 == (nmu + 7) // 2 - 4 and s[0] % nni + 1 == (nni + 7) // 2 - 4
+        c = pyoracle.mesh(nni, nmu, "col-wise")
+        assert np.array_equal(c, np.array([(mu - 1) * nni + nu - 1 for mu in range(2, nmu - 3)
+                                           for nu in range(2, nni)]))

@@ -185,3 +185,52 @@ def test_fh_config3_481x901_mixed_precision_matches_reference_energy():
     assert res["iterations"] == len(ref["iterations"])
     assert abs(res["etot"] - ref["etot"]) < TOL_E
     assert np.abs(res["eps"] - np.array(ref["eps"])).max() < 1e-9
+
+
+def _listing(name):
+    with open(os.path.join(GOLDEN, "ref_listings.json")) as f:
+        return json.load(f)[name]
+
+
+@pytest.mark.parametrize("name", ["h2_set-01", "he_set-02"])
+def test_reference_default_scheme_decks_match_listings(name):
+    """The reference's own decks that relax with its default SOR (and, for
+    He, one-thread MCSOR for the orbital) in the default middle ordering run
+    through the same schemes on the GPU (x2g_relax_ordered's level kernel)
+    and stop at the total energy their reference listings print, within
+    1e-10 Eh."""
+    from x2dhf_amd import deck, scf
+    ref = _listing(name)
+    d = deck.load(os.path.join(GOLDEN, "decks", ref["deck"]))
+    print(f"{name}: {d.nni}x{d.nmu}, {d.notes[-1]}", flush=True)
+    job = scf.Job(d.spec(prec=0, schedule=0))
+    try:
+        res = job.run()
+    finally:
+        job.close()
+    print(f"{name}: E = {res['etot']:.16f} (listing {ref['etot']:.16f}, {ref['source']}) "
+          f"dE = {res['etot'] - ref['etot']:.2e}, iterations {res['iterations']}, stop '{res['stop']}'")
+    assert abs(res["etot"] - ref["etot"]) < TOL_E
+
+
+def test_reference_be_continuation_matches_listing():
+    """be/set-01: 10 SCF iterations with SOR in the middle ordering from the
+    reference's converged Be checkpoint (orbpot old) end at the total energy
+    of its listing within 1e-10 Eh."""
+    from x2dhf_amd import checkpoint as ckm, deck
+    ref = _listing("be_set-01")
+    d = deck.load(os.path.join(GOLDEN, "decks", ref["deck"]))
+    ck = ckm.read(os.path.join(GOLDEN, "be-hf", "be-hf"), ms=[m for m, _ in d.orbitals])
+    job = ckm.resume(d.spec(), ck)
+    try:
+        for _ in range(d.maxscf):
+            if job.iterate(1):
+                break
+        job.finish()
+        res = job.result()
+    finally:
+        job.close()
+    print(f"be_set-01: E = {res['etot']:.16f} (listing {ref['etot']:.16f}) dE = {res['etot'] - ref['etot']:.2e}, "
+          f"iterations {res['iterations']}, stop '{res['stop']}'")
+    assert res["iterations"] == 10
+    assert abs(res["etot"] - ref["etot"]) < TOL_E

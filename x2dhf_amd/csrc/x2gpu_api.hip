@@ -422,6 +422,10 @@ extern "C" int x2g_grid_destroy(x2g_grid *g)
                     g->dnu_f, g->vxi, g->vxisq, g->veta, g->vetasq};
     for (void *p : ptrs)
         if (p) x2g_free(c, p);
+    for (auto &kv : g->levels) {
+        if (kv.second.off) x2g_free(c, kv.second.off);
+        if (kv.second.pts) x2g_free(c, kv.second.pts);
+    }
     delete g;
     return X2G_OK;
 }

@@ -74,8 +74,18 @@ struct x2g_ctx {
     size_t scratch_bytes = 0;
 };
 
+// Level schedule of an ordered (MC)SOR sweep on one grid (device arrays):
+// phases off[k] .. off[k+1]-1 of pts (unpadded point indices) are mutually
+// independent and run in order.
+struct x2g_levels {
+    int *off = nullptr;
+    int *pts = nullptr;
+    int nphase = 0;
+};
+
 struct x2g_grid {
     x2g_ctx *ctx = nullptr;
+    std::map<int, x2g_levels> levels;  // by (method, ordering), built on first use
     int nni = 0, nmu = 0;
     double coef[25];               // dmu1,dmu2,dni1,dni2 (16), exeven (5), diag, hni, hmu, r
     double *suppl = nullptr;       // 14 * nni*nmu

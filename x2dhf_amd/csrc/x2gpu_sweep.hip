@@ -1951,6 +1951,214 @@ static int launch_restride(x2g_ctx *ctx, const std::vector<RestrideDesc> &v)
 using namespace x2g;
 
 // ---------------------------------------------------------------------------
+// Ordered (MC)SOR: the reference's sequential point order, exactly.
+//
+// sor relaxes the points one after another in the mesh ordering; mcsor (one
+// thread) in five passes over every fifth position of that ordering
+// (sormcsor.F90:20-132, 143-386).  A point reads the new value of a
+// neighbour visited before it and the old value of one visited after it, so
+// any schedule that updates a point after every earlier-visited neighbour
+// and before every later-visited one reproduces the sequential result bit
+// for bit.  The host gives every point a level (0 if no earlier-visited
+// neighbour shares its pass, else one more than the largest such
+// neighbour's level), and a sweep runs the passes in order, each pass its
+// levels in order; the points of one level are independent.  For sor in
+// col-wise order the levels are the anti-diagonals nu + mu; mcsor in
+// col-wise order needs one level per pass, in middle order a few more where
+// the ordering folds back.
+//
+// Mesh orderings (mesh, initCBAllocArrays.F90:2408-2540), rows mu = 2..nmu-4
+// and columns nu = 2..nni-1 of the relaxed interior:
+//   col-wise: rows in increasing mu, in each row nu increasing;
+//   middle: rows from the middle one down to mu = 2, then from the next one up
+//           to nmu-4; in each row nu from the middle down to 2, then up to
+//           nni-1 (the middle row / column is (n+7)/2 - 4 of n+8 padded).
+static void mesh_sequence(int nni, int nmu, int ordering, std::vector<int> &seq)
+{
+    seq.clear();
+    const int midr = (nmu + 7) / 2, midc = (nni + 7) / 2;   // padded indices
+    for (int i = 6; i <= nmu; ++i) {
+        const int ii = (ordering == X2G_ORDER_MIDDLE && i <= midr) ? midr - i + 6 : i;
+        for (int j = 6; j <= nni + 3; ++j) {
+            const int jj = (ordering == X2G_ORDER_MIDDLE && j <= midc) ? midc - j + 6 : j;
+            seq.push_back((ii - 5) * nni + (jj - 5));          // (mu-1)*nni + nu-1
+        }
+    }
+}
+
+static void build_levels(int nni, int nmu, int method, int ordering, std::vector<int> &off,
+                         std::vector<int> &pts)
+{
+    std::vector<int> seq;
+    mesh_sequence(nni, nmu, ordering, seq);
+    const int n = (int)seq.size(), npass = method == X2G_MCSOR ? 5 : 1;
+    std::vector<int> pos((size_t)nni * nmu, -1), lvl((size_t)nni * nmu, 0);
+    for (int k = 0; k < n; ++k) pos[seq[k]] = k;
+    off.assign(1, 0);
+    pts.clear();
+    std::vector<int> pl, cnt;
+    for (int c = 0; c < npass; ++c) {
+        pl.clear();
+        for (int k = c; k < n; k += npass) pl.push_back(seq[k]);
+        int maxl = 0;
+        for (int ik : pl) {
+            const int mu = ik / nni + 1, nu = ik % nni + 1, me = pos[ik];
+            int l = 0;
+            auto see = [&](int nu2, int mu2) {
+                if (nu2 < 2 || nu2 > nni - 1 || mu2 < 2 || mu2 > nmu - 4) return;   // not relaxed
+                const int q = (mu2 - 1) * nni + nu2 - 1, pq = pos[q];
+                if (pq < me && (pq % npass) == c) l = std::max(l, lvl[q] + 1);
+            };
+            for (int k = 1; k <= 4; ++k) {
+                see(nu - k, mu);
+                see(nu + k, mu);
+                see(nu, mu - k);
+                see(nu, mu + k);
+            }
+            lvl[ik] = l;
+            maxl = std::max(maxl, l);
+        }
+        // the pass's points by level, in sequence order within a level
+        cnt.assign(maxl + 2, 0);
+        for (int ik : pl) ++cnt[lvl[ik] + 1];
+        for (int l = 0; l <= maxl; ++l) cnt[l + 1] += cnt[l];
+        const size_t base = pts.size();
+        pts.resize(base + pl.size());
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int ik : pl) pts[base + fill[lvl[ik]]++] = ik;
+        for (int l = 0; l <= maxl; ++l) off.push_back((int)base + cnt[l + 1]);
+    }
+}
+
+// The level schedule of (method, ordering) on grid g, built and uploaded on
+// first use.
+static int grid_levels(x2g_grid *g, int method, int ordering, const x2g_levels **out)
+{
+    const int key = method * 16 + ordering;
+    auto it = g->levels.find(key);
+    if (it == g->levels.end()) {
+        std::vector<int> off, pts;
+        build_levels(g->nni, g->nmu, method, ordering, off, pts);
+        x2g_levels L;
+        int rc = x2g_malloc(g->ctx, off.size() * sizeof(int), (void **)&L.off);
+        if (!rc) rc = x2g_malloc(g->ctx, pts.size() * sizeof(int), (void **)&L.pts);
+        if (!rc) rc = x2g_h2d(g->ctx, L.off, off.data(), off.size() * sizeof(int));
+        if (!rc) rc = x2g_h2d(g->ctx, L.pts, pts.data(), pts.size() * sizeof(int));
+        if (rc) {
+            if (L.off) x2g_free(g->ctx, L.off);
+            if (L.pts) x2g_free(g->ctx, L.pts);
+            return rc;
+        }
+        L.nphase = (int)off.size() - 1;
+        it = g->levels.emplace(key, L).first;
+    }
+    *out = &it->second;
+    return X2G_OK;
+}
+
+struct LevelDesc {
+    double *f;                 // padded array (row stride nni+8), relaxed in place
+    const double *lhs, *rhs, *bmu, *dnu;
+    const int *off, *pts;      // level schedule
+    double c[16], ex[5];
+    double omega, omega1;
+    int nni, nmu, isym, nphase;
+};
+
+constexpr int kLevelThreads = 512;
+
+// All nsweeps sweeps of one problem per workgroup: the phases of the level
+// schedule in order, a barrier after each, then the boundary lines
+// (sormcsor.F90:94-129: the mu = 1 row, then the nu = 1 and nu = nni
+// columns, whose mu = 1 corners read the new row).  The point update keeps
+// the reference's expression and no fused multiply-add is formed, so the
+// result is the reference's bit for bit.
+__global__ void __launch_bounds__(kLevelThreads) k_levels(const LevelDesc *__restrict__ ld, int nsweeps)
+{
+#pragma clang fp contract(off)
+    const LevelDesc &d = ld[blockIdx.x];
+    const int nni = d.nni, nmu = d.nmu, s1 = nni + 8;
+    double *f = d.f;
+    const double *c = d.c;
+    const bool even = d.isym == 1;
+    for (int sw = 0; sw < nsweeps; ++sw) {
+        for (int ph = 0; ph < d.nphase; ++ph) {
+            const int a = d.off[ph], b = d.off[ph + 1];
+            for (int k = a + (int)threadIdx.x; k < b; k += blockDim.x) {
+                const int ik = d.pts[k];
+                const int mu = ik / nni + 1, nu = ik - (mu - 1) * nni + 1;
+                const int p = (mu + 3) * s1 + nu + 3;
+                const int q1 = s1, q2 = 2 * s1, q3 = 3 * s1, q4 = 4 * s1;
+                const double ddmi2 = c[4] * (f[p - q4] + f[p + q4]) + c[5] * (f[p - q3] + f[p + q3]) +
+                                     c[6] * (f[p - q2] + f[p + q2]) + c[7] * (f[p - q1] + f[p + q1]);
+                const double ddmi1 = c[0] * (f[p - q4] - f[p + q4]) + c[1] * (f[p - q3] - f[p + q3]) +
+                                     c[2] * (f[p - q2] - f[p + q2]) + c[3] * (f[p - q1] - f[p + q1]);
+                const double ddni2 = c[12] * (f[p - 4] + f[p + 4]) + c[13] * (f[p - 3] + f[p + 3]) +
+                                     c[14] * (f[p - 2] + f[p + 2]) + c[15] * (f[p - 1] + f[p + 1]);
+                const double ddni1 = c[8] * (f[p - 4] - f[p + 4]) + c[9] * (f[p - 3] - f[p + 3]) +
+                                     c[10] * (f[p - 2] - f[p + 2]) + c[11] * (f[p - 1] - f[p + 1]);
+                const double cc = ddmi2 + d.bmu[mu - 1] * ddmi1 + ddni2 + d.dnu[nu - 1] * ddni1;
+                f[p] = d.omega * (d.rhs[ik] - cc) / d.lhs[ik] + d.omega1 * f[p];
+            }
+            __syncthreads();
+        }
+        const double *ex = d.ex;
+        for (int nu = 2 + (int)threadIdx.x; nu <= nni - 1; nu += blockDim.x) {
+            const int p = 4 * s1 + nu + 3;
+            f[p] = even ? ex[0] * f[p + s1] + ex[1] * f[p + 2 * s1] + ex[2] * f[p + 3 * s1] +
+                              ex[3] * f[p + 4 * s1] + ex[4] * f[p + 5 * s1]
+                        : 0.0;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < 2 * (nmu - 4); t += blockDim.x) {
+            const int side = t / (nmu - 4), mu = 1 + t % (nmu - 4);
+            const int p = (mu + 3) * s1 + (side == 0 ? 1 : nni) + 3, dir = side == 0 ? 1 : -1;
+            f[p] = even ? ex[0] * f[p + dir] + ex[1] * f[p + 2 * dir] + ex[2] * f[p + 3 * dir] +
+                              ex[3] * f[p + 4 * dir] + ex[4] * f[p + 5 * dir]
+                        : 0.0;
+        }
+        __syncthreads();
+    }
+}
+
+static void fill_level_desc(LevelDesc &d, x2g_grid *g, const x2g_levels *L, double *f, const double *lhs,
+                            const double *rhs, int bkind, int isym, double omega)
+{
+    d.f = f;
+    d.lhs = lhs;
+    d.rhs = rhs;
+    d.bmu = bkind == 0 ? g->bmu_orb : g->bmu_pot;
+    d.dnu = g->dnu;
+    d.off = L->off;
+    d.pts = L->pts;
+    for (int i = 0; i < 16; ++i) d.c[i] = g->coef[i];
+    for (int i = 0; i < 5; ++i) d.ex[i] = g->coef[16 + i];
+    d.omega = omega;
+    d.omega1 = 1.0 - omega;
+    d.nni = g->nni;
+    d.nmu = g->nmu;
+    d.isym = isym;
+    d.nphase = L->nphase;
+}
+
+// nsweeps ordered sweeps of every problem in v (one workgroup each).
+static int launch_levels(x2g_ctx *ctx, const std::vector<LevelDesc> &v, int nsweeps)
+{
+    if (v.empty() || nsweeps <= 0) return X2G_OK;
+    const LevelDesc *dd;
+    int rc = stage_n(ctx, v.data(), v.size(), &dd);
+    if (rc) return rc;
+    if ((rc = stage_flush(ctx))) return rc;
+    double bytes = 0.0;
+    for (auto &d : v) bytes += 32.0 * (double)(d.nni - 2) * (double)(d.nmu - 5) * nsweeps;
+    hipEvent_t ea, eb;
+    if ((rc = stats_begin(ctx, &ea, &eb))) return rc;
+    X2G_LAUNCH(ctx, k_levels, dim3((int)v.size()), dim3(kLevelThreads), 0, dd, nsweeps);
+    X2G_HIP(hipGetLastError());
+    return stats_end(ctx, ea, eb, bytes, 0, nsweeps);
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 
 extern "C" int x2g_set_sweep_engine(int engine, int strip_rows)
@@ -2115,6 +2323,22 @@ extern "C" int x2g_mcsor_noalloc(x2g_grid *g, int isym, int bkind, double *work_
     return rc;
 }
 
+extern "C" int x2g_relax_ordered(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
+                                 const double *rhs_d, double omega, int nsweeps, int method, int ordering)
+{
+    int rc = mcsor_check(g, isym, bkind, work_d, lhs_d, rhs_d, omega, nsweeps, "x2g_relax_ordered");
+    if (rc) return rc;
+    X2G_CHECK(method == X2G_MCSOR || method == X2G_SOR, "x2g_relax_ordered: method must be X2G_MCSOR or X2G_SOR");
+    X2G_CHECK(ordering == X2G_ORDER_COLWISE || ordering == X2G_ORDER_MIDDLE,
+              "x2g_relax_ordered: ordering must be X2G_ORDER_COLWISE or X2G_ORDER_MIDDLE");
+    if (nsweeps == 0) return X2G_OK;
+    const x2g_levels *L;
+    if ((rc = grid_levels(g, method, ordering, &L))) return rc;
+    std::vector<LevelDesc> v(1);
+    fill_level_desc(v[0], g, L, work_d, lhs_d, rhs_d, bkind, isym, omega);
+    return launch_levels(g->ctx, v, nsweeps);
+}
+
 extern "C" int x2g_mcsor_host(x2g_grid *g, int isym, int bkind, double *work, const double *lhs,
                               const double *rhs, double omega, int nsweeps)
 {
@@ -2135,6 +2359,13 @@ extern "C" int x2g_mcsor_host(x2g_grid *g, int isym, int bkind, double *work, co
     if (l_d) x2g_free(ctx, l_d);
     if (r_d) x2g_free(ctx, r_d);
     return rc;
+}
+
+// An item relaxed in the reference's sequential order (any scheme other than
+// the col-wise MCSOR colour sweep).
+static inline bool ordered(const x2g_relax_item &it)
+{
+    return it.method != X2G_MCSOR || it.ordering != X2G_ORDER_COLWISE;
 }
 
 extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
@@ -2160,8 +2391,13 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         X2G_CHECK(it.bkind == 0 || it.bkind == 1, "x2g_relax_batch: item %d bkind", i);
         X2G_CHECK(it.prec == 0 || it.prec == 1, "x2g_relax_batch: item %d prec", i);
         X2G_CHECK(it.omega > 0.0 && it.omega < 2.0, "x2g_relax_batch: item %d omega must lie in (0, 2)", i);
+        X2G_CHECK(it.method == X2G_MCSOR || it.method == X2G_SOR, "x2g_relax_batch: item %d method", i);
+        X2G_CHECK(it.ordering == X2G_ORDER_COLWISE || it.ordering == X2G_ORDER_MIDDLE,
+                  "x2g_relax_batch: item %d ordering", i);
+        X2G_CHECK(it.prec == 0 || (it.method == X2G_MCSOR && it.ordering == X2G_ORDER_COLWISE),
+                  "x2g_relax_batch: item %d: FP32 correction sweeps are col-wise MCSOR only", i);
         if ((rc = check_grid_sweepable(g))) return rc;
-        (it.prec ? maxnni32 : maxnni64) = std::max(it.prec ? maxnni32 : maxnni64, g->nni);
+        if (!ordered(it)) (it.prec ? maxnni32 : maxnni64) = std::max(it.prec ? maxnni32 : maxnni64, g->nni);
     }
     const int kind64 = maxnni64 ? sweep_kind<double>(ctx, maxnni64) : 0;
     const int kind32 = maxnni32 ? sweep_kind<float>(ctx, maxnni32) : 0;
@@ -2171,6 +2407,7 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     };
     auto padbytes = [&](int i) {  // one padded array of item i
         x2g_grid *g = grids[items[i].grid_index];
+        if (ordered(items[i])) return g->npad() * sizeof(double);  // relaxed in place, stride nni+8
         return (size_t)pstride(i) * (g->nmu + 8) * (items[i].prec ? sizeof(float) : sizeof(double));
     };
     auto lrbytes = [&](int i) {  // (omega/lhs, rhs) pairs of item i
@@ -2183,7 +2420,9 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
         x2g_grid *g = grids[items[i].grid_index];
         o64[i] = off64;
         o32[i] = off32;
-        if (items[i].prec == 0)
+        if (ordered(items[i]))
+            off64 += align256(padbytes(i));
+        else if (items[i].prec == 0)
             off64 += 2 * align256(padbytes(i)) + align256(lrbytes(i));
         else
             off32 += 2 * align256(padbytes(i)) +
@@ -2194,8 +2433,8 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     char *w64 = (char *)ctx->ws[0].ptr;
     char *w32 = (char *)ctx->ws[3].ptr;
 
-    std::vector<int> i64, i32;  // item indices by precision
-    for (int i = 0; i < n; ++i) (items[i].prec ? i32 : i64).push_back(i);
+    std::vector<int> i64, i32, iord;  // item indices by precision; ordered (MC)SOR items
+    for (int i = 0; i < n; ++i) (ordered(items[i]) ? iord : items[i].prec ? i32 : i64).push_back(i);
 
     auto bufA = [&](int i) { return (double *)(w64 + o64[i]); };
     auto bufB = [&](int i) { return (double *)(w64 + o64[i] + align256(padbytes(i))); };
@@ -2219,7 +2458,10 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     for (int i = 0; i < n; ++i) {
         const x2g_relax_item &it = items[i];
         x2g_grid *g = grids[it.grid_index];
-        if (it.prec == 0) pin.push_back({it.fun_d, bufA(i), bufB(i), g->nni, g->nmu, it.isym, pstride(i)});
+        if (ordered(it))
+            pin.push_back({it.fun_d, bufA(i), nullptr, g->nni, g->nmu, it.isym, g->nni + 8});
+        else if (it.prec == 0)
+            pin.push_back({it.fun_d, bufA(i), bufB(i), g->nni, g->nmu, it.isym, pstride(i)});
         if (it.tail_fock1_d)
             tails.push_back({it.fun_d, it.tail_fock1_d, g->suppl + 5 * g->n(), g->vxisq, g->vetasq, g->nni, g->nmu});
     }
@@ -2232,7 +2474,9 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     {
         std::vector<ProdDesc> pv;
         long maxn = 0;
-        for (int i : i64) {
+        std::vector<int> prod_items(i64);
+        prod_items.insert(prod_items.end(), iord.begin(), iord.end());
+        for (int i : prod_items) {
             const x2g_relax_item &it = items[i];
             if (!it.rhs_f1_d) continue;
             const long np = grids[it.grid_index]->n();
@@ -2289,6 +2533,22 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
     }
     const PadDesc *d_pout64 = nullptr;
     if (!pout64.empty() && (rc = stage_descs(ctx, pout64, doff, &d_pout64))) return rc;
+
+    // ordered (MC)SOR items: level schedules, putout descriptors
+    std::vector<LevelDesc> lev;
+    std::vector<PadDesc> pout_ord;
+    for (int i : iord) {
+        const x2g_relax_item &it = items[i];
+        x2g_grid *g = grids[it.grid_index];
+        const x2g_levels *L;
+        if ((rc = grid_levels(g, it.method, it.ordering, &L))) return rc;
+        LevelDesc d{};
+        fill_level_desc(d, g, L, bufA(i), it.lhs_d, it.rhs_d, it.bkind, it.isym, it.omega);
+        lev.push_back(d);
+        pout_ord.push_back({it.fun_d, bufA(i), nullptr, g->nni, g->nmu, 1, g->nni + 8});
+    }
+    const PadDesc *d_pout_ord = nullptr;
+    if (!pout_ord.empty() && (rc = stage_descs(ctx, pout_ord, doff, &d_pout_ord))) return rc;
 
     // FP32 items: conversion, residual, sweeps, correction
     SweepPlan<float> p32;
@@ -2406,6 +2666,12 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             if ((rc = launch_sweeps<double>(ctx, d64e, d64o, p64, maxsor2))) return rc;
             X2G_LAUNCH(ctx, k_putout, dim3(grid_x_for(maxn, 256), (int)i64.size()), dim3(256), 0,
                                d_pout64);
+            X2G_HIP(hipGetLastError());
+        }
+        if (!iord.empty() && maxsor2 > 0) {
+            if ((rc = launch_levels(ctx, lev, maxsor2))) return rc;
+            X2G_LAUNCH(ctx, k_putout, dim3(grid_x_for(maxn, 256), (int)iord.size()), dim3(256), 0,
+                               d_pout_ord);
             X2G_HIP(hipGetLastError());
         }
         // (no sweeps: u stays as it is, as in the FP64 path; the correction

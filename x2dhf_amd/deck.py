@@ -24,8 +24,11 @@ relaxation path and builds the scf.JobSpec the GPU driver runs:
   omega w_orb [w_pot]                          (read_omega, 2186-2222)
   multipol recalc [mpole]                      (read_multipol, 2124-2153)
   sor [orb [pot]], sor4orb n [m], sor4pot n [m]   sweep counts (read_sor, 2806-2880)
-  mcsor, mcsor-o, mcsor-ce, order <name>      accepted: the GPU path always relaxes
-                                              with MCSOR in the col-wise colouring
+  mcsor [n ..], mcsor-o [..]         orbitals by MCSOR (read_mcsor, read_mcsor_o, 1915-1995)
+  mcsor-ce [..]                      potentials by MCSOR (read_mcsor_ce, 1997-2028); without
+                                     these cards both are relaxed by SOR, the reference's default
+  order col-wise|middle              mesh ordering (read_order; default middle,
+                                     initVariables.F90:206)
   print, prtevery, mmoments, plot, out4pair, debug, inout, stop   ignored
 
 Any other card (DFT, external fields, fixed orbitals, other start orbitals,
@@ -40,7 +43,7 @@ from . import scf
 BOHR2ANG = 0.529177210671212  # commons.F90:21
 _SYM_M = {"sigma": 0, "pi": 1, "delta": 2, "phi": 3}
 _IGNORED = {"title", "print", "prtevery", "mmoments", "plot", "out4pair", "debug", "inout", "stop"}
-_SOLVER = {"mcsor", "mcsor-o", "mcsor-ce", "order"}
+_ORDERINGS = {"col-wise": 0, "middle": 1}
 
 
 class DeckError(ValueError):
@@ -95,6 +98,9 @@ class Deck:
     maxsororb: tuple = (1, 10)
     maxsorpot: tuple = (1, 10)
     start: str = "hydrogen"                        # orbpot: hydrogen | old
+    orb_method: int = 1                            # 0 MCSOR, 1 SOR (the reference's default)
+    pot_method: int = 1
+    ordering: int = 1                              # 0 col-wise, 1 middle (the default)
     notes: list = field(default_factory=list)
 
     def spec(self, prec=0, schedule=0, device=0, polish=0):
@@ -112,7 +118,8 @@ class Deck:
                              ienterm=self.ienterm, inoterm=self.inoterm, omega=self.omega, prec=prec,
                              schedule=schedule, device=device, maxsor=self.maxsororb, maxsorpot=self.maxsorpot,
                              mpole=self.mpole, ienterm4init=self.ienterm4init, recalc_mm=self.recalc_mm,
-                             conv=self.conv, polish=polish)
+                             conv=self.conv, polish=polish, orb_method=self.orb_method,
+                             pot_method=self.pot_method, ordering=self.ordering)
 
 
 def _cards(text):
@@ -286,8 +293,15 @@ def parse(text):
                 d.maxsororb = cur
             else:
                 d.maxsorpot = cur
-        elif label in _SOLVER:
-            d.notes.append(f"{label} {' '.join(items)}".strip())
+        elif label in ("mcsor", "mcsor-o"):
+            d.orb_method = 0
+        elif label == "mcsor-ce":
+            d.pot_method = 0
+        elif label == "order":
+            name = _word(items[0]) if items else ""
+            if name not in _ORDERINGS:
+                raise DeckError(f"order '{name}': this build relaxes in the col-wise or middle ordering")
+            d.ordering = _ORDERINGS[name]
         else:
             raise DeckError(f"card '{label}' is outside the relaxation path of this build")
     for need in ("nuclei", "config", "grid", "orbpot") + (("lcao",) if d.start == "hydrogen" else ()):
@@ -295,9 +309,9 @@ def parse(text):
             raise DeckError(f"the deck has no '{need}' card")
     d.nni = nnu_calc(d.nnu_card)
     d.nmu = nmu_calc(d.nmu_card, d.nni, d.r, d.rinf)
-    if not ("mcsor" in seen and "mcsor-ce" in seen and any(n.startswith("order col-wise") for n in d.notes)):
-        d.notes.append("relaxed with MCSOR in the col-wise colouring (the deck asks for another scheme: "
-                       "same converged solution, different iteration history)")
+    scheme = {0: "MCSOR", 1: "SOR"}
+    order = {v: k for k, v in _ORDERINGS.items()}[d.ordering]
+    d.notes.append(f"orbitals by {scheme[d.orb_method]}, potentials by {scheme[d.pot_method]}, {order} ordering")
     return d
 
 

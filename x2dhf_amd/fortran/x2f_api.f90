@@ -25,6 +25,9 @@ module x2f_api
      integer(c_int) :: nscf2skip, nenlast, nnolast
      ! FP64 finish of a mixed-precision job (scf_opts%polish); 0 = none
      integer(c_int) :: polish
+     ! relaxation scheme of the FP64 sweeps: orbitals / potentials 0 MCSOR,
+     ! 1 SOR; mesh ordering 0 col-wise, 1 middle (x2g_relax_item)
+     integer(c_int) :: orb_method, pot_method, ordering
   end type x2f_spec
 
   ! job results (mirrored by x2dhf_amd/scf.py: JobResult)
@@ -216,6 +219,7 @@ contains
       j%o%mpole = spec%mpole; j%o%prec = spec%prec; j%o%schedule = spec%schedule
       j%o%omega_orb = spec%omega_orb; j%o%omega_pot = spec%omega_pot; j%o%recalc_mm = spec%recalc_mm
       if (spec%polish > 0) j%o%polish = spec%polish
+      j%o%orb_method = spec%orb_method; j%o%pot_method = spec%pot_method; j%o%ordering = spec%ordering
       if (spec%nscf2skip > 0) j%o%nscf2skip = spec%nscf2skip
       if (spec%nenlast > 0) j%o%nenlast = spec%nenlast
       if (spec%nnolast > 0) j%o%nnolast = spec%nnolast

@@ -25,7 +25,8 @@ class RelaxItem(C.Structure):
     """x2g_relax_item"""
     _fields_ = [("fun_d", _vp), ("lhs_d", _vp), ("rhs_d", _vp), ("tail_fock1_d", _vp),
                 ("omega", _dbl), ("isym", _int), ("bkind", _int), ("prec", _int),
-                ("grid_index", _int), ("rhs_f1_d", _vp), ("rhs_f2_d", _vp), ("rhs_f3_d", _vp)]
+                ("grid_index", _int), ("rhs_f1_d", _vp), ("rhs_f2_d", _vp), ("rhs_f3_d", _vp),
+                ("method", _int), ("ordering", _int)]
 
 
 class Term(C.Structure):
@@ -104,6 +105,7 @@ _PROTOS = {
     "x2g_mcsor": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
     "x2g_mcsor_host": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
     "x2g_mcsor_noalloc": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
+    "x2g_relax_ordered": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int, _int, _int]),
     "x2g_relax_batch": (_int, [_vp, _vp, C.POINTER(RelaxItem), _int, _int, _int]),
     "x2g_residual_norms": (_int, [_vp, _vp, C.POINTER(RelaxItem), _int, _vp]),
     "x2g_relax_workspace_bytes": (C.c_longlong, [_int, _int, _int]),
