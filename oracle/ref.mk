@@ -5,6 +5,15 @@
 #
 #   make -f oracle/ref.mk            # serial build   -> oracle/_ref/x2dhf
 #   make -f oracle/ref.mk omp        # OpenMP build   -> oracle/_ref/x2dhf_omp
+#   make -f oracle/ref.mk pthread    # pthreads build -> oracle/_ref/x2dhf_pt
+#     (the reference's USE_PTHREAD variant: Fortran with -DPTHREAD, the MCSOR
+#      and potential threads of src/c with gcc -pthread)
+#   make -f oracle/ref.mk tpool      # pthreads thread-pool build -> oracle/_ref/x2dhf_tp
+#     (USE_TPOOL: the same relaxation routines on persistent pthreads)
+# The USE_PTHREAD build aborts ("stack smashing detected") as soon as it
+# relaxes with more than one MCSOR thread: its pthread_join writes a void *
+# into an int (src/c/mcsor_pthread.c:83-87, src/c/coulExch_pthread.c:119-120).
+# The thread-pool build joins with NULL and runs; it is the CPU baseline.
 #
 # Internal BLAS is used (the reference's USE_BLAS=OFF), so every operation on
 # the relaxation path is the reference's own code.
@@ -24,10 +33,17 @@ SRCS = dgamit.F params.F90 blas.F90 card.f90 discret.f90 checkOrbSym.f90 memory.
   prepGauss.F90 storedOrbitals.F90 initOrbitalsPotentials.F90 initVariables.F90 \
   inputInterface.F90 machine.F90 printCase.F90 summary.F90 x2dhf.F90
 
-.PHONY: serial omp all
+.PHONY: serial omp pthread tpool all
 serial: $(OUT)/x2dhf
 omp: $(OUT)/x2dhf_omp
-all: serial omp
+pthread: $(OUT)/x2dhf_pt
+tpool: $(OUT)/x2dhf_tp
+all: serial omp pthread tpool
+
+CSRCS = sorpt.c mcsor_pthread.c coulExch_pthread.c relax_single_pot_pthread.c mcsor_single_colour_pthread.c
+TSRCS = sorpt.c mcsor_tpool.c coulExch_tpool.c relax_single_pot_tpool.c mcsor_single_colour_tpool.c
+REF_CC ?= gcc
+CFLAGS ?= -O2
 
 # Sources are compiled one at a time in dependency order (module files are
 # produced as a side effect), so a single recipe per variant is enough.
@@ -42,6 +58,22 @@ $(OUT)/x2dhf_omp: $(addprefix $(REFSRC)/,$(SRCS))
 	cd $(OUT)/omp && for f in $(SRCS); do \
 	  $(REF_FC) $(FFLAGS) -fopenmp -I$(REFSRC) -c $(REFSRC)/$$f || exit 1; done
 	cd $(OUT)/omp && $(REF_FC) $(FFLAGS) -fopenmp -o ../x2dhf_omp *.o
+
+$(OUT)/x2dhf_pt: $(addprefix $(REFSRC)/,$(SRCS)) $(addprefix $(REFSRC)/c/,$(CSRCS))
+	mkdir -p $(OUT)/pt
+	cd $(OUT)/pt && for f in $(SRCS); do \
+	  $(REF_FC) $(FFLAGS) -DPTHREAD -I$(REFSRC) -c $(REFSRC)/$$f || exit 1; done
+	cd $(OUT)/pt && for f in $(CSRCS); do \
+	  $(REF_CC) $(CFLAGS) -pthread -DPTHREAD -I$(REFSRC) -I$(REFSRC)/c -c $(REFSRC)/c/$$f || exit 1; done
+	cd $(OUT)/pt && $(REF_FC) $(FFLAGS) -o ../x2dhf_pt *.o -lpthread
+
+$(OUT)/x2dhf_tp: $(addprefix $(REFSRC)/,$(SRCS)) $(addprefix $(REFSRC)/c/,$(TSRCS))
+	mkdir -p $(OUT)/tp
+	cd $(OUT)/tp && for f in $(SRCS); do \
+	  $(REF_FC) $(FFLAGS) -DTPOOL -I$(REFSRC) -c $(REFSRC)/$$f || exit 1; done
+	cd $(OUT)/tp && for f in $(TSRCS); do \
+	  $(REF_CC) $(CFLAGS) -pthread -DTPOOL -I$(REFSRC) -I$(REFSRC)/c -c $(REFSRC)/c/$$f || exit 1; done
+	cd $(OUT)/tp && $(REF_FC) $(FFLAGS) -o ../x2dhf_tp *.o -lpthread
 
 # Golden-vector harness: oracle/ref_harness.f90 linked against the OpenMP
 # objects of the reference (everything except its main program).

@@ -153,3 +153,21 @@ def test_relaxation_scheme_cards():
         assert (s.orb_method, s.pot_method, s.ordering) == want
     with pytest.raises(deck.DeckError):
         deck.parse(base + "order row-wise\nstop\n")
+
+
+def test_spec_to_deck_round_trip():
+    """to_deck writes the reference deck of a job specification: parsing it
+    gives the same specification (served-stream molecules, both schemes)."""
+    from x2dhf_amd import deck, serve
+    for _, name, spec in serve.poisson_stream(12, 10.0, seed=4, prec=0):
+        spec.schedule = 0
+        txt = deck.to_deck(spec, title=name)
+        back = deck.parse(txt).spec()
+        # the reference rounds the grid down to 1 mod 30 (nnucalc / nmucalc)
+        assert (back.nni, back.nmu) == (deck.nnu_calc(spec.nni), deck.round_to_mesh(spec.nmu))
+        back.nni, back.nmu = spec.nni, spec.nmu
+        assert _fields(back) == _fields(spec), name
+    d = deck.load(os.path.join(DECKS, "config3_fh_481x901.data"))
+    s = d.spec()
+    assert _fields(deck.parse(deck.to_deck(s)).spec()) == _fields(s)
+    assert "mcsor 8\nmcsor-ce" in deck.to_deck(s, mcsor_threads=8)

@@ -315,6 +315,43 @@ def parse(text):
     return d
 
 
+def to_deck(spec, title="job", mcsor_threads=None):
+    """The reference deck of a job specification (the reverse of parse, for
+    the cards this path reads): the reference program runs it to the same
+    SCF.  mcsor_threads: the thread count of the reference's mcsor card
+    (orbitals by MCSOR with that many threads); None keeps the spec's
+    schemes as cards.  Only the hydrogenic LCAO start is written."""
+    sym = {0: "sigma", 1: "pi", 2: "delta", 3: "phi"}
+    lines = [f"title {title}", "method hf", f"nuclei {spec.z1!r} {spec.z2!r} {spec.r!r}", "config 0"]
+    for k in range(spec.norb):
+        gu = {1: " g", -1: " u"}.get(spec.gu[k], "")
+        lines.append(f"  1 {sym[spec.m[k]]}{gu}" + (" end" if k == spec.norb - 1 else ""))
+    lines.append(f"grid {spec.nni} {spec.nmu} {spec.rinf!r}")
+    lines.append("orbpot hydrogen" + (f" {spec.ienterm4init}" if spec.ienterm4init != 1 else ""))
+    lines.append("lcao")
+    for k in range(spec.norb):
+        lines.append(f"  {spec.c1[k]!r} {spec.n1[k]} {spec.l1[k]} {spec.za[k]!r}   "
+                     f"{spec.c2[k]!r} {spec.n2[k]} {spec.l2[k]} {spec.zb[k]!r}")
+    if spec.omega_orb > 0:
+        lines.append(f"omega {spec.omega_orb!r}" + (f" {spec.omega_pot!r}" if spec.omega_pot > 0 else ""))
+    lines.append(f"scf {spec.maxscf} 1000 {spec.ienterm} {spec.inoterm} 3")
+    if spec.nscf2skip > 0:
+        lines.append(f"conv {spec.nscf2skip}" + (f" {spec.nenlast}" if spec.nenlast > 0 else "") +
+                     (f" {spec.nnolast}" if spec.nnolast > 0 else ""))
+    if spec.maxsororb[1] != 10 or spec.maxsorpot[1] != 10:
+        lines.append(f"sor {spec.maxsororb[1]} {spec.maxsorpot[1]}")
+    lines.append(f"order {'middle' if spec.ordering == 1 else 'col-wise'}")
+    if mcsor_threads is not None:
+        lines += [f"mcsor {int(mcsor_threads)}", "mcsor-ce"]
+    else:
+        if spec.orb_method == 0:
+            lines.append("mcsor")
+        if spec.pot_method == 0:
+            lines.append("mcsor-ce")
+    lines.append("stop")
+    return "\n".join(lines) + "\n"
+
+
 def load(path):
     """Parse the deck file at path."""
     with open(path) as f:
