@@ -5,7 +5,7 @@ Default workload (--workload stream): the served stream of closed-shell
 diatomic HF requests -- Poisson arrivals at 20/s (seeded) of neutral
 closed-shell diatomics with Z1, Z2 in [1, 10], R in [1, 6] bohr, 1-5 orbitals,
 grids from 169x241 to 505x901 (x2dhf_amd.serve.poisson_stream), each
-converged to orbital-energy and norm changes below 1e-12 by the
+converged to orbital-energy changes below 1e-11 by the
 continuous-batching server with FP32 MCSOR correction sweeps and FP64
 residuals, norms, orthogonalisation, Lagrange multipliers and energies.  One
 step = one stream of --count requests.  value = converged solves per hour.
@@ -68,7 +68,7 @@ def parse():
                          "n2: waves of the N2 331x601 deck; poisson: one batch of seeded synthetic Poisson "
                          "problems on mixed grids")
     ap.add_argument("--count", type=int, default=None,
-                    help="problems (poisson, default 256) or requests per stream (stream, default 24)")
+                    help="problems (poisson, default 256) or requests per stream (stream, default 64)")
     ap.add_argument("--rate", type=float, default=20.0, help="stream: mean arrivals per second")
     ap.add_argument("--plain-sample", type=int, default=6,
                     help="stream: requests also run one at a time on the plain FP64 GPU path")
@@ -262,9 +262,6 @@ def bench_poisson(args):
     print(json.dumps(line), flush=True)
 
 
-CONVERGED = ("orbital energy threshold", "orbital norm threshold", "saturation", "resets")
-
-
 def serve_stream(reqs, prec, stats=False):
     """Serve (arrival_s, name, spec) requests with the continuous-batching
     server at the given precision; returns (handles, elapsed_s, sweep stats)."""
@@ -345,7 +342,7 @@ def bench_stream(args):
     sweeps, served by the continuous-batching server; latency = arrival ->
     converged energy."""
     from x2dhf_amd import scf, serve
-    count = args.count or 24
+    count = args.count or 64
     for w in range(args.warmup):
         serve_stream(serve.poisson_stream(min(8, count), args.rate, seed=1000 + w, prec=args.prec), args.prec)
         print(f"# warmup stream {w + 1} done", file=sys.stderr, flush=True)
@@ -359,7 +356,8 @@ def bench_stream(args):
         st = [a + b for a, b in zip(st, s)]
         print(f"# timed stream {k + 1}: {count} requests in {e:.2f} s", file=sys.stderr, flush=True)
     conv = [h.result["stop"] for h in handles]
-    nconv = sum(1 for c in conv if c in CONVERGED)
+    ok = [serve.converged(h.result) for h in handles]
+    nconv = sum(ok)
     lat = [h.latency for h in handles]
     value = nconv / elapsed * 3600.0
     fp64 = None
@@ -371,14 +369,16 @@ def bench_stream(args):
             h64, e64, _ = serve_stream(reqs, 0)
             h64_all += h64
             e64_all += e64
-        dE64 = [abs(a.result["etot"] - b.result["etot"]) for a, b in zip(handles, h64_all)]
-        n64 = sum(1 for h in h64_all if h.result["stop"] in CONVERGED)
+        dE64 = [abs(a.result["etot"] - b.result["etot"]) for a, b, c in zip(handles, h64_all, ok)
+                if c and serve.converged(b.result)]
+        n64 = sum(1 for h in h64_all if serve.converged(h.result))
         fp64 = {"solves_per_hr": n64 / e64_all * 3600.0, "elapsed_s": e64_all,
                 "p50_latency_s": serve.percentile([h.latency for h in h64_all], 50),
                 "p99_latency_s": serve.percentile([h.latency for h in h64_all], 99),
-                "max_abs_dE_mixed_vs_fp64": max(dE64)}
-        print(f"# FP64 served: {e64_all:.2f} s, max |dE| mixed vs FP64 {max(dE64):.2e}", file=sys.stderr,
-              flush=True)
+                "max_abs_dE_mixed_vs_fp64": max(dE64) if dE64 else None,
+                "compared": len(dE64)}
+        print(f"# FP64 served: {e64_all:.2f} s, max |dE| mixed vs FP64 {fp64['max_abs_dE_mixed_vs_fp64']} over "
+              f"{len(dE64)} requests converged in both", file=sys.stderr, flush=True)
     plain = None
     if not args.no_plain and args.plain_sample > 0:
         plain_s, dEp, its = 0.0, [], 0
@@ -392,13 +392,14 @@ def bench_stream(args):
                 plain_s += time.perf_counter() - t1
             finally:
                 job.close()
-            dEp.append(abs(res["etot"] - h.result["etot"]))
+            if serve.converged(res) and serve.converged(h.result):
+                dEp.append(abs(res["etot"] - h.result["etot"]))
             its += res["iterations"]
-        n = len(dEp)
+        n = min(args.plain_sample, len(timed))
         plain = {"sample": f"the first {n} requests of the timed stream, one at a time", "solve_s_total": plain_s,
                  "seconds_per_solve": plain_s / n, "solves_per_hr": n / plain_s * 3600.0,
-                 "max_abs_dE_vs_served": max(dEp)}
-        print(f"# plain FP64 GPU, {n} requests one at a time: {plain_s:.2f} s, max |dE| {max(dEp):.2e}",
+                 "max_abs_dE_vs_served": max(dEp) if dEp else None}
+        print(f"# plain FP64 GPU, {n} requests one at a time: {plain_s:.2f} s, max |dE| {plain['max_abs_dE_vs_served']}",
               file=sys.stderr, flush=True)
     roof = sweep_roofline(st, args.prec)
     if os.path.exists(TRAFFIC) and roof.get("algorithmic_bytes_per_launch"):
@@ -421,7 +422,7 @@ def bench_stream(args):
                 f"{args.rate}/s, hydrogenic LCAO starts",
         "config": {"workload": "served stream (configuration 5): neutral closed-shell diatomics, Z1 and Z2 in "
                                "[1,10], R in [1,6] bohr, 1-5 orbitals, grids 169x241..505x901, R_inf 40, HF SCF "
-                               "to orbital-energy and norm changes < 1e-12 (10 MCSOR sweeps per relaxation), "
+                               "to orbital-energy changes < 1e-11 (10 MCSOR sweeps per relaxation), "
                                "continuous batching (at most 16 jobs stepped together, admission by HBM "
                                "footprint)", "requests_per_step": count, "rate_per_s": args.rate,
                    "precision": "FP32 MCSOR correction sweeps, FP64 residual/norm/orthogonalisation/"
@@ -434,8 +435,8 @@ def bench_stream(args):
         "speedup_vs_fp64_served": value / fp64["solves_per_hr"] if fp64 else None,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain_fp64_gpu": value / plain["solves_per_hr"] if plain else None,
-        "max_abs_dE_vs_fp64": max(dE64 + ([plain["max_abs_dE_vs_served"]] if plain else [])) if (dE64 or plain)
-        else None,
+        "max_abs_dE_vs_fp64": max(dE64 + ([plain["max_abs_dE_vs_served"]] if plain and plain["max_abs_dE_vs_served"]
+                                          is not None else [])) if dE64 else None,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
