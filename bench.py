@@ -335,13 +335,26 @@ def cpu_baseline_stream(reqs, budget_s):
                        f"runs of the stream's smallest requests, one at a time: " + "; ".join(names))
 
 
+def autotune_summary(report):
+    """The sweep plans the autotuner chose: batches served by each choice of
+    (block rows, workgroups per CU, split) over the tuned batch shapes."""
+    import re
+    chosen = {}
+    for ln in report:
+        m = re.search(r"br (\d+) wpc (\d+) split (\d+) \(batches (\d+)\)", ln)
+        if m:
+            k = f"br{m.group(1)} wpc{m.group(2)} split{m.group(3)}"
+            chosen[k] = chosen.get(k, 0) + int(m.group(4))
+    return {"shapes": len(report), "batches_by_plan": chosen}
+
+
 def bench_stream(args):
     """Configuration 5, the headline: Poisson-arrival SCF requests (seeded,
     --rate per second) of closed-shell diatomics with Z1, Z2 in [1, 10], R in
     [1, 6] bohr, 1-5 orbitals, grids 169x241 .. 505x901, FP32 correction
     sweeps, served by the continuous-batching server; latency = arrival ->
     converged energy."""
-    from x2dhf_amd import scf, serve
+    from x2dhf_amd import gpu, scf, serve
     count = args.count or 64
     for w in range(args.warmup):
         serve_stream(serve.poisson_stream(min(8, count), args.rate, seed=1000 + w, prec=args.prec), args.prec)
@@ -438,6 +451,7 @@ def bench_stream(args):
         "max_abs_dE_vs_fp64": max(dE64 + ([plain["max_abs_dE_vs_served"]] if plain and plain["max_abs_dE_vs_served"]
                                           is not None else [])) if dE64 else None,
         "roofline": roof,
+        "autotune": autotune_summary(gpu.tune_report()),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

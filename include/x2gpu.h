@@ -139,6 +139,26 @@ const double *x2g_grid_suppl(x2g_grid *g, int k);
  * multiple of 4), <= 0 lets the library choose. */
 int x2g_set_sweep_engine(int engine, int strip_rows);
 
+/* Sweep autotuner (on by default; X2G_AUTOTUNE=0 turns it off): on the first
+ * relaxation of a batch shape (precision, ring class of the widest grid,
+ * number of ring classes, total rows rounded to a power of sqrt 2) the
+ * candidate k_sweep plans -- blocks of 4 or 2 rows, workgroups per CU 1, 2, 3
+ * or as many as the LDS ring allows, one launch or one per ring class -- are
+ * timed on that batch and the fastest is kept for the shape.  on = 0 / 1
+ * turns it off / on, -1 forgets the tuned shapes.  No reference counterpart. */
+int x2g_set_autotune(int on);
+/* Fix the k_sweep plan for later calls of this process: block_rows 4 or 2
+ * (0: 4), wg_per_cu workgroups per CU the segment planner fills (0: as many
+ * as the LDS ring allows), split 1 = one launch per ring class.  Any nonzero
+ * value keeps the autotuner out; (0, 0, 0) hands the choice back to it.  The
+ * results do not depend on the plan.  No reference counterpart. */
+int x2g_set_sweep_plan(int block_rows, int wg_per_cu, int split);
+/* The tuned shapes as text, one line per shape: "key: br B wpc W split S
+ * (batches N); br/wpc/split time ..." (times per sweep of the candidates).
+ * Writes at most len bytes including the terminating 0; returns the full
+ * length. */
+int x2g_tune_report(char *buf, int len);
+
 /* putin (inout.F90:23): embed fun_d (nni*nmu) into the padded work_d and fill
  * the mirror halos by parity isym (+1 even, -1 odd). */
 int x2g_putin(x2g_grid *g, int isym, const double *fun_d, double *work_d);

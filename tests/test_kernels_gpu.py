@@ -24,6 +24,7 @@ def ctx():
     c = gpu.Context(0)
     yield c
     gpu.load().x2g_set_sweep_engine(0, -1)
+    gpu.load().x2g_set_sweep_plan(0, 0, 0)
     c.close()
 
 
@@ -38,10 +39,18 @@ def relerr(a, b):
 
 
 def engines():
-    # (engine, strip rows): the default sweep kernel with the library's strip
-    # height and with a forced small strip (several strips per problem), the
-    # first strip kernel likewise, and the per-pass kernels
-    return [(0, -1), (0, 24), (2, -1), (2, 32), (1, -1)]
+    # (engine, strip rows, block rows): the default sweep kernel with the
+    # library's strip height and with a forced small strip (several strips per
+    # problem), in blocks of 4 and of 2 rows, the first strip kernel likewise,
+    # and the per-pass kernels
+    return [(0, -1, 4), (0, 24, 4), (0, -1, 2), (0, 22, 2), (2, -1, 4), (2, 32, 4), (1, -1, 4)]
+
+
+def set_engine(ctx, eng):
+    from x2dhf_amd import gpu
+    engine, rows, br = eng
+    gpu.check(ctx.lib.x2g_set_sweep_engine(engine, rows))
+    gpu.check(ctx.lib.x2g_set_sweep_plan(br if br != 4 else 0, 0, 0))
 
 
 @pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
@@ -53,7 +62,7 @@ def test_mcsor_vs_reference(ctx, case, pfx, eng):
     if pfx + "win" not in z:
         pytest.skip("single potential")
     print(f"case={case} {pfx} engine={eng}", flush=True)
-    gpu.check(ctx.lib.x2g_set_sweep_engine(*eng))
+    set_engine(ctx, eng)
     g = make_grid(ctx, z)
     isym = int(z[pfx + "isym"][0])
     omega, nsw = float(z[pfx + "scal"][0]), int(z[pfx + "scal"][1])
@@ -100,7 +109,7 @@ def test_potential_batch_vs_reference(ctx, eng):
     parities, Delta m = 0, 1, 2) relaxed in one batch = the reference's
     coulExchMCSOR (relaxPots.F90:196-322)."""
     from x2dhf_amd import gpu
-    gpu.check(ctx.lib.x2g_set_sweep_engine(*eng))
+    set_engine(ctx, eng)
     for case in ("h2_61x91", "n2_61x91"):
         z = load_golden(case)
         print(f"case={case} engine={eng}", flush=True)
@@ -133,7 +142,7 @@ def test_orbital_relaxation_vs_reference(ctx, eng):
     """orbMCSOR (relaxOrbs.F90:150-241): lhs = fock1+diag, rhs = fock2, ten
     sweeps, then the exponential tail of the orbital."""
     from x2dhf_amd import gpu
-    gpu.check(ctx.lib.x2g_set_sweep_engine(*eng))
+    set_engine(ctx, eng)
     for case, isym in (("h2_61x91", 1), ("n2_61x91", -1)):
         z = load_golden(case)
         g = make_grid(ctx, z)
@@ -312,7 +321,7 @@ def synthetic_problem(nni, nmu, seed, rinf=40.0, r=2.0):
 
 @pytest.mark.parametrize("shape", [(91, 121), (169, 241), (331, 601), (481, 901), (505, 901)])
 @pytest.mark.parametrize("isym", [1, -1])
-@pytest.mark.parametrize("engine", [0, 2])
+@pytest.mark.parametrize("engine", [(0, -1, 4), (0, -1, 2), (2, -1, 4)])
 def test_relax_batch_large_vs_oracle(ctx, shape, isym, engine):
     """Batched relaxation (maxsor1=2 x maxsor2=10) at configuration sizes against the
     oracle's putin -> mcsor -> putout macro-iterations.  FP64 grids wider than the
@@ -321,7 +330,7 @@ def test_relax_batch_large_vs_oracle(ctx, shape, isym, engine):
     from x2dhf_amd import gpu
     nni, nmu = shape
     print(f"shape={shape} isym={isym} engine={engine}", flush=True)
-    gpu.check(ctx.lib.x2g_set_sweep_engine(engine, -1))
+    set_engine(ctx, engine)
     P = synthetic_problem(nni, nmu, seed=nni + nmu)
     g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
     fun, l, r = ctx.upload(P["fun"]), ctx.upload(P["lhs"]), ctx.upload(P["rhs"])
@@ -342,7 +351,7 @@ def test_uneven_batch_matches_single_problems(ctx):
     """Problems on four different grids relaxed in one batch give the same
     result as each relaxed alone."""
     from x2dhf_amd import gpu
-    gpu.check(ctx.lib.x2g_set_sweep_engine(0, -1))
+    set_engine(ctx, (0, -1, 4))
     shapes = [(91, 121), (151, 241), (61, 91), (331, 601)]
     probs = [synthetic_problem(n, m, seed=7 + i) for i, (n, m) in enumerate(shapes)]
     grids = [gpu.Grid(ctx, p["nni"], p["nmu"], p["coef"], p["suppl"], p["vxi"], p["veta"]) for p in probs]
@@ -364,14 +373,14 @@ def test_uneven_batch_matches_single_problems(ctx):
 
 
 @pytest.mark.parametrize("shape", [(91, 121), (331, 601), (505, 901)])
-@pytest.mark.parametrize("engine", [0, 2])
+@pytest.mark.parametrize("engine", [(0, -1, 4), (0, -1, 2), (2, -1, 4)])
 def test_fp32_correction_sweeps_vs_oracle(ctx, shape, engine):
     """With u = 0 one mixed-precision macro-iteration is exactly FP32 MCSOR from
     zero on rhs (the residual), so its interior must match the FP32 oracle."""
     from x2dhf_amd import gpu
     nni, nmu = shape
     print(f"shape={shape} engine={engine}", flush=True)
-    gpu.check(ctx.lib.x2g_set_sweep_engine(engine, -1))
+    set_engine(ctx, engine)
     P = synthetic_problem(nni, nmu, seed=11)
     g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
     f = ctx.upload(np.zeros(nni * nmu))
@@ -392,7 +401,7 @@ def test_mixed_precision_refines_to_fp64(ctx):
     """FP32 correction sweeps with FP64 residuals converge to the FP64 solution:
     after enough macro-iterations the FP64 residual is at FP64 level."""
     from x2dhf_amd import gpu
-    gpu.check(ctx.lib.x2g_set_sweep_engine(0, -1))
+    set_engine(ctx, (0, -1, 4))
     nni, nmu = 169, 241
     P = synthetic_problem(nni, nmu, seed=13)
     g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
@@ -443,7 +452,7 @@ def test_rhs_as_product_of_factors(ctx, nni, nmu, nf):
     formed into rhs_d first) agree with the same item given rhs = the product
     computed by x2g_lincomb, bit for bit, and so do the residual norms."""
     from x2dhf_amd import gpu
-    gpu.check(ctx.lib.x2g_set_sweep_engine(0, -1))
+    set_engine(ctx, (0, -1, 4))
     P = synthetic_problem(nni, nmu, seed=7 * nni + nf)
     rng = np.random.default_rng(nni)
     print(f"rhs factors {nni}x{nmu} nf={nf}", flush=True)
@@ -480,7 +489,7 @@ def test_zero_sweeps_leave_the_function_unchanged(ctx):
     in FP64 and with the FP32 correction (whose start array is then never
     cleared or read)."""
     from x2dhf_amd import gpu
-    gpu.check(ctx.lib.x2g_set_sweep_engine(0, -1))
+    set_engine(ctx, (0, -1, 4))
     nni, nmu = 169, 241
     P = synthetic_problem(nni, nmu, seed=3)
     g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])

@@ -30,6 +30,8 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <cmath>
+#include <cstdio>
 
 namespace x2g {
 
@@ -403,16 +405,16 @@ __global__ void k_rhs_product(const ProdDesc *__restrict__ pd)
 }
 
 // ---------------------------------------------------------------------------
-// k_sweep: the pipelined strip sweep (same block schedule as k_strip with
-// SKEW = 2) with a fixed lane -> work map.  Wave p < 5 serves colour pass p:
-// its lane l the colour points t = l + 64 j (j < NT) of each of the four rows
-// of the pass's current block.  With the pass uniform across the wave, all
+// k_sweep: the pipelined strip sweep (the block schedule of k_strip, with
+// blocks of BR = 4 or 2 rows) with a fixed lane -> work map.  Wave p < 5
+// serves colour pass p: its lane l the colour points t = l + 64 j (j < NT) of
+// each of the BR rows of the pass's current block.  With the pass uniform across the wave, all
 // that depends on it (block, ring slot, rows, colour offset, lhs / rhs row
 // base) is scalar work.  The points t >= 64 NT of each pass (MS of them per
 // row at most) go to spill waves, one point per lane, pass (tid-320) / MS.
 // The ring rows have a compile-time
 // stride W >= nni+16, so a point reads its 17 stencil values with immediate
-// offsets from three block bases, and every point of a step is processed
+// offsets from 2 (4/BR) + 1 block bases, and every point of a step is processed
 // without branches: points outside the pass's block range or the grid are
 // computed on in-bounds garbage and written to an LDS sink.
 //
@@ -422,7 +424,7 @@ __global__ void k_rhs_product(const ProdDesc *__restrict__ pd)
 // an out-of-range one, so each wave issues the same memory operations in
 // every step and the compiler's waits count exactly the loads they need.
 // src / dst have a row stride P = nni+8 rounded up to 16 bytes: a block of
-// 4 rows is a contiguous run of 16-byte chunks, copied into the ring with one
+// BR rows is a contiguous run of 16-byte chunks, copied into the ring with one
 // 16-byte load and one LDS store per chunk (NSV chunks per thread) and
 // written back the same way, pad columns included (src and dst hold the same
 // pads, which the sweep never changes).  lhs / rhs come as one array of
@@ -439,12 +441,25 @@ __global__ void k_rhs_product(const ProdDesc *__restrict__ pd)
 // so the steps issue a fixed number of memory operations.
 //
 // LDS (elements of T): d(nu) at index nu-1 [W] (so a point's column base also
-// addresses its d(nu)), the row weights of the ring's rows [48 x 8, written
-// with each block], the coefficients [32], the ring of 12 blocks of 4 rows
-// [48 W], the sink [one per thread].  Independent of the strip height.
-__host__ __device__ inline size_t sweep_lds_elems(int W, int nth)
+// addresses its d(nu)), the row weights of the ring's rows [8 per row, written
+// with each block], the coefficients [32], the ring [its rows x W], the sink
+// [one per thread].  Independent of the strip height.  Two-row blocks halve
+// the per-lane registers of a step (the (omega/lhs, rhs) pairs and the
+// prefetched chunks), so more workgroups share a CU, at the price of a
+// longer pipeline fill per strip and two more ring rows per pass skew.
+//
+// Pipeline shape of k_sweep for blocks of BR rows (4 or 2): passes SKEW
+// blocks apart (at least one block more than the stencil's reach of 4 rows),
+// and the ring holds the live window of 4 SKEW + 2 (4/BR) + 1 blocks plus the
+// one being prefetched: 12 blocks of 4 rows (48 rows) or 18 of 2 (36 rows).
+__host__ __device__ constexpr int sweep_skew(int BR) { return BR == 4 ? 2 : 3; }
+__host__ __device__ constexpr int sweep_ring_blocks(int BR) { return 4 * sweep_skew(BR) + 2 * (4 / BR) + 2; }
+// LDS of k_sweep (elements of T): d(nu) [W], the row weights [8 per ring
+// row], the coefficients [32], the ring [ring rows x W], the sink [nth]
+__host__ __device__ inline size_t sweep_lds_elems(int W, int nth, int BR)
 {
-    return (size_t)W + 48 * 8 + 32 + (size_t)48 * W + (size_t)nth;
+    const int rows = sweep_ring_blocks(BR) * BR;
+    return (size_t)W + (size_t)rows * 8 + 32 + (size_t)rows * W + (size_t)nth;
 }
 
 // elements of T per 16-byte chunk, and chunks per thread in a block copy (a
@@ -453,7 +468,7 @@ __host__ __device__ inline size_t sweep_lds_elems(int W, int nth)
 template <typename T>
 constexpr int sweep_vec() { return 16 / (int)sizeof(T); }
 template <typename T>
-constexpr int sweep_nsv() { return sizeof(T) == 4 ? 2 : 3; }
+constexpr int sweep_nsv(int BR) { return sizeof(T) == 4 ? (BR == 4 ? 2 : 1) : (BR == 4 ? 3 : 2); }
 // row stride of k_sweep's padded arrays: nni+8 rounded up to 16 bytes
 __host__ __device__ inline int sweep_row_stride(int nni, int vec) { return (nni + 8 + vec - 1) / vec * vec; }
 
@@ -514,12 +529,22 @@ struct Chunk {
 struct SweepSeg {
     int prob, a, L;
 };
-template <typename T, int NT, int W>
+template <typename T, int NT, int W, int BR>
 __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a, const int L, const int MS,
                                               const bool zsrc, unsigned char *ks_smem);
 
-template <typename T, int NT, int W>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && W <= 352 ? 4 : 1)))
+// Waves per SIMD the register allocation leaves room for, from the
+// workgroups the LDS lets share a CU: FP32 rings of stride <= 352 fit two
+// 6-wave workgroups (4-row blocks), with 2-row blocks up to stride 256 three
+// and more, stride 352 two and stride 528 two of 8 waves.
+template <typename T, int W, int BR>
+constexpr int sweep_waves_per_eu()
+{
+    return sizeof(T) != 4 ? 1 : (BR == 4 ? (W <= 352 ? 4 : 1) : (W <= 256 ? 5 : (W <= 528 ? 4 : 1)));
+}
+
+template <typename T, int NT, int W, int BR>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<T, W, BR>())))
 k_sweep(const SweepDesc<T> *__restrict__ descs, const SweepSeg *__restrict__ segs, const int *__restrict__ wg_seg,
         int MS, int zero_src)
 {
@@ -528,26 +553,30 @@ k_sweep(const SweepDesc<T> *__restrict__ descs, const SweepSeg *__restrict__ seg
     for (int si = s0; si < s1; ++si) {
         const SweepSeg sg = segs[si];
         if (si > s0) __syncthreads();  // the previous segment is done with the LDS
-        sweep_segment<T, NT, W>(descs[sg.prob], sg.a, sg.L, MS, zero_src != 0, ks_smem);
+        sweep_segment<T, NT, W, BR>(descs[sg.prob], sg.a, sg.L, MS, zero_src != 0, ks_smem);
     }
 }
 
-// One strip of rows [a, a+L) of problem d (L a multiple of 4, or the rest of
-// the problem): the pipelined sweep described above.  zsrc: src is known to
-// be zero everywhere (the first sweep of a correction), so it is not read.
-template <typename T, int NT, int W>
+// One strip of rows [a, a+L) of problem d (L a multiple of BR, or the rest of
+// the problem): the pipelined sweep described above, with blocks of BR rows
+// (4 or 2).  zsrc: src is known to be zero everywhere (the first sweep of a
+// correction), so it is not read.
+template <typename T, int NT, int W, int BR>
 __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a, const int L, const int MS,
                                               const bool zsrc, unsigned char *ks_smem)
 {
-    constexpr int SKEW = 2, RB = 4 * SKEW + 4;
-    constexpr int VEC = sweep_vec<T>(), NSV = sweep_nsv<T>();
+    // R: stencil reach in blocks; SKEW: blocks between consecutive passes
+    // (>= 1 + R, so one barrier per step suffices); RB: ring blocks (the
+    // live window s-4 SKEW-R .. s+R plus the block being prefetched)
+    constexpr int R = 4 / BR, SKEW = sweep_skew(BR), RB = sweep_ring_blocks(BR), RROWS = RB * BR;
+    constexpr int VEC = sweep_vec<T>(), NSV = sweep_nsv<T>(BR);
     T *const dnu_s = reinterpret_cast<T *>(ks_smem);
 
     const int nni = d.nni, nmu = d.nmu, P = d.P;
     const int aend = min(a + L, nmu - 3);
     const int r0 = a - 16;
-    const int K = L / 4 + 8;
-    const int S = K + 4 * SKEW - 3;
+    const int K = (L + 32) / BR;          // blocks of the strip and its 16-row halos
+    const int S = K + 4 * SKEW - 3 * R;   // steps: the last block's write-out at K-1-4R + 4 SKEW + R
     const int tid = threadIdx.x, nth = blockDim.x;
     const int M = split_m(nni);
     const int cA = d.cA;
@@ -557,43 +586,43 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     const __amdgpu_buffer_rsrc_t rs_lr = buf_rsrc(d.lr, d.lr_bytes);
     const __amdgpu_buffer_rsrc_t rs_bmu = buf_rsrc(d.bmu, (unsigned)(nmu * sizeof(T)));
     T *const ab_r = dnu_s + W;                 // row weights, 8 per ring row
-    T *const coef_s = ab_r + 48 * 8;           // c[8..15] (nu weights), 1-omega
+    T *const coef_s = ab_r + RROWS * 8;        // c[8..15] (nu weights), 1-omega
     T *const ring = coef_s + 32;
-    T *const sink = ring + 48 * W;
+    T *const sink = ring + RROWS * W;
 
     auto slot_of = [&](int b) { return (b + RB) % RB; };  // b >= -RB
 
     // Chunk q = tid + j*nth of a block (elements VEC*q .. VEC*q+VEC-1 of its
-    // 4 x P values): its LDS offset rr*W + col within a ring block (-1: no
-    // such chunk), and in cinfo its row rr (bits 0..2), the element of the
-    // chunk holding nu = 1 (bits 3..5) and nu = nni (bits 6..8) of that row
-    // (7: none).  Fixed for the whole strip.
+    // BR x P values): its LDS offset rr*W + col within a ring block (-1: no
+    // such chunk), and in cinfo its row rr (bits 0..2; BR: none), the element
+    // of the chunk holding nu = 1 (bits 3..5) and nu = nni (bits 6..8) of that
+    // row (7: none).  Fixed for the whole strip.
     int cl[NSV], cinfo[NSV];
 #pragma unroll
     for (int j = 0; j < NSV; ++j) {
         const int e0 = (tid + j * nth) * VEC;
         const int rr = e0 / P, col = e0 - rr * P;
-        const bool v = e0 < 4 * P;
+        const bool v = e0 < BR * P;
         const int b1 = 4 - col, b2 = nni + 3 - col;
         cl[j] = v ? rr * W + col : -1;
-        cinfo[j] = (v ? rr : 4) | ((v && b1 >= 0 && b1 < VEC ? b1 : 7) << 3) |
+        cinfo[j] = (v ? rr : BR) | ((v && b1 >= 0 && b1 < VEC ? b1 : 7) << 3) |
                    ((v && b2 >= 0 && b2 < VEC ? b2 : 7) << 6);
     }
 
     // Row weights of row mu: distance k = 4,3,2,1 -> c2 = c[4..7], c1 = c[0..3];
     // ab[0..3] = c2 + b(mu) c1 (rows mu-k), ab[4..7] = c2 - b(mu) c1 (rows mu+k).
-    // Thread tid < 32 forms entry tid & 7 of row tid >> 3 of a block; its two
-    // coefficients are loaded once.
+    // Thread tid < 8 BR forms entry tid & 7 of row tid >> 3 of a block; its
+    // two coefficients are loaded once.
     const T rw_c2 = d.c[4 + (tid & 3)], rw_c1 = d.c[tid & 3];
     auto row_weight = [&](T b) -> T { return (tid & 7) < 4 ? rw_c2 + b * rw_c1 : rw_c2 - b * rw_c1; };
 
-    // the chunks of block bl, plus b(mu) of row (tid >> 3) & 3 of the block
+    // the chunks of block bl, plus b(mu) of row (tid >> 3) % BR of the block
     // for the row weights (rows outside the grid read b = 0)
     auto load_block = [&](int bl, u32x4 (&pr)[NSV], T &bv) {
-        const int mub = r0 + 4 * bl;
+        const int mub = r0 + BR * bl;
         {
-            const int mu = mub + ((tid >> 3) & 3);
-            const bool ok = tid < 32 && mu >= 1 && mu <= nmu;
+            const int mu = mub + ((tid >> 3) & (BR - 1));
+            const bool ok = tid < 8 * BR && mu >= 1 && mu <= nmu;
             bv = buf_load1(rs_bmu, oob_unless(ok, (mu - 1) * (int)sizeof(T)), T(0));
         }
         const int gb = (mub + 3) * P * (int)sizeof(T);  // negative above the padded array
@@ -605,10 +634,10 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
         }
     };
     auto commit_block = [&](int bl, const u32x4 (&pr)[NSV], T bv) {
-        if (bl > K) return;
+        if (bl > K - 1 + R) return;
         const int sl = slot_of(bl);
-        if (tid < 32) ab_r[sl * 32 + tid] = row_weight(bv);
-        T *bp = ring + sl * 4 * W;
+        if (tid < 8 * BR) ab_r[sl * 8 * BR + tid] = row_weight(bv);
+        T *bp = ring + sl * BR * W;
 #pragma unroll
         for (int j = 0; j < NSV; ++j)
             if (cl[j] >= 0) *reinterpret_cast<u32x4 *>(bp + cl[j]) = pr[j];
@@ -616,10 +645,10 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     // rows [a, aend) of block bw (final) to dst, with the nu = 1 and nu = nni
     // values extrapolated (sormcsor.F90:352-383; zero for odd symmetry)
     auto write_out = [&](int bw) {
-        const bool wact = bw >= 4 && bw <= K - 5;
-        const int mub = r0 + 4 * bw;
+        const bool wact = bw >= 4 * R && bw <= K - 1 - 4 * R;
+        const int mub = r0 + BR * bw;
         const int gb = (mub + 3) * P * (int)sizeof(T);
-        const T *wbase = ring + slot_of(bw) * 4 * W;
+        const T *wbase = ring + slot_of(bw) * BR * W;
 #pragma unroll
         for (int j = 0; j < NSV; ++j) {
             const int rr = cinfo[j] & 7;
@@ -645,11 +674,11 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
         }
     };
 
-    // prologue: coefficients, d(nu), blocks -1, 0, 1 and their row weights
+    // prologue: coefficients, d(nu), blocks -R .. R and their row weights
     if (tid < 8) coef_s[tid] = d.c[8 + tid];
     if (tid == 8) coef_s[8] = d.omega1;
     for (int e = tid; e < nni; e += nth) dnu_s[e] = d.dnu[e];
-    for (int b = -1; b <= 1; ++b) {
+    for (int b = -R; b <= R; ++b) {
         u32x4 pr[NSV];
         T bv;
         load_block(b, pr, bv);
@@ -679,15 +708,15 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     }
     const int M5 = 5 * M;
 
-    // (omega/lhs, rhs) of pass p's points in step s (rows of block s - 2p);
-    // rows outside the pass's range or the grid and absent points read
-    // zeros, which are never used
-    auto load_lr = [&](int s, int p, T (&lv)[4][NT], T (&rv)[4][NT]) {
+    // (omega/lhs, rhs) of pass p's points in step s (rows of block
+    // s - SKEW p); rows outside the pass's range or the grid and absent points
+    // read zeros, which are never used
+    auto load_lr = [&](int s, int p, T (&lv)[BR][NT], T (&rv)[BR][NT]) {
         const int b = s - SKEW * p;
-        const bool act = p < 5 && b >= p && b <= K - 1 - p;
-        const int mu0 = r0 + 4 * b;
+        const bool act = p < 5 && b >= R * p && b <= K - 1 - R * p;
+        const int mu0 = r0 + BR * b;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
+        for (int rr = 0; rr < BR; ++rr) {
             const int mu = mu0 + rr;
             const bool rowok = act && mu >= 2 && mu <= nmu - 4;
             const unsigned rb = rowok ? (unsigned)(((mu - 2) * M5 + p * M) * (int)(2 * sizeof(T))) : kOOB;
@@ -708,29 +737,34 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     for (int k = 0; k < 8; ++k) cf[k] = coef_s[k];
     const T om1 = coef_s[8];
 
-    // pass p on block s - 2p with the (omega/lhs, rhs) of step s; within an
+    // pass p on block s - SKEW p with the (omega/lhs, rhs) of step s; within an
     // active pass the points are processed without branches (points outside
     // the grid compute on in-bounds values and write to the sink).  The points
     // of one pass in a row do not read each other (a point's neighbours have
     // other colours), so a row's stencil values are gathered first and its
     // arithmetic follows, which keeps the LDS reads in flight together.
-    auto compute = [&](int s, int p, const T (&lv)[4][NT], const T (&rv)[4][NT]) {
+    auto compute = [&](int s, int p, const T (&lv)[BR][NT], const T (&rv)[BR][NT]) {
         const int b = s - SKEW * p;
-        if (p >= 5 || b < p || b > K - 1 - p) return;  // idle lanes and passes outside their block range
+        if (p >= 5 || b < R * p || b > K - 1 - R * p) return;  // idle lanes and passes outside their range
         const int sb = slot_of(b);
-        const int sm = sb == 0 ? RB - 1 : sb - 1, sp = sb == RB - 1 ? 0 : sb + 1;
-        const T *const bm = ring + sm * 4 * W;   // block b-1
-        T *const b0 = ring + sb * 4 * W;         // block b
-        const T *const bp = ring + sp * 4 * W;   // block b+1
-        const int mu0 = r0 + 4 * b;
-        const T *ab = ab_r + sb * 32;           // row weights of block b
+        // the ring blocks b-R .. b+R
+        const T *blk[2 * R + 1];
+#pragma unroll
+        for (int k = -R; k <= R; ++k) {
+            int sl = sb + k;
+            sl = sl < 0 ? sl + RB : (sl >= RB ? sl - RB : sl);
+            blk[k + R] = ring + sl * BR * W;
+        }
+        T *const b0 = ring + sb * BR * W;         // block b
+        const int mu0 = r0 + BR * b;
+        const T *ab = ab_r + sb * 8 * BR;        // row weights of block b
         int off = mod5(p - cA * (mu0 - 2));     // (nu-2) mod 5 of the pass's points in row mu0
         // Rows are processed in groups of G: the stencil values of a group are
         // gathered before its arithmetic (the points of one pass never read
         // each other, in one row or in neighbouring rows).
         constexpr int G = 2;
 #pragma unroll
-        for (int rg = 0; rg < 4; rg += G) {
+        for (int rg = 0; rg < BR; rg += G) {
             int cbv[G][NT], ov[G][NT];
             bool okv[G][NT];
             T x[G][NT][17];  // 0..8: own row at nu-4..nu+4; 9..12: rows mu-1..mu-4; 13..16: rows mu+1..mu+4
@@ -745,7 +779,7 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
                     const int cb = 1 + off + 5 * tq[j];  // = nu - 1 = (column of nu) - 4
                     cbv[g][j] = cb;
                     okv[g][j] = lim[j] <= lim_r;
-                    const T *qm = bm + cb, *qp = bp + cb, *q0 = b0 + cb;
+                    const T *q0 = b0 + cb;
                     if constexpr (sizeof(T) == 4) {
                         // the own row's nine values from one opaque base, so that
                         // they are read with immediate offsets of that base
@@ -760,9 +794,11 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
                     }
 #pragma unroll
                     for (int k = 1; k <= 4; ++k) {
+                        // rows rr -/+ k: block offset and row within it (compile-time)
                         const int rdn = rr - k, rup = rr + k;
-                        x[g][j][8 + k] = rdn < 0 ? qm[(rdn + 4) * W + 4] : q0[rdn * W + 4];
-                        x[g][j][12 + k] = rup > 3 ? qp[(rup - 4) * W + 4] : q0[rup * W + 4];
+                        const int bdn = rdn >= 0 ? 0 : -((-rdn + BR - 1) / BR), bup = rup / BR;
+                        x[g][j][8 + k] = blk[bdn + R][cb + (rdn - bdn * BR) * W + 4];
+                        x[g][j][12 + k] = blk[bup + R][cb + (rup - bup * BR) * W + 4];
                     }
                 }
             }
@@ -816,11 +852,11 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
         }
     };
     // the two lane roles: the pass of a pass wave is a scalar
-    auto load_lr_any = [&](int s, T (&lv)[4][NT], T (&rv)[4][NT]) {
+    auto load_lr_any = [&](int s, T (&lv)[BR][NT], T (&rv)[BR][NT]) {
         if (pass_wave) load_lr(s, wave, lv, rv);
         else load_lr(s, p_spill, lv, rv);
     };
-    auto compute_any = [&](int s, const T (&lv)[4][NT], const T (&rv)[4][NT]) {
+    auto compute_any = [&](int s, const T (&lv)[BR][NT], const T (&rv)[BR][NT]) {
         if (pass_wave) compute(s, wave, lv, rv);
         else compute(s, p_spill, lv, rv);
     };
@@ -857,34 +893,35 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
         }
     };
 
-    // Block s+2 and the (omega/lhs, rhs) of step s+1 are loaded at the start
-    // of step s and land during its compute phase.  Block bw = s - 9 is final
-    // from the start of step s (pass 4 finished it in step s-1).  FP64 writes
-    // it out right after issuing the loads, so its stores overlap the compute
-    // phase; FP32 writes it out at the end of the step (each measured faster
-    // that way).  The commit of block s+2 reuses the slot of block s-10,
-    // which was written out in step s-1.
+    // Block s+R+1 and the (omega/lhs, rhs) of step s+1 are loaded at the
+    // start of step s and land during its compute phase.  Block
+    // bw = s - 4 SKEW - R is final from the start of step s (pass 4 wrote it
+    // in step s-R; in step s it only reads it, as a neighbour of block bw+R).
+    // FP64 writes it out right after issuing the loads, so its stores overlap
+    // the compute phase; FP32 writes it out at the end of the step (each
+    // measured faster that way).  The commit of block s+R+1 reuses the slot of
+    // block s+R+1-RB = s-4 SKEW-R-1, written out in step s-1.
     constexpr bool early_out = sizeof(T) == 8;
-    T lv[4][NT], rv[4][NT];
+    T lv[BR][NT], rv[BR][NT];
     load_lr_any(0, lv, rv);
     __builtin_amdgcn_s_waitcnt(0);
     for (int s = 0; s < S; ++s) {
-        T ln[4][NT], rn[4][NT], bpre;
+        T ln[BR][NT], rn[BR][NT], bpre;
         u32x4 pre[NSV];
         load_lr_any(s + 1, ln, rn);
-        load_block(s + 2, pre, bpre);
-        if (early_out) write_out(s - 4 * SKEW - 1);
+        load_block(s + R + 1, pre, bpre);
+        if (early_out) write_out(s - 4 * SKEW - R);
         compute_any(s, lv, rv);
         __builtin_amdgcn_s_waitcnt(0);
-        commit_block(s + 2, pre, bpre);
+        commit_block(s + R + 1, pre, bpre);
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr)
+        for (int rr = 0; rr < BR; ++rr)
 #pragma unroll
             for (int j = 0; j < NT; ++j) {
                 lv[rr][j] = ln[rr][j];
                 rv[rr][j] = rn[rr][j];
             }
-        if (!early_out) write_out(s - 4 * SKEW - 1);
+        if (!early_out) write_out(s - 4 * SKEW - R);
         __syncthreads();
     }
     if (a == 2) {
@@ -1500,6 +1537,21 @@ static int g_engine = -1, g_strip_rows = -1;
 // k_sweep workgroups per CU for the balanced schedule (0: as many as the LDS
 // holds); from X2G_SWEEP_WPC.
 static int g_sweep_wpc = 0;
+// k_sweep batches launch once per ring-stride class with X2G_SWEEP_SPLIT=1
+// (default: one launch in the widest grid's class)
+static int g_sweep_split = 0;
+// The k_sweep plan's free choices: workgroups per CU the segment planner
+// fills (0: as many as the LDS ring allows) and the launch split by ring
+// class.  The autotuner picks them per batch shape.
+struct SweepKnobs {
+    int wpc = 0, split = 0, br = 4;
+};
+// k_sweep block height (4 or 2 rows; X2G_SWEEP_BR) when not tuned
+static int g_sweep_br = 4;
+// the plan's choices are fixed (x2g_set_sweep_plan, or any of X2G_SWEEP_BR,
+// X2G_SWEEP_WPC, X2G_SWEEP_SPLIT set): the autotuner stays out
+static bool g_plan_fixed = false;
+static int g_autotune = -1;  // -1: from X2G_AUTOTUNE (default on)
 static void engine_defaults()
 {
     if (g_engine >= 0) return;
@@ -1511,6 +1563,15 @@ static void engine_defaults()
     g_strip_rows = l ? atoi(l) : -1;
     const char *w = getenv("X2G_SWEEP_WPC");
     g_sweep_wpc = w ? atoi(w) : 0;
+    const char *brv = getenv("X2G_SWEEP_BR");
+    g_sweep_br = brv && atoi(brv) == 2 ? 2 : 4;
+    const char *sp = getenv("X2G_SWEEP_SPLIT");
+    g_sweep_split = sp ? atoi(sp) != 0 : 0;
+    g_plan_fixed = w || brv || sp;
+    if (g_autotune < 0) {
+        const char *at = getenv("X2G_AUTOTUNE");
+        g_autotune = at ? atoi(at) != 0 : 1;
+    }
 }
 static int env_engine()
 {
@@ -1558,10 +1619,16 @@ static int choose_L(x2g_ctx *ctx, long total_rows)
 // in nwg workgroups (those the chip holds at once: wpc per CU, at most one
 // per 8 rows).  Results do not depend on the cut: a strip's rows come out
 // the same whatever lies beyond its halos.
-static void segments_for_budget(const std::vector<int> &rows, long B, std::vector<SweepSeg> &segs,
+static void segments_for_budget(const std::vector<int> &rows, long B, int br, std::vector<SweepSeg> &segs,
                                 std::vector<int> &wgseg)
 {
-    constexpr int kMinRows = 8, kFill = 13;
+    // a segment of L rows costs L/br + kFill steps (the halos and the
+    // pipeline fill: 13 with 4-row blocks, 22 with 2-row blocks)
+    constexpr int kMinRows = 8;
+    const int kFill = 32 / br + 4 * sweep_skew(br) - 3 * (4 / br);
+    // an empty workgroup always takes a segment of at least kMinRows rows,
+    // so every problem is placed
+    B = std::max<long>(B, kFill + (kMinRows + br - 1) / br);
     segs.clear();
     wgseg.assign(1, 0);
     long left_budget = B;
@@ -1572,13 +1639,13 @@ static void segments_for_budget(const std::vector<int> &rows, long B, std::vecto
     for (int p = 0; p < (int)rows.size(); ++p) {
         int a = 2, left = rows[p];
         while (left > 0) {
-            const long fit = 4 * (left_budget - kFill);
+            const long fit = br * (left_budget - kFill);
             if (left <= fit) {
-                const int L = (left + 3) & ~3;
+                const int L = (left + br - 1) / br * br;
                 segs.push_back({p, a, L});
-                left_budget -= L / 4 + kFill;
+                left_budget -= L / br + kFill;
                 left = 0;
-                if (left_budget - kFill < kMinRows / 4) close_wg();
+                if (left_budget - kFill < kMinRows / br) close_wg();
             } else if (fit >= kMinRows) {
                 segs.push_back({p, a, (int)fit});
                 a += (int)fit;
@@ -1592,14 +1659,14 @@ static void segments_for_budget(const std::vector<int> &rows, long B, std::vecto
     if (wgseg.back() != (int)segs.size()) wgseg.push_back((int)segs.size());
 }
 
-static void plan_segments(x2g_ctx *ctx, const std::vector<int> &rows, int wpc, std::vector<SweepSeg> &segs,
+static void plan_segments(x2g_ctx *ctx, const std::vector<int> &rows, int wpc, int br, std::vector<SweepSeg> &segs,
                           std::vector<int> &wgseg)
 {
     segs.clear();
     wgseg.clear();
     const int forced = env_strip_l();
     if (forced > 0) {
-        const int L = (forced + 3) & ~3;
+        const int L = (forced + br - 1) / br * br;
         for (int p = 0; p < (int)rows.size(); ++p)
             for (int a = 2; a < 2 + rows[p]; a += L) {
                 wgseg.push_back((int)segs.size());
@@ -1614,13 +1681,14 @@ static void plan_segments(x2g_ctx *ctx, const std::vector<int> &rows, int wpc, s
     // workgroup pays more than the halo re-reads cost while CUs are idle)
     const long nwg = std::max<long>(1, std::min<long>((long)wpc * ctx->num_cu, (R + 7) / 8));
     // a budget of all rows plus a fill per problem fits in one workgroup
-    long lo = 15, hi = R / 4 + 14 * (long)rows.size() + 15;
+    const long fill = 32 / br + 4 * sweep_skew(br) - 3 * (4 / br);
+    long lo = fill + 8 / br, hi = R / br + (fill + 1) * (long)rows.size() + fill + 8 / br;
     while (lo < hi) {
         const long mid = (lo + hi) / 2;
-        segments_for_budget(rows, mid, segs, wgseg);
+        segments_for_budget(rows, mid, br, segs, wgseg);
         if ((long)wgseg.size() - 1 <= nwg) hi = mid; else lo = mid + 1;
     }
-    segments_for_budget(rows, lo, segs, wgseg);
+    segments_for_budget(rows, lo, br, segs, wgseg);
 }
 
 // kind: 0 per-pass kernels (in place), 1 k_strip, 2 k_sweep (residue-split lhs/rhs)
@@ -1630,14 +1698,21 @@ struct SweepPlan {
     int kind = 0;
     int skew = 0;
     int total_wg = 0;
-    int nth = 0, MS = 0, NT = 0, W = 0;
     size_t lds = 0;
-    // k_sweep: the row segments and each workgroup's first segment
-    // (wgseg[w] .. wgseg[w+1]-1), with their staged device copies
-    std::vector<SweepSeg> segs;
-    std::vector<int> wgseg;
-    const SweepSeg *d_segs = nullptr;
-    const int *d_wgseg = nullptr;
+    // k_sweep: one launch per ring-stride class of the batch's problems (a
+    // narrow grid's ring then leaves room for more workgroups per CU than
+    // the batch's widest grid would), each with its row segments and every
+    // workgroup's first segment (wgseg[w] .. wgseg[w+1]-1; segments index
+    // the batch's descriptors), and their staged device copies
+    struct Sub {
+        int W = 0, br = 4, nth = 0, MS = 0, NT = 0, total_wg = 0;
+        size_t lds = 0;
+        std::vector<SweepSeg> segs;
+        std::vector<int> wgseg;
+        const SweepSeg *d_segs = nullptr;
+        const int *d_wgseg = nullptr;
+    };
+    std::vector<Sub> subs;
 };
 
 // Does k_sweep apply to a batch whose widest grid has nni points?
@@ -1676,8 +1751,8 @@ static bool sweep_fits(x2g_ctx *ctx, int maxnni)
     if (W == 0 || nth > 512 || (NT == 2) != (W == 768)) return false;
     // a block's 16-byte chunks must fit the threads' copy slots
     const int P = sweep_row_stride(maxnni, sweep_vec<T>());
-    if (4 * P / sweep_vec<T>() > sweep_nsv<T>() * nth || P > W) return false;
-    return sweep_lds_elems(W, nth) * sizeof(T) <= ctx->max_lds;
+    if (4 * P / sweep_vec<T>() > sweep_nsv<T>(4) * nth || P > W) return false;
+    return sweep_lds_elems(W, nth, 4) * sizeof(T) <= ctx->max_lds;
 }
 
 // Which kernel family a batch of this precision and width uses (engine setting
@@ -1694,29 +1769,36 @@ static int sweep_kind(x2g_ctx *ctx, int maxnni)
 // Launch one sweep of every problem in `descs` (device copy dd, host copy hv).
 // k_sweep for the plan's stride class and depth (the instantiations are
 // listed once here and in set_lds_attr).
-template <typename T, int NT, int W>
-static void launch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, const SweepPlan<T> &plan, int zero_src)
+template <typename T, int NT, int W, int BR>
+static void launch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, const typename SweepPlan<T>::Sub &plan, int zero_src)
 {
     // the caller has flushed the staged descriptors
-    hipLaunchKernelGGL((k_sweep<T, NT, W>), dim3(plan.total_wg), dim3(plan.nth), plan.lds, ctx->stream, dd,
+    hipLaunchKernelGGL((k_sweep<T, NT, W, BR>), dim3(plan.total_wg), dim3(plan.nth), plan.lds, ctx->stream, dd,
                        plan.d_segs, plan.d_wgseg, plan.MS, zero_src);
 }
 
-template <typename T>
-static bool dispatch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, const SweepPlan<T> &plan, int zero_src)
+template <typename T, int BR>
+static bool dispatch_ks_br(x2g_ctx *ctx, const SweepDesc<T> *dd, const typename SweepPlan<T>::Sub &plan,
+                           int zero_src)
 {
     switch (plan.W) {
-    case 128: launch_ks<T, 1, 128>(ctx, dd, plan, zero_src); return true;
-    case 192: launch_ks<T, 1, 192>(ctx, dd, plan, zero_src); return true;
-    case 256: launch_ks<T, 1, 256>(ctx, dd, plan, zero_src); return true;
-    case 352: launch_ks<T, 1, 352>(ctx, dd, plan, zero_src); return true;
+    case 128: launch_ks<T, 1, 128, BR>(ctx, dd, plan, zero_src); return true;
+    case 192: launch_ks<T, 1, 192, BR>(ctx, dd, plan, zero_src); return true;
+    case 256: launch_ks<T, 1, 256, BR>(ctx, dd, plan, zero_src); return true;
+    case 352: launch_ks<T, 1, 352, BR>(ctx, dd, plan, zero_src); return true;
     default: break;
     }
     if constexpr (sizeof(T) == 4) {
-        if (plan.W == 528) { launch_ks<T, 1, 528>(ctx, dd, plan, zero_src); return true; }
-        if (plan.W == 768) { launch_ks<T, 2, 768>(ctx, dd, plan, zero_src); return true; }
+        if (plan.W == 528) { launch_ks<T, 1, 528, BR>(ctx, dd, plan, zero_src); return true; }
+        if (plan.W == 768) { launch_ks<T, 2, 768, BR>(ctx, dd, plan, zero_src); return true; }
     }
     return false;
+}
+
+template <typename T>
+static bool dispatch_ks(x2g_ctx *ctx, const SweepDesc<T> *dd, const typename SweepPlan<T>::Sub &plan, int zero_src)
+{
+    return plan.br == 2 ? dispatch_ks_br<T, 2>(ctx, dd, plan, zero_src) : dispatch_ks_br<T, 4>(ctx, dd, plan, zero_src);
 }
 
 // One sweep of every problem in the plan (device descriptors dd, host copy
@@ -1730,9 +1812,11 @@ static int launch_sweep_once(x2g_ctx *ctx, const SweepDesc<T> *dd, const std::ve
     const bool in_place = plan.kind == 0;
     const int n = (int)hv.size();
     if (plan.kind == 2) {
-        const bool ok = dispatch_ks<T>(ctx, dd, plan, zero_src ? 1 : 0);
-        X2G_CHECK(ok, "no sweep kernel for ring stride %d", plan.W);
-        X2G_HIP(hipGetLastError());
+        for (const auto &sub : plan.subs) {
+            const bool ok = dispatch_ks<T>(ctx, dd, sub, zero_src ? 1 : 0);
+            X2G_CHECK(ok, "no sweep kernel for ring stride %d", sub.W);
+            X2G_HIP(hipGetLastError());
+        }
         return X2G_OK;
     }
     if (!in_place && skew > 0) {
@@ -1801,11 +1885,16 @@ static int set_lds_attr(x2g_ctx *ctx)
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     const void *ks[] = {
-        (const void *)k_sweep<double, 1, 128>, (const void *)k_sweep<double, 1, 192>,
-        (const void *)k_sweep<double, 1, 256>, (const void *)k_sweep<double, 1, 352>,
-        (const void *)k_sweep<float, 1, 128>,  (const void *)k_sweep<float, 1, 192>,
-        (const void *)k_sweep<float, 1, 256>,  (const void *)k_sweep<float, 1, 352>,
-        (const void *)k_sweep<float, 1, 528>,  (const void *)k_sweep<float, 2, 768>};
+        (const void *)k_sweep<double, 1, 128, 4>, (const void *)k_sweep<double, 1, 192, 4>,
+        (const void *)k_sweep<double, 1, 256, 4>, (const void *)k_sweep<double, 1, 352, 4>,
+        (const void *)k_sweep<float, 1, 128, 4>,  (const void *)k_sweep<float, 1, 192, 4>,
+        (const void *)k_sweep<float, 1, 256, 4>,  (const void *)k_sweep<float, 1, 352, 4>,
+        (const void *)k_sweep<float, 1, 528, 4>,  (const void *)k_sweep<float, 2, 768, 4>,
+        (const void *)k_sweep<double, 1, 128, 2>, (const void *)k_sweep<double, 1, 192, 2>,
+        (const void *)k_sweep<double, 1, 256, 2>, (const void *)k_sweep<double, 1, 352, 2>,
+        (const void *)k_sweep<float, 1, 128, 2>,  (const void *)k_sweep<float, 1, 192, 2>,
+        (const void *)k_sweep<float, 1, 256, 2>,  (const void *)k_sweep<float, 1, 352, 2>,
+        (const void *)k_sweep<float, 1, 528, 2>,  (const void *)k_sweep<float, 2, 768, 2>};
     for (const void *k : ks) X2G_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     done = true;
     return X2G_OK;
@@ -1843,7 +1932,7 @@ static int pad_row_stride(int nni, int kind)
 // of the batch's kind sweep_kind<T>(ctx, widest nni).
 template <typename T>
 static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vector<T *> &a_bufs,
-                       std::vector<T *> &b_bufs, SweepPlan<T> &plan)
+                       std::vector<T *> &b_bufs, SweepPlan<T> &plan, const SweepKnobs *kn = nullptr)
 {
     int maxnni = 0;
     long rows = 0;
@@ -1854,13 +1943,11 @@ static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vecto
         prow.push_back(d.nmu - 5);
     }
     plan.kind = sweep_kind<T>(ctx, maxnni);
+    plan.subs.clear();
     int L = 0;
     if (plan.kind == 2) {
-        sweep_lanes(maxnni, plan.MS, plan.NT, plan.nth);
-        plan.W = sweep_stride<T>(maxnni);
         plan.skew = 2;
         L = 0;  // the row segments below
-        plan.lds = sweep_lds_elems(plan.W, plan.nth) * sizeof(T);
     } else if (plan.kind == 1) {
         plan.skew = choose_skew<T>(ctx, maxnni);
         L = choose_L(ctx, rows);
@@ -1880,10 +1967,34 @@ static int plan_sweeps(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vecto
     }
     plan.total_wg = wg;
     if (plan.kind == 2) {
-        int wpc = g_sweep_wpc;
-        if (wpc <= 0) wpc = std::max<int>(1, (int)(ctx->max_lds / std::max<size_t>(plan.lds, 1)));
-        plan_segments(ctx, prow, wpc, plan.segs, plan.wgseg);
-        plan.total_wg = (int)plan.wgseg.size() - 1;
+        // one launch per ring class (all problems in the widest grid's class
+        // when the split is off), each planned to fill the chip
+        std::map<int, std::vector<int>> cls;  // W -> problem indices
+        for (int i = 0; i < (int)base.size(); ++i)
+            cls[(kn ? kn->split : g_sweep_split) ? sweep_stride<T>(base[i].nni) : sweep_stride<T>(maxnni)]
+                .push_back(i);
+        plan.total_wg = 0;
+        for (auto &kv : cls) {
+            typename SweepPlan<T>::Sub sub;
+            int cmax = 0;
+            std::vector<int> rows;
+            for (int i : kv.second) {
+                cmax = std::max(cmax, base[i].nni);
+                rows.push_back(prow[i]);
+            }
+            sweep_lanes(cmax, sub.MS, sub.NT, sub.nth);
+            sub.W = kv.first;
+            sub.br = (kn ? kn->br : g_sweep_br) == 2 ? 2 : 4;
+            sub.lds = sweep_lds_elems(sub.W, sub.nth, sub.br) * sizeof(T);
+            int wpc = kn ? kn->wpc : g_sweep_wpc;
+            if (wpc <= 0) wpc = std::max<int>(1, (int)(ctx->max_lds / std::max<size_t>(sub.lds, 1)));
+            plan_segments(ctx, rows, wpc, sub.br, sub.segs, sub.wgseg);
+            for (auto &sg : sub.segs) sg.prob = kv.second[sg.prob];  // class index -> batch index
+            sub.total_wg = (int)sub.wgseg.size() - 1;
+            plan.total_wg += sub.total_wg;
+            plan.lds = std::max(plan.lds, sub.lds);
+            plan.subs.push_back(std::move(sub));
+        }
     }
     plan.even = base;
     plan.odd = base;
@@ -1910,10 +2021,133 @@ static int stage_plan(x2g_ctx *ctx, SweepPlan<T> &plan, const SweepDesc<T> **de,
     if ((rc = stage_descs(ctx, plan.even, off, de))) return rc;
     if ((rc = stage_descs(ctx, plan.odd, off, dodd))) return rc;
     if (plan.kind == 2) {
-        X2G_CHECK(!plan.segs.empty() && plan.total_wg > 0, "sweep plan without segments");
-        if ((rc = stage_descs(ctx, plan.segs, off, &plan.d_segs))) return rc;
-        if ((rc = stage_descs(ctx, plan.wgseg, off, &plan.d_wgseg))) return rc;
+        X2G_CHECK(!plan.subs.empty(), "sweep plan without launches");
+        for (auto &sub : plan.subs) {
+            X2G_CHECK(!sub.segs.empty() && sub.total_wg > 0, "sweep plan without segments");
+            if ((rc = stage_descs(ctx, sub.segs, off, &sub.d_segs))) return rc;
+            if ((rc = stage_descs(ctx, sub.wgseg, off, &sub.d_wgseg))) return rc;
+        }
     }
+    return X2G_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner of the k_sweep plan.  On the first batch of a shape (precision,
+// ring class of the widest grid, number of ring classes, total relaxed rows
+// rounded to a power of sqrt 2) every candidate (workgroups per CU 1, 2, 3
+// or as many as the LDS allows; one launch or one per ring class) sweeps
+// the batch's own buffers a few times under HIP events, and the fastest is
+// kept for that shape for the rest of the process.  The candidates' sweeps
+// run before the relaxation proper, on work arrays that it rewrites before
+// reading (putin, and the zeroed correction arrays), so results do not
+// depend on tuning.
+struct TuneEntry {
+    SweepKnobs best;
+    std::vector<std::pair<SweepKnobs, float>> tried;  // candidate, ms per sweep
+    long uses = 0;
+};
+static std::mutex g_tune_mutex;
+static std::map<std::string, TuneEntry> g_tune;
+
+template <typename T>
+static std::string tune_key(const std::vector<SweepDesc<T>> &base)
+{
+    int maxnni = 0;
+    long rows = 0;
+    std::map<int, int> cls;
+    for (auto &d : base) {
+        maxnni = std::max(maxnni, d.nni);
+        rows += d.nmu - 5;
+        cls[sweep_stride<T>(d.nni)]++;
+    }
+    const int bucket = (int)std::lround(2.0 * std::log2((double)std::max<long>(rows, 1)));
+    char buf[96];
+    snprintf(buf, sizeof(buf), "fp%d W%d classes%d rows~%ld", (int)(8 * sizeof(T)), sweep_stride<T>(maxnni),
+             (int)cls.size(), (long)std::lround(std::pow(2.0, bucket / 2.0)));
+    return buf;
+}
+
+// The knobs for this batch (tuning it if its shape is new).  zsrc: the first
+// sweep takes its source as zero (FP32 corrections).
+template <typename T>
+static int tuned_knobs(x2g_ctx *ctx, std::vector<SweepDesc<T>> &base, std::vector<T *> &A, std::vector<T *> &B,
+                       bool zsrc, SweepKnobs &out)
+{
+    engine_defaults();
+    out = SweepKnobs{g_sweep_wpc, g_sweep_split, g_sweep_br};
+    if (!g_autotune || g_plan_fixed || env_strip_l() > 0) return X2G_OK;
+    int maxnni = 0;
+    for (auto &d : base) maxnni = std::max(maxnni, d.nni);
+    if (sweep_kind<T>(ctx, maxnni) != 2) return X2G_OK;
+    const std::string key = tune_key<T>(base);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            out = it->second.best;
+            it->second.uses++;
+            return X2G_OK;
+        }
+    }
+    std::vector<SweepKnobs> cand;
+    std::map<int, int> cls;
+    for (auto &d : base) cls[sweep_stride<T>(d.nni)]++;
+    for (int br : {4, 2})
+        for (int split = 0; split <= (cls.size() > 1 ? 1 : 0); ++split)
+            for (int wpc : {0, 1, 2, 3}) cand.push_back({wpc, split, br});
+    TuneEntry e;
+    float best_ms = 1e30f;
+    hipEvent_t ea, eb;
+    X2G_HIP(hipEventCreate(&ea));
+    X2G_HIP(hipEventCreate(&eb));
+    std::vector<std::vector<int>> seen;  // block height and workgroups per launch of the plans tried
+    int rc = X2G_OK;
+    for (const auto &kn : cand) {
+        SweepPlan<T> plan;
+        std::vector<SweepDesc<T>> b2(base);
+        plan_sweeps(ctx, b2, A, B, plan, &kn);
+        std::vector<int> sig{kn.br};
+        for (auto &sub : plan.subs) sig.push_back(sub.total_wg);
+        if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;  // same plan as an earlier one
+        seen.push_back(sig);
+        const SweepDesc<T> *de, *dodd;
+        if ((rc = stage_plan(ctx, plan, &de, &dodd))) break;
+        if ((rc = stage_flush(ctx))) break;
+        // one untimed sweep, then three timed ones
+        if ((rc = launch_sweep_once<T>(ctx, de, plan.even, plan, zsrc))) break;
+        if (hipEventRecord(ea, ctx->stream) != hipSuccess) { rc = X2G_EHIP; break; }
+        for (int k = 0; k < 3 && !rc; ++k) rc = launch_sweep_once<T>(ctx, (k % 2) ? de : dodd,
+                                                                    (k % 2) ? plan.even : plan.odd, plan, false);
+        if (rc) break;
+        if (hipEventRecord(eb, ctx->stream) != hipSuccess || hipEventSynchronize(eb) != hipSuccess) {
+            rc = X2G_EHIP;
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, ea, eb);
+        ms /= 3.0f;
+        e.tried.push_back({kn, ms});
+        if (ms < best_ms) {
+            best_ms = ms;
+            e.best = kn;
+        }
+    }
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    if (rc) {
+        set_error("sweep autotuner: a candidate launch failed");
+        return rc;
+    }
+    e.uses = 1;
+    out = e.best;
+    if (getenv("X2G_TUNE_LOG")) {
+        fprintf(stderr, "x2g autotune %s:", key.c_str());
+        for (auto &t : e.tried)
+            fprintf(stderr, " (br %d wpc %d split %d) %.1f us", t.first.br, t.first.wpc, t.first.split, 1e3 * t.second);
+        fprintf(stderr, " -> br %d wpc %d split %d\n", e.best.br, e.best.wpc, e.best.split);
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    g_tune.emplace(key, std::move(e));
     return X2G_OK;
 }
 
@@ -2160,6 +2394,54 @@ static int launch_levels(x2g_ctx *ctx, const std::vector<LevelDesc> &v, int nswe
 
 // ---------------------------------------------------------------------------
 // C ABI
+
+extern "C" int x2g_set_autotune(int on)
+{
+    engine_defaults();
+    g_autotune = on != 0;
+    if (on < 0) {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        g_tune.clear();
+        g_autotune = 1;
+    }
+    return X2G_OK;
+}
+
+extern "C" int x2g_set_sweep_plan(int block_rows, int wg_per_cu, int split)
+{
+    X2G_CHECK(block_rows == 0 || block_rows == 2 || block_rows == 4,
+              "x2g_set_sweep_plan: block_rows must be 0, 2 or 4");
+    X2G_CHECK(wg_per_cu >= 0 && wg_per_cu <= 16, "x2g_set_sweep_plan: wg_per_cu must be 0..16");
+    X2G_CHECK(split == 0 || split == 1, "x2g_set_sweep_plan: split must be 0 or 1");
+    engine_defaults();
+    g_sweep_br = block_rows ? block_rows : 4;
+    g_sweep_wpc = wg_per_cu;
+    g_sweep_split = split;
+    g_plan_fixed = block_rows || wg_per_cu || split;
+    return X2G_OK;
+}
+
+extern "C" int x2g_tune_report(char *buf, int len)
+{
+    X2G_CHECK(buf && len > 0, "x2g_tune_report: null buffer");
+    std::string out;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        for (auto &kv : g_tune) {
+            char line[512];
+            int n = snprintf(line, sizeof(line), "%s: br %d wpc %d split %d (batches %ld);", kv.first.c_str(),
+                             kv.second.best.br, kv.second.best.wpc, kv.second.best.split, kv.second.uses);
+            for (auto &t : kv.second.tried)
+                if (n < (int)sizeof(line) - 40)
+                    n += snprintf(line + n, sizeof(line) - n, " %d/%d/%d %.1fus", t.first.br, t.first.wpc,
+                                  t.first.split, 1e3 * t.second);
+            out += line;
+            out += "\n";
+        }
+    }
+    snprintf(buf, (size_t)len, "%s", out.c_str());
+    return (int)out.size();
+}
 
 extern "C" int x2g_set_sweep_engine(int engine, int strip_rows)
 {
@@ -2505,9 +2787,10 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
                               it.isym, it.omega);
             A.push_back(bufA(i64[k]));
             B.push_back(bufB(i64[k]));
+            base[k].lr = lrp(i64[k]);
+            base[k].lr_bytes = (unsigned)lrbytes(i64[k]);
         }
-        plan_sweeps(ctx, base, A, B, p64);
-        if (p64.kind == 2) {
+        if (kind64 == 2) {
             // lhs / rhs do not change during the call: one residue-split copy
             std::vector<PackDesc<double>> pk;
             for (size_t k = 0; k < i64.size(); ++k) {
@@ -2515,13 +2798,12 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
                 x2g_grid *g = grids[it.grid_index];
                 pk.push_back({it.lhs_d, lrp(i64[k]), it.omega, g->nni, g->nmu, 0});
                 pk.push_back({it.rhs_d, lrp(i64[k]), 0.0, g->nni, g->nmu, 1});
-                for (auto *v : {&p64.even, &p64.odd}) {
-                    (*v)[k].lr = lrp(i64[k]);
-                    (*v)[k].lr_bytes = (unsigned)lrbytes(i64[k]);
-                }
             }
             if ((rc = launch_pack(ctx, pk))) return rc;
         }
+        SweepKnobs kn64;
+        if ((rc = tuned_knobs<double>(ctx, base, A, B, false, kn64))) return rc;
+        plan_sweeps(ctx, base, A, B, p64, &kn64);
         if ((rc = stage_plan(ctx, p64, &d64e, &d64o))) return rc;
         const bool final_in_b = p64.kind != 0 && (maxsor2 % 2) == 1;
         for (size_t k = 0; k < i64.size(); ++k) {
@@ -2618,17 +2900,18 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             bd.isym = it.isym;
             bnd.push_back(bd);
         }
-        plan_sweeps(ctx, base, A, B, p32);
+        if (kind32 == 2)
+            for (size_t k = 0; k < i32.size(); ++k) {
+                base[k].lr = f32p(i32[k], 2);
+                base[k].lr_bytes = (unsigned)lrbytes(i32[k]);
+            }
+        SweepKnobs kn32;
+        if ((rc = tuned_knobs<float>(ctx, base, A, B, true, kn32))) return rc;
+        plan_sweeps(ctx, base, A, B, p32, &kn32);
         // the strip engines ping-pong: an odd sweep count leaves the correction in B
         if (p32.kind != 0 && (maxsor2 % 2) == 1)
             for (size_t k = 0; k < i32.size(); ++k) corr[k].e = f32p(i32[k], 1);
         const bool split32 = p32.kind == 2;
-        if (split32)
-            for (size_t k = 0; k < i32.size(); ++k)
-                for (auto *v : {&p32.even, &p32.odd}) {
-                    (*v)[k].lr = f32p(i32[k], 2);
-                    (*v)[k].lr_bytes = (unsigned)lrbytes(i32[k]);
-                }
         if ((rc = stage_plan(ctx, p32, &d32e, &d32o))) return rc;
         if ((rc = stage_descs(ctx, res, doff, &d_res))) return rc;
         if ((rc = stage_descs(ctx, corr, doff, &d_corr))) return rc;

@@ -105,6 +105,9 @@ _PROTOS = {
     "x2g_mcsor": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
     "x2g_mcsor_host": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
     "x2g_mcsor_noalloc": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
+    "x2g_set_autotune": (_int, [_int]),
+    "x2g_set_sweep_plan": (_int, [_int, _int, _int]),
+    "x2g_tune_report": (_int, [C.c_char_p, _int]),
     "x2g_relax_ordered": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int, _int, _int]),
     "x2g_relax_batch": (_int, [_vp, _vp, C.POINTER(RelaxItem), _int, _int, _int]),
     "x2g_residual_norms": (_int, [_vp, _vp, C.POINTER(RelaxItem), _int, _vp]),
@@ -144,6 +147,14 @@ def load():
 
 class X2GError(RuntimeError):
     pass
+
+
+def tune_report():
+    """The sweep autotuner's tuned batch shapes (x2g_tune_report), one line each."""
+    L = load()
+    buf = C.create_string_buffer(1 << 16)
+    L.x2g_tune_report(buf, len(buf))
+    return [ln for ln in buf.value.decode().splitlines() if ln]
 
 
 def check(rc):
