@@ -686,7 +686,10 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     }
 
     // Lane roles: pass waves (wave < 5, pass = wave, uniform) and spill lanes
-    // (pass q / MS, point 64 NT + q % MS; pass 5 = idle).  Per point j:
+    // q (pass q / MSp, point 64 NT + q % MSp for q < 5 MSp, else pass 5 =
+    // idle; MSp = this problem's points per pass and row beyond 64 NT, at
+    // most the launch's MS: a problem narrower than the launch's widest
+    // leaves whole spill waves idle).  Per point j:
     // tq = t clamped to M-1 (keeps the LDS reads of absent points inside the
     // row), lim = 5t for a point of the grid's colour blocks, else a value no
     // row accepts, and toff = its byte offset in a row of (omega/lhs, rhs)
@@ -694,19 +697,25 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool pass_wave = wave < 5;
     const int q = tid - 320;
-    const int p_spill = pass_wave || MS == 0 ? 5 : min(q / MS, 5);
+    const int MSp = max(0, min(MS, M - 64 * NT));
+    const int p_spill = pass_wave || q >= 5 * MSp ? 5 : q / MSp;
     constexpr int kNoPoint = 1 << 20;
     int tq[NT], lim[NT];
     unsigned toff[NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
-        const int t = pass_wave ? (tid & 63) + 64 * j : (j == 0 && p_spill < 5 ? 64 * NT + q % MS : M);
+        const int t = pass_wave ? (tid & 63) + 64 * j : (j == 0 && p_spill < 5 ? 64 * NT + q % MSp : M);
         const bool ex = t < M;
         tq[j] = min(t, M - 1);
         lim[j] = ex ? 5 * t : kNoPoint;
         toff[j] = ex ? (unsigned)(t * (int)(2 * sizeof(T))) : kOOBLane;
     }
     const int M5 = 5 * M;
+    // A spill wave none of whose lanes holds a point of this problem (a
+    // problem narrower than the launch's widest) skips the compute phase;
+    // its loads and stores stay, so every wave issues the same memory
+    // operations.
+    const bool spill_idle = !pass_wave && __ballot(lim[0] != kNoPoint) == 0;
 
     // (omega/lhs, rhs) of pass p's points in step s (rows of block
     // s - SKEW p); rows outside the pass's range or the grid and absent points
@@ -858,7 +867,7 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     };
     auto compute_any = [&](int s, const T (&lv)[BR][NT], const T (&rv)[BR][NT]) {
         if (pass_wave) compute(s, wave, lv, rv);
-        else compute(s, p_spill, lv, rv);
+        else if (!spill_idle) compute(s, p_spill, lv, rv);
     };
     // The mu=1 row (sormcsor.F90:333-350) of the first strip, after its last
     // step: rows 2..6 are final in dst, written by this workgroup (the
@@ -894,7 +903,8 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     };
 
     // Block s+R+1 and the (omega/lhs, rhs) of step s+1 are loaded at the
-    // start of step s and land during its compute phase.  Block
+    // start of step s and land during its compute phase (the compiler's
+    // waits hold back on them at their first use, after the compute).  Block
     // bw = s - 4 SKEW - R is final from the start of step s (pass 4 wrote it
     // in step s-R; in step s it only reads it, as a neighbour of block bw+R).
     // FP64 writes it out right after issuing the loads, so its stores overlap
@@ -904,7 +914,6 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     constexpr bool early_out = sizeof(T) == 8;
     T lv[BR][NT], rv[BR][NT];
     load_lr_any(0, lv, rv);
-    __builtin_amdgcn_s_waitcnt(0);
     for (int s = 0; s < S; ++s) {
         T ln[BR][NT], rn[BR][NT], bpre;
         u32x4 pre[NSV];
@@ -912,7 +921,6 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
         load_block(s + R + 1, pre, bpre);
         if (early_out) write_out(s - 4 * SKEW - R);
         compute_any(s, lv, rv);
-        __builtin_amdgcn_s_waitcnt(0);
         commit_block(s + R + 1, pre, bpre);
 #pragma unroll
         for (int rr = 0; rr < BR; ++rr)
@@ -1548,6 +1556,7 @@ struct SweepKnobs {
 };
 // k_sweep block height (4 or 2 rows; X2G_SWEEP_BR) when not tuned
 static int g_sweep_br = 4;
+
 // the plan's choices are fixed (x2g_set_sweep_plan, or any of X2G_SWEEP_BR,
 // X2G_SWEEP_WPC, X2G_SWEEP_SPLIT set): the autotuner stays out
 static bool g_plan_fixed = false;
@@ -1885,16 +1894,26 @@ static int set_lds_attr(x2g_ctx *ctx)
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     X2G_HIP(hipFuncSetAttribute((const void *)k_strip<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     const void *ks[] = {
-        (const void *)k_sweep<double, 1, 128, 4>, (const void *)k_sweep<double, 1, 192, 4>,
-        (const void *)k_sweep<double, 1, 256, 4>, (const void *)k_sweep<double, 1, 352, 4>,
-        (const void *)k_sweep<float, 1, 128, 4>,  (const void *)k_sweep<float, 1, 192, 4>,
-        (const void *)k_sweep<float, 1, 256, 4>,  (const void *)k_sweep<float, 1, 352, 4>,
-        (const void *)k_sweep<float, 1, 528, 4>,  (const void *)k_sweep<float, 2, 768, 4>,
-        (const void *)k_sweep<double, 1, 128, 2>, (const void *)k_sweep<double, 1, 192, 2>,
-        (const void *)k_sweep<double, 1, 256, 2>, (const void *)k_sweep<double, 1, 352, 2>,
-        (const void *)k_sweep<float, 1, 128, 2>,  (const void *)k_sweep<float, 1, 192, 2>,
-        (const void *)k_sweep<float, 1, 256, 2>,  (const void *)k_sweep<float, 1, 352, 2>,
-        (const void *)k_sweep<float, 1, 528, 2>,  (const void *)k_sweep<float, 2, 768, 2>};
+        (const void *)k_sweep<double, 1, 128, 4>,
+        (const void *)k_sweep<double, 1, 192, 4>,
+        (const void *)k_sweep<double, 1, 256, 4>,
+        (const void *)k_sweep<double, 1, 352, 4>,
+        (const void *)k_sweep<float, 1, 128, 4>,
+        (const void *)k_sweep<float, 1, 192, 4>,
+        (const void *)k_sweep<float, 1, 256, 4>,
+        (const void *)k_sweep<float, 1, 352, 4>,
+        (const void *)k_sweep<float, 1, 528, 4>,
+        (const void *)k_sweep<float, 2, 768, 4>,
+        (const void *)k_sweep<double, 1, 128, 2>,
+        (const void *)k_sweep<double, 1, 192, 2>,
+        (const void *)k_sweep<double, 1, 256, 2>,
+        (const void *)k_sweep<double, 1, 352, 2>,
+        (const void *)k_sweep<float, 1, 128, 2>,
+        (const void *)k_sweep<float, 1, 192, 2>,
+        (const void *)k_sweep<float, 1, 256, 2>,
+        (const void *)k_sweep<float, 1, 352, 2>,
+        (const void *)k_sweep<float, 1, 528, 2>,
+        (const void *)k_sweep<float, 2, 768, 2>};
     for (const void *k : ks) X2G_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     done = true;
     return X2G_OK;
