@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--rate", type=float, default=20.0, help="stream: mean arrivals per second")
     ap.add_argument("--plain-sample", type=int, default=6,
                     help="stream: requests also run one at a time on the plain FP64 GPU path")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="stream: skip serving the requests again with graph replay of the relaxations")
     ap.add_argument("--no-fp64", action="store_true", help="stream: skip serving the requests with FP64 sweeps")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="stream: CPU-baseline sample budget (whole converged reference runs, smallest first)")
@@ -262,12 +264,13 @@ def bench_poisson(args):
     print(json.dumps(line), flush=True)
 
 
-def serve_stream(reqs, prec, stats=False):
+def serve_stream(reqs, prec, stats=False, graphs=False):
     """Serve (arrival_s, name, spec) requests with the continuous-batching
-    server at the given precision; returns (handles, elapsed_s, sweep stats)."""
+    server at the given precision (graphs: relaxations replayed from captured
+    HIP graphs); returns (handles, elapsed_s, sweep stats)."""
     import ctypes as C
     from x2dhf_amd import gpu, scf, serve
-    server = serve.Server(max_active=16)
+    server = serve.Server(max_active=16, graphs=graphs)
     lib = server.lib
     if stats:
         gpu.check(lib.x2g_stats_enable(server.ctx.h, 1))
@@ -392,6 +395,23 @@ def bench_stream(args):
                 "compared": len(dE64)}
         print(f"# FP64 served: {e64_all:.2f} s, max |dE| mixed vs FP64 {fp64['max_abs_dE_mixed_vs_fp64']} over "
               f"{len(dE64)} requests converged in both", file=sys.stderr, flush=True)
+    graphs = None
+    if not args.no_graphs:
+        # the same requests again, relaxations replayed from captured HIP
+        # graphs (sweep statistics off: they time eager launches only)
+        hg_all, eg_all, caps, reps = [], 0.0, 0, 0
+        for k in range(args.steps):
+            reqs = serve.poisson_stream(count, args.rate, seed=args.seed + k, prec=args.prec)
+            hg, eg, _ = serve_stream(reqs, args.prec, graphs=True)
+            hg_all += hg
+            eg_all += eg
+        ng = sum(1 for h in hg_all if serve.converged(h.result))
+        same = all(a.result["etot"] == b.result["etot"] for a, b in zip(handles, hg_all))
+        graphs = {"solves_per_hr": ng / eg_all * 3600.0, "elapsed_s": eg_all,
+                  "p50_latency_s": serve.percentile([h.latency for h in hg_all], 50),
+                  "energies_identical_to_eager": same}
+        print(f"# graph replay: {eg_all:.2f} s, {graphs['solves_per_hr']:.0f} solves/hr, energies identical: {same}",
+              file=sys.stderr, flush=True)
     plain = None
     if not args.no_plain and args.plain_sample > 0:
         plain_s, dEp, its = 0.0, [], 0
@@ -446,6 +466,7 @@ def bench_stream(args):
         "latency_p50_by_molecule_s": {k: serve.percentile(v, 50) for k, v in sorted(by_mol.items())},
         "fp64_served": fp64,
         "speedup_vs_fp64_served": value / fp64["solves_per_hr"] if fp64 else None,
+        "graph_replay": graphs,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain_fp64_gpu": value / plain["solves_per_hr"] if plain else None,
         "max_abs_dE_vs_fp64": max(dE64 + ([plain["max_abs_dE_vs_served"]] if plain and plain["max_abs_dE_vs_served"]

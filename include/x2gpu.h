@@ -153,6 +153,18 @@ int x2g_set_autotune(int on);
  * value keeps the autotuner out; (0, 0, 0) hands the choice back to it.  The
  * results do not depend on the plan.  No reference counterpart. */
 int x2g_set_sweep_plan(int block_rows, int wg_per_cu, int split);
+/* Replay x2g_relax_batch from captured HIP graphs on this context (default
+ * off; X2G_GRAPHS=1 at context creation turns it on).  The second call with
+ * the same signature -- items, grids, sweep counts, sweep-plan settings and
+ * workspace -- is captured into a graph of its kernel launches, with the
+ * launch descriptors in a device arena of the graph's own; later calls with
+ * that signature launch the graph.  Results are those of the eager calls.
+ * Not while sweep statistics are on.  on = 0 releases the graphs.  No
+ * reference counterpart (the reference keeps its worker threads across
+ * calls, c/mcsor_tpool.c:41-199; this keeps the launch sequence). */
+int x2g_set_graphs(x2g_ctx *ctx, int on);
+/* Graphs captured and replays launched on this context so far. */
+int x2g_graph_counts(x2g_ctx *ctx, long long *captures, long long *replays);
 /* The tuned shapes as text, one line per shape: "key: br B wpc W split S
  * (batches N); br/wpc/split time ..." (times per sweep of the candidates).
  * Writes at most len bytes including the terminating 0; returns the full

@@ -4,6 +4,8 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 
 namespace x2g {
 
@@ -72,6 +74,17 @@ int pinned_out_reserve(x2g_ctx *ctx, size_t bytes)
 
 int stage_bytes(x2g_ctx *ctx, const void *src, size_t bytes, const void **dev_out)
 {
+    if (ctx->cap_arena) {
+        // graph capture: the descriptors belong to the graph's own arena
+        const size_t off = (ctx->cap_off + 255) & ~size_t(255);
+        X2G_CHECK(off + bytes <= ctx->cap_bytes, "launch descriptors exceed the graph arena (%zu bytes)",
+                  ctx->cap_bytes);
+        if (ctx->cap_host.size() < off + bytes) ctx->cap_host.resize(off + bytes);
+        std::memcpy(ctx->cap_host.data() + off, src, bytes);
+        *dev_out = ctx->cap_arena + off;
+        ctx->cap_off = off + bytes;
+        return X2G_OK;
+    }
     const size_t cap = std::min(ctx->ws[2].bytes, ctx->pinned_bytes);
     X2G_CHECK(bytes <= cap, "launch descriptors (%zu bytes) exceed the staging ring", bytes);
     size_t off = (ctx->stage_off + 255) & ~size_t(255);
@@ -92,7 +105,7 @@ int stage_bytes(x2g_ctx *ctx, const void *src, size_t bytes, const void **dev_ou
 
 int stage_flush(x2g_ctx *ctx)
 {
-    if (ctx->stage_hi > ctx->stage_lo) {
+    if (ctx->stage_hi > ctx->stage_lo && !ctx->cap_arena) {
         const size_t lo = ctx->stage_lo;
         X2G_HIP(hipMemcpyAsync((char *)ctx->ws[2].ptr + lo, (char *)ctx->pinned + lo, ctx->stage_hi - lo,
                                hipMemcpyHostToDevice, ctx->stream));
@@ -213,6 +226,8 @@ extern "C" int x2g_ctx_create(int device, x2g_ctx **out)
         set_error("hipStreamCreateWithFlags failed");
         return X2G_EHIP;
     }
+    const char *gr = getenv("X2G_GRAPHS");
+    c->graphs_on = gr && atoi(gr) != 0;
     int rc = ws_reserve(c, 2, kStageBytes);
     if (!rc) rc = pinned_reserve(c, kStageBytes);
     if (!rc) rc = scratch_init(c);
@@ -224,10 +239,21 @@ extern "C" int x2g_ctx_create(int device, x2g_ctx **out)
     return X2G_OK;
 }
 
+void x2g::graphs_clear(x2g_ctx *ctx)
+{
+    for (auto &kv : ctx->graphs) {
+        (void)hipGraphExecDestroy(kv.second.exec);
+        (void)hipFree(kv.second.arena);
+    }
+    ctx->graphs.clear();
+    ctx->graph_seen.clear();
+}
+
 extern "C" int x2g_ctx_destroy(x2g_ctx *ctx)
 {
     if (!ctx) return X2G_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    graphs_clear(ctx);
     for (auto &p : ctx->pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -356,8 +382,10 @@ extern "C" int x2g_grid_create(x2g_ctx *ctx, int nni, int nmu, const double *coe
     X2G_CHECK(nni >= 7 && nmu >= 11, "x2g_grid_create: grid %dx%d too small", nni, nmu);
     X2G_CHECK((long)nni * nmu < (1L << 30), "x2g_grid_create: grid too large");
     *out = nullptr;
+    static std::atomic<unsigned long long> next_uid{1};
     x2g_grid *g = new x2g_grid();
     g->ctx = ctx;
+    g->uid = next_uid++;
     g->nni = nni;
     g->nmu = nmu;
     std::memcpy(g->coef, coef, sizeof(g->coef));

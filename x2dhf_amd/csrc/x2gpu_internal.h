@@ -72,6 +72,26 @@ struct x2g_ctx {
     // this context's device (shared by every context on that device)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // x2g_relax_batch replayed from captured HIP graphs (x2g_set_graphs):
+    // the instantiated graph of each call signature with the device arena
+    // that holds its launch descriptors, and the signatures seen once (a
+    // signature is captured on its second call, after the first has tuned
+    // the sweep plan and sized the workspace)
+    struct GraphEntry {
+        hipGraphExec_t exec = nullptr;
+        void *arena = nullptr;
+        void *ws0 = nullptr, *ws3 = nullptr;  // workspace the graph addresses
+        long last_use = 0;
+    };
+    bool graphs_on = false;
+    std::unordered_map<std::string, GraphEntry> graphs;
+    std::unordered_map<std::string, int> graph_seen;
+    long graph_clock = 0, graph_captures = 0, graph_replays = 0;
+    // during a capture: descriptors are staged into cap_host and addressed in
+    // cap_arena, where they are copied once the capture has ended
+    char *cap_arena = nullptr;
+    size_t cap_bytes = 0, cap_off = 0;
+    std::vector<char> cap_host;
 };
 
 // Level schedule of an ordered (MC)SOR sweep on one grid (device arrays):
@@ -85,6 +105,7 @@ struct x2g_levels {
 
 struct x2g_grid {
     x2g_ctx *ctx = nullptr;
+    unsigned long long uid = 0;    // unique per grid created in this process
     std::map<int, x2g_levels> levels;  // by (method, ordering), built on first use
     int nni = 0, nmu = 0;
     double coef[25];               // dmu1,dmu2,dni1,dni2 (16), exeven (5), diag, hni, hmu, r
@@ -109,6 +130,8 @@ int pinned_reserve(x2g_ctx *ctx, size_t bytes);
 int stage_bytes(x2g_ctx *ctx, const void *src, size_t bytes, const void **dev_out);
 // Issue one stream-ordered copy of everything staged since the last flush.
 int stage_flush(x2g_ctx *ctx);
+// Release every captured graph of the context.
+void graphs_clear(x2g_ctx *ctx);
 // Launch a kernel on the context's stream once the staged descriptors are on
 // their way (every launch goes through here, so a call's descriptors cost one
 // copy however many arrays it stages).

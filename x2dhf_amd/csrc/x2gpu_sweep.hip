@@ -28,6 +28,8 @@
 //    the batch's problems.
 #include "x2gpu_internal.h"
 #include <algorithm>
+#include <string>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <cmath>
@@ -1561,6 +1563,9 @@ static int g_sweep_br = 4;
 // X2G_SWEEP_WPC, X2G_SWEEP_SPLIT set): the autotuner stays out
 static bool g_plan_fixed = false;
 static int g_autotune = -1;  // -1: from X2G_AUTOTUNE (default on)
+// bumped by every setting above (a captured relaxation graph is only
+// replayed under the settings it was captured with)
+static std::atomic<long> g_plan_gen{0};
 static void engine_defaults()
 {
     if (g_engine >= 0) return;
@@ -2418,6 +2423,7 @@ extern "C" int x2g_set_autotune(int on)
 {
     engine_defaults();
     g_autotune = on != 0;
+    ++g_plan_gen;
     if (on < 0) {
         std::lock_guard<std::mutex> lk(g_tune_mutex);
         g_tune.clear();
@@ -2437,6 +2443,7 @@ extern "C" int x2g_set_sweep_plan(int block_rows, int wg_per_cu, int split)
     g_sweep_wpc = wg_per_cu;
     g_sweep_split = split;
     g_plan_fixed = block_rows || wg_per_cu || split;
+    ++g_plan_gen;
     return X2G_OK;
 }
 
@@ -2468,6 +2475,7 @@ extern "C" int x2g_set_sweep_engine(int engine, int strip_rows)
     engine_defaults();
     g_engine = engine;
     g_strip_rows = strip_rows;
+    ++g_plan_gen;
     return X2G_OK;
 }
 
@@ -2669,8 +2677,8 @@ static inline bool ordered(const x2g_relax_item &it)
     return it.method != X2G_MCSOR || it.ordering != X2G_ORDER_COLWISE;
 }
 
-extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
-                               int maxsor1, int maxsor2)
+static int relax_batch_eager(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
+                             int maxsor1, int maxsor2)
 {
     X2G_CHECK(ctx && grids && items && n >= 0, "x2g_relax_batch: null argument");
     X2G_CHECK(maxsor1 >= 0 && maxsor2 >= 0, "x2g_relax_batch: negative sweep counts");
@@ -2996,6 +3004,168 @@ extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_r
             X2G_HIP(hipGetLastError());
         }
     }
+    return X2G_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Graph replay of x2g_relax_batch.  A call's launch sequence depends only on
+// its signature: the items (arrays, parameters, scheme), their grids, the
+// sweep counts, the sweep-plan settings and the workspace addresses.  The
+// second call of a signature is captured into a HIP graph whose launch
+// descriptors live in a device arena of the graph's own (the staging ring
+// is reused by later calls, so a graph cannot point into it); later calls of
+// the signature replay the graph.  The first call runs eagerly: it tunes the
+// sweep plan, builds level schedules and sizes the workspace, so the capture
+// makes no allocation and no synchronisation.  The captured sequence holds
+// kernels only (no memset or copy nodes).
+constexpr size_t kGraphArena = 1 << 20;   // first arena size; doubled on overflow
+constexpr size_t kMaxGraphs = 64;         // cached graphs per context (LRU)
+
+static std::string relax_signature(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
+                                   int maxsor1, int maxsor2)
+{
+    std::string k;
+    auto put = [&](const auto &v) { k.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
+    put(n);
+    put(maxsor1);
+    put(maxsor2);
+    put(g_plan_gen.load());
+    put(ctx->ws[0].ptr);
+    put(ctx->ws[3].ptr);
+    for (int i = 0; i < n; ++i) {
+        const x2g_relax_item &it = items[i];
+        const x2g_grid *g = grids[it.grid_index];
+        put(g);
+        put(g->uid);
+        put(it.fun_d);
+        put(it.lhs_d);
+        put(it.rhs_d);
+        put(it.tail_fock1_d);
+        put(it.rhs_f1_d);
+        put(it.rhs_f2_d);
+        put(it.rhs_f3_d);
+        put(it.omega);
+        put(it.isym);
+        put(it.bkind);
+        put(it.prec);
+        put(it.method);
+        put(it.ordering);
+    }
+    return k;
+}
+
+// Capture relax_batch_eager into a graph (the caller has checked the
+// arguments).  On success the graph is cached under `key`; the captured work
+// has not run yet.
+static int relax_capture(x2g_ctx *ctx, const std::string &key, x2g_grid *const *grids,
+                         const x2g_relax_item *items, int n, int maxsor1, int maxsor2,
+                         x2g_ctx::GraphEntry **out)
+{
+    int rc = stage_flush(ctx);  // descriptors staged before the capture go the usual way
+    if (rc) return rc;
+    for (size_t cap = kGraphArena; cap <= 64 * kGraphArena; cap *= 2) {
+        void *arena = nullptr;
+        X2G_HIP(hipMalloc(&arena, cap));
+        ctx->cap_arena = (char *)arena;
+        ctx->cap_bytes = cap;
+        ctx->cap_off = 0;
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            rc = relax_batch_eager(ctx, grids, items, n, maxsor1, maxsor2);
+            e = hipStreamEndCapture(ctx->stream, &graph);
+        }
+        ctx->cap_arena = nullptr;
+        const bool overflow = rc && ctx->cap_off + 256 > cap;
+        if (e != hipSuccess || rc || !graph) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipFree(arena);
+            (void)hipGetLastError();
+            if (overflow) continue;
+            set_error("x2g_relax_batch: graph capture failed (%s)", e != hipSuccess ? hipGetErrorString(e)
+                                                                                      : "launch error");
+            return rc ? rc : X2G_EHIP;
+        }
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e == hipSuccess)
+            e = hipMemcpy(arena, ctx->cap_host.data(), ctx->cap_off, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            (void)hipFree(arena);
+            set_error("x2g_relax_batch: graph instantiation failed (%s)", hipGetErrorString(e));
+            return X2G_EHIP;
+        }
+        if (ctx->graphs.size() >= kMaxGraphs) {
+            auto victim = ctx->graphs.begin();
+            for (auto it = ctx->graphs.begin(); it != ctx->graphs.end(); ++it)
+                if (it->second.last_use < victim->second.last_use) victim = it;
+            (void)hipGraphExecDestroy(victim->second.exec);
+            (void)hipFree(victim->second.arena);
+            ctx->graphs.erase(victim);
+        }
+        x2g_ctx::GraphEntry &g = ctx->graphs[key];
+        g.exec = exec;
+        g.arena = arena;
+        g.ws0 = ctx->ws[0].ptr;
+        g.ws3 = ctx->ws[3].ptr;
+        ++ctx->graph_captures;
+        *out = &g;
+        return X2G_OK;
+    }
+    set_error("x2g_relax_batch: launch descriptors too large for a graph arena");
+    return X2G_EINVAL;
+}
+
+extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
+                               int maxsor1, int maxsor2)
+{
+    X2G_CHECK(ctx && grids && items && n >= 0, "x2g_relax_batch: null argument");
+    // statistics time launches with events on the stream: eager only
+    if (!ctx->graphs_on || ctx->stats_on || n == 0 || maxsor1 == 0)
+        return relax_batch_eager(ctx, grids, items, n, maxsor1, maxsor2);
+    for (int i = 0; i < n; ++i)
+        X2G_CHECK(items[i].grid_index >= 0 && grids[items[i].grid_index], "x2g_relax_batch: bad grid index %d",
+                  items[i].grid_index);
+    const std::string key = relax_signature(ctx, grids, items, n, maxsor1, maxsor2);
+    x2g_ctx::GraphEntry *ge = nullptr;
+    auto it = ctx->graphs.find(key);
+    if (it != ctx->graphs.end()) {
+        ge = &it->second;
+    } else {
+        int &seen = ctx->graph_seen[key];
+        if (seen < 0 || ++seen < 2) return relax_batch_eager(ctx, grids, items, n, maxsor1, maxsor2);
+        if (ctx->graph_seen.size() > 4096) ctx->graph_seen.clear();
+        int rc = relax_capture(ctx, key, grids, items, n, maxsor1, maxsor2, &ge);
+        if (rc) {
+            // the signature stays eager (the failed capture ran nothing)
+            ctx->graph_seen[key] = -1;
+            return relax_batch_eager(ctx, grids, items, n, maxsor1, maxsor2);
+        }
+    }
+    ge->last_use = ++ctx->graph_clock;
+    X2G_HIP(hipGraphLaunch(ge->exec, ctx->stream));
+    ++ctx->graph_replays;
+    return X2G_OK;
+}
+
+extern "C" int x2g_set_graphs(x2g_ctx *ctx, int on)
+{
+    X2G_CHECK(ctx, "x2g_set_graphs: null ctx");
+    if (ctx->graphs_on && !on) {
+        X2G_HIP(hipStreamSynchronize(ctx->stream));
+        graphs_clear(ctx);
+    }
+    ctx->graphs_on = on != 0;
+    return X2G_OK;
+}
+
+extern "C" int x2g_graph_counts(x2g_ctx *ctx, long long *captures, long long *replays)
+{
+    X2G_CHECK(ctx && captures && replays, "x2g_graph_counts: null argument");
+    *captures = ctx->graph_captures;
+    *replays = ctx->graph_replays;
     return X2G_OK;
 }
 

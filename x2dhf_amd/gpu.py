@@ -107,6 +107,8 @@ _PROTOS = {
     "x2g_mcsor_noalloc": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
     "x2g_set_autotune": (_int, [_int]),
     "x2g_set_sweep_plan": (_int, [_int, _int, _int]),
+    "x2g_set_graphs": (_int, [_vp, _int]),
+    "x2g_graph_counts": (_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "x2g_tune_report": (_int, [C.c_char_p, _int]),
     "x2g_relax_ordered": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int, _int, _int]),
     "x2g_relax_batch": (_int, [_vp, _vp, C.POINTER(RelaxItem), _int, _int, _int]),

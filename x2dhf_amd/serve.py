@@ -53,11 +53,16 @@ class Request:
 
 class Server:
     """Continuous-batching server.  max_active caps the jobs stepped together
-    (slots); hbm_budget (bytes) caps the footprint of the admitted jobs."""
+    (slots); hbm_budget (bytes) caps the footprint of the admitted jobs;
+    graphs replays the batched relaxations from captured HIP graphs."""
 
-    def __init__(self, device=0, max_active=16, hbm_budget=None, hbm_fraction=0.8, clock=time.perf_counter):
+    def __init__(self, device=0, max_active=16, hbm_budget=None, hbm_fraction=0.8, clock=time.perf_counter,
+                 graphs=False):
         self.ctx = gpu.Context(device)
         self.lib = gpu.load()
+        # graphs: replay each batch composition's relaxation launches from a
+        # captured HIP graph (x2g_set_graphs)
+        gpu.check(self.lib.x2g_set_graphs(self.ctx.h, 1 if graphs else 0))
         if hbm_budget is None:
             free, total = gpu._szt(), gpu._szt()
             gpu.check(self.lib.x2g_mem_info(self.ctx.h, free, total))
@@ -148,6 +153,13 @@ class Server:
                 continue
             self.step()
         return self.done
+
+    def graph_counts(self):
+        """(graphs captured, graph launches) of the server's context."""
+        import ctypes as C
+        c, r = C.c_longlong(), C.c_longlong()
+        gpu.check(self.lib.x2g_graph_counts(self.ctx.h, C.byref(c), C.byref(r)))
+        return c.value, r.value
 
     def close(self):
         for r in self.active:
