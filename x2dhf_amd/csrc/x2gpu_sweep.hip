@@ -34,6 +34,7 @@
 #include <mutex>
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 
 namespace x2g {
 
@@ -862,15 +863,6 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
             }
         }
     };
-    // the two lane roles: the pass of a pass wave is a scalar
-    auto load_lr_any = [&](int s, T (&lv)[BR][NT], T (&rv)[BR][NT]) {
-        if (pass_wave) load_lr(s, wave, lv, rv);
-        else load_lr(s, p_spill, lv, rv);
-    };
-    auto compute_any = [&](int s, const T (&lv)[BR][NT], const T (&rv)[BR][NT]) {
-        if (pass_wave) compute(s, wave, lv, rv);
-        else if (!spill_idle) compute(s, p_spill, lv, rv);
-    };
     // The mu=1 row (sormcsor.F90:333-350) of the first strip, after its last
     // step: rows 2..6 are final in dst, written by this workgroup (the
     // barrier makes its stores visible to all of its threads).  A fixed number
@@ -914,26 +906,36 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     // measured faster that way).  The commit of block s+R+1 reuses the slot of
     // block s+R+1-RB = s-4 SKEW-R-1, written out in step s-1.
     constexpr bool early_out = sizeof(T) == 8;
-    T lv[BR][NT], rv[BR][NT];
-    load_lr_any(0, lv, rv);
-    for (int s = 0; s < S; ++s) {
-        T ln[BR][NT], rn[BR][NT], bpre;
-        u32x4 pre[NSV];
-        load_lr_any(s + 1, ln, rn);
-        load_block(s + R + 1, pre, bpre);
-        if (early_out) write_out(s - 4 * SKEW - R);
-        compute_any(s, lv, rv);
-        commit_block(s + R + 1, pre, bpre);
+    // The two lane roles run separate copies of the step loop (each wave is
+    // wholly in one role and every copy meets the same barriers), so that in
+    // a pass wave the pass and all that depends on it stay scalar.
+    auto steps = [&](auto role) {
+        constexpr bool PASS = decltype(role)::value;
+        const int p = PASS ? wave : p_spill;
+        const bool busy = PASS || !spill_idle;
+        T lv[BR][NT], rv[BR][NT];
+        load_lr(0, p, lv, rv);
+        for (int s = 0; s < S; ++s) {
+            T ln[BR][NT], rn[BR][NT], bpre;
+            u32x4 pre[NSV];
+            load_lr(s + 1, p, ln, rn);
+            load_block(s + R + 1, pre, bpre);
+            if (early_out) write_out(s - 4 * SKEW - R);
+            if (busy) compute(s, p, lv, rv);
+            commit_block(s + R + 1, pre, bpre);
 #pragma unroll
-        for (int rr = 0; rr < BR; ++rr)
+            for (int rr = 0; rr < BR; ++rr)
 #pragma unroll
-            for (int j = 0; j < NT; ++j) {
-                lv[rr][j] = ln[rr][j];
-                rv[rr][j] = rn[rr][j];
-            }
-        if (!early_out) write_out(s - 4 * SKEW - R);
-        __syncthreads();
-    }
+                for (int j = 0; j < NT; ++j) {
+                    lv[rr][j] = ln[rr][j];
+                    rv[rr][j] = rn[rr][j];
+                }
+            if (!early_out) write_out(s - 4 * SKEW - R);
+            __syncthreads();
+        }
+    };
+    if (pass_wave) steps(std::true_type{});
+    else steps(std::false_type{});
     if (a == 2) {
         __syncthreads();
         row_one();
