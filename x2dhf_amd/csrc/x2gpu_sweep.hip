@@ -755,10 +755,10 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
     // of one pass in a row do not read each other (a point's neighbours have
     // other colours), so a row's stencil values are gathered first and its
     // arithmetic follows, which keeps the LDS reads in flight together.
-    auto compute = [&](int s, int p, const T (&lv)[BR][NT], const T (&rv)[BR][NT]) {
-        const int b = s - SKEW * p;
-        if (p >= 5 || b < R * p || b > K - 1 - R * p) return;  // idle lanes and passes outside their range
+    auto compute_rows = [&](int b, int p, const T (&lv)[BR][NT], const T (&rv)[BR][NT], auto contig) {
+        constexpr bool CONTIG = decltype(contig)::value;
         const int sb = slot_of(b);
+        const T *const wb = ring + slot_of(b - R) * BR * W;  // the window's first block (CONTIG)
         // the ring blocks b-R .. b+R
         const T *blk[2 * R + 1];
 #pragma unroll
@@ -792,7 +792,22 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
                     cbv[g][j] = cb;
                     okv[g][j] = lim[j] <= lim_r;
                     const T *q0 = b0 + cb;
-                    if constexpr (sizeof(T) == 4) {
+                    if constexpr (CONTIG) {
+                        // all 17 values from one opaque base at the window's
+                        // first row, with immediate offsets (own row 4 R ... of
+                        // the window: R BR = 4 rows above block b)
+                        int o = (int)(wb - dnu_s) + cb;  // from the start of the LDS
+                        asm("" : "+v"(o));
+                        ov[g][j] = o + (4 + rr) * W;
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) x[g][j][k] = dnu_s[o + (4 + rr) * W + k];
+#pragma unroll
+                        for (int k = 1; k <= 4; ++k) {
+                            x[g][j][8 + k] = dnu_s[o + (4 + rr - k) * W + 4];
+                            x[g][j][12 + k] = dnu_s[o + (4 + rr + k) * W + 4];
+                        }
+                        continue;
+                    } else if constexpr (sizeof(T) == 4) {
                         // the own row's nine values from one opaque base, so that
                         // they are read with immediate offsets of that base
                         int o = (int)(b0 - dnu_s) + rr * W + cb;  // from the start of the LDS
@@ -863,6 +878,24 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
             }
         }
     };
+    //
+    // The window of blocks b-R .. b+R occupies consecutive ring slots except
+    // when it wraps around the ring's end.  A pass wave (its block is a
+    // scalar) whose window does not wrap reads all 17 stencil values of a
+    // point from one base register with immediate offsets (FP32); otherwise
+    // each block has its own base.
+    auto compute = [&](int s, int p, const T (&lv)[BR][NT], const T (&rv)[BR][NT], auto role) {
+        const int b = s - SKEW * p;
+        if (p >= 5 || b < R * p || b > K - 1 - R * p) return;  // idle lanes and passes outside their range
+        const int sw = slot_of(b - R);
+        if constexpr (decltype(role)::value && sizeof(T) == 4) {
+            if (sw + 2 * R < RB) {
+                compute_rows(b, p, lv, rv, std::true_type{});
+                return;
+            }
+        }
+        compute_rows(b, p, lv, rv, std::false_type{});
+    };
     // The mu=1 row (sormcsor.F90:333-350) of the first strip, after its last
     // step: rows 2..6 are final in dst, written by this workgroup (the
     // barrier makes its stores visible to all of its threads).  A fixed number
@@ -921,7 +954,7 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
             load_lr(s + 1, p, ln, rn);
             load_block(s + R + 1, pre, bpre);
             if (early_out) write_out(s - 4 * SKEW - R);
-            if (busy) compute(s, p, lv, rv);
+            if (busy) compute(s, p, lv, rv, role);
             commit_block(s + R + 1, pre, bpre);
 #pragma unroll
             for (int rr = 0; rr < BR; ++rr)
