@@ -49,6 +49,10 @@ GOLDEN = os.path.join(HERE, "tests", "golden", "scf_config2_n2_331x601.json")
 REF_EXE = os.path.join(HERE, "oracle", "_ref", "x2dhf_omp")
 REF_TPOOL = os.path.join(HERE, "oracle", "_ref", "x2dhf_tp")
 TRAFFIC = os.path.join(HERE, "profiles", "traffic_sweep_n2_331x601.json")
+# the stream's dominant sweep launches: 2-row blocks of ring stride 528
+TRAFFIC_STREAM = os.path.join(HERE, "profiles", "traffic_sweep_stream_505x901.json")
+# problems of the stream traffic probe: (nni, nmu, count)
+STREAM_PROBE = ((505, 901, 12), (361, 601, 12))
 HBM_PEAK_GBS = 8000.0
 METRIC = "converged SCF solves/hr + p50/p99 latency vs plain FP64 GPU at ΔE<1e-8 Eh; GB/s"
 
@@ -78,11 +82,15 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="stream: CPU-baseline sample budget (whole converged reference runs, smallest first)")
     ap.add_argument("--seed", type=int, default=0, help="poisson / stream: seed of the synthetic workload")
-    ap.add_argument("--traffic-probe", action="store_true",
-                    help="only run a copy kernel of known size and a few sweeps (for rocprofv3 --pmc)")
+    ap.add_argument("--traffic-probe", nargs="?", const="n2", choices=["n2", "stream"],
+                    help="only run a copy kernel of known size and a few sweeps (for rocprofv3 --pmc): n2 = "
+                         "--batch x 22 N2 331x601 potentials (FP32 and FP64), stream = the stream's widest "
+                         "launch class (12 problems on 505x901 and 12 on 361x601, FP32, 2-row blocks)")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="turn the counter_collection.csv files of the FETCH_SIZE and WRITE_SIZE runs of "
                          "--traffic-probe into the per-launch HBM traffic file read by the benchmark")
+    ap.add_argument("--probe", choices=["n2", "stream"], default="n2",
+                    help="which --traffic-probe the --traffic-summary files come from")
     return ap.parse_args()
 
 
@@ -174,6 +182,82 @@ def traffic_probe(batch):
     big.destroy()
     ctx.close()
     print("traffic probe done", flush=True)
+
+
+def traffic_probe_stream():
+    """For the rocprofv3 counter runs: the copy kernel of traffic_probe, then
+    one relaxation call (5 FP32 sweeps) of the problems of STREAM_PROBE in one
+    batch, with the stream's usual plan for this class fixed (2-row blocks,
+    as many workgroups per CU as the LDS holds; no autotuning launches)."""
+    import numpy as np
+    from x2dhf_amd import gpu, scf
+    ctx = gpu.Context(0)
+    lib = ctx.lib
+    gpu.check(lib.x2g_set_autotune(0))
+    gpu.check(lib.x2g_set_sweep_plan(2, 0, 0))
+    rng = np.random.default_rng(0)
+    cb = scf.grid_build(1201, 1201, 7.0, 7.0, 2.068, 40.0)
+    big = gpu.Grid(ctx, 1201, 1201, *cb)
+    a, out = ctx.upload(rng.standard_normal(big.n)), ctx.alloc(big.n)
+    copy = (gpu.Term * 1)(gpu.Term(1.0, a.ptr, None, None))
+    for _ in range(3):
+        gpu.check(lib.x2g_lincomb(big.h, out.ptr, None, 0.0, copy, 1))
+    gpu.check(lib.x2g_ctx_sync(ctx.h))
+    gs, items, keep = [], [], []
+    for nni, nmu, cnt in STREAM_PROBE:
+        coef, suppl, vxi, veta = scf.grid_build(nni, nmu, 5.0, 3.0, 2.0, 40.0)
+        g = gpu.Grid(ctx, nni, nmu, coef, suppl, vxi, veta)
+        lhs = ctx.upload(suppl[7])
+        keep.append(lhs)
+        for _ in range(cnt):
+            f, r = ctx.upload(0.01 * rng.standard_normal(g.n)), ctx.upload(rng.standard_normal(g.n))
+            keep += [f, r]
+            items.append(gpu.RelaxItem(f.ptr, lhs.ptr, r.ptr, None, 1.9, 1, 1, 1, len(gs)))
+        gs.append(g)
+    gh = (gpu._vp * len(gs))(*[g.h.value for g in gs])
+    n = len(items)
+    gpu.check(lib.x2g_relax_batch(ctx.h, gh, (gpu.RelaxItem * n)(*items), n, 1, 5))
+    gpu.check(lib.x2g_ctx_sync(ctx.h))
+    for g in gs:
+        g.destroy()
+    big.destroy()
+    gpu.check(lib.x2g_set_sweep_plan(0, 0, 0))
+    ctx.close()
+    print("traffic probe done", flush=True)
+
+
+def traffic_summary_stream(fetch_csv, write_csv):
+    """HBM bytes per sweep launch of traffic_probe_stream, as traffic_summary:
+    counters scaled by the copy kernel, averaged over the four FP32 sweeps
+    that read their source (the first takes its zero start)."""
+    import csv
+    n_copy = 1201 * 1201 * 8
+
+    def per_kernel(path):
+        rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
+
+        def named(r, k):
+            n = r["Kernel_Name"]
+            return n.startswith(k) or f"::{k}(" in n or f"::{k}<" in n
+
+        copy = [float(r["Counter_Value"]) * 1024 for r in rows if named(r, "k_lincomb")]
+        sweep = [float(r["Counter_Value"]) * 1024 for r in rows if named(r, "k_sweep")]
+        if len(copy) != 3 or len(sweep) != 5:
+            raise SystemExit(f"{path}: expected 3 copy and 5 sweep dispatches, found {len(copy)} and {len(sweep)}")
+        return sum(copy) / len(copy), sweep[1:]
+
+    fc, fv = per_kernel(fetch_csv)
+    wc, wv = per_kernel(write_csv)
+    kf, kw = n_copy / fc, n_copy / wc
+    algo = sum(16.0 * cnt * (nni - 2) * (nmu - 5) for nni, nmu, cnt in STREAM_PROBE)
+    rd, wr = kf * sum(fv) / len(fv), kw * sum(wv) / len(wv)
+    out = {"workload": " + ".join(f"{cnt} x {nni}x{nmu}" for nni, nmu, cnt in STREAM_PROBE) +
+           " FP32 correction problems, one k_sweep<float,1,528,2> launch", "fetch_factor": kf, "write_factor": kw,
+           "fp32": {"hbm_read_bytes": rd, "hbm_write_bytes": wr, "hbm_bytes_per_launch": rd + wr,
+                    "algorithmic_bytes_per_launch": algo, "traffic_over_algorithmic": (rd + wr) / algo}}
+    with open(TRAFFIC_STREAM, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
 
 
 def traffic_summary(fetch_csv, write_csv, batch):
@@ -435,13 +519,15 @@ def bench_stream(args):
         print(f"# plain FP64 GPU, {n} requests one at a time: {plain_s:.2f} s, max |dE| {plain['max_abs_dE_vs_served']}",
               file=sys.stderr, flush=True)
     roof = sweep_roofline(st, args.prec)
-    if os.path.exists(TRAFFIC) and roof.get("algorithmic_bytes_per_launch"):
-        ratio = json.load(open(TRAFFIC)).get("fp32" if args.prec else "fp64", {}).get("traffic_over_algorithmic")
+    tfile = TRAFFIC_STREAM if args.prec else TRAFFIC
+    if os.path.exists(tfile) and roof.get("algorithmic_bytes_per_launch"):
+        ratio = json.load(open(tfile)).get("fp32" if args.prec else "fp64", {}).get("traffic_over_algorithmic")
         if ratio:
             roof["traffic"] = ratio * roof["algorithmic_bytes_per_launch"]
-            roof["traffic_source"] = ("measured/algorithmic HBM bytes of the 352-potential N2 launch (rocprofv3 "
-                                      "FETCH_SIZE / WRITE_SIZE, " + os.path.relpath(TRAFFIC, HERE) +
-                                      ") times this run's average algorithmic bytes per launch")
+            roof["traffic_source"] = ("measured/algorithmic HBM bytes of the probe launch in " +
+                                      os.path.relpath(tfile, HERE) + " (rocprofv3 FETCH_SIZE / WRITE_SIZE runs of "
+                                      "bench.py --traffic-probe) times this run's average algorithmic bytes per "
+                                      "launch")
     cpu = None if args.no_cpu else cpu_baseline_stream(timed, args.cpu_seconds)
     by_mol = {}
     for (t, name, spec), h in zip(timed, handles):
@@ -481,13 +567,19 @@ def bench_stream(args):
 def main():
     args = parse()
     if args.traffic_summary:
-        traffic_summary(*args.traffic_summary, args.batch)
+        if args.probe == "stream":
+            traffic_summary_stream(*args.traffic_summary)
+        else:
+            traffic_summary(*args.traffic_summary, args.batch)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (use --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe == "stream":
+        traffic_probe_stream()
+        return
     if args.traffic_probe:
         traffic_probe(args.batch)
         return
