@@ -13,7 +13,10 @@
 # The USE_PTHREAD build aborts ("stack smashing detected") as soon as it
 # relaxes with more than one MCSOR thread: its pthread_join writes a void *
 # into an int (src/c/mcsor_pthread.c:83-87, src/c/coulExch_pthread.c:119-120).
-# The thread-pool build joins with NULL and runs; it is the CPU baseline.
+# The thread-pool build joins with NULL and runs, but its orbital sweeps
+# diverge on the served-stream decks (orbital norms of 1e14 and then NaN in
+# the iteration after its MCSOR threads start), so the CPU baseline of the
+# stream is the OpenMP build on the same number of threads.
 #
 # Internal BLAS is used (the reference's USE_BLAS=OFF), so every operation on
 # the relaxation path is the reference's own code.
