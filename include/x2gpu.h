@@ -198,16 +198,27 @@ int x2g_mcsor_noalloc(x2g_grid *g, int isym, int bkind, double *work_d, const do
                       const double *rhs_d, double omega, int nsweeps);
 /* sor (sormcsor.F90:20-132, method X2G_SOR) or mcsor (sormcsor.F90:143-386,
  * method X2G_MCSOR, one thread) in mesh ordering `ordering`
- * (initCBAllocArrays.F90:2408-2540; the reference's default is middle) on the
+ * (initCBAllocArrays.F90:2408-2810; the reference's default is middle) on the
  * padded work_d, in place: nsweeps sweeps, each followed by the boundary
  * lines.  The points are updated in exactly the reference's sequential order
  * of dependencies: each sweep runs as a list of phases of mutually
  * independent points (for sor: the anti-diagonals of the ordering's
- * (position in row, row) plane), all in one workgroup. */
+ * (position in row, row) plane), all in one workgroup.  The orderings are
+ * the reference's `order` card values (read_order, inputInterface.F90:2397):
+ * col-wise, middle, middlemt, row-wise, rrow-wis, test1, test2, test3,
+ * test4, middle2. */
 #define X2G_MCSOR 0
 #define X2G_SOR 1
 #define X2G_ORDER_COLWISE 0
 #define X2G_ORDER_MIDDLE 1
+#define X2G_ORDER_MIDDLEMT 2
+#define X2G_ORDER_ROWWISE 3
+#define X2G_ORDER_RROWWISE 4
+#define X2G_ORDER_TEST1 5
+#define X2G_ORDER_TEST2 6
+#define X2G_ORDER_TEST3 7
+#define X2G_ORDER_TEST4 8
+#define X2G_ORDER_MIDDLE2 9
 int x2g_relax_ordered(x2g_grid *g, int isym, int bkind, double *work_d, const double *lhs_d,
                       const double *rhs_d, double omega, int nsweeps, int method, int ordering);
 /* Host-pointer drop-in for mcsor/sor: work (padded), lhs, rhs on the host;
@@ -243,8 +254,8 @@ typedef struct {
     const double *rhs_f2_d;
     const double *rhs_f3_d;
     /* Relaxation scheme (prec = 0 items): method X2G_MCSOR (five passes) or
-     * X2G_SOR, over the points in mesh ordering X2G_ORDER_COLWISE or
-     * X2G_ORDER_MIDDLE (initCBAllocArrays.F90:2408-2540).  Zero (MCSOR,
+     * X2G_SOR, over the points in mesh ordering X2G_ORDER_* (one of the
+     * reference's ten, initCBAllocArrays.F90:2408-2810).  Zero (MCSOR,
      * col-wise) is the pipelined colour sweep; any other pair runs the
      * reference's sequential point order exactly (x2g_relax_ordered). */
     int method;

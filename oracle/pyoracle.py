@@ -88,7 +88,10 @@ def orb_tail(nni, nmu, vxisq, vetasq, fa, psi):
     return out
 
 
-ORDERINGS = {"col-wise": 0, "middle": 1}
+# the reference's mesh orderings (initCBAllocArrays.F90:2458-2810) by the
+# 8-character name of its `order` card
+ORDERINGS = {"col-wise": 0, "middle": 1, "middlemt": 2, "row-wise": 3, "rrow-wis": 4, "test1": 5, "test2": 6,
+             "test3": 7, "test4": 8, "middle2": 9}
 
 
 def mesh(nni, nmu, ordering):

@@ -257,27 +257,76 @@ void x2o_orb_asympt_set(int nni, int nmu, double *psi, const double *edecay)
     }
 }
 
-/* Sequence of relaxed points of mesh (initCBAllocArrays.F90:2408-2540) as
- * unpadded 0-based indices ik = (mu-1)*nni + (nu-1), ordering 0 = col-wise
- * (2467-2497: mu = 2..nmu-4 outer, nu = 2..nni-1 inner), 1 = middle
- * (2499-2540: rows from the middle row down to mu = 2, then from the one
- * above it up to nmu-4; in every row nu from the middle column down to 2,
- * then from the one after it up to nni-1).  Returns the count,
- * (nni-2)*(nmu-5). */
+/* Sequence of relaxed points of mesh (initCBAllocArrays.F90:2408-2810) as
+ * unpadded 0-based indices ik = (mu-1)*nni + (nu-1), for the reference's
+ * orderings 0 col-wise (2467-2497: mu = 2..nmu-4 outer, nu = 2..nni-1
+ * inner), 1 middle (2499-2540: rows from the middle row down to mu = 2, then
+ * from the one above it up to nmu-4; in every row nu likewise from the middle
+ * column), 2 middlemt (2543-2575: the middle split one row and column
+ * further, the upper rows walked from nmu-4 back to the middle), 3 row-wise
+ * (2578-2599: nu outer), 4 rrow-wis (2601-2627: row-wise reversed), 5 test1,
+ * 7 test3 and 9 middle2 (2630-2665, 2698-2733, 2774-2810: the middle walks
+ * with nu outer), 6 test2 (2667-2696: nu outer from the middle column, mu
+ * from nmu-4 down to 2), 8 test4 (2735-2771: mu from nmu-4 down, nu from the
+ * middle column).  Returns the count, (nni-2)*(nmu-5), or -1 for an unknown
+ * ordering. */
 long x2o_mesh(int nni, int nmu, int ordering, int *seq)
 {
-    const int nni8 = nni + 8, nmu8 = nmu + 8;
-    const int iend1 = nmu;                         /* padded rows 6..nmu <-> mu = 2..nmu-4 */
+    /* padded rows i = 6..nmu (mu = i-4 = 2..nmu-4), columns j = 6..nni+3
+     * (nu = 2..nni-1); the "mirrored" walks start at the middle row / column
+     * and go outwards, first towards index 6, then towards the end
+     * (initCBAllocArrays.F90:2458-2810) */
+    const int nni8 = nni + 8, nmu8 = nmu + 8, iend1 = nmu, jend = nni + 3;
     const int nmux = (nmu8 - 1) / 2, nnix = (nni8 - 1) / 2;
     long k = 0;
-    for (int i = 6; i <= iend1; ++i) {
-        const int ii = (ordering == 1 && i <= nmux) ? nmux - i + 6 : i;
-        for (int j = 6; j <= nni + 3; ++j) {
-            const int jj = (ordering == 1 && j <= nnix) ? nnix - j + 6 : j;
-            const int mu = ii - 4, nu = jj - 4;
-            seq[k++] = (mu - 1) * nni + (nu - 1);
+#define EMIT(ii, jj) (seq[k++] = ((ii) - 5) * nni + ((jj) - 5))
+#define MIR_I(i) ((i) <= nmux ? nmux - (i) + 6 : (i))
+#define MIR_J(j) ((j) <= nnix ? nnix - (j) + 6 : (j))
+    switch (ordering) {
+    case 0: /* col-wise */
+        for (int i = 6; i <= iend1; ++i)
+            for (int j = 6; j <= jend; ++j) EMIT(i, j);
+        break;
+    case 1: /* middle */
+        for (int i = 6; i <= iend1; ++i)
+            for (int j = 6; j <= jend; ++j) EMIT(MIR_I(i), MIR_J(j));
+        break;
+    case 2: { /* middlemt: the upper half walks from the last row back to the middle */
+        const int mx = nmux + 1, nx = nnix + 1;
+        for (int i = 6; i <= iend1; ++i) {
+            const int ii = i <= mx ? mx - i + 6 : nmu8 - 7 - (i - mx);
+            for (int j = 6; j <= jend; ++j) EMIT(ii, j <= nx ? nx - j + 6 : j);
         }
+        break;
     }
+    case 3: /* row-wise */
+        for (int j = 6; j <= jend; ++j)
+            for (int i = 6; i <= iend1; ++i) EMIT(i, j);
+        break;
+    case 4: /* rrow-wis */
+        for (int j = jend; j >= 6; --j)
+            for (int i = iend1; i >= 6; --i) EMIT(i, j);
+        break;
+    case 5: /* test1 */
+    case 7: /* test3 */
+    case 9: /* middle2 */
+        for (int j = 6; j <= jend; ++j)
+            for (int i = 6; i <= iend1; ++i) EMIT(MIR_I(i), MIR_J(j));
+        break;
+    case 6: /* test2 */
+        for (int j = 6; j <= jend; ++j)
+            for (int i = iend1; i >= 6; --i) EMIT(i, MIR_J(j));
+        break;
+    case 8: /* test4 */
+        for (int i = iend1; i >= 6; --i)
+            for (int j = 6; j <= jend; ++j) EMIT(i, MIR_J(j));
+        break;
+    default:
+        return -1;
+    }
+#undef EMIT
+#undef MIR_I
+#undef MIR_J
     return k;
 }
 

@@ -8,7 +8,7 @@ the fixtures it writes are plain arrays (numpy .npz, no pickles) and travel
 with the repository.
 
     make -f oracle/ref.mk harness
-    python tests/golden/make_golden.py [case ...]
+    python tests/golden/make_golden.py [case ...]     (h2_61x91_orderings: every other ordering)
 """
 import os
 import subprocess
@@ -34,16 +34,24 @@ LIGHT = {
     "n2_61x91_middle": ("n2_61x91_middle.data", 1),
 }
 LIGHT_KEYS = ("sizes", "scal", "coef", "suppl", "vxi", "veta")
+# the H2 61x91 deck in each of the reference's other mesh orderings (its
+# `order` card; one MCSOR thread): the relaxation vectors of each go into
+# one file, h2_61x91_orderings.npz, under "<ordering>/<key>"
+ORDER_BASE = "h2_61x91_middle.data"
+ORDER_NAMES = ("middlemt", "row-wise", "rrow-wis", "test1", "test2", "test3", "test4", "middle2")
+ORDER_KEYS = ("k1_win", "k1_lhs", "k1_rhs", "k1_wout_mcsor1", "k1_wout_sor", "k1_isym", "k1_scal")
 
 
-def run_case(name, deck, iorb, light=False):
+def run_case(name, deck, iorb, light=False, deck_text=None):
     with tempfile.TemporaryDirectory() as d:
         env = dict(os.environ, X2_DUMP=d, OMP_NUM_THREADS="2")
         if iorb is not None:
             env["X2_IORB"] = str(iorb)
-        with open(os.path.join(HERE, "decks", deck)) as fin:
-            subprocess.run([HARNESS], stdin=fin, cwd=d, env=env, check=True,
-                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        if deck_text is None:
+            with open(os.path.join(HERE, "decks", deck)) as f:
+                deck_text = f.read()
+        subprocess.run([HARNESS], input=deck_text, text=True, cwd=d, env=env, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
 
         def f64(n):
             return np.fromfile(os.path.join(d, n + ".f64"), dtype="<f8")
@@ -77,6 +85,8 @@ def run_case(name, deck, iorb, light=False):
             out[pfx + "isym"] = i32(pfx + "isym")
             out[pfx + "scal"] = f64(pfx + "scal")
 
+        if light == "vectors":
+            return {k: out[k] for k in ORDER_KEYS}
         if light:
             keep = {k: v for k, v in out.items() if k in LIGHT_KEYS or k.startswith("k")}
             np.savez_compressed(os.path.join(HERE, name + ".npz"), **keep)
@@ -127,6 +137,16 @@ if __name__ == "__main__":
     for name, (deck, iorb) in CASES.items():
         if not only or name in only:
             run_case(name, deck, iorb)
+    if not only or "h2_61x91_orderings" in only:
+        with open(os.path.join(HERE, "decks", ORDER_BASE)) as f:
+            base = f.read()
+        assert "order middle" in base
+        vec = {}
+        for o in ORDER_NAMES:
+            got = run_case(o, None, None, light="vectors", deck_text=base.replace("order middle", f"order {o}"))
+            vec.update({f"{o}/{k}": v for k, v in got.items()})
+        np.savez_compressed(os.path.join(HERE, "h2_61x91_orderings.npz"), **vec)
+        print("h2_61x91_orderings ->", os.path.getsize(os.path.join(HERE, "h2_61x91_orderings.npz")), "bytes")
     for name, (deck, iorb) in LIGHT.items():
         if not only or name in only:
             run_case(name, deck, iorb, light=True)

@@ -151,8 +151,13 @@ def test_relaxation_scheme_cards():
         assert (d.orb_method, d.pot_method, d.ordering) == want, extra
         s = d.spec()
         assert (s.orb_method, s.pot_method, s.ordering) == want
-    with pytest.raises(deck.DeckError):
-        deck.parse(base + "order row-wise\nstop\n")
+    # every ordering of the reference's mesh, read as 8 characters
+    for name, code in (("row-wise", 3), ("rrow-wise", 4), ("middlemt", 2), ("test4", 8), ("middle2", 9)):
+        d = deck.parse(base + f"order {name}\nstop\n")
+        assert d.ordering == code and d.spec().ordering == code
+        assert deck.parse(deck.to_deck(d.spec())).ordering == code
+    with pytest.raises(deck.DeckError, match="undefined type of ordering"):
+        deck.parse(base + "order diagonal\nstop\n")
 
 
 def test_spec_to_deck_round_trip():

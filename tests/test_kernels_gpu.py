@@ -551,6 +551,37 @@ def test_ordered_relaxation_bitexact_vs_reference(ctx, case, pfx):
     g.destroy()
 
 
+ORDER_CODES = {"middlemt": 2, "row-wise": 3, "rrow-wis": 4, "test1": 5, "test2": 6, "test3": 7, "test4": 8,
+               "middle2": 9}
+
+
+@pytest.mark.parametrize("ordering", list(ORDER_CODES))
+def test_other_orderings_bitexact_vs_reference(ctx, ordering):
+    """x2g_relax_ordered in each of the reference's other mesh orderings:
+    sor (10 sweeps) and one-thread mcsor (1 sweep) on the H2 61x91 deck with
+    that `order` card reproduce the reference's own routines bit for bit
+    (tests/golden/h2_61x91_orderings.npz)."""
+    import os
+    from conftest import GOLDEN
+    from x2dhf_amd import gpu
+    z = load_golden("h2_61x91_middle")
+    v = np.load(os.path.join(GOLDEN, "h2_61x91_orderings.npz"))
+    vec = lambda k: v[f"{ordering}/k1_{k}"]
+    print(f"ordering={ordering}", flush=True)
+    g = make_grid(ctx, z)
+    isym = int(vec("isym")[0])
+    omega, nsw = float(vec("scal")[0]), int(vec("scal")[1])
+    lhs, rhs = ctx.upload(vec("lhs")), ctx.upload(vec("rhs"))
+    for method, name, ns in ((1, "wout_sor", nsw), (0, "wout_mcsor1", 1)):
+        w = ctx.upload(vec("win"))
+        gpu.check(ctx.lib.x2g_relax_ordered(g.h, isym, 1, w.ptr, lhs.ptr, rhs.ptr, omega, ns, method,
+                                            ORDER_CODES[ordering]))
+        got = w.get()
+        assert np.array_equal(got, vec(name)), (name, relerr(got, vec(name)))
+        w.free()
+    g.destroy()
+
+
 def test_ordered_potential_batch_vs_oracle(ctx):
     """All potentials of the N2 pi orbital in one x2g_relax_batch call with
     SOR in the middle ordering (each a workgroup of the level kernel), two

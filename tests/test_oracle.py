@@ -6,10 +6,12 @@
   boundary extrapolation weights and the 7-point Newton-Cotes weights on
   polynomials, and the fixed point of the sweep (zero residual).
 """
+import os
+
 import numpy as np
 import pytest
 
-from conftest import load_golden
+from conftest import GOLDEN, load_golden
 from oracle import pyoracle
 
 
@@ -215,12 +217,50 @@ def test_oracle_middle_ordering_bitexact_vs_reference(case, pfx):
         assert _ok(w, z[pfx + name]), (case, pfx, name, np.abs(w - z[pfx + name]).max())
 
 
+OTHER_ORDERINGS = ("middlemt", "row-wise", "rrow-wis", "test1", "test2", "test3", "test4", "middle2")
+
+
+@pytest.mark.parametrize("ordering", OTHER_ORDERINGS)
+def test_oracle_other_orderings_bitexact_vs_reference(ordering):
+    """sor (10 sweeps) and one-thread mcsor (1 sweep) in each of the
+    reference's other mesh orderings (initCBAllocArrays.F90:2543-2810) against
+    its own routines on the H2 61x91 deck with that `order` card
+    (tests/golden/h2_61x91_orderings.npz; grid arrays from the same deck)."""
+    z = load_golden("h2_61x91_middle")
+    v = np.load(os.path.join(GOLDEN, "h2_61x91_orderings.npz"))
+    g = lambda k: v[f"{ordering}/k1_{k}"]
+    nni, nmu = int(z["sizes"][0]), int(z["sizes"][1])
+    isym = int(g("isym")[0])
+    omega, nsw = float(g("scal")[0]), int(g("scal")[1])
+    args = (g("lhs"), g("rhs"), z["suppl"][1], z["suppl"][2], z["coef"][:16], z["coef"][16:21], omega)
+    for mode, name, ns in [(0, "wout_mcsor1", 1), (1, "wout_sor", nsw)]:
+        w = pyoracle.relax_ordered(nni, nmu, isym, g("win").copy(), *args, ns, mode, ordering)
+        assert _ok(w, g(name)), (ordering, name, np.abs(w - g(name)).max())
+    # the MCSOR vector pins the order itself: the middle ordering's passes
+    # give another result.  (SOR results of orderings that keep the same
+    # precedence between stencil neighbours coincide, e.g. col-wise and
+    # row-wise: the cross stencil has no diagonal couplings.)
+    w = pyoracle.relax_ordered(nni, nmu, isym, g("win").copy(), *args, 1, 0, "middle")
+    assert not np.array_equal(w, g("wout_mcsor1"))
+
+
 def test_mesh_orderings_are_permutations():
-    """Both orderings visit every relaxed point once; col-wise is mu-major
+    """Every ordering visits every relaxed point once; col-wise is mu-major
     with nu fastest, middle starts at the middle point and ends at the corner
-    (nu, mu) = (nni-1, nmu-4)."""
+    (nu, mu) = (nni-1, nmu-4); test1, test3 and middle2 are one sequence
+    (the reference writes it three times); no two other orderings agree."""
     for nni, nmu in ((61, 91), (91, 121), (151, 211)):
         n = (nni - 2) * (nmu - 5)
+        seqs = {}
+        for o in pyoracle.ORDERINGS:
+            s = pyoracle.mesh(nni, nmu, o)
+            seqs[o] = tuple(s.tolist())
+            assert len(set(s.tolist())) == n, o
+        assert seqs["test1"] == seqs["test3"] == seqs["middle2"]
+        distinct = {seqs[o] for o in pyoracle.ORDERINGS if o not in ("test3", "middle2")}
+        assert len(distinct) == len(pyoracle.ORDERINGS) - 2
+        r = pyoracle.mesh(nni, nmu, "row-wise")
+        assert np.array_equal(pyoracle.mesh(nni, nmu, "rrow-wis"), r[::-1])
         for o in ("col-wise", "middle"):
             s = pyoracle.mesh(nni, nmu, o)
             mu, nu = s // nni + 1, s % nni + 1

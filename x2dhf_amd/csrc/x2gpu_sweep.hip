@@ -2260,22 +2260,52 @@ using namespace x2g;
 // col-wise order needs one level per pass, in middle order a few more where
 // the ordering folds back.
 //
-// Mesh orderings (mesh, initCBAllocArrays.F90:2408-2540), rows mu = 2..nmu-4
-// and columns nu = 2..nni-1 of the relaxed interior:
-//   col-wise: rows in increasing mu, in each row nu increasing;
-//   middle: rows from the middle one down to mu = 2, then from the next one up
-//           to nmu-4; in each row nu from the middle down to 2, then up to
-//           nni-1 (the middle row / column is (n+7)/2 - 4 of n+8 padded).
+// Relaxation sequence of mesh ordering `ordering` (the reference's mesh,
+// initCBAllocArrays.F90:2458-2810) as unpadded indices (mu-1)*nni + nu-1.
+// Padded rows i = 6..nmu and columns j = 6..nni+3 are walked in one of four
+// patterns: rows outer or columns outer, each index either straight,
+// reversed, or "mirrored" (from the middle index down to 6, then from the
+// one after the middle up to the end).  middlemt mirrors around one row and
+// column further and walks its upper rows from the end back to the middle.
 static void mesh_sequence(int nni, int nmu, int ordering, std::vector<int> &seq)
 {
     seq.clear();
-    const int midr = (nmu + 7) / 2, midc = (nni + 7) / 2;   // padded indices
-    for (int i = 6; i <= nmu; ++i) {
-        const int ii = (ordering == X2G_ORDER_MIDDLE && i <= midr) ? midr - i + 6 : i;
-        for (int j = 6; j <= nni + 3; ++j) {
-            const int jj = (ordering == X2G_ORDER_MIDDLE && j <= midc) ? midc - j + 6 : j;
-            seq.push_back((ii - 5) * nni + (jj - 5));          // (mu-1)*nni + nu-1
-        }
+    seq.reserve((size_t)(nni - 2) * (nmu - 5));
+    const int rmid = (nmu + 7) / 2, cmid = (nni + 7) / 2;  // padded middle row / column
+    const int rlast = nmu, clast = nni + 3;
+    // index k (0-based) of a walk of the rows or columns 6..last
+    auto straight = [](int k, int) { return 6 + k; };
+    auto reversed = [](int k, int last) { return last - k; };
+    auto mirrored = [](int mid) {
+        return [mid](int k, int) { const int i = 6 + k; return i <= mid ? mid - i + 6 : i; };
+    };
+    auto emit = [&](int row, int col) { seq.push_back((row - 5) * nni + (col - 5)); };
+    const int nr = rlast - 5, nc = clast - 5;
+    auto rows_outer = [&](auto rw, auto cw) {
+        for (int a = 0; a < nr; ++a)
+            for (int b = 0; b < nc; ++b) emit(rw(a, rlast), cw(b, clast));
+    };
+    auto cols_outer = [&](auto rw, auto cw) {
+        for (int b = 0; b < nc; ++b)
+            for (int a = 0; a < nr; ++a) emit(rw(a, rlast), cw(b, clast));
+    };
+    switch (ordering) {
+    case X2G_ORDER_COLWISE: rows_outer(straight, straight); break;
+    case X2G_ORDER_MIDDLE: rows_outer(mirrored(rmid), mirrored(cmid)); break;
+    case X2G_ORDER_MIDDLEMT: {
+        const int rm = rmid + 1;
+        auto rw = [rm](int k, int last) { const int i = 6 + k; return i <= rm ? rm - i + 6 : last - (i - rm - 1); };
+        rows_outer(rw, mirrored(cmid + 1));
+        break;
+    }
+    case X2G_ORDER_ROWWISE: cols_outer(straight, straight); break;
+    case X2G_ORDER_RROWWISE: cols_outer(reversed, reversed); break;
+    case X2G_ORDER_TEST1:
+    case X2G_ORDER_TEST3:
+    case X2G_ORDER_MIDDLE2: cols_outer(mirrored(rmid), mirrored(cmid)); break;
+    case X2G_ORDER_TEST2: cols_outer(reversed, mirrored(cmid)); break;
+    case X2G_ORDER_TEST4: rows_outer(reversed, mirrored(cmid)); break;
+    default: break;
     }
 }
 
@@ -2673,8 +2703,8 @@ extern "C" int x2g_relax_ordered(x2g_grid *g, int isym, int bkind, double *work_
     int rc = mcsor_check(g, isym, bkind, work_d, lhs_d, rhs_d, omega, nsweeps, "x2g_relax_ordered");
     if (rc) return rc;
     X2G_CHECK(method == X2G_MCSOR || method == X2G_SOR, "x2g_relax_ordered: method must be X2G_MCSOR or X2G_SOR");
-    X2G_CHECK(ordering == X2G_ORDER_COLWISE || ordering == X2G_ORDER_MIDDLE,
-              "x2g_relax_ordered: ordering must be X2G_ORDER_COLWISE or X2G_ORDER_MIDDLE");
+    X2G_CHECK(ordering >= X2G_ORDER_COLWISE && ordering <= X2G_ORDER_MIDDLE2,
+              "x2g_relax_ordered: ordering must be one of X2G_ORDER_*");
     if (nsweeps == 0) return X2G_OK;
     const x2g_levels *L;
     if ((rc = grid_levels(g, method, ordering, &L))) return rc;
@@ -2736,7 +2766,7 @@ static int relax_batch_eager(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_rel
         X2G_CHECK(it.prec == 0 || it.prec == 1, "x2g_relax_batch: item %d prec", i);
         X2G_CHECK(it.omega > 0.0 && it.omega < 2.0, "x2g_relax_batch: item %d omega must lie in (0, 2)", i);
         X2G_CHECK(it.method == X2G_MCSOR || it.method == X2G_SOR, "x2g_relax_batch: item %d method", i);
-        X2G_CHECK(it.ordering == X2G_ORDER_COLWISE || it.ordering == X2G_ORDER_MIDDLE,
+        X2G_CHECK(it.ordering >= X2G_ORDER_COLWISE && it.ordering <= X2G_ORDER_MIDDLE2,
                   "x2g_relax_batch: item %d ordering", i);
         X2G_CHECK(it.prec == 0 || (it.method == X2G_MCSOR && it.ordering == X2G_ORDER_COLWISE),
                   "x2g_relax_batch: item %d: FP32 correction sweeps are col-wise MCSOR only", i);

@@ -27,8 +27,9 @@ relaxation path and builds the scf.JobSpec the GPU driver runs:
   mcsor [n ..], mcsor-o [..]         orbitals by MCSOR (read_mcsor, read_mcsor_o, 1915-1995)
   mcsor-ce [..]                      potentials by MCSOR (read_mcsor_ce, 1997-2028); without
                                      these cards both are relaxed by SOR, the reference's default
-  order col-wise|middle              mesh ordering (read_order; default middle,
-                                     initVariables.F90:206)
+  order name                         mesh ordering (read_order, 8 characters: col-wise,
+                                     middle (the default, initVariables.F90:206), middlemt,
+                                     row-wise, rrow-wis(e), test1..test4, middle2)
   print, prtevery, mmoments, plot, out4pair, debug, inout, stop   ignored
 
 Any other card (DFT, external fields, fixed orbitals, other start orbitals,
@@ -43,7 +44,10 @@ from . import scf
 BOHR2ANG = 0.529177210671212  # commons.F90:21
 _SYM_M = {"sigma": 0, "pi": 1, "delta": 2, "phi": 3}
 _IGNORED = {"title", "print", "prtevery", "mmoments", "plot", "out4pair", "debug", "inout", "stop"}
-_ORDERINGS = {"col-wise": 0, "middle": 1}
+# the reference's mesh orderings by their 8-character `order` names
+# (initCBAllocArrays.F90:2458-2810), as the X2G_ORDER_* codes
+_ORDERINGS = {"col-wise": 0, "middle": 1, "middlemt": 2, "row-wise": 3, "rrow-wis": 4, "test1": 5, "test2": 6,
+              "test3": 7, "test4": 8, "middle2": 9}
 
 
 class DeckError(ValueError):
@@ -298,9 +302,9 @@ def parse(text):
         elif label == "mcsor-ce":
             d.pot_method = 0
         elif label == "order":
-            name = _word(items[0]) if items else ""
+            name = (_word(items[0]) if items else "")[:8]  # read as character*8
             if name not in _ORDERINGS:
-                raise DeckError(f"order '{name}': this build relaxes in the col-wise or middle ordering")
+                raise DeckError(f"order '{name}': undefined type of ordering")
             d.ordering = _ORDERINGS[name]
         else:
             raise DeckError(f"card '{label}' is outside the relaxation path of this build")
@@ -340,7 +344,7 @@ def to_deck(spec, title="job", mcsor_threads=None):
                      (f" {spec.nnolast}" if spec.nnolast > 0 else ""))
     if spec.maxsororb[1] != 10 or spec.maxsorpot[1] != 10:
         lines.append(f"sor {spec.maxsororb[1]} {spec.maxsorpot[1]}")
-    lines.append(f"order {'middle' if spec.ordering == 1 else 'col-wise'}")
+    lines.append("order " + {v: k for k, v in _ORDERINGS.items()}[spec.ordering])
     if mcsor_threads is not None:
         lines += [f"mcsor {int(mcsor_threads)}", "mcsor-ce"]
     else:

@@ -26,7 +26,7 @@ module x2f_api
      ! FP64 finish of a mixed-precision job (scf_opts%polish); 0 = none
      integer(c_int) :: polish
      ! relaxation scheme of the FP64 sweeps: orbitals / potentials 0 MCSOR,
-     ! 1 SOR; mesh ordering 0 col-wise, 1 middle (x2g_relax_item)
+     ! 1 SOR; mesh ordering: an X2G_ORDER_* code, 0 col-wise .. 9 middle2 (x2g_relax_item)
      integer(c_int) :: orb_method, pot_method, ordering
   end type x2f_spec
 

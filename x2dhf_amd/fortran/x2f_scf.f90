@@ -51,7 +51,7 @@ module x2f_scf
      ! the reference's FP64 fixed point; 0 = none
      integer :: polish = 0
      ! relaxation scheme of FP64 sweeps (x2g_relax_item method / ordering):
-     ! orbitals and potentials 0 MCSOR, 1 SOR; ordering 0 col-wise, 1 middle
+     ! orbitals and potentials 0 MCSOR, 1 SOR; ordering an X2G_ORDER_* code
      integer :: orb_method = 0, pot_method = 0, ordering = 0
   end type scf_opts
 

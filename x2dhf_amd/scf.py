@@ -130,8 +130,9 @@ def make_spec(z1, z2, r, nni, nmu, rinf, orbitals, maxscf=3000, ienterm=10, inot
     conv: (nscf2skip, nenlast, nnolast) of the saturation test, 0 = default;
     polish: with prec = 1, the number of FP64-sweep iterations the job ends
     with (it then stops at the FP64 path's fixed point), 0 = none;
-    orb_method / pot_method: 0 MCSOR, 1 SOR, and ordering: 0 col-wise,
-    1 middle, the relaxation scheme of the FP64 sweeps (0/0/0 is the
+    orb_method / pot_method: 0 MCSOR, 1 SOR, and ordering: an X2G_ORDER_*
+    code (0 col-wise, 1 middle, ... 9 middle2; deck._ORDERINGS), the
+    relaxation scheme of the FP64 sweeps (0/0/0 is the
     pipelined col-wise colour sweep; the FP32 correction sweeps are always
     col-wise MCSOR)."""
     s = JobSpec()
