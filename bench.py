@@ -152,6 +152,7 @@ def traffic_probe(batch):
     import numpy as np
     from x2dhf_amd import gpu, scf
     ctx = gpu.Context(0)
+    gpu.check(ctx.lib.x2g_set_autotune(0))  # the default plan (4-row blocks), no tuning launches
     rng = np.random.default_rng(0)
     cb = scf.grid_build(1201, 1201, 7.0, 7.0, 2.068, 40.0)
     big = gpu.Grid(ctx, 1201, 1201, *cb)
