@@ -50,3 +50,24 @@ def test_bad_arguments_are_rejected_before_any_device_call():
     assert b"null" in lib.x2g_last_error()
     assert lib.x2g_set_sweep_engine(7, 0) == -1
     assert lib.x2g_set_sweep_engine(0, -1) == 0
+
+
+def test_plan_graph_and_workspace_entry_points_check_their_arguments():
+    """The sweep-plan, autotuner, graph and workspace entry points are host
+    logic: their argument checks run without a device."""
+    from x2dhf_amd import gpu
+    lib = gpu.load()
+    assert lib.x2g_set_sweep_plan(3, 0, 0) == -1 and b"block_rows" in lib.x2g_last_error()
+    assert lib.x2g_set_sweep_plan(2, 17, 0) == -1 and b"wg_per_cu" in lib.x2g_last_error()
+    assert lib.x2g_set_sweep_plan(2, 0, 2) == -1 and b"split" in lib.x2g_last_error()
+    assert lib.x2g_set_sweep_plan(2, 1, 1) == 0 and lib.x2g_set_sweep_plan(0, 0, 0) == 0
+    assert lib.x2g_set_graphs(None, 1) == -1 and b"null" in lib.x2g_last_error()
+    c = C.c_longlong()
+    assert lib.x2g_graph_counts(None, C.byref(c), C.byref(c)) == -1
+    assert lib.x2g_tune_report(None, 10) == -1
+    assert lib.x2g_set_autotune(1) == 0
+    # workspace of one relaxation problem: FP64 two padded arrays + pairs,
+    # FP32 two padded correction arrays + lhs/residual; bad sizes give -1
+    w64, w32 = lib.x2g_relax_workspace_bytes(331, 601, 0), lib.x2g_relax_workspace_bytes(331, 601, 1)
+    assert w64 > 2 * 8 * 339 * 609 and w32 > 2 * 4 * 339 * 609 and w64 > w32
+    assert lib.x2g_relax_workspace_bytes(5, 601, 0) == -1 and lib.x2g_relax_workspace_bytes(331, 601, 2) == -1
