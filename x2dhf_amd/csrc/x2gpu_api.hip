@@ -77,6 +77,7 @@ int stage_bytes(x2g_ctx *ctx, const void *src, size_t bytes, const void **dev_ou
     if (ctx->cap_arena) {
         // graph capture: the descriptors belong to the graph's own arena
         const size_t off = (ctx->cap_off + 255) & ~size_t(255);
+        if (off + bytes > ctx->cap_bytes) ctx->cap_overflow = true;
         X2G_CHECK(off + bytes <= ctx->cap_bytes, "launch descriptors exceed the graph arena (%zu bytes)",
                   ctx->cap_bytes);
         if (ctx->cap_host.size() < off + bytes) ctx->cap_host.resize(off + bytes);

@@ -91,6 +91,7 @@ struct x2g_ctx {
     // cap_arena, where they are copied once the capture has ended
     char *cap_arena = nullptr;
     size_t cap_bytes = 0, cap_off = 0;
+    bool cap_overflow = false;     // a capture's descriptors did not fit its arena
     std::vector<char> cap_host;
 };
 

@@ -3134,6 +3134,7 @@ static int relax_capture(x2g_ctx *ctx, const std::string &key, x2g_grid *const *
         ctx->cap_arena = (char *)arena;
         ctx->cap_bytes = cap;
         ctx->cap_off = 0;
+        ctx->cap_overflow = false;
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
@@ -3141,7 +3142,7 @@ static int relax_capture(x2g_ctx *ctx, const std::string &key, x2g_grid *const *
             e = hipStreamEndCapture(ctx->stream, &graph);
         }
         ctx->cap_arena = nullptr;
-        const bool overflow = rc && ctx->cap_off + 256 > cap;
+        const bool overflow = ctx->cap_overflow;
         if (e != hipSuccess || rc || !graph) {
             if (graph) (void)hipGraphDestroy(graph);
             (void)hipFree(arena);
@@ -3163,6 +3164,15 @@ static int relax_capture(x2g_ctx *ctx, const std::string &key, x2g_grid *const *
             return X2G_EHIP;
         }
         if (ctx->graphs.size() >= kMaxGraphs) {
+            // the least recently used graph may still run: wait for the
+            // stream before releasing it and its arena
+            e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) {
+                (void)hipGraphExecDestroy(exec);
+                (void)hipFree(arena);
+                set_error("x2g_relax_batch: %s", hipGetErrorString(e));
+                return X2G_EHIP;
+            }
             auto victim = ctx->graphs.begin();
             for (auto it = ctx->graphs.begin(); it != ctx->graphs.end(); ++it)
                 if (it->second.last_use < victim->second.last_use) victim = it;
