@@ -351,16 +351,24 @@ def bench_poisson(args):
     print(json.dumps(line), flush=True)
 
 
-def serve_stream(reqs, prec, stats=False, graphs=False):
+def serve_stream(reqs, prec, stats=False, graphs=False, groups=1):
     """Serve (arrival_s, name, spec) requests with the continuous-batching
     server at the given precision (graphs: relaxations replayed from captured
-    HIP graphs); returns (handles, elapsed_s, sweep stats)."""
+    HIP graphs; groups > 1: that many server groups on their own streams and
+    host threads, serve.GroupServer); returns (handles, elapsed_s, sweep
+    stats summed over the groups' contexts)."""
     import ctypes as C
     from x2dhf_amd import gpu, scf, serve
-    server = serve.Server(max_active=STREAM_SLOTS, graphs=graphs)
-    lib = server.lib
+    if groups > 1:
+        server = serve.GroupServer(groups=groups, max_active=STREAM_SLOTS, graphs=graphs)
+        ctxs = [srv.ctx for srv in server.servers]
+    else:
+        server = serve.Server(max_active=STREAM_SLOTS, graphs=graphs)
+        ctxs = [server.ctx]
+    lib = gpu.load()
     if stats:
-        gpu.check(lib.x2g_stats_enable(server.ctx.h, 1))
+        for c in ctxs:
+            gpu.check(lib.x2g_stats_enable(c.h, 1))
     t0 = server.now()
     handles = []
     for t, name, spec in reqs:
@@ -368,13 +376,17 @@ def serve_stream(reqs, prec, stats=False, graphs=False):
         sp.prec = prec
         handles.append(server.submit(sp, arrival=t0 + t))
     server.run()
-    gpu.check(lib.x2g_ctx_sync(server.ctx.h))
+    for c in ctxs:
+        gpu.check(lib.x2g_ctx_sync(c.h))
     elapsed = max(h.finished for h in handles) - t0
-    st = (C.c_double * 6)()
+    tot = [0.0] * 6
     if stats:
-        gpu.check(lib.x2g_stats_get(server.ctx.h, st))
+        for c in ctxs:
+            st = (C.c_double * 6)()
+            gpu.check(lib.x2g_stats_get(c.h, st))
+            tot = [a + b for a, b in zip(tot, st)]
     server.close()
-    return handles, elapsed, list(st)
+    return handles, elapsed, tot
 
 
 def cpu_baseline_stream(reqs, budget_s):
@@ -514,6 +526,26 @@ def bench_stream(args):
                   "energies_identical_to_eager": same}
         print(f"# graph replay: {eg_all:.2f} s, {graphs['solves_per_hr']:.0f} solves/hr, energies identical: {same}",
               file=sys.stderr, flush=True)
+    groups2 = None
+    if not args.no_graphs:
+        # the same requests again by two server groups on their own streams and
+        # host threads (serve.GroupServer), relaxations replayed from graphs:
+        # one group's host turn overlaps the other's kernels (per-launch sweep
+        # times are not comparable with concurrent streams, so the roofline
+        # comes from the single-server timed stream above)
+        h2_all, e2_all = [], 0.0
+        for k in range(args.steps):
+            reqs = serve.poisson_stream(count, args.rate, seed=args.seed + k, prec=args.prec)
+            h2, e2, _ = serve_stream(reqs, args.prec, graphs=True, groups=2)
+            h2_all += h2
+            e2_all += e2
+        n2 = sum(1 for h in h2_all if serve.converged(h.result))
+        same2 = all(a.result["etot"] == b.result["etot"] for a, b in zip(handles, h2_all))
+        groups2 = {"groups": 2, "slots": STREAM_SLOTS, "solves_per_hr": n2 / e2_all * 3600.0, "elapsed_s": e2_all,
+                   "p50_latency_s": serve.percentile([h.latency for h in h2_all], 50),
+                   "energies_identical_to_eager": same2}
+        print(f"# two groups + graph replay: {e2_all:.2f} s, {groups2['solves_per_hr']:.0f} solves/hr, energies "
+              f"identical: {same2}", file=sys.stderr, flush=True)
     plain = None
     if not args.no_plain and args.plain_sample > 0:
         plain_s, dEp, its = 0.0, [], 0
@@ -571,6 +603,7 @@ def bench_stream(args):
         "fp64_served": fp64,
         "speedup_vs_fp64_served": value / fp64["solves_per_hr"] if fp64 else None,
         "graph_replay": graphs,
+        "two_groups_graph_replay": groups2,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain_fp64_gpu": value / plain["solves_per_hr"] if plain else None,
         "max_abs_dE_vs_fp64": max(dE64 + ([plain["max_abs_dE_vs_served"]] if plain and plain["max_abs_dE_vs_served"]

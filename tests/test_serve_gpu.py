@@ -119,3 +119,27 @@ def test_config5_widened_stream_each_result_as_alone_and_within_1e8_of_fp64():
         assert np.array_equal(m["eps"], alone["eps"])
         assert m["stop"] not in ("diverged", "not a number")
         assert abs(m["etot"] - fp64["etot"]) < 1e-8
+
+
+def test_group_server_gives_the_single_server_results():
+    """Two server groups on their own streams and host threads, fed from one
+    queue, give every request the result it gets from a single server (and
+    so, served alone), and both groups take requests."""
+    from x2dhf_amd import serve
+    specs = _specs()
+    print("single server", flush=True)
+    single = _serve(specs, [0.0] * len(specs))
+    print("two groups", flush=True)
+    grp = serve.GroupServer(groups=2, max_active=4)
+    try:
+        reqs = [grp.submit(s, arrival=0.0) for s in specs]
+        grp.run()
+        per_group = [len(srv.done) for srv in grp.servers]
+    finally:
+        grp.close()
+    print(f"requests per group {per_group}")
+    assert sorted(per_group) == [2, 2]
+    for a, b in zip(single, reqs):
+        assert b.result["iterations"] == a.result["iterations"]
+        assert b.result["etot"] == a.result["etot"]
+        assert np.array_equal(b.result["eps"], a.result["eps"])

@@ -7,6 +7,7 @@ x2dhf_amd/fortran/x2f_scf.f90.
 """
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -232,6 +233,11 @@ def control_probe(ee, norms, ee0, maxscf=1000, ienterm=10, inoterm=8, nscf2skip=
     return STOPCODES.get(sc.value, str(sc.value)), it.value
 
 
+# The Fortran job table is modified only by job creation and destruction;
+# host threads that step different jobs serialise those two through this lock.
+JOB_TABLE_LOCK = threading.Lock()
+
+
 class Job:
     """One SCF job on the GPU.  share: another Job whose device context this
     job uses; ctx: an x2gpu context (gpu.Context) owned by the caller.  Jobs
@@ -241,11 +247,12 @@ class Job:
         self.L = load()
         self.spec = spec
         h = _i()
-        if ctx is not None:
-            rc = self.L.x2f_job_create_ctx(C.byref(spec), ctx.h.value if hasattr(ctx.h, "value") else ctx.h,
-                                           C.byref(h))
-        else:
-            rc = self.L.x2f_job_create(C.byref(spec), share.h if share is not None else 0, C.byref(h))
+        with JOB_TABLE_LOCK:
+            if ctx is not None:
+                rc = self.L.x2f_job_create_ctx(C.byref(spec), ctx.h.value if hasattr(ctx.h, "value") else ctx.h,
+                                               C.byref(h))
+            else:
+                rc = self.L.x2f_job_create(C.byref(spec), share.h if share is not None else 0, C.byref(h))
         if rc != 0:
             raise ValueError(f"x2f_job_create failed with code {rc}")
         self.h = h.value
@@ -309,7 +316,8 @@ class Job:
 
     def close(self):
         if self.h:
-            self.L.x2f_job_destroy(self.h)
+            with JOB_TABLE_LOCK:
+                self.L.x2f_job_destroy(self.h)
             self.h = 0
 
     @staticmethod

@@ -26,6 +26,8 @@ Every job's result is independent of which other jobs share its batch: the
 sweeps, quadratures and linear combinations of a job read and write only that
 job's arrays, with a partition of the work that does not depend on the batch.
 """
+import contextlib
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -78,6 +80,9 @@ class Server:
         self.iterations = 0
         self.deferred = 0      # admissions put back for lack of device memory
         self._next_id = 0
+        # guards `pending` when it is a queue shared with other servers
+        # (GroupServer); a server of its own needs none
+        self._queue_lock = contextlib.nullcontext()
 
     def now(self):
         return self.clock() - self.t0
@@ -97,11 +102,15 @@ class Server:
 
     def _admit(self):
         t = self.now()
-        while self.pending and self.pending[0].arrival <= t and len(self.active) < self.max_active:
-            r = self.pending[0]
-            if self.held + r.footprint > self.budget:
-                break
-            self.pending.pop(0)
+        admitted = 0
+        while len(self.active) < self.max_active:
+            with self._queue_lock:
+                if not self.pending or self.pending[0].arrival > t:
+                    break
+                r = self.pending[0]
+                if self.held + r.footprint > self.budget:
+                    break
+                self.pending.pop(0)
             try:
                 r.job = scf.Job(r.spec, ctx=self.ctx)
                 r.job.prep()
@@ -114,12 +123,15 @@ class Server:
                     r.job = None
                 if not self.active:
                     raise RuntimeError(f"request {r.rid} cannot be set up on an idle device: {e}") from e
-                self.pending.insert(0, r)
+                with self._queue_lock:
+                    self.pending.insert(0, r)
                 self.deferred += 1
                 break
             r.admitted = self.now()
             self.held += r.footprint
             self.active.append(r)
+            admitted += 1
+        return admitted
 
     def _retire(self, r):
         r.job.finish()
@@ -147,11 +159,17 @@ class Server:
     def run(self):
         """Serve until every submitted request is finished; idles (sleeps)
         until the next arrival when nothing is admitted."""
-        while self.pending or self.active:
-            if not self.active and self.pending and self.pending[0].arrival > self.now():
-                time.sleep(min(0.05, self.pending[0].arrival - self.now()))
+        while True:
+            with self._queue_lock:
+                head = self.pending[0].arrival if self.pending else None
+            if head is None and not self.active:
+                break
+            if not self.active and head is not None and head > self.now():
+                time.sleep(min(0.05, max(0.0, head - self.now())))
                 continue
             self.step()
+            if not self.active:
+                time.sleep(0.001)  # a shared queue's head went to another server, or does not fit
         return self.done
 
     def graph_counts(self):
@@ -322,3 +340,91 @@ def percentile(values, q):
     lo = int(k)
     hi = min(lo + 1, len(v) - 1)
     return v[lo] + (v[hi] - v[lo]) * (k - lo)
+
+
+class GroupServer:
+    """`groups` servers on one GPU, each with its own device context (its own
+    stream) and its own host thread, taking requests from one queue in
+    arrival order: whichever group has a free slot and room in its share of
+    the memory budget admits the head request.  While one group reads back
+    its iteration's results and runs the host bookkeeping, and while it
+    issues its many small launches, the other groups' kernels keep the GPU
+    busy.  max_active and the budget are split evenly between the groups;
+    results are those of a single server (no job depends on its batch)."""
+
+    def __init__(self, groups=2, device=0, max_active=32, hbm_budget=None, hbm_fraction=0.8,
+                 clock=time.perf_counter, graphs=False):
+        self.clock = clock
+        slots = max(1, max_active // groups)
+        first = Server(device, slots, hbm_budget=hbm_budget, hbm_fraction=hbm_fraction, clock=clock, graphs=graphs)
+        share = first.budget // groups
+        first.budget = share
+        self.servers = [first] + [Server(device, slots, hbm_budget=share, clock=clock, graphs=graphs)
+                                  for _ in range(groups - 1)]
+        self.budget = share
+        self.t0 = clock()
+        self.pending = []
+        self._lock = threading.Lock()
+        for srv in self.servers:
+            srv.t0 = self.t0
+            srv.pending = self.pending
+            srv._queue_lock = self._lock
+        self._next_id = 0
+
+    def now(self):
+        return self.clock() - self.t0
+
+    def submit(self, spec, arrival=None):
+        """Queue a request (as Server.submit; it must fit one group's budget)."""
+        fp = footprint(spec)
+        if fp > self.budget:
+            raise ValueError(f"request needs {fp} bytes, more than a group's budget {self.budget}")
+        r = Request(self._next_id, spec, self.now() if arrival is None else float(arrival), fp)
+        self._next_id += 1
+        with self._lock:
+            self.pending.append(r)
+            self.pending.sort(key=lambda q: (q.arrival, q.rid))
+        return r
+
+    def run(self):
+        """Serve until every request is finished, one host thread per group."""
+        errors = []
+
+        def work(srv):
+            try:
+                srv.run()
+            except BaseException as e:  # re-raised by run() after the other groups finish
+                errors.append(e)
+                with self._lock:
+                    self.pending.clear()
+
+        threads = [threading.Thread(target=work, args=(srv,)) for srv in self.servers]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        return self.done
+
+    @property
+    def done(self):
+        return [r for srv in self.servers for r in srv.done]
+
+    @property
+    def iterations(self):
+        return sum(srv.iterations for srv in self.servers)
+
+    def graph_counts(self):
+        c = [srv.graph_counts() for srv in self.servers]
+        return sum(a for a, _ in c), sum(b for _, b in c)
+
+    def close(self):
+        for srv in self.servers:
+            srv.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
