@@ -543,7 +543,7 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
 template <typename T, int W, int BR>
 constexpr int sweep_waves_per_eu()
 {
-    return sizeof(T) != 4 ? 1 : (BR == 4 ? (W <= 352 ? 4 : 1) : (W <= 256 ? 5 : (W <= 528 ? 4 : 1)));
+    return sizeof(T) != 4 ? 1 : (BR == 4 ? (W <= 384 ? 4 : 1) : (W <= 256 ? 5 : (W <= 528 ? 4 : 1)));
 }
 
 template <typename T, int NT, int W, int BR>
@@ -1771,7 +1771,7 @@ struct SweepPlan {
 template <typename T>
 static int sweep_stride(int maxnni)
 {
-    static const int cls32[] = {128, 192, 256, 352, 528, 768};
+    static const int cls32[] = {128, 192, 256, 384, 528, 768};
     static const int cls64[] = {128, 192, 256, 352};
     const int *c = sizeof(T) == 4 ? cls32 : cls64;
     const int nc = sizeof(T) == 4 ? 6 : 4;
@@ -1834,10 +1834,13 @@ static bool dispatch_ks_br(x2g_ctx *ctx, const SweepDesc<T> *dd, const typename 
     case 128: launch_ks<T, 1, 128, BR>(ctx, dd, plan, zero_src); return true;
     case 192: launch_ks<T, 1, 192, BR>(ctx, dd, plan, zero_src); return true;
     case 256: launch_ks<T, 1, 256, BR>(ctx, dd, plan, zero_src); return true;
-    case 352: launch_ks<T, 1, 352, BR>(ctx, dd, plan, zero_src); return true;
     default: break;
     }
+    if constexpr (sizeof(T) == 8) {
+        if (plan.W == 352) { launch_ks<T, 1, 352, BR>(ctx, dd, plan, zero_src); return true; }
+    }
     if constexpr (sizeof(T) == 4) {
+        if (plan.W == 384) { launch_ks<T, 1, 384, BR>(ctx, dd, plan, zero_src); return true; }
         if (plan.W == 528) { launch_ks<T, 1, 528, BR>(ctx, dd, plan, zero_src); return true; }
         if (plan.W == 768) { launch_ks<T, 2, 768, BR>(ctx, dd, plan, zero_src); return true; }
     }
@@ -1941,7 +1944,7 @@ static int set_lds_attr(x2g_ctx *ctx)
         (const void *)k_sweep<float, 1, 128, 4>,
         (const void *)k_sweep<float, 1, 192, 4>,
         (const void *)k_sweep<float, 1, 256, 4>,
-        (const void *)k_sweep<float, 1, 352, 4>,
+        (const void *)k_sweep<float, 1, 384, 4>,
         (const void *)k_sweep<float, 1, 528, 4>,
         (const void *)k_sweep<float, 2, 768, 4>,
         (const void *)k_sweep<double, 1, 128, 2>,
@@ -1951,7 +1954,7 @@ static int set_lds_attr(x2g_ctx *ctx)
         (const void *)k_sweep<float, 1, 128, 2>,
         (const void *)k_sweep<float, 1, 192, 2>,
         (const void *)k_sweep<float, 1, 256, 2>,
-        (const void *)k_sweep<float, 1, 352, 2>,
+        (const void *)k_sweep<float, 1, 384, 2>,
         (const void *)k_sweep<float, 1, 528, 2>,
         (const void *)k_sweep<float, 2, 768, 2>};
     for (const void *k : ks) X2G_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
