@@ -206,9 +206,13 @@ int x2g_mcsor_noalloc(x2g_grid *g, int isym, int bkind, double *work_d, const do
  * (position in row, row) plane), all in one workgroup.  The orderings are
  * the reference's `order` card values (read_order, inputInterface.F90:2397):
  * col-wise, middle, middlemt, row-wise, rrow-wis, test1, test2, test3,
- * test4, middle2. */
+ * test4, middle2.  Method X2G_SOR_BACKWARD is sor walking the ordering from
+ * its last point to its first: the backward sweeps of the reference's
+ * altsweeps option (isstart = ngrid1, isstop = 1, isstep = -1 for odd
+ * orbitals, doSCF.F90:403-411, read by sor at sormcsor.F90:42). */
 #define X2G_MCSOR 0
 #define X2G_SOR 1
+#define X2G_SOR_BACKWARD 2
 #define X2G_ORDER_COLWISE 0
 #define X2G_ORDER_MIDDLE 1
 #define X2G_ORDER_MIDDLEMT 2
@@ -253,8 +257,8 @@ typedef struct {
     const double *rhs_f1_d;
     const double *rhs_f2_d;
     const double *rhs_f3_d;
-    /* Relaxation scheme (prec = 0 items): method X2G_MCSOR (five passes) or
-     * X2G_SOR, over the points in mesh ordering X2G_ORDER_* (one of the
+    /* Relaxation scheme (prec = 0 items): method X2G_MCSOR (five passes),
+     * X2G_SOR or X2G_SOR_BACKWARD, over the points in mesh ordering X2G_ORDER_* (one of the
      * reference's ten, initCBAllocArrays.F90:2408-2810).  Zero (MCSOR,
      * col-wise) is the pipelined colour sweep; any other pair runs the
      * reference's sequential point order exactly (x2g_relax_ordered). */

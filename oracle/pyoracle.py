@@ -103,7 +103,8 @@ def mesh(nni, nmu, ordering):
 
 
 def relax_ordered(nni, nmu, isym, work, lhs, rhs, b2d, d2d, coef16, exeven, omega, nsweeps, mode, ordering):
-    """In-place sor (mode 1) / mcsor (mode 0) of `work` in mesh ordering `ordering`."""
+    """In-place sor (mode 1) / mcsor (mode 0) of `work` in mesh ordering
+    `ordering`; mode 2 is sor walking the ordering backwards (altsweeps)."""
     code = ORDERINGS.get(ordering, ordering)
     seq = np.empty((nni - 2) * (nmu - 5), dtype=np.int32)
     lib().x2o_relax_ordered(nni, nmu, isym, _p(work), _p(np.ascontiguousarray(lhs, np.float64)),

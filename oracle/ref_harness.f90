@@ -142,8 +142,8 @@ program ref_harness
   call dumpi('nexchpots', 1, (/ nexchpots(iorb) /))
 
   ! raw kernel goldens: the first two potential problems of iorb, one mcsor
-  ! call of maxsor2 sweeps, one of a single sweep and one sor call each, on
-  ! identical input (prefix k1_ / k2_)
+  ! call of maxsor2 sweeps, one of a single sweep, one sor call and one
+  ! backward sor call each, on identical input (prefix k1_ / k2_)
   allocate(lhs(mxsize), rhs(mxsize), wk(mxsize8), wk0(mxsize8))
   do ip = 1, min(2, nexchpots(iorb))
      pfx = 'k1_'
@@ -177,6 +177,13 @@ program ref_harness
      call sor(isym, wk0, lhs, rhs, supplptr(i4b(2):), supplptr(i4b(3):), &
           sorptr(iadext(1):), sorptr(iadnor(1):), sorptr(iadex1(1):), sorptr(iadex2(1):), sorptr(iadex3(1):))
      call dumpd(pfx//'wout_sor', mxsize8, wk0)
+     ! the backward sweep of the altsweeps option (doSCF.F90:403-411)
+     isstart = ngrid1; isstop = 1; isstep = -1
+     wk0 = wk
+     call sor(isym, wk0, lhs, rhs, supplptr(i4b(2):), supplptr(i4b(3):), &
+          sorptr(iadext(1):), sorptr(iadnor(1):), sorptr(iadex1(1):), sorptr(iadex2(1):), sorptr(iadex3(1):))
+     call dumpd(pfx//'wout_sor_bwd', mxsize8, wk0)
+     isstart = 1; isstop = ngrid1; isstep = 1
      xv = 0; xv(1) = omega; xv(2) = dble(maxsor2)
      call dumpd(pfx//'scal', 8, xv)
   enddo

@@ -333,7 +333,10 @@ long x2o_mesh(int nni, int nmu, int ordering, int *seq)
 /* sor (sormcsor.F90:20-132, mode 1) or mcsor (sormcsor.F90:143-386, mode 0:
  * five passes, pass c over the sequence positions i = c, c+5, c+10, ...)
  * of nsweeps sweeps over the points of mesh ordering `ordering` (x2o_mesh),
- * each sweep followed by the boundary lines. */
+ * each sweep followed by the boundary lines.  Mode 2 is sor with the
+ * reference's backward sweep (isstart = ngrid1, isstop = 1, isstep = -1 of
+ * the altsweeps option, doSCF.F90:403-411, used by sormcsor.F90:42): the
+ * sequence from its last position to its first. */
 void x2o_relax_ordered(int nni, int nmu, int isym, double *work, const double *lhs, const double *rhs,
                        const double *b, const double *d, const double *coef, const double *exeven,
                        double omega, int nsweeps, int mode, int ordering, int *seq)
@@ -344,7 +347,7 @@ void x2o_relax_ordered(int nni, int nmu, int isym, double *work, const double *l
     for (int it = 0; it < nsweeps; ++it) {
         for (int c = 0; c < (mode == 0 ? 5 : 1); ++c)
             for (long k = c; k < n; k += (mode == 0 ? 5 : 1)) {
-                const int ik = seq[k];
+                const int ik = mode == 2 ? seq[n - 1 - k] : seq[k];
                 const int mu = ik / nni + 1, nu = ik % nni + 1;
                 update_point(work, PIDX(nu, mu, N8), (size_t)ik, N8, lhs, rhs, b, d, coef, omega, omega1);
             }

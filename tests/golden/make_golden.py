@@ -39,7 +39,8 @@ LIGHT_KEYS = ("sizes", "scal", "coef", "suppl", "vxi", "veta")
 # one file, h2_61x91_orderings.npz, under "<ordering>/<key>"
 ORDER_BASE = "h2_61x91_middle.data"
 ORDER_NAMES = ("middlemt", "row-wise", "rrow-wis", "test1", "test2", "test3", "test4", "middle2")
-ORDER_KEYS = ("k1_win", "k1_lhs", "k1_rhs", "k1_wout_mcsor1", "k1_wout_sor", "k1_isym", "k1_scal")
+ORDER_KEYS = ("k1_win", "k1_lhs", "k1_rhs", "k1_wout_mcsor1", "k1_wout_sor", "k1_wout_sor_bwd", "k1_isym",
+              "k1_scal")
 
 
 def run_case(name, deck, iorb, light=False, deck_text=None):
@@ -78,7 +79,7 @@ def run_case(name, deck, iorb, light=False, deck_text=None):
         for pfx in ("k1_", "k2_"):
             if not os.path.exists(os.path.join(d, pfx + "win.f64")):
                 continue
-            for k in ("win", "wout_mcsor", "wout_mcsor1", "wout_sor"):
+            for k in ("win", "wout_mcsor", "wout_mcsor1", "wout_sor", "wout_sor_bwd"):
                 out[pfx + k] = f64(pfx + k)
             for k in ("lhs", "rhs"):
                 out[pfx + k] = f64(pfx + k)

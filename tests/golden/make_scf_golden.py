@@ -1,9 +1,11 @@
 """Write SCF-level golden fixtures tests/golden/scf_<case>.json from the reference.
 
 Runs the reference program (oracle/_ref/x2dhf_omp, built from the reference's
-own sources by oracle/ref.mk) on the decks of tests/golden/decks -- all of
-them relax orbitals and potentials with MCSOR in the col-wise order, the
-algorithm of the GPU path -- and keeps, from its listing:
+own sources by oracle/ref.mk) on the decks of tests/golden/decks -- all but
+one relax orbitals and potentials with MCSOR in the col-wise order, the
+algorithm of the pipelined GPU sweep; altsweeps_fh_91x121 uses the
+reference's default SOR in the middle ordering with its altsweeps option
+(backward sweeps for odd orbitals) -- and keeps, from its listing:
   * the orbital energy of every orbital after every SCF iteration,
   * the final total energy and its components, orbital energies and the
     number of (MC)SOR sweeps.
@@ -35,6 +37,7 @@ CASES = {
     "fh_91x121": "fh_91x121.data",
     "config2_n2_331x601": "config2_n2_331x601.data",
     "config3_fh_481x901": "config3_fh_481x901.data",
+    "altsweeps_fh_91x121": "altsweeps_fh_91x121.data",
 }
 
 NUM = r"(-?\d\.\d+E[+-]\d+)"

@@ -158,6 +158,12 @@ def test_relaxation_scheme_cards():
         assert deck.parse(deck.to_deck(d.spec())).ordering == code
     with pytest.raises(deck.DeckError, match="undefined type of ordering"):
         deck.parse(base + "order diagonal\nstop\n")
+    # altsweeps (read as its first 8 characters, inputInterface.F90:736)
+    d = deck.parse(base + "altsweeps\nstop\n")
+    assert d.alt_sweeps == 1 and d.spec().alt_sweeps == 1
+    assert d.notes[-1].endswith("forward/backward SOR sweeps")
+    assert deck.parse(deck.to_deck(d.spec())).alt_sweeps == 1
+    assert deck.parse(base + "stop\n").spec().alt_sweeps == 0
 
 
 def test_spec_to_deck_round_trip():

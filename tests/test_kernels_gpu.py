@@ -528,8 +528,8 @@ def test_mcsor_host_dropin_vs_reference(ctx, case):
 @pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91", "h2_61x91_middle", "n2_61x91_middle"])
 @pytest.mark.parametrize("pfx", ["k1_", "k2_"])
 def test_ordered_relaxation_bitexact_vs_reference(ctx, case, pfx):
-    """x2g_relax_ordered: sor (10 sweeps) and one-thread mcsor (10 and 1
-    sweeps) in the col-wise and the reference's default middle ordering
+    """x2g_relax_ordered: sor (10 sweeps, forward and backward) and one-thread
+    mcsor (10 and 1 sweeps) in the col-wise and the reference's default middle ordering
     reproduce the reference's own routines bit for bit (the level schedule
     keeps its sequential order of dependencies; no fused multiply-adds)."""
     from x2dhf_amd import gpu
@@ -542,7 +542,8 @@ def test_ordered_relaxation_bitexact_vs_reference(ctx, case, pfx):
     isym = int(z[pfx + "isym"][0])
     omega, nsw = float(z[pfx + "scal"][0]), int(z[pfx + "scal"][1])
     lhs, rhs = ctx.upload(z[pfx + "lhs"]), ctx.upload(z[pfx + "rhs"])
-    for method, name, ns in ((1, "wout_sor", nsw), (0, "wout_mcsor", nsw), (0, "wout_mcsor1", 1)):
+    for method, name, ns in ((1, "wout_sor", nsw), (2, "wout_sor_bwd", nsw), (0, "wout_mcsor", nsw),
+                             (0, "wout_mcsor1", 1)):
         w = ctx.upload(z[pfx + "win"])
         gpu.check(ctx.lib.x2g_relax_ordered(g.h, isym, 1, w.ptr, lhs.ptr, rhs.ptr, omega, ns, method, ordering))
         got = w.get()
@@ -558,8 +559,8 @@ ORDER_CODES = {"middlemt": 2, "row-wise": 3, "rrow-wis": 4, "test1": 5, "test2":
 @pytest.mark.parametrize("ordering", list(ORDER_CODES))
 def test_other_orderings_bitexact_vs_reference(ctx, ordering):
     """x2g_relax_ordered in each of the reference's other mesh orderings:
-    sor (10 sweeps) and one-thread mcsor (1 sweep) on the H2 61x91 deck with
-    that `order` card reproduce the reference's own routines bit for bit
+    sor (10 sweeps, forward and backward) and one-thread mcsor (1 sweep) on
+    the H2 61x91 deck with that `order` card reproduce the reference's own routines bit for bit
     (tests/golden/h2_61x91_orderings.npz)."""
     import os
     from conftest import GOLDEN
@@ -572,7 +573,7 @@ def test_other_orderings_bitexact_vs_reference(ctx, ordering):
     isym = int(vec("isym")[0])
     omega, nsw = float(vec("scal")[0]), int(vec("scal")[1])
     lhs, rhs = ctx.upload(vec("lhs")), ctx.upload(vec("rhs"))
-    for method, name, ns in ((1, "wout_sor", nsw), (0, "wout_mcsor1", 1)):
+    for method, name, ns in ((1, "wout_sor", nsw), (2, "wout_sor_bwd", nsw), (0, "wout_mcsor1", 1)):
         w = ctx.upload(vec("win"))
         gpu.check(ctx.lib.x2g_relax_ordered(g.h, isym, 1, w.ptr, lhs.ptr, rhs.ptr, omega, ns, method,
                                             ORDER_CODES[ordering]))
@@ -584,7 +585,8 @@ def test_other_orderings_bitexact_vs_reference(ctx, ordering):
 
 def test_ordered_potential_batch_vs_oracle(ctx):
     """All potentials of the N2 pi orbital in one x2g_relax_batch call with
-    SOR in the middle ordering (each a workgroup of the level kernel), two
+    SOR in the middle ordering (each a workgroup of the level kernel; every
+    other potential with the backward sweeps of the altsweeps option), two
     macro-iterations of putin, 10 sweeps, putout: equal to the oracle's
     sequential sor on each, bit for bit."""
     from x2dhf_amd import gpu
@@ -599,18 +601,19 @@ def test_ordered_potential_batch_vs_oracle(ctx):
         rhs = orb[in1 - 1] * orb[in2 - 1] * gg
         lhs = f3 + float(dm * dm) * e if dm else f3.copy()
         k = len(want)
+        method = 1 if k % 2 == 0 else 2
         fun0 = z["pots_asympt"][k]
         w = fun0.copy()
         for _ in range(2):
             wp = pyoracle.putin(nni, nmu, int(isym), w)
             pyoracle.relax_ordered(nni, nmu, int(isym), wp, lhs, rhs, z["suppl"][1], z["suppl"][2], z["coef"][:16],
-                                   z["coef"][16:21], omega, 10, 1, "middle")
+                                   z["coef"][16:21], omega, 10, method, "middle")
             w = pyoracle.putout(nni, nmu, wp)
         want.append(w)
         fun, l, r = ctx.upload(fun0), ctx.upload(lhs), ctx.upload(rhs)
         keep += [fun, l, r]
         it = gpu.RelaxItem(fun.ptr, l.ptr, r.ptr, None, omega, int(isym), 1, 0, 0)
-        it.method, it.ordering = 1, 1
+        it.method, it.ordering = method, 1
         items.append(it)
     arr = (gpu.RelaxItem * len(items))(*items)
     gh = (C.c_void_p * 1)(g.h.value)

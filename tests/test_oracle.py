@@ -32,6 +32,9 @@ def test_oracle_sweeps_bitexact_vs_reference(case, pfx):
     for mode, name, ns in [(0, "wout_mcsor", nsw), (0, "wout_mcsor1", 1), (1, "wout_sor", nsw)]:
         w = pyoracle.mcsor(nni, nmu, isym, z[pfx + "win"].copy(), *args, ns, mode)
         assert _ok(w, z[pfx + name]), (case, pfx, name, np.abs(w - z[pfx + name]).max())
+    # sor with the altsweeps backward sweep (isstep = -1, doSCF.F90:403-411)
+    w = pyoracle.relax_ordered(nni, nmu, isym, z[pfx + "win"].copy(), *args, nsw, 2, "col-wise")
+    assert _ok(w, z[pfx + "wout_sor_bwd"]), (case, pfx, np.abs(w - z[pfx + "wout_sor_bwd"]).max())
 
 
 @pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
@@ -200,8 +203,8 @@ def test_oracle_f32_twin_fixed_point():
 @pytest.mark.parametrize("case", ["h2_61x91_middle", "n2_61x91_middle"])
 @pytest.mark.parametrize("pfx", ["k1_", "k2_"])
 def test_oracle_middle_ordering_bitexact_vs_reference(case, pfx):
-    """sor and mcsor in the reference's default `middle` mesh ordering
-    (initCBAllocArrays.F90:2499-2540), 10 and 1 sweeps, against the reference's
+    """sor (forward and backward) and mcsor in the reference's default `middle`
+    mesh ordering (initCBAllocArrays.F90:2499-2540), 10 and 1 sweeps, against the reference's
     own routines run on the same deck (mcsor with one thread: with several,
     the reference's OpenMP passes race on the neighbouring points that this
     ordering puts into one pass)."""
@@ -212,7 +215,7 @@ def test_oracle_middle_ordering_bitexact_vs_reference(case, pfx):
     isym = int(z[pfx + "isym"][0])
     omega, nsw = float(z[pfx + "scal"][0]), int(z[pfx + "scal"][1])
     args = (z[pfx + "lhs"], z[pfx + "rhs"], z["suppl"][1], z["suppl"][2], z["coef"][:16], z["coef"][16:21], omega)
-    for mode, name, ns in [(0, "wout_mcsor", nsw), (0, "wout_mcsor1", 1), (1, "wout_sor", nsw)]:
+    for mode, name, ns in [(0, "wout_mcsor", nsw), (0, "wout_mcsor1", 1), (1, "wout_sor", nsw), (2, "wout_sor_bwd", nsw)]:
         w = pyoracle.relax_ordered(nni, nmu, isym, z[pfx + "win"].copy(), *args, ns, mode, "middle")
         assert _ok(w, z[pfx + name]), (case, pfx, name, np.abs(w - z[pfx + name]).max())
 
@@ -222,7 +225,7 @@ OTHER_ORDERINGS = ("middlemt", "row-wise", "rrow-wis", "test1", "test2", "test3"
 
 @pytest.mark.parametrize("ordering", OTHER_ORDERINGS)
 def test_oracle_other_orderings_bitexact_vs_reference(ordering):
-    """sor (10 sweeps) and one-thread mcsor (1 sweep) in each of the
+    """sor (10 sweeps, forward and backward) and one-thread mcsor (1 sweep) in each of the
     reference's other mesh orderings (initCBAllocArrays.F90:2543-2810) against
     its own routines on the H2 61x91 deck with that `order` card
     (tests/golden/h2_61x91_orderings.npz; grid arrays from the same deck)."""
@@ -233,7 +236,7 @@ def test_oracle_other_orderings_bitexact_vs_reference(ordering):
     isym = int(g("isym")[0])
     omega, nsw = float(g("scal")[0]), int(g("scal")[1])
     args = (g("lhs"), g("rhs"), z["suppl"][1], z["suppl"][2], z["coef"][:16], z["coef"][16:21], omega)
-    for mode, name, ns in [(0, "wout_mcsor1", 1), (1, "wout_sor", nsw)]:
+    for mode, name, ns in [(0, "wout_mcsor1", 1), (1, "wout_sor", nsw), (2, "wout_sor_bwd", nsw)]:
         w = pyoracle.relax_ordered(nni, nmu, isym, g("win").copy(), *args, ns, mode, ordering)
         assert _ok(w, g(name)), (ordering, name, np.abs(w - g(name)).max())
     # the MCSOR vector pins the order itself: the middle ordering's passes

@@ -213,6 +213,37 @@ def test_reference_default_scheme_decks_match_listings(name):
     assert abs(res["etot"] - ref["etot"]) < TOL_E
 
 
+def test_altsweeps_deck_matches_reference_trajectory_and_energy():
+    """FH 91x121 with the reference's default schemes (SOR for orbitals and
+    potentials, middle ordering) and its altsweeps card: the SOR of the odd
+    orbitals and of their potentials walks the ordering backwards
+    (doSCF.F90:403-411), x2g_relax_item method X2G_SOR_BACKWARD.  The
+    orbital energies of the first 40 iterations (relative 1e-9, the bound of
+    the other FH and N2 trajectories: rounding differences outside the
+    relaxation grow over the iterations), the final orbital energies and the
+    total energy agree with the reference program's run of the same deck
+    (tests/golden/scf_altsweeps_fh_91x121.json); the deck without the card
+    departs from that trajectory by about 3% in its first iteration."""
+    from x2dhf_amd import deck
+    ref = golden("altsweeps_fh_91x121")
+    d = deck.load(os.path.join(GOLDEN, "decks", "altsweeps_fh_91x121.data"))
+    assert d.alt_sweeps == 1 and d.orb_method == 1 and d.pot_method == 1 and d.ordering == 1
+    print(f"altsweeps FH: {d.nni}x{d.nmu}, {d.notes[-1]}", flush=True)
+    res, worst = run_plain(d.spec(prec=0, schedule=0), ref, check_iters=40)
+    print(f"altsweeps FH 91x121: E = {res['etot']:.14f} (ref {ref['etot']:.14f}) dE = {res['etot'] - ref['etot']:.2e}, "
+          f"iterations {res['iterations']} (ref {len(ref['iterations'])}), trajectory rel err {worst:.2e}, "
+          f"stop '{res['stop']}'", flush=True)
+    assert worst < 1e-9
+    assert abs(res["etot"] - ref["etot"]) < TOL_E
+    assert np.abs(res["eps"] - np.array(ref["eps"])).max() < 1e-9
+    d.alt_sweeps = 0
+    spec = d.spec(prec=0, schedule=0)
+    spec.maxscf = 3
+    _, worst_fwd = run_plain(spec, ref, check_iters=3)
+    print(f"without altsweeps: trajectory rel err {worst_fwd:.2e} in 3 iterations")
+    assert worst_fwd > 1e-3
+
+
 def test_reference_be_continuation_matches_listing():
     """be/set-01: 10 SCF iterations with SOR in the middle ordering from the
     reference's converged Be checkpoint (orbpot old) end at the total energy

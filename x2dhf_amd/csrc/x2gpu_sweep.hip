@@ -2261,7 +2261,8 @@ using namespace x2g;
 // levels in order; the points of one level are independent.  For sor in
 // col-wise order the levels are the anti-diagonals nu + mu; mcsor in
 // col-wise order needs one level per pass, in middle order a few more where
-// the ordering folds back.
+// the ordering folds back.  Backward sor (the altsweeps option) is sor over
+// the reversed sequence.
 //
 // Relaxation sequence of mesh ordering `ordering` (the reference's mesh,
 // initCBAllocArrays.F90:2458-2810) as unpadded indices (mu-1)*nni + nu-1.
@@ -2317,6 +2318,7 @@ static void build_levels(int nni, int nmu, int method, int ordering, std::vector
 {
     std::vector<int> seq;
     mesh_sequence(nni, nmu, ordering, seq);
+    if (method == X2G_SOR_BACKWARD) std::reverse(seq.begin(), seq.end());
     const int n = (int)seq.size(), npass = method == X2G_MCSOR ? 5 : 1;
     std::vector<int> pos((size_t)nni * nmu, -1), lvl((size_t)nni * nmu, 0);
     for (int k = 0; k < n; ++k) pos[seq[k]] = k;
@@ -2705,7 +2707,8 @@ extern "C" int x2g_relax_ordered(x2g_grid *g, int isym, int bkind, double *work_
 {
     int rc = mcsor_check(g, isym, bkind, work_d, lhs_d, rhs_d, omega, nsweeps, "x2g_relax_ordered");
     if (rc) return rc;
-    X2G_CHECK(method == X2G_MCSOR || method == X2G_SOR, "x2g_relax_ordered: method must be X2G_MCSOR or X2G_SOR");
+    X2G_CHECK(method == X2G_MCSOR || method == X2G_SOR || method == X2G_SOR_BACKWARD,
+              "x2g_relax_ordered: method must be X2G_MCSOR, X2G_SOR or X2G_SOR_BACKWARD");
     X2G_CHECK(ordering >= X2G_ORDER_COLWISE && ordering <= X2G_ORDER_MIDDLE2,
               "x2g_relax_ordered: ordering must be one of X2G_ORDER_*");
     if (nsweeps == 0) return X2G_OK;
@@ -2768,7 +2771,8 @@ static int relax_batch_eager(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_rel
         X2G_CHECK(it.bkind == 0 || it.bkind == 1, "x2g_relax_batch: item %d bkind", i);
         X2G_CHECK(it.prec == 0 || it.prec == 1, "x2g_relax_batch: item %d prec", i);
         X2G_CHECK(it.omega > 0.0 && it.omega < 2.0, "x2g_relax_batch: item %d omega must lie in (0, 2)", i);
-        X2G_CHECK(it.method == X2G_MCSOR || it.method == X2G_SOR, "x2g_relax_batch: item %d method", i);
+        X2G_CHECK(it.method == X2G_MCSOR || it.method == X2G_SOR || it.method == X2G_SOR_BACKWARD,
+                  "x2g_relax_batch: item %d method", i);
         X2G_CHECK(it.ordering >= X2G_ORDER_COLWISE && it.ordering <= X2G_ORDER_MIDDLE2,
                   "x2g_relax_batch: item %d ordering", i);
         X2G_CHECK(it.prec == 0 || (it.method == X2G_MCSOR && it.ordering == X2G_ORDER_COLWISE),

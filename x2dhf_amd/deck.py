@@ -30,6 +30,8 @@ relaxation path and builds the scf.JobSpec the GPU driver runs:
   order name                         mesh ordering (read_order, 8 characters: col-wise,
                                      middle (the default, initVariables.F90:206), middlemt,
                                      row-wise, rrow-wis(e), test1..test4, middle2)
+  altsweeps                          SOR sweeps forward for even and backward for odd
+                                     orbitals (inputInterface.F90:736, doSCF.F90:403-411)
   print, prtevery, mmoments, plot, out4pair, debug, inout, stop   ignored
 
 Any other card (DFT, external fields, fixed orbitals, other start orbitals,
@@ -105,6 +107,7 @@ class Deck:
     orb_method: int = 1                            # 0 MCSOR, 1 SOR (the reference's default)
     pot_method: int = 1
     ordering: int = 1                              # 0 col-wise, 1 middle (the default)
+    alt_sweeps: int = 0                            # altsweeps card
     notes: list = field(default_factory=list)
 
     def spec(self, prec=0, schedule=0, device=0, polish=0):
@@ -123,7 +126,7 @@ class Deck:
                              schedule=schedule, device=device, maxsor=self.maxsororb, maxsorpot=self.maxsorpot,
                              mpole=self.mpole, ienterm4init=self.ienterm4init, recalc_mm=self.recalc_mm,
                              conv=self.conv, polish=polish, orb_method=self.orb_method,
-                             pot_method=self.pot_method, ordering=self.ordering)
+                             pot_method=self.pot_method, ordering=self.ordering, alt_sweeps=self.alt_sweeps)
 
 
 def _cards(text):
@@ -306,6 +309,8 @@ def parse(text):
             if name not in _ORDERINGS:
                 raise DeckError(f"order '{name}': undefined type of ordering")
             d.ordering = _ORDERINGS[name]
+        elif label == "altsweep":
+            d.alt_sweeps = 1
         else:
             raise DeckError(f"card '{label}' is outside the relaxation path of this build")
     for need in ("nuclei", "config", "grid", "orbpot") + (("lcao",) if d.start == "hydrogen" else ()):
@@ -315,7 +320,8 @@ def parse(text):
     d.nmu = nmu_calc(d.nmu_card, d.nni, d.r, d.rinf)
     scheme = {0: "MCSOR", 1: "SOR"}
     order = {v: k for k, v in _ORDERINGS.items()}[d.ordering]
-    d.notes.append(f"orbitals by {scheme[d.orb_method]}, potentials by {scheme[d.pot_method]}, {order} ordering")
+    d.notes.append(f"orbitals by {scheme[d.orb_method]}, potentials by {scheme[d.pot_method]}, {order} ordering"
+                   + (", forward/backward SOR sweeps" if d.alt_sweeps else ""))
     return d
 
 
@@ -352,6 +358,8 @@ def to_deck(spec, title="job", mcsor_threads=None):
             lines.append("mcsor")
         if spec.pot_method == 0:
             lines.append("mcsor-ce")
+    if spec.alt_sweeps:
+        lines.append("altsweeps")
     lines.append("stop")
     return "\n".join(lines) + "\n"
 

@@ -28,6 +28,8 @@ module x2f_api
      ! relaxation scheme of the FP64 sweeps: orbitals / potentials 0 MCSOR,
      ! 1 SOR; mesh ordering: an X2G_ORDER_* code, 0 col-wise .. 9 middle2 (x2g_relax_item)
      integer(c_int) :: orb_method, pot_method, ordering
+     ! altsweeps (doSCF.F90:403-411): 1 = SOR sweeps backward for odd orbitals
+     integer(c_int) :: alt_sweeps
   end type x2f_spec
 
   ! job results (mirrored by x2dhf_amd/scf.py: JobResult)
@@ -220,6 +222,7 @@ contains
       j%o%omega_orb = spec%omega_orb; j%o%omega_pot = spec%omega_pot; j%o%recalc_mm = spec%recalc_mm
       if (spec%polish > 0) j%o%polish = spec%polish
       j%o%orb_method = spec%orb_method; j%o%pot_method = spec%pot_method; j%o%ordering = spec%ordering
+      j%o%alt_sweeps = spec%alt_sweeps
       if (spec%nscf2skip > 0) j%o%nscf2skip = spec%nscf2skip
       if (spec%nenlast > 0) j%o%nenlast = spec%nenlast
       if (spec%nnolast > 0) j%o%nnolast = spec%nnolast

@@ -20,7 +20,7 @@ module x2f_capi
      type(c_ptr) :: rhs_f1_d = c_null_ptr   ! rhs = f1*f2*f3 formed by the residual (optional)
      type(c_ptr) :: rhs_f2_d = c_null_ptr
      type(c_ptr) :: rhs_f3_d = c_null_ptr
-     integer(c_int) :: method = 0            ! 0 MCSOR, 1 SOR (x2g_relax_ordered)
+     integer(c_int) :: method = 0            ! 0 MCSOR, 1 SOR, 2 backward SOR (x2g_relax_ordered)
      integer(c_int) :: ordering = 0          ! X2G_ORDER_*: 0 col-wise, 1 middle, .. 9 middle2
   end type x2g_relax_item
 

@@ -53,6 +53,9 @@ module x2f_scf
      ! relaxation scheme of FP64 sweeps (x2g_relax_item method / ordering):
      ! orbitals and potentials 0 MCSOR, 1 SOR; ordering an X2G_ORDER_* code
      integer :: orb_method = 0, pot_method = 0, ordering = 0
+     ! altsweeps (doSCF.F90:403-411): SOR runs backward over the ordering
+     ! for the odd orbitals and their potentials
+     integer :: alt_sweeps = 0
   end type scf_opts
 
   type :: lc_batch
@@ -405,7 +408,7 @@ contains
        n = n + 1
        items(n) = x2g_relax_item(fun, j%plhs(ip), j%prhs(ip), c_null_ptr, j%omega_pot, &
             int(j%c%psym(i,k), c_int), 1_c_int, int(j%o%prec, c_int), int(gidx, c_int))
-       call set_scheme(j, items(n), j%o%pot_method)
+       call set_scheme(j, items(n), j%o%pot_method, i)
        if (present(lb) .and. j%o%prec == 1) then
           items(n)%rhs_f1_d = j%orb(j%c%pin1(i,k))
           items(n)%rhs_f2_d = j%orb(j%c%pin2(i,k))
@@ -426,18 +429,21 @@ contains
     n = n + 1
     items(n) = x2g_relax_item(j%orb(i), j%lo(i), j%f2o(i), j%f1o(i), j%omega_orb, &
          int(j%c%isym(i), c_int), 0_c_int, int(j%o%prec, c_int), int(gidx, c_int))
-    call set_scheme(j, items(n), j%o%orb_method)
+    call set_scheme(j, items(n), j%o%orb_method, i)
     j%nsweep_orb = j%nsweep_orb + int(j%o%maxsororb(1)*j%o%maxsororb(2), c_long_long)
   end subroutine orb_item
 
-  ! The relaxation scheme of an FP64 item (the FP32 correction sweeps are
-  ! col-wise MCSOR whatever the job asks for).
-  subroutine set_scheme(j, it, method)
+  ! The relaxation scheme of an FP64 item of orbital iorb (the FP32
+  ! correction sweeps are col-wise MCSOR whatever the job asks for).  With
+  ! altsweeps the SOR of an odd orbital and of its potentials walks the
+  ! ordering backwards (isstep = -1, doSCF.F90:403-411); MCSOR ignores it.
+  subroutine set_scheme(j, it, method, iorb)
     type(scf_job), intent(in) :: j
     type(x2g_relax_item), intent(inout) :: it
-    integer, intent(in) :: method
+    integer, intent(in) :: method, iorb
     if (j%o%prec /= 0) return
     it%method = int(method, c_int)
+    if (method == 1 .and. j%o%alt_sweeps /= 0 .and. mod(iorb, 2) == 1) it%method = 2_c_int
     it%ordering = int(j%o%ordering, c_int)
   end subroutine set_scheme
 
@@ -1204,7 +1210,7 @@ contains
           n = n + 1
           items(n) = x2g_relax_item(js(h)%orb(i), js(h)%lo(i), js(h)%f2o(i), js(h)%f1o(i), js(h)%omega_orb, &
                int(js(h)%c%isym(i), c_int), 0_c_int, int(js(h)%o%prec, c_int), int(q-1, c_int))
-          call set_scheme(js(h), items(n), js(h)%o%orb_method)
+          call set_scheme(js(h), items(n), js(h)%o%orb_method, i)
           if (js(h)%o%prec == 1 .and. js(h)%c%nel /= 1) then
              items(n)%rhs_f1_d = js(h)%S(7)
              items(n)%rhs_f2_d = js(h)%xpot(i)
