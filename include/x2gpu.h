@@ -393,6 +393,57 @@ int x2g_pot_tail_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_tail_item
  * one device-to-host transfer (synchronises). */
 int x2g_gather(x2g_ctx *ctx, const double *const *src_d, const int *counts, int n, double *out_h);
 
+/* ---- SCF control on the device ------------------------------------------ */
+/* The end-of-iteration bookkeeping of the SCF loop (doSCF.F90:546-744: the
+ * orbital-energy changes, the energy / norm threshold tests with their
+ * nscfextra confirmations and resets, the saturation test, the iteration
+ * limit, the release of the frozen starting orbitals, the multipole-refresh
+ * decision) plus the failure guards (a NaN norm or energy, detect.F90:9-58;
+ * an energy change above orbEnergyIncThld, doSCF.F90:612-616) and the FP64
+ * finish of a mixed-precision job, for one job per record.  The record
+ * lives in device memory and is the job's control state between iterations;
+ * the caller writes it once (settings and the state after the set-up pass)
+ * and reads it back to learn the decisions.  Orbital i is index i-1. */
+#define X2G_CTRL_MAXO 16
+typedef struct {
+    /* settings */
+    double ethld;                  /* 10^-ienterm: orbital-energy threshold */
+    double nthld;                  /* 10^-inoterm: orbital-norm threshold */
+    double thren4init;             /* 10^-ienterm4init: frees the frozen starting orbitals */
+    double recalc_mm;              /* multipole refresh when |eeconv / max d eps| exceeds it */
+    double inc_thld;               /* orbEnergyIncThld: a larger energy change is divergence */
+    /* state */
+    double ee[X2G_CTRL_MAXO];      /* orbital energies (diagonal Lagrange multipliers) */
+    double eeold[X2G_CTRL_MAXO];
+    double deltaee[X2G_CTRL_MAXO];
+    double orbnorm[X2G_CTRL_MAXO]; /* <psi|psi> before normalisation */
+    double eeconv, eeconvprev, dnmaxprev, deltaeemax, deltanormmax;
+    int norb, maxscf, nscf2skip, nenlast, nnolast, nscfextra, maxresets, polish;
+    int ifixorb[X2G_CTRL_MAXO];    /* 1: frozen (a hydrogenic start until the potentials settle) */
+    int itouch[X2G_CTRL_MAXO];
+    int mm_touch[X2G_CTRL_MAXO];   /* orbitals whose moments the refresh recomputes */
+    int iscf, nresets, nresets4no, norbenergythld, norbnormthld, noenergydec, nonormdec;
+    int lenergythld, lnormthld, lfixorb, lfixorb4init;
+    int mm_due;                    /* out: refresh the multipole moments now */
+    int prec;                      /* 1: FP32 correction sweeps; set to 0 by the FP64 finish */
+    int polish_end;                /* no threshold stop before this iteration (FP64 finish) */
+    int stopcode;                  /* 1 energy, 2 norm, 3 limit, 4 saturation, 5 resets, 6 diverged, 7 NaN */
+    int stop;                      /* out: the job stops after this iteration */
+} x2g_scf_ctrl;
+
+/* One SCF iteration's control of n jobs in one launch: for job k, with
+ * rec_d[k][0..norb-1] the norms and rec_d[k][16..16+norb-1] the orbital
+ * energies this iteration produced (x2g_normortho_batch norm_out_d,
+ * x2g_eahf_batch out_d), advance ctrl_d[k] by one iteration.  The records
+ * stay on the device; read them with x2g_gather (sizeof(x2g_scf_ctrl) / 8
+ * doubles each).  No reference counterpart as an interface: the reference
+ * runs these rules on the host inside scf (doSCF.F90:546-744). */
+int x2g_scf_control(x2g_ctx *ctx, x2g_scf_ctrl *const *ctrl_d, const double *const *rec_d, int n);
+
+/* sizeof(x2g_scf_ctrl), for callers that mirror the record in another
+ * language (the Fortran driver binds it with ISO_C_BINDING). */
+int x2g_scf_ctrl_bytes(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,3 +71,15 @@ def test_plan_graph_and_workspace_entry_points_check_their_arguments():
     w64, w32 = lib.x2g_relax_workspace_bytes(331, 601, 0), lib.x2g_relax_workspace_bytes(331, 601, 1)
     assert w64 > 2 * 8 * 339 * 609 and w32 > 2 * 4 * 339 * 609 and w64 > w32
     assert lib.x2g_relax_workspace_bytes(5, 601, 0) == -1 and lib.x2g_relax_workspace_bytes(331, 601, 2) == -1
+
+
+def test_scf_control_record_layout_agrees():
+    """x2g_scf_ctrl has one layout in the library, the ctypes mirror and the
+    Fortran driver's bind(c) type (which checks it at job creation): 74
+    doubles then 72 ints, 880 bytes, no padding; x2g_scf_control checks its arguments
+    before any device call."""
+    from x2dhf_amd import gpu
+    lib = gpu.load()
+    assert lib.x2g_scf_ctrl_bytes() == C.sizeof(gpu.ScfCtrl) == 880
+    assert gpu.ScfCtrl.norb.offset == 74 * 8 and gpu.ScfCtrl.stop.offset == 880 - 4
+    assert lib.x2g_scf_control(None, None, None, 1) == -1 and b"null" in lib.x2g_last_error()

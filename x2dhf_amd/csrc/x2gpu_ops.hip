@@ -972,6 +972,221 @@ extern "C" int x2g_pot_tail_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2
     return X2G_OK;
 }
 
+// ---------------------------------------------------------------------------
+// SCF control on the device: one thread per job advances the job's control
+// record by one iteration.  The rules are those of the Fortran driver's
+// energy_change / convergence_control / scf_control (the reference's
+// doSCF.F90:546-744 with the failure guards and the FP64 finish), in the
+// same order of comparisons and assignments, so every decision is the host
+// rules' decision bit for bit (only comparisons, one subtraction per orbital
+// and one division; no contraction is formed).
+struct CtrlDesc {
+    x2g_scf_ctrl *c;
+    const double *rec;
+};
+
+__device__ static bool ctrl_rules(x2g_scf_ctrl &c)
+{
+#pragma clang fp contract(off)
+    const double eps0 = 2.220446049250313e-16;  // epsilon(1.0_dp)
+    const int norb = c.norb;
+    c.deltaeemax = 0.0;
+    for (int i = 0; i < norb; ++i)
+        if (fabs(c.deltaee[i]) >= fabs(c.deltaeemax)) c.deltaeemax = c.deltaee[i];
+    c.deltanormmax = 0.0;
+    for (int i = 0; i < norb; ++i)
+        if (c.ifixorb[i] == 0 && fabs(c.orbnorm[i] - 1.0) >= c.deltanormmax) {
+            c.deltanormmax = fabs(c.orbnorm[i] - 1.0);
+            if (fabs(c.deltanormmax) < eps0) c.deltanormmax = eps0;
+        }
+    if (fabs(c.eeconv) < eps0) c.eeconv = c.deltaeemax;
+
+    // failure guards: a NaN norm or energy, or divergence
+    for (int i = 0; i < norb; ++i)
+        if (isnan(c.ee[i]) || isnan(c.orbnorm[i]) || isnan(c.deltaee[i])) {
+            c.stopcode = 7;
+            return true;
+        }
+    if (fabs(c.deltaeemax) > c.inc_thld) {
+        c.stopcode = 6;
+        return true;
+    }
+
+    // saturation: no decrease for nenlast / nnolast iterations after nscf2skip
+    if (c.iscf > c.nscf2skip) {
+        if (fabs(c.deltaeemax) >= fabs(c.eeconvprev)) {
+            ++c.noenergydec;
+        } else {
+            c.eeconvprev = c.deltaeemax;
+            c.noenergydec = 0;
+        }
+        if (fabs(c.deltanormmax) >= fabs(c.dnmaxprev)) {
+            ++c.nonormdec;
+        } else {
+            c.dnmaxprev = c.deltanormmax;
+            c.nonormdec = 0;
+        }
+        if (c.noenergydec > c.nenlast || c.nonormdec > c.nnolast) {
+            c.stopcode = 4;
+            return true;
+        }
+    } else {
+        c.eeconvprev = 100.0;
+        c.dnmaxprev = c.deltanormmax;
+    }
+    if (c.iscf == c.maxscf) {
+        c.stopcode = 3;
+        return true;
+    }
+
+    // the frozen starting orbitals are released once the energies settle
+    if (c.lfixorb4init && fabs(c.deltaeemax) < c.thren4init) {
+        if (c.norbenergythld == c.nscfextra) {
+            for (int i = 0; i < X2G_CTRL_MAXO; ++i) c.ifixorb[i] = 0;
+            c.lfixorb = 0;
+            c.lfixorb4init = 0;
+            c.eeconv = 10.0;
+            c.deltaeemax = 100.0;
+        } else {
+            ++c.norbenergythld;
+        }
+    }
+
+    if (!c.lenergythld) {
+        if (fabs(c.deltaeemax) < c.ethld) {
+            c.lenergythld = 1;
+            c.eeconv = c.deltaeemax;
+            for (int i = 0; i < X2G_CTRL_MAXO; ++i) c.itouch[i] = 0;
+            c.norbenergythld = 0;
+        }
+    } else {
+        if (fabs(c.deltaeemax) < c.ethld) {
+            if (c.norbenergythld == c.nscfextra) {
+                c.stopcode = 1;
+                return true;
+            }
+            ++c.norbenergythld;
+        } else if (fabs(c.deltanormmax) < c.nthld) {
+            if (c.norbnormthld == c.nscfextra) {
+                c.stopcode = 2;
+                return true;
+            }
+            ++c.norbnormthld;
+        } else {
+            c.lenergythld = 0;
+            if (c.nresets > c.maxresets) {
+                c.stopcode = 5;
+                return true;
+            }
+            ++c.nresets;
+        }
+    }
+
+    if (!c.lfixorb) {
+        if (!c.lnormthld) {
+            if (fabs(c.deltanormmax) < c.nthld) {
+                c.lnormthld = 1;
+                for (int i = 0; i < X2G_CTRL_MAXO; ++i) c.itouch[i] = 0;
+                c.norbnormthld = 0;
+            }
+        } else {
+            if (fabs(c.deltanormmax) < c.nthld) {
+                if (c.norbnormthld == c.nscfextra) {
+                    c.stopcode = 2;
+                    return true;
+                }
+                ++c.norbnormthld;
+            } else {
+                c.lnormthld = 0;
+                if (c.nresets4no > c.maxresets) {
+                    c.stopcode = 5;
+                    return true;
+                }
+                ++c.nresets4no;
+            }
+        }
+    }
+
+    // refresh the multipole moments when the energy change dropped enough
+    if (fabs(c.eeconv / c.deltaeemax) > c.recalc_mm) {
+        c.eeconv = c.deltaeemax;
+        c.mm_due = 1;
+        for (int i = 0; i < X2G_CTRL_MAXO; ++i) c.mm_touch[i] = c.itouch[i];
+    }
+    for (int i = 0; i < X2G_CTRL_MAXO; ++i) c.itouch[i] = 0;
+    return false;
+}
+
+__global__ void k_scf_control(const CtrlDesc *__restrict__ cd, int n)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    x2g_scf_ctrl c = *cd[k].c;
+    const double *rec = cd[k].rec;
+    c.norb = max(0, min(c.norb, X2G_CTRL_MAXO));
+    // the iteration: every orbital was touched; the new norms (of the
+    // orbitals not frozen) and energies, and their changes
+    ++c.iscf;
+    c.mm_due = 0;
+    for (int i = 0; i < c.norb; ++i) {
+        c.itouch[i] = 1;
+        if (c.ifixorb[i] == 0) c.orbnorm[i] = rec[i];
+        c.ee[i] = rec[X2G_CTRL_MAXO + i];
+        c.deltaee[i] = c.ee[i] - c.eeold[i];
+        if (fabs(c.deltaee[i]) < 2.220446049250313e-16) c.deltaee[i] = 2.0 * c.ethld;
+        c.eeold[i] = c.ee[i];
+    }
+    bool stop = ctrl_rules(c);
+    // the FP64 finish of a mixed-precision job
+    if (c.polish > 0) {
+        const bool threshold_stop = stop && c.stopcode != 3 && c.stopcode < 6;
+        if (c.prec == 1) {
+            if (threshold_stop || (!stop && c.iscf >= c.maxscf - c.polish)) {
+                c.prec = 0;
+                c.polish_end = min(c.iscf + c.polish, c.maxscf);
+                c.lenergythld = 0;
+                c.lnormthld = 0;
+                c.norbenergythld = 0;
+                c.norbnormthld = 0;
+                c.nresets = 0;
+                c.nresets4no = 0;
+                c.noenergydec = 0;
+                c.nonormdec = 0;
+                c.eeconvprev = 1.0e35;
+                c.dnmaxprev = 1.0e35;
+                c.stopcode = 0;
+                stop = false;
+            }
+        } else if (threshold_stop && c.iscf < c.polish_end) {
+            c.stopcode = 0;
+            stop = false;
+        }
+    }
+    c.stop = stop ? 1 : 0;
+    *cd[k].c = c;
+}
+
+extern "C" int x2g_scf_control(x2g_ctx *ctx, x2g_scf_ctrl *const *ctrl_d, const double *const *rec_d, int n)
+{
+    X2G_CHECK(ctx && n >= 0 && (n == 0 || (ctrl_d && rec_d)), "x2g_scf_control: null argument");
+    if (n == 0) return X2G_OK;
+    std::vector<CtrlDesc> v(n);
+    for (int k = 0; k < n; ++k) {
+        X2G_CHECK(ctrl_d[k] && rec_d[k], "x2g_scf_control: job %d has a null record", k);
+        v[k] = {ctrl_d[k], rec_d[k]};
+    }
+    size_t off = 0;
+    const CtrlDesc *dd;
+    int rc = stage(ctx, v.data(), v.size(), off, &dd);
+    if (rc) return rc;
+    X2G_LAUNCH(ctx, k_scf_control, dim3((n + 63) / 64), dim3(64), 0, dd, n);
+    X2G_HIP(hipGetLastError());
+    return X2G_OK;
+}
+
+extern "C" int x2g_scf_ctrl_bytes(void) { return (int)sizeof(x2g_scf_ctrl); }
+
 extern "C" int x2g_gather(x2g_ctx *ctx, const double *const *src_d, const int *counts, int n, double *out_h)
 {
     X2G_CHECK(ctx && (n == 0 || (src_d && counts && out_h)) && n >= 0, "x2g_gather: null argument");

@@ -30,6 +30,9 @@ module x2f_api
      integer(c_int) :: orb_method, pot_method, ordering
      ! altsweeps (doSCF.F90:403-411): 1 = SOR sweeps backward for odd orbitals
      integer(c_int) :: alt_sweeps
+     ! 1: the batched schedule runs its end-of-iteration control on the host
+     ! (the default 0 runs it on the device, x2g_scf_control)
+     integer(c_int) :: host_control
   end type x2f_spec
 
   ! job results (mirrored by x2dhf_amd/scf.py: JobResult)
@@ -223,6 +226,7 @@ contains
       if (spec%polish > 0) j%o%polish = spec%polish
       j%o%orb_method = spec%orb_method; j%o%pot_method = spec%pot_method; j%o%ordering = spec%ordering
       j%o%alt_sweeps = spec%alt_sweeps
+      j%o%host_control = spec%host_control
       if (spec%nscf2skip > 0) j%o%nscf2skip = spec%nscf2skip
       if (spec%nenlast > 0) j%o%nenlast = spec%nenlast
       if (spec%nnolast > 0) j%o%nnolast = spec%nnolast

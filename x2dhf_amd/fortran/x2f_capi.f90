@@ -24,6 +24,22 @@ module x2f_capi
      integer(c_int) :: ordering = 0          ! X2G_ORDER_*: 0 col-wise, 1 middle, .. 9 middle2
   end type x2g_relax_item
 
+  ! the device SCF control record (x2g_scf_ctrl); orbital i is index i
+  integer, parameter :: X2G_CTRL_MAXO = 16
+  type, bind(c) :: x2g_scf_ctrl
+     real(c_double) :: ethld = 0, nthld = 0, thren4init = 0, recalc_mm = 0, inc_thld = 0
+     real(c_double) :: ee(X2G_CTRL_MAXO) = 0, eeold(X2G_CTRL_MAXO) = 0
+     real(c_double) :: deltaee(X2G_CTRL_MAXO) = 0, orbnorm(X2G_CTRL_MAXO) = 0
+     real(c_double) :: eeconv = 0, eeconvprev = 0, dnmaxprev = 0, deltaeemax = 0, deltanormmax = 0
+     integer(c_int) :: norb = 0, maxscf = 0, nscf2skip = 0, nenlast = 0, nnolast = 0, nscfextra = 0
+     integer(c_int) :: maxresets = 0, polish = 0
+     integer(c_int) :: ifixorb(X2G_CTRL_MAXO) = 0, itouch(X2G_CTRL_MAXO) = 0, mm_touch(X2G_CTRL_MAXO) = 0
+     integer(c_int) :: iscf = 0, nresets = 0, nresets4no = 0, norbenergythld = 0, norbnormthld = 0
+     integer(c_int) :: noenergydec = 0, nonormdec = 0
+     integer(c_int) :: lenergythld = 0, lnormthld = 0, lfixorb = 0, lfixorb4init = 0
+     integer(c_int) :: mm_due = 0, prec = 0, polish_end = 0, stopcode = 0, stop_flag = 0
+  end type x2g_scf_ctrl
+
   type, bind(c) :: x2g_term
      real(c_double) :: c = 0.0_c_double
      type(c_ptr) :: a_d = c_null_ptr
@@ -294,6 +310,17 @@ module x2f_capi
        integer(c_int), value :: n
        real(c_double), intent(out) :: out(*)
      end function x2g_gather
+
+     integer(c_int) function x2g_scf_control(ctx, ctrl, rec, n) bind(c, name='x2g_scf_control')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr), intent(in) :: ctrl(*), rec(*)
+       integer(c_int), value :: n
+     end function x2g_scf_control
+
+     integer(c_int) function x2g_scf_ctrl_bytes() bind(c, name='x2g_scf_ctrl_bytes')
+       import :: c_int
+     end function x2g_scf_ctrl_bytes
 
      integer(c_int) function x2g_setci(g, psi, ihsym) bind(c, name='x2g_setci')
        import :: c_int, c_ptr

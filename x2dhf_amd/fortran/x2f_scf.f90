@@ -56,6 +56,10 @@ module x2f_scf
      ! altsweeps (doSCF.F90:403-411): SOR runs backward over the ordering
      ! for the odd orbitals and their potentials
      integer :: alt_sweeps = 0
+     ! the batched schedule's end-of-iteration control: 0 on the device
+     ! (x2g_scf_control), 1 by the host rules below (the plain schedule always
+     ! uses the host rules)
+     integer :: host_control = 0
   end type scf_opts
 
   type :: lc_batch
@@ -102,8 +106,12 @@ module x2f_scf
                                    ! 6 divergence, 7 not a number
      integer(c_long_long) :: nsweep_orb = 0, nsweep_pot = 0
      ! device record written by the batched schedule: <psi|psi> before
-     ! normalisation (1..MAXO) and orbital energies (MAXO+1..2*MAXO)
-     type(c_ptr) :: rec = c_null_ptr
+     ! normalisation (1..MAXO) and orbital energies (MAXO+1..2*MAXO),
+     ! followed by the job's device control record (x2g_scf_ctrl), which the
+     ! batched schedule advances on the device; ctrl_dirty: the host state
+     ! changed since the record was last written
+     type(c_ptr) :: rec = c_null_ptr, ctrl = c_null_ptr
+     logical :: ctrl_dirty = .true.
      ! results
      real(dp) :: etot = 0, vkt = 0, vnt = 0, ecoul = 0, eexch = 0, virial = 0
   end type scf_job
@@ -155,8 +163,15 @@ contains
     enddo
     j%ones = dalloc(j)
     call x2check(x2g_fill(j%ctx, j%ones, j%n, 1.0_dp), 'x2g_fill')
-    call x2check(x2g_malloc(j%ctx, int(16*MAXO, c_size_t), j%rec), 'x2g_malloc')
-    call x2check(x2g_fill(j%ctx, j%rec, int(2*MAXO, c_size_t), 0.0_dp), 'x2g_fill')
+    block
+      type(x2g_scf_ctrl), target :: c0
+      if (x2g_scf_ctrl_bytes() /= int(c_sizeof(c0), c_int) .or. X2G_CTRL_MAXO /= MAXO) &
+           error stop 'x2g_scf_ctrl: the C library and the Fortran driver lay the record out differently'
+      call x2check(x2g_malloc(j%ctx, int(16*MAXO, c_size_t) + c_sizeof(c0), j%rec), 'x2g_malloc')
+      call x2check(x2g_fill(j%ctx, j%rec, int(2*MAXO, c_size_t), 0.0_dp), 'x2g_fill')
+      j%ctrl = ptr_off(j%rec, 16*MAXO)
+    end block
+    j%ctrl_dirty = .true.
     j%ndev = j%ndev + 1
     j%devbufs(j%ndev) = j%rec
     ! potentials of the iteration, in the SCF's orbital order
@@ -848,6 +863,7 @@ contains
     j%eeconvprev = 1.e35_dp; j%dnmaxprev = 1.e35_dp
     j%noenergydec = 0; j%nonormdec = 0
     j%done = .false.; j%stopcode = 0
+    j%ctrl_dirty = .true.
     call setci(j, 0)
   end subroutine scf_init_state
 
@@ -890,6 +906,7 @@ contains
        call orbital_step(j, j%c%norb - ii + 1)
     enddo
     stop_now = scf_control(j)
+    j%ctrl_dirty = .true.
     if (j%mm_due) then
        call mpole_moments(j, j%mm_touch)
        j%mm_due = .false.
@@ -1110,12 +1127,14 @@ contains
     type(x2g_tail_item), allocatable :: tails(:)
     type(x2g_normortho_item), allocatable :: no(:)
     type(x2g_eahf_item), allocatable :: ea(:)
-    type(c_ptr), allocatable :: grids(:), recs(:)
+    type(c_ptr), allocatable :: grids(:), recs(:), cptr(:), rptr(:)
     integer(c_int), allocatable :: cnt(:)
     integer, allocatable :: recq(:)
     real(dp), allocatable :: rech(:)
     logical, allocatable :: done(:)
     type(lc_batch) :: lb
+    type(x2g_scf_ctrl) :: cc
+    integer :: ndc
     type(x2g_term) :: t1(8), tc(MAXO), tx(4*MAXO)
     type(c_ptr) :: ctx
     integer :: q, h, i, ii, n, nt, nmax, ntmax, ms1, ms2, maxnorb, mpmax, i2, i3, jor, nno, nea, nrec
@@ -1141,6 +1160,7 @@ contains
     do q = 1, nj
        if (stopped(q)) cycle
        h = idx(q)
+       if (js(h)%o%host_control == 0 .and. js(h)%ctrl_dirty) call ctrl_upload(js(h))
        js(h)%iscf = js(h)%iscf + 1
        js(h)%iasympt = js(h)%c%norb
        allocate(done(max(1, js(h)%c%nslots)))
@@ -1279,34 +1299,115 @@ contains
     enddo
     call x2check(x2g_eahf_batch(ctx, grids, ea, int(nea, c_int)), 'batched EaHF')
 
-    ! 5. one transfer of the norms and orbital energies, then each job's
-    ! bookkeeping in the reference's orbital order
-    allocate(recs(nj), cnt(nj), recq(nj))
-    nrec = 0
+    ! 5. the end-of-iteration control of every job in one launch
+    ! (x2g_scf_control, on the records the norms and energies went to), then
+    ! one transfer of the control records; a job with host_control set
+    ! transfers its norms and energies instead and runs the host rules
+    allocate(recs(nj), cnt(nj), recq(nj), cptr(nj), rptr(nj))
+    nrec = 0; ndc = 0
     do q = 1, nj
        if (stopped(q)) cycle
+       h = idx(q)
        nrec = nrec + 1
-       recs(nrec) = js(idx(q))%rec
-       cnt(nrec) = int(2*MAXO, c_int)
        recq(nrec) = q
+       if (js(h)%o%host_control == 0) then
+          ndc = ndc + 1
+          cptr(ndc) = js(h)%ctrl
+          rptr(ndc) = js(h)%rec
+          recs(nrec) = js(h)%ctrl
+          cnt(nrec) = int(c_sizeof(cc)/8, c_int)
+       else
+          recs(nrec) = js(h)%rec
+          cnt(nrec) = int(2*MAXO, c_int)
+       endif
     enddo
-    allocate(rech(2*MAXO*nrec))
+    if (ndc > 0) call x2check(x2g_scf_control(ctx, cptr, rptr, int(ndc, c_int)), 'SCF control')
+    allocate(rech(max(1, sum(cnt(1:nrec)))))
     call x2check(x2g_gather(ctx, recs, cnt, int(nrec, c_int), rech), 'gather')
+    base = 0
     do n = 1, nrec
        q = recq(n)
        h = idx(q)
-       base = 2*MAXO*(n-1)
-       do ii = 1, js(h)%c%norb
-          i = js(h)%c%norb - ii + 1
-          if (js(h)%ifixorb(i) == 0) js(h)%orbnorm(i) = rech(base + i)
-          js(h)%ee(i,i) = rech(base + MAXO + i)
-          call energy_change(js(h), i)
-       enddo
-       stopped(q) = scf_control(js(h))
+       if (js(h)%o%host_control == 0) then
+          cc = transfer(rech(base + 1:base + cnt(n)), cc)
+          call ctrl_unpack(js(h), cc)
+          stopped(q) = cc%stop_flag /= 0
+       else
+          do ii = 1, js(h)%c%norb
+             i = js(h)%c%norb - ii + 1
+             if (js(h)%ifixorb(i) == 0) js(h)%orbnorm(i) = rech(base + i)
+             js(h)%ee(i,i) = rech(base + MAXO + i)
+             call energy_change(js(h), i)
+          enddo
+          stopped(q) = scf_control(js(h))
+          js(h)%ctrl_dirty = .true.
+       endif
+       base = base + cnt(n)
     enddo
     call mm_refresh(js, idx, nj)
-    deallocate(items, tails, grids, no, ea, recs, cnt, recq, rech)
+    deallocate(items, tails, grids, no, ea, recs, cnt, recq, rech, cptr, rptr)
   end subroutine batched_iteration
+
+  ! The job's control state as the device control record (x2g_scf_ctrl):
+  ! the settings of the rules and the state the host rules keep.
+  subroutine ctrl_pack(j, c)
+    type(scf_job), intent(in) :: j
+    type(x2g_scf_ctrl), intent(out) :: c
+    integer :: i
+    c%ethld = 10.0_dp**dble(-j%o%ienterm)
+    c%nthld = 10.0_dp**dble(-j%o%inoterm)
+    c%thren4init = 10.0_dp**dble(-j%o%ienterm4init)
+    c%recalc_mm = j%o%recalc_mm
+    c%inc_thld = j%o%orb_energy_inc_thld
+    do i = 1, MAXO
+       c%ee(i) = j%ee(i,i)
+    enddo
+    c%eeold = j%eeold; c%deltaee = j%deltaee; c%orbnorm = j%orbnorm
+    c%eeconv = j%eeconv; c%eeconvprev = j%eeconvprev; c%dnmaxprev = j%dnmaxprev
+    c%deltaeemax = j%deltaeemax; c%deltanormmax = j%deltanormmax
+    c%norb = j%c%norb; c%maxscf = j%o%maxscf; c%nscf2skip = j%o%nscf2skip
+    c%nenlast = j%o%nenlast; c%nnolast = j%o%nnolast; c%nscfextra = j%o%nscfextra
+    c%maxresets = j%o%maxresets; c%polish = j%o%polish
+    c%ifixorb = j%ifixorb; c%itouch = j%itouch; c%mm_touch = j%mm_touch
+    c%iscf = j%iscf; c%nresets = j%nresets; c%nresets4no = j%nresets4no
+    c%norbenergythld = j%norbenergythld; c%norbnormthld = j%norbnormthld
+    c%noenergydec = j%noenergydec; c%nonormdec = j%nonormdec
+    c%lenergythld = merge(1, 0, j%lenergythld); c%lnormthld = merge(1, 0, j%lnormthld)
+    c%lfixorb = merge(1, 0, j%lfixorb); c%lfixorb4init = merge(1, 0, j%lfixorb4init)
+    c%mm_due = merge(1, 0, j%mm_due); c%prec = j%o%prec; c%polish_end = j%polish_end
+    c%stopcode = j%stopcode; c%stop_flag = 0
+  end subroutine ctrl_pack
+
+  ! the state of a device control record back into the job
+  subroutine ctrl_unpack(j, c)
+    type(scf_job), intent(inout) :: j
+    type(x2g_scf_ctrl), intent(in) :: c
+    integer :: i
+    do i = 1, MAXO
+       j%ee(i,i) = c%ee(i)
+    enddo
+    j%eeold = c%eeold; j%deltaee = c%deltaee; j%orbnorm = c%orbnorm
+    j%eeconv = c%eeconv; j%eeconvprev = c%eeconvprev; j%dnmaxprev = c%dnmaxprev
+    j%deltaeemax = c%deltaeemax; j%deltanormmax = c%deltanormmax
+    j%ifixorb = c%ifixorb; j%itouch = c%itouch; j%mm_touch = c%mm_touch
+    j%iscf = c%iscf; j%nresets = c%nresets; j%nresets4no = c%nresets4no
+    j%norbenergythld = c%norbenergythld; j%norbnormthld = c%norbnormthld
+    j%noenergydec = c%noenergydec; j%nonormdec = c%nonormdec
+    j%lenergythld = c%lenergythld /= 0; j%lnormthld = c%lnormthld /= 0
+    j%lfixorb = c%lfixorb /= 0; j%lfixorb4init = c%lfixorb4init /= 0
+    j%mm_due = c%mm_due /= 0; j%o%prec = c%prec; j%polish_end = c%polish_end
+    j%stopcode = c%stopcode
+    j%iasympt = j%c%norb
+  end subroutine ctrl_unpack
+
+  ! write the job's device control record from its host state
+  subroutine ctrl_upload(j)
+    type(scf_job), intent(inout) :: j
+    type(x2g_scf_ctrl), target :: c
+    call ctrl_pack(j, c)
+    call x2check(x2g_h2d(j%ctx, j%ctrl, c_loc(c), c_sizeof(c)), 'x2g_h2d')
+    j%ctrl_dirty = .false.
+  end subroutine ctrl_upload
 
   ! The host bookkeeping of the SCF loop (energy_change + convergence_control)
   ! driven by given per-iteration orbital energies ee(norb, nit) and norms

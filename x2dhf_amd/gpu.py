@@ -76,6 +76,25 @@ class Lcao(C.Structure):
                 ("c1", _dbl), ("za", _dbl), ("c2", _dbl), ("zb", _dbl)]
 
 
+CTRL_MAXO = 16
+
+
+class ScfCtrl(C.Structure):
+    """x2g_scf_ctrl"""
+    _fields_ = [("ethld", _dbl), ("nthld", _dbl), ("thren4init", _dbl), ("recalc_mm", _dbl), ("inc_thld", _dbl),
+                ("ee", _dbl * CTRL_MAXO), ("eeold", _dbl * CTRL_MAXO), ("deltaee", _dbl * CTRL_MAXO),
+                ("orbnorm", _dbl * CTRL_MAXO),
+                ("eeconv", _dbl), ("eeconvprev", _dbl), ("dnmaxprev", _dbl), ("deltaeemax", _dbl),
+                ("deltanormmax", _dbl),
+                ("norb", _int), ("maxscf", _int), ("nscf2skip", _int), ("nenlast", _int), ("nnolast", _int),
+                ("nscfextra", _int), ("maxresets", _int), ("polish", _int),
+                ("ifixorb", _int * CTRL_MAXO), ("itouch", _int * CTRL_MAXO), ("mm_touch", _int * CTRL_MAXO),
+                ("iscf", _int), ("nresets", _int), ("nresets4no", _int), ("norbenergythld", _int),
+                ("norbnormthld", _int), ("noenergydec", _int), ("nonormdec", _int),
+                ("lenergythld", _int), ("lnormthld", _int), ("lfixorb", _int), ("lfixorb4init", _int),
+                ("mm_due", _int), ("prec", _int), ("polish_end", _int), ("stopcode", _int), ("stop", _int)]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "x2g_version": (_int, []),
@@ -126,6 +145,8 @@ _PROTOS = {
     "x2g_moments_batch": (_int, [_vp, _vp, C.POINTER(MomentItem), _int, _int, _vp]),
     "x2g_pot_tail_batch": (_int, [_vp, _vp, C.POINTER(TailItem), _int, _int]),
     "x2g_gather": (_int, [_vp, _vp, _vp, _int, _vp]),
+    "x2g_scf_control": (_int, [_vp, _vp, _vp, _int]),
+    "x2g_scf_ctrl_bytes": (_int, []),
 }
 
 _lib = None

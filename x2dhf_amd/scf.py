@@ -31,7 +31,8 @@ class JobSpec(C.Structure):
                 ("n2", _i * MAXO), ("l2", _i * MAXO),
                 ("occ", _d * MAXO), ("c1", _d * MAXO), ("c2", _d * MAXO), ("za", _d * MAXO), ("zb", _d * MAXO),
                 ("nscf2skip", _i), ("nenlast", _i), ("nnolast", _i), ("polish", _i),
-                ("orb_method", _i), ("pot_method", _i), ("ordering", _i), ("alt_sweeps", _i)]
+                ("orb_method", _i), ("pot_method", _i), ("ordering", _i), ("alt_sweeps", _i),
+                ("host_control", _i)]
 
 
 class JobResult(C.Structure):
@@ -126,7 +127,7 @@ class Orbital:
 def make_spec(z1, z2, r, nni, nmu, rinf, orbitals, maxscf=3000, ienterm=10, inoterm=8, omega=(-1.0, -1.0),
               prec=0, schedule=0, device=0, maxsor=(1, 10), mpole=4, nscfextra=3, ienterm4init=1,
               maxsorpot=None, recalc_mm=1.15, conv=(0, 0, 0), polish=0, orb_method=0, pot_method=0, ordering=0,
-              alt_sweeps=0):
+              alt_sweeps=0, host_control=0):
     """Job specification.  maxsor / maxsorpot: (macro-iterations, sweeps) of
     the orbital / potential relaxations (maxsorpot defaults to maxsor);
     conv: (nscf2skip, nenlast, nnolast) of the saturation test, 0 = default;
@@ -137,7 +138,10 @@ def make_spec(z1, z2, r, nni, nmu, rinf, orbitals, maxscf=3000, ienterm=10, inot
     relaxation scheme of the FP64 sweeps (0/0/0 is the
     pipelined col-wise colour sweep; the FP32 correction sweeps are always
     col-wise MCSOR); alt_sweeps: the altsweeps option, SOR sweeps backward
-    for the odd orbitals (doSCF.F90:403-411; no effect on MCSOR)."""
+    for the odd orbitals (doSCF.F90:403-411; no effect on MCSOR);
+    host_control: run the batched schedule's end-of-iteration control with
+    the host rules instead of on the device (x2g_scf_control); the plain
+    schedule always uses the host rules."""
     s = JobSpec()
     s.nni, s.nmu, s.norb = nni, nmu, len(orbitals)
     s.maxscf, s.ienterm, s.inoterm, s.nscfextra, s.ienterm4init = maxscf, ienterm, inoterm, nscfextra, ienterm4init
@@ -151,6 +155,7 @@ def make_spec(z1, z2, r, nni, nmu, rinf, orbitals, maxscf=3000, ienterm=10, inot
     s.polish = polish
     s.orb_method, s.pot_method, s.ordering = orb_method, pot_method, ordering
     s.alt_sweeps = 1 if alt_sweeps else 0
+    s.host_control = 1 if host_control else 0
     for k, o in enumerate(orbitals):
         s.m[k], s.gu[k], s.occ[k] = o.m, o.gu, o.occ
         s.c1[k], s.n1[k], s.l1[k], s.za[k] = o.c1, o.n1, o.l1, o.za
