@@ -979,7 +979,11 @@ extern "C" int x2g_pot_tail_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2
 // doSCF.F90:546-744 with the failure guards and the FP64 finish), in the
 // same order of comparisons and assignments, so every decision is the host
 // rules' decision bit for bit (only comparisons, one subtraction per orbital
-// and one division; no contraction is formed).
+// and one division; no contraction is formed).  A workgroup per job copies
+// the record into LDS with all its lanes, one lane applies the rules there
+// (a private copy with indexed arrays would live in scratch memory), and
+// the lanes write it back.
+namespace x2g {
 struct CtrlDesc {
     x2g_scf_ctrl *c;
     const double *rec;
@@ -1117,13 +1121,9 @@ __device__ static bool ctrl_rules(x2g_scf_ctrl &c)
     return false;
 }
 
-__global__ void k_scf_control(const CtrlDesc *__restrict__ cd, int n)
+__device__ static void ctrl_step(x2g_scf_ctrl &c, const double *rec)
 {
 #pragma clang fp contract(off)
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    x2g_scf_ctrl c = *cd[k].c;
-    const double *rec = cd[k].rec;
     c.norb = max(0, min(c.norb, X2G_CTRL_MAXO));
     // the iteration: every orbital was touched; the new norms (of the
     // orbitals not frozen) and energies, and their changes
@@ -1164,23 +1164,42 @@ __global__ void k_scf_control(const CtrlDesc *__restrict__ cd, int n)
         }
     }
     c.stop = stop ? 1 : 0;
-    *cd[k].c = c;
 }
+
+constexpr int kCtrlWords = (int)(sizeof(x2g_scf_ctrl) / sizeof(int));
+static_assert(sizeof(x2g_scf_ctrl) % sizeof(double) == 0, "x2g_scf_ctrl: whole doubles");
+
+__global__ void __launch_bounds__(64) k_scf_control(const CtrlDesc *__restrict__ cd)
+{
+    __shared__ x2g_scf_ctrl sc;
+    __shared__ double srec[2 * X2G_CTRL_MAXO];
+    const CtrlDesc d = cd[blockIdx.x];
+    int *w = reinterpret_cast<int *>(&sc);
+    const int *src = reinterpret_cast<const int *>(d.c);
+    for (int i = threadIdx.x; i < kCtrlWords; i += blockDim.x) w[i] = src[i];
+    if (threadIdx.x < 2 * X2G_CTRL_MAXO) srec[threadIdx.x] = d.rec[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) ctrl_step(sc, srec);
+    __syncthreads();
+    int *dst = reinterpret_cast<int *>(d.c);
+    for (int i = threadIdx.x; i < kCtrlWords; i += blockDim.x) dst[i] = w[i];
+}
+}  // namespace x2g
 
 extern "C" int x2g_scf_control(x2g_ctx *ctx, x2g_scf_ctrl *const *ctrl_d, const double *const *rec_d, int n)
 {
     X2G_CHECK(ctx && n >= 0 && (n == 0 || (ctrl_d && rec_d)), "x2g_scf_control: null argument");
     if (n == 0) return X2G_OK;
-    std::vector<CtrlDesc> v(n);
+    std::vector<x2g::CtrlDesc> v(n);
     for (int k = 0; k < n; ++k) {
         X2G_CHECK(ctrl_d[k] && rec_d[k], "x2g_scf_control: job %d has a null record", k);
         v[k] = {ctrl_d[k], rec_d[k]};
     }
     size_t off = 0;
-    const CtrlDesc *dd;
+    const x2g::CtrlDesc *dd;
     int rc = stage(ctx, v.data(), v.size(), off, &dd);
     if (rc) return rc;
-    X2G_LAUNCH(ctx, k_scf_control, dim3((n + 63) / 64), dim3(64), 0, dd, n);
+    X2G_LAUNCH(ctx, x2g::k_scf_control, dim3(n), dim3(64), 0, dd);
     X2G_HIP(hipGetLastError());
     return X2G_OK;
 }
