@@ -148,7 +148,7 @@ def _batched(specs):
             j.finish()
         return [j.result() for j in jobs]
     finally:
-        for j in jobs:
+        for j in reversed(jobs):
             j.close()
 
 

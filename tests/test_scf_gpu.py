@@ -265,3 +265,26 @@ def test_reference_be_continuation_matches_listing():
           f"iterations {res['iterations']}, stop '{res['stop']}'")
     assert res["iterations"] == 10
     assert abs(res["etot"] - ref["etot"]) < TOL_E
+
+
+def test_shared_context_outlives_the_job_that_created_it():
+    """Jobs that share a context may be destroyed in any order: destroying
+    the job that created the context hands it to a job still using it, which
+    then runs to the energy it has alone and frees the context last."""
+    from x2dhf_amd import scf
+    spec = lambda: scf.h2_spec(61, 91, 40.0, schedule=1, ienterm=11, inoterm=11, maxscf=2000)
+    alone = scf.Job(spec())
+    try:
+        want = run_batched([alone])[0]
+    finally:
+        alone.close()
+    owner = scf.Job(spec())
+    other = scf.Job(spec(), share=owner)
+    try:
+        owner.close()
+        got = run_batched([other])[0]
+    finally:
+        other.close()
+        owner.close()
+    print(f"alone E={want['etot']:.15f}, after the owner was destroyed E={got['etot']:.15f}")
+    assert got["iterations"] == want["iterations"] and got["etot"] == want["etot"]

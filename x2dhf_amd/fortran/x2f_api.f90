@@ -432,10 +432,24 @@ contains
     x2f_job_bytes = int(job_bytes(jobs(handle)), c_long_long)
   end function x2f_job_bytes
 
+  ! Destroying the job that created a shared context while other jobs still
+  ! use it hands the context to one of them, which destroys it in turn; the
+  ! context ends with the last job that uses it, in any order of destruction.
   integer(c_int) function x2f_job_destroy(handle) bind(c, name='x2f_job_destroy') result(rc)
     integer(c_int), value :: handle
+    integer :: k
     rc = 1
     if (.not. valid(handle)) return
+    if (jobs(handle)%own_ctx) then
+       do k = 1, MAXJOBS
+          if (k == handle .or. .not. used(k)) cycle
+          if (c_associated(jobs(k)%ctx, jobs(handle)%ctx)) then
+             jobs(k)%own_ctx = .true.
+             jobs(handle)%own_ctx = .false.
+             exit
+          endif
+       enddo
+    endif
     call job_free(jobs(handle))
     used(handle) = .false.
     rc = 0
