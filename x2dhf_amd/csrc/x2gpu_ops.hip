@@ -403,32 +403,47 @@ __device__ __forceinline__ double psi_sym(const NoDesc &d, long e)
 // blockIdx.z = 0: s = sum w*(psi*psi*f) over the orbital as it arrives (the
 // reference normalises before imposing the symmetry); blockIdx.z = 1+q:
 // sum w*(psi_sym*oth_q*f) with the symmetry imposed.
+// Terms k0 .. k0+kNoGroup-1 of one item per workgroup (blockIdx.z = group):
+// term 0 is <psi|psi>, term k >= 1 the overlap with other k-1.  The terms of
+// a group share one pass over psi, f and w; each term's partial sums the
+// same elements in the same order as a workgroup of its own would.
+constexpr int kNoGroup = 4;
 __global__ void __launch_bounds__(kRedBlock) k_no_dot(const NoDesc *__restrict__ nd, double *partial)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[4 * kNoGroup];
     const NoDesc &d = nd[blockIdx.y];
     if (!d.normalise) return;
-    const int k = blockIdx.z;
-    if (k > d.noth) return;
-    double s[1] = {0.0};
-    if (k == 0) {
-        for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
-            double x = gmem(d.psi)[e];
-            x *= gmem(d.psi)[e];
-            x *= gmem(d.f)[e];
-            s[0] += gmem(d.w)[e] * x;
-        }
-    } else {
-        const double *b = d.oth[k - 1];
-        for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
-            double x = psi_sym(d, e);
-            x *= gmem(b)[e];
-            x *= gmem(d.f)[e];
-            s[0] += gmem(d.w)[e] * x;
+    const int k0 = blockIdx.z * kNoGroup;
+    if (k0 > d.noth) return;
+    const int nk = min(kNoGroup, d.noth + 1 - k0);
+    const bool self = k0 == 0, overlaps = k0 + nk > 1;
+    const double *oth[kNoGroup];
+#pragma unroll
+    for (int g = 0; g < kNoGroup; ++g) oth[g] = g < nk && k0 + g >= 1 ? d.oth[k0 + g - 1] : nullptr;
+    double s[kNoGroup] = {0.0, 0.0, 0.0, 0.0};
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (long)gridDim.x * blockDim.x) {
+        const double f = gmem(d.f)[e], w = gmem(d.w)[e];
+        const double p = self ? gmem(d.psi)[e] : 0.0;
+        const double ps = overlaps ? psi_sym(d, e) : 0.0;
+#pragma unroll
+        for (int g = 0; g < kNoGroup; ++g) {
+            if (g >= nk) break;
+            double x;
+            if (self && g == 0) {
+                x = p;
+                x *= p;
+            } else {
+                x = ps;
+                x *= gmem(oth[g])[e];
+            }
+            x *= f;
+            s[g] += w * x;
         }
     }
-    block_sum<1>(s, sh);
-    if (threadIdx.x == 0) partial[((long)blockIdx.y * (kMaxOrtho + 1) + k) * gridDim.x + blockIdx.x] = s[0];
+    block_sum<kNoGroup>(s, sh);
+    if (threadIdx.x == 0)
+        for (int g = 0; g < nk; ++g)
+            partial[((long)blockIdx.y * (kMaxOrtho + 1) + k0 + g) * gridDim.x + blockIdx.x] = s[g];
 }
 
 // The update psi <- c0 psi_sym + sum_k c_k other_k (with normalise = 0: psi
@@ -815,7 +830,8 @@ extern "C" int x2g_normortho_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x
     size_t off = 0;
     const NoDesc *dd;
     if ((rc = stage(ctx, v.data(), v.size(), off, &dd))) return rc;
-    X2G_LAUNCH(ctx, k_no_dot, dim3(kRedBlocks, n, 1 + maxoth), dim3(kRedBlock), 0, dd, partial);
+    X2G_LAUNCH(ctx, k_no_dot, dim3(kRedBlocks, n, (1 + maxoth + kNoGroup - 1) / kNoGroup), dim3(kRedBlock), 0, dd,
+               partial);
     X2G_HIP(hipGetLastError());
     X2G_LAUNCH(ctx, k_no_apply, dim3(gx(maxn, kRedBlock), n), dim3(kRedBlock), 0, dd, partial, kRedBlocks);
     X2G_HIP(hipGetLastError());

@@ -1434,19 +1434,18 @@ struct CorrDesc {
     int ps;             // row stride of e
 };
 
-// u += e on the relaxed interior.
-// u += e over the relaxed points (32-bit index arithmetic: a grid has fewer
-// than 2^31 points; global loads and stores).
+// u += e over the relaxed points, a row per workgroup at a time (row
+// r: mu = 2 + r; column c: nu = 2 + c; 32-bit index arithmetic: a grid has
+// fewer than 2^31 points; global loads and stores).
 __global__ void __launch_bounds__(256) k_correct(const CorrDesc *__restrict__ cd)
 {
     const CorrDesc &d = cd[blockIdx.y];
-    const int nni = d.nni, w = nni - 2, s1 = d.ps;
-    const int tot = w * (d.nmu - 5);
+    const int nni = d.nni, w = nni - 2, s1 = d.ps, rows = d.nmu - 5;
     const gfptr e = gmem(d.e);
     const gdptr_w u = gmem_w(d.fun);
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += gridDim.x * blockDim.x) {
-        const int r = k / w, c = k - r * w;   // mu = 2 + r, nu = 2 + c
-        u[(1 + r) * nni + 1 + c] += (double)e[(5 + r) * s1 + 5 + c];
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int ub = (1 + r) * nni + 1, eb = (5 + r) * s1 + 5;
+        for (int c = threadIdx.x; c < w; c += blockDim.x) u[ub + c] += (double)e[eb + c];
     }
 }
 
@@ -3064,7 +3063,7 @@ static int relax_batch_eager(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_rel
             X2G_LAUNCH(ctx, k_zero_f32, dim3(std::min(grid_x_for(maxzero, 256), 128), nzero), dim3(256), 0, d_zero);
             X2G_HIP(hipGetLastError());
             if ((rc = launch_sweeps<float>(ctx, d32e, d32o, p32, maxsor2, /*a_zero=*/true))) return rc;
-            X2G_LAUNCH(ctx, k_correct, dim3(grid_x_for(maxn, 256), n32), dim3(256), 0, d_corr);
+            X2G_LAUNCH(ctx, k_correct, dim3((int)std::min<long>(maxnmu, 1024), n32), dim3(256), 0, d_corr);
             X2G_HIP(hipGetLastError());
             X2G_LAUNCH(ctx, k_boundary_fun, dim3(grid_x_for(2 * maxn / std::max<long>(maxnni, 1) + maxnni, 256), n32),
                                dim3(256), 0, d_bnd);
