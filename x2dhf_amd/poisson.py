@@ -164,7 +164,7 @@ class Batch:
         return calls, res
 
     def solved(self, res, tol=1e-12, floors=None):
-        """Per-problem verdict: residual <= max(tol, FLOOR_FACTOR * floor)."""
+        """Per-problem outcome: residual <= max(tol, FLOOR_FACTOR * floor)."""
         res = np.asarray(res)
         if floors is None:
             floors = self.floors([i for i in range(len(res)) if res[i] > tol])
