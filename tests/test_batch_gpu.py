@@ -174,3 +174,24 @@ def test_gather(env):
     cnt = (C.c_int * 2)(5, 3)
     gpu.check(ctx.lib.x2g_gather(ctx.h, src, cnt, 2, out.ctypes.data))
     assert np.array_equal(out, [0, 1, 2, 3, 4, 10, 11, 12])
+
+
+def test_empty_batches_are_no_ops(env):
+    """Every batched entry point accepts an empty batch (n = 0: a server step
+    with no job in some phase) and returns success without touching the
+    device or its arguments."""
+    from x2dhf_amd import gpu
+    ctx, gs = env
+    g = grids_arg(gs)
+    out = np.full(4, 7.0)
+    assert ctx.lib.x2g_relax_batch(ctx.h, g, None, 0, 1, 10) == 0
+    assert ctx.lib.x2g_residual_norms(ctx.h, g, None, 0, out.ctypes.data) == 0
+    assert ctx.lib.x2g_lincomb_batch(ctx.h, g, None, 0, None, 0) == 0
+    assert ctx.lib.x2g_normortho_batch(ctx.h, g, None, 0) == 0
+    assert ctx.lib.x2g_eahf_batch(ctx.h, g, None, 0) == 0
+    assert ctx.lib.x2g_moments_batch(ctx.h, g, None, 0, 4, out.ctypes.data) == 0
+    assert ctx.lib.x2g_pot_tail_batch(ctx.h, g, None, 0, 4) == 0
+    assert ctx.lib.x2g_gather(ctx.h, None, None, 0, out.ctypes.data) == 0
+    assert ctx.lib.x2g_scf_control(ctx.h, None, None, 0) == 0
+    ctx.sync()
+    assert np.array_equal(out, np.full(4, 7.0))

@@ -2750,7 +2750,7 @@ static inline bool ordered(const x2g_relax_item &it)
 static int relax_batch_eager(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
                              int maxsor1, int maxsor2)
 {
-    X2G_CHECK(ctx && grids && items && n >= 0, "x2g_relax_batch: null argument");
+    X2G_CHECK(ctx && grids && (items || n == 0) && n >= 0, "x2g_relax_batch: null argument");
     X2G_CHECK(maxsor1 >= 0 && maxsor2 >= 0, "x2g_relax_batch: negative sweep counts");
     if (n == 0 || maxsor1 == 0) return X2G_OK;
     int rc = set_lds_attr(ctx);
@@ -3202,7 +3202,7 @@ static int relax_capture(x2g_ctx *ctx, const std::string &key, x2g_grid *const *
 extern "C" int x2g_relax_batch(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
                                int maxsor1, int maxsor2)
 {
-    X2G_CHECK(ctx && grids && items && n >= 0, "x2g_relax_batch: null argument");
+    X2G_CHECK(ctx && grids && (items || n == 0) && n >= 0, "x2g_relax_batch: null argument");
     // statistics time launches with events on the stream: eager only
     if (!ctx->graphs_on || ctx->stats_on || n == 0 || maxsor1 == 0)
         return relax_batch_eager(ctx, grids, items, n, maxsor1, maxsor2);
@@ -3266,7 +3266,7 @@ extern "C" long long x2g_relax_workspace_bytes(int nni, int nmu, int prec)
 extern "C" int x2g_residual_norms(x2g_ctx *ctx, x2g_grid *const *grids, const x2g_relax_item *items, int n,
                                   double *out_h)
 {
-    X2G_CHECK(ctx && grids && items && out_h && n >= 0, "x2g_residual_norms: null argument");
+    X2G_CHECK(ctx && grids && (items || n == 0) && out_h && n >= 0, "x2g_residual_norms: null argument");
     if (n == 0) return X2G_OK;
     long maxn = 0, maxnni = 0, maxnmu = 0;
     for (int i = 0; i < n; ++i) {
