@@ -537,9 +537,9 @@ __device__ __forceinline__ void sweep_segment(const SweepDesc<T> &d, const int a
                                               const bool zsrc, unsigned char *ks_smem);
 
 // Waves per SIMD the register allocation leaves room for, from the
-// workgroups the LDS lets share a CU: FP32 rings of stride <= 352 fit two
+// workgroups the LDS lets share a CU: FP32 rings of stride <= 384 fit two
 // 6-wave workgroups (4-row blocks), with 2-row blocks up to stride 256 three
-// and more, stride 352 two and stride 528 two of 8 waves.
+// and more, stride 384 two and stride 528 two of 8 waves.
 template <typename T, int W, int BR>
 constexpr int sweep_waves_per_eu()
 {
