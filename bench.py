@@ -49,12 +49,14 @@ GOLDEN = os.path.join(HERE, "tests", "golden", "scf_config2_n2_331x601.json")
 REF_EXE = os.path.join(HERE, "oracle", "_ref", "x2dhf_omp")
 TRAFFIC = os.path.join(HERE, "profiles", "traffic_sweep_n2_331x601.json")
 # the stream's dominant sweep launches: 2-row blocks of ring stride 528
-TRAFFIC_STREAM = os.path.join(HERE, "profiles", "traffic_sweep_stream_505x901.json")
+TRAFFIC_STREAM = os.path.join(HERE, "profiles", "traffic_sweep_stream_mix.json")
 # jobs the stream's server steps together (32 measured 3% faster than 16 on
 # the default stream; 64 the same as 32)
 STREAM_SLOTS = 32
-# problems of the stream traffic probe: (nni, nmu, count)
-STREAM_PROBE = ((505, 901, 12), (361, 601, 12))
+# problems of the stream traffic probe: (nni, nmu, count), a mix of the
+# stream's grid classes whose relaxed points (24.6 M, 394 MB of algorithmic
+# bytes per FP32 sweep) match the stream's average sweep launch
+STREAM_PROBE = ((505, 901, 28), (409, 721, 20), (361, 601, 20), (217, 361, 16), (169, 241, 16))
 HBM_PEAK_GBS = 8000.0
 METRIC = "converged SCF solves/hr + p50/p99 latency vs plain FP64 GPU at ΔE<1e-8 Eh; GB/s"
 
