@@ -73,6 +73,46 @@ def test_graph_replays_equal_eager_calls():
             assert np.array_equal(x, y), (call, k)
 
 
+def test_graph_replays_of_ordered_sor_equal_eager_calls():
+    """Items relaxed in the reference's sequential point order (k_levels:
+    SOR forward and backward, one-thread MCSOR, in the middle ordering) are
+    replayed from the captured graph with the eager results, bit for bit."""
+    from x2dhf_amd import gpu
+    res = {}
+    for graphs in (False, True):
+        ctx = gpu.Context(0)
+        try:
+            gpu.check(ctx.lib.x2g_set_graphs(ctx.h, 1 if graphs else 0))
+            probs = [synthetic_problem(n, m, seed=61 + k) for k, (n, m) in enumerate(SHAPES[:2])]
+            grids = [gpu.Grid(ctx, p["nni"], p["nmu"], p["coef"], p["suppl"], p["vxi"], p["veta"]) for p in probs]
+            gh = (gpu._vp * len(grids))(*[g.h.value for g in grids])
+            keep, items = [], []
+            for k, p in enumerate(probs):
+                for method in (1, 2, 0):
+                    f, lo, r = ctx.upload(p["fun"]), ctx.upload(p["lhs"]), ctx.upload(p["rhs"])
+                    keep.append((f, lo, r))
+                    it = gpu.RelaxItem(f.ptr, lo.ptr, r.ptr, None, 1.9, 1 if method != 2 else -1, 1, 0, k)
+                    it.method, it.ordering = method, 1
+                    items.append(it)
+            arr = (gpu.RelaxItem * len(items))(*items)
+            out = []
+            print(f"ordered items, graphs={graphs}", flush=True)
+            for call in range(4):
+                gpu.check(ctx.lib.x2g_relax_batch(ctx.h, gh, arr, len(items), 1, 3))
+                ctx.sync()
+                out.append([f.get() for f, _, _ in keep])
+            res[graphs] = (out, _counts(ctx))
+            for g in grids:
+                g.destroy()
+        finally:
+            ctx.close()
+    print(f"captures/replays {res[True][1]}")
+    assert res[False][1] == (0, 0) and res[True][1][0] == 1 and res[True][1][1] >= 2
+    for call, (a, b) in enumerate(zip(res[False][0], res[True][0])):
+        for k, (x, y) in enumerate(zip(a, b)):
+            assert np.array_equal(x, y), (call, k)
+
+
 def test_new_sweep_plan_is_captured_anew():
     """Changing the sweep plan mid-sequence makes a new signature: the graph of
     the old plan is not replayed, and the results still equal the eager ones."""
