@@ -32,9 +32,11 @@ def test_oracle_sweeps_bitexact_vs_reference(case, pfx):
     for mode, name, ns in [(0, "wout_mcsor", nsw), (0, "wout_mcsor1", 1), (1, "wout_sor", nsw)]:
         w = pyoracle.mcsor(nni, nmu, isym, z[pfx + "win"].copy(), *args, ns, mode)
         assert _ok(w, z[pfx + name]), (case, pfx, name, np.abs(w - z[pfx + name]).max())
-    # sor with the altsweeps backward sweep (isstep = -1, doSCF.F90:403-411)
+    # sor with the altsweeps backward sweep (isstep = -1, doSCF.F90:403-411); the
+    # reference's backward result is not its forward one, so the direction is pinned
     w = pyoracle.relax_ordered(nni, nmu, isym, z[pfx + "win"].copy(), *args, nsw, 2, "col-wise")
     assert _ok(w, z[pfx + "wout_sor_bwd"]), (case, pfx, np.abs(w - z[pfx + "wout_sor_bwd"]).max())
+    assert np.abs(z[pfx + "wout_sor_bwd"] - z[pfx + "wout_sor"]).max() > 1e-3
 
 
 @pytest.mark.parametrize("case", ["h2_61x91", "n2_61x91"])
