@@ -267,6 +267,34 @@ def test_reference_be_continuation_matches_listing():
     assert abs(res["etot"] - ref["etot"]) < TOL_E
 
 
+def test_reference_be_mcsor_continuation_converges_to_listing():
+    """be/set-12: MCSOR for the orbitals and SOR for the potentials in the
+    middle ordering, from the reference's converged Be checkpoint (the same
+    files as be/set-01) to the orbital-energy threshold 1e-10: the total
+    energy of its listing (reference.lst:454, after 92 iterations) within
+    1e-10 Eh.  The reference ran its MCSOR on 8 threads, whose passes race in
+    this ordering, so only the converged energy is comparable; this build
+    runs the sequential one-thread order."""
+    from x2dhf_amd import checkpoint as ckm, deck
+    ref = _listing("be_set-12")
+    d = deck.load(os.path.join(GOLDEN, "decks", ref["deck"]))
+    assert (d.orb_method, d.pot_method, d.ordering, d.nni, d.nmu) == (0, 1, 1, 151, 181)
+    ck = ckm.read(os.path.join(GOLDEN, "be-hf", "be-hf"), ms=[m for m, _ in d.orbitals])
+    job = ckm.resume(d.spec(), ck)
+    try:
+        for _ in range(d.maxscf):
+            if job.iterate(1):
+                break
+        job.finish()
+        res = job.result()
+    finally:
+        job.close()
+    print(f"be_set-12: E = {res['etot']:.16f} (listing {ref['etot']:.16f}) dE = {res['etot'] - ref['etot']:.2e}, "
+          f"iterations {res['iterations']} (listing 92), stop '{res['stop']}'")
+    assert res["stop"] == "orbital energy threshold"
+    assert abs(res["etot"] - ref["etot"]) < TOL_E
+
+
 def test_shared_context_outlives_the_job_that_created_it():
     """Jobs that share a context may be destroyed in any order: destroying
     the job that created the context hands it to a job still using it, which
