@@ -32,17 +32,40 @@ contains
   ! Build the grid of an (nni x nmu) mesh with practical infinity rinf for
   ! nuclei z1 (at nu = 0) and z2 (at nu = pi) a distance r apart.
   ! Returns 0, or 1 when the sizes do not fit the 7-point quadrature.
+  !
+  ! The relaxation solves, in xi = cosh(mu), eta = cos(nu), the equations
+  ! multiplied through by the metric factor r^2 (xi^2 - eta^2)/4, which
+  ! leaves d2/dmu2 + d2/dnu2 with unit weight.  With q = xi^2 - eta^2 the
+  ! 14 coefficient arrays are, point by point:
+  !   borb   coth(mu)                         d/dmu of an orbital
+  !   bpot   coth(mu) - 2 tanh(mu)            d/dmu of a potential
+  !   d      cot(nu)                          d/dnu
+  !   e      -1/sinh^2(mu) - 1/sin^2(nu)      the m^2 term
+  !   f0     r (xi (z1+z2) + eta (z2-z1))     (z1/r1 + z2/r2) r^2 q/2
+  !   f1     r^2 q/2                          the orbital energy's factor
+  !   f2     -r q/xi
+  !   f3     -2/xi^2 + the centre weight of both 9-point second differences
+  !   f4     r xi/2
+  !   g      -pi r^3 xi q/2                   the Poisson source's factor
+  !   wjac1  -(pi r/2) sinh(mu) sin(nu)                      volume element
+  !   wjac2  (pi r^2/2) sinh(mu) sin(nu) (sinh^2 mu + sin^2 nu)/xi
+  ! wjac1, wjac2 carry the mu quadrature weights; wgt1, wgt2 are those times
+  ! the nu weights.  On the axis row mu = 0, bpot and f2 are zero and e keeps
+  ! only its nu term; where sin(nu) = 0, d and e are zero.  Every product is
+  ! evaluated in the same order as the reference's initSuppl
+  ! (initCBAllocArrays.F90:1755-1990), so the arrays agree with its to the
+  ! last bit and the relaxation follows the same trajectory.
   integer function grid_build(g, nni, nmu, z1, z2, r, rinf) result(ierr)
     type(grid_t), intent(inout) :: g
     integer, intent(in) :: nni, nmu
     real(dp), intent(in) :: z1, z2, r, rinf
-    real(dp), parameter :: aa1(9) = [3.0_dp, -32.0_dp, 168.0_dp, -672.0_dp, 0.0_dp, &
-         672.0_dp, -168.0_dp, 32.0_dp, -3.0_dp]
-    real(dp), parameter :: aa2(9) = [-9.0_dp, 128.0_dp, -1008.0_dp, 8064.0_dp, -14350.0_dp, &
-         8064.0_dp, -1008.0_dp, 128.0_dp, -9.0_dp]
-    real(dp) :: xmi0, w1, w2, w4, w5, w6, w7, w8, w9, w10, wj1, wj2, xrr, wmup
-    real(dp), allocatable :: wmu(:), wni(:)
-    integer :: i, j, k, n
+    ! 9-point central differences: points k-4..k-1 of the first (/840 h) and
+    ! second (/5040 h^2) derivative; the centre of the second is -14350
+    real(dp), parameter :: c1(4) = [3.0_dp, -32.0_dp, 168.0_dp, -672.0_dp]
+    real(dp), parameter :: c2(4) = [-9.0_dp, 128.0_dp, -1008.0_dp, 8064.0_dp]
+    real(dp) :: x
+    real(dp), allocatable :: xi(:,:), eta(:,:), sh(:,:), sn(:,:), q(:,:), c(:,:), wmu(:), wni(:)
+    integer :: i
 
     ierr = 0
     if (nni < 13 .or. nmu < 13 .or. mod(nni-1, 6) /= 0 .or. mod(nmu-1, 6) /= 0) then
@@ -50,151 +73,110 @@ contains
        return
     endif
     g%nni = nni; g%nmu = nmu; g%z1 = z1; g%z2 = z2; g%r = r; g%rinf = rinf
-    n = nni*nmu
     if (allocated(g%vmu)) deallocate(g%vmu, g%vxi, g%vxisq, g%vxi1, g%vxi2, g%vni, g%veta, &
          g%vetasq, g%veta1, g%veta2, g%suppl)
     allocate(g%vmu(nmu), g%vxi(nmu), g%vxisq(nmu), g%vxi1(nmu), g%vxi2(nmu))
     allocate(g%vni(nni), g%veta(nni), g%vetasq(nni), g%veta1(nni), g%veta2(nni))
-    allocate(g%suppl(n, 14), wmu(nmu), wni(nni))
+    allocate(g%suppl(nni*nmu, 14))
 
-    ! steps: h_nu = pi/(nni-1); h_mu from the practical infinity
-    ! (initCBAllocArrays.F90:813-822)
+    ! steps: h_nu = pi/(nni-1); the last mu node sits at the practical
+    ! infinity, xi_max = 2 rinf/r (initCBAllocArrays.F90:813-822)
     g%hni = pii/dble(nni-1)
-    xmi0 = 2.0_dp*rinf/r
-    xmi0 = log(xmi0 + sqrt(xmi0*xmi0 - 1.0_dp))
-    g%hmu = xmi0/dble(nmu-1)
+    x = 2.0_dp*rinf/r
+    g%hmu = log(x + sqrt(x*x - 1.0_dp))/dble(nmu-1)
 
-    ! mu, xi tables: mu accumulated step by step (initCBAllocArrays.F90:826-844)
+    ! axis tables (initCBAllocArrays.F90:826-860).  mu is accumulated node by
+    ! node, as the reference's mesh is, not formed as (j-1)*h_mu.
     g%vmu(1) = 0.0_dp
-    g%vxi(1) = cosh(g%vmu(1))
-    g%vxisq(1) = g%vxi(1)*g%vxi(1)
-    g%vxi1(1) = sqrt(g%vxi(1)*g%vxi(1) - 1.0_dp)
-    g%vxi2(1) = 0.0_dp
     do i = 2, nmu
        g%vmu(i) = g%vmu(i-1) + g%hmu
-       g%vxi(i) = cosh(g%vmu(i))
-       g%vxisq(i) = g%vxi(i)*g%vxi(i)
-       g%vxi1(i) = sqrt(g%vxi(i)*g%vxi(i) - 1.0_dp)
-       g%vxi2(i) = g%vxi(i)/g%vxi1(i)
     enddo
-    ! nu, eta tables (initCBAllocArrays.F90:848-860)
-    do i = 1, nni
-       g%vni(i) = dble(i-1)*g%hni
-       g%veta(i) = cos(g%vni(i))
-       g%vetasq(i) = g%veta(i)*g%veta(i)
-       g%veta1(i) = sqrt(1.0_dp - g%veta(i)*g%veta(i))
-       if (g%veta1(i) < epsilon(1.0_dp)) then
-          g%veta2(i) = 0.0_dp
-       else
-          g%veta2(i) = g%veta(i)/g%veta1(i)
-       endif
-    enddo
+    g%vxi = cosh(g%vmu)
+    g%vxisq = g%vxi*g%vxi
+    g%vxi1 = sqrt(g%vxi*g%vxi - 1.0_dp)              ! sinh(mu)
+    g%vxi2(1) = 0.0_dp
+    g%vxi2(2:) = g%vxi(2:)/g%vxi1(2:)                 ! coth(mu)
+    g%vni = [(dble(i-1)*g%hni, i = 1, nni)]
+    g%veta = cos(g%vni)
+    g%vetasq = g%veta*g%veta
+    g%veta1 = sqrt(1.0_dp - g%veta*g%veta)           ! sin(nu)
+    where (g%veta1 < epsilon(1.0_dp))
+       g%veta2 = 0.0_dp
+    elsewhere
+       g%veta2 = g%veta/g%veta1                       ! cot(nu)
+    end where
 
-    ! coefficient arrays (initSuppl, initCBAllocArrays.F90:1755-1903)
-    wj1 = -r*pii/2.0_dp
-    wj2 = r*r*pii/2.0_dp
-    xrr = 0.5_dp*r
-    g%diag = -14350.0_dp/(5040.0_dp)*(1.0_dp/(g%hmu*g%hmu) + 1.0_dp/(g%hni*g%hni))
-    do i = 1, nmu
-       w1 = g%vxi1(i)
-       w2 = g%vxi(i)*g%vxi(i)
-       w9 = xrr*g%vxi(i)
-       w4 = g%vxi2(i)
-       if (i == 1) then
-          w5 = 0.0_dp
-          w6 = 0.0_dp
-       else
-          w5 = g%vxi(i)/g%vxi1(i) - 2.0_dp*g%vxi1(i)/g%vxi(i)
-          w6 = 1.0_dp/(g%vxi1(i)*g%vxi1(i))
-       endif
-       w8 = -r*r*r*pii*g%vxi(i)/2.0_dp
-       do j = 1, nni
-          k = (i-1)*nni + j
-          w10 = g%veta1(j)
-          g%suppl(k, 1) = w4                                    ! borb
-          g%suppl(k, 2) = w5                                    ! bpot
-          g%suppl(k, 3) = g%veta2(j)                            ! d
-          if (w10 < epsilon(1.0_dp)) then
-             g%suppl(k, 4) = 0.0_dp                             ! e
-          else
-             g%suppl(k, 4) = -w6 - 1.0_dp/(w10*w10)
-          endif
-          w7 = w2 - g%veta(j)*g%veta(j)
-          g%suppl(k, 6) = r*r*w7/2.0_dp                         ! f1
-          g%suppl(k, 5) = r*(g%vxi(i)*(z1+z2) + g%veta(j)*(z2-z1))  ! f0, point nuclei
-          if (i == 1) then
-             g%suppl(k, 7) = 0.0_dp                             ! f2
-          else
-             g%suppl(k, 7) = -r*w7/g%vxi(i)
-          endif
-          g%suppl(k, 8) = -2.0_dp/w2 + g%diag                   ! f3 (+diag)
-          g%suppl(k, 10) = w8*w7                                ! g
-          g%suppl(k, 9) = w9                                    ! f4
-          g%suppl(k, 11) = wj1*w1*w10                           ! wjac1
-          g%suppl(k, 12) = wj2*w1*w10*(w1*w1 + w10*w10)/g%vxi(i) ! wjac2
-       enddo
-    enddo
+    ! the coefficient arrays as (nu, mu) fields
+    xi = spread(g%vxi, 1, nni);   eta = spread(g%veta, 2, nmu)
+    sh = spread(g%vxi1, 1, nni);  sn = spread(g%veta1, 2, nmu)
+    q = xi*xi - eta*eta
+    g%diag = -14350.0_dp/5040.0_dp*(1.0_dp/(g%hmu*g%hmu) + 1.0_dp/(g%hni*g%hni))
+    allocate(c(nni, nmu))
 
-    ! composite 7-point Newton-Cotes weights (initCBAllocArrays.F90:1917-1952)
-    wmup = 0.0_dp
-    w1 = g%hmu/140.0_dp
-    wmu(1) = wmup + w1*41.0_dp
-    do i = 2, nmu, 6
-       wmu(i) = w1*216.0_dp
-       wmu(i+1) = w1*27.0_dp
-       wmu(i+2) = w1*272.0_dp
-       wmu(i+3) = w1*27.0_dp
-       wmu(i+4) = w1*216.0_dp
-       if (i+5 == nmu) then
-          wmu(i+5) = w1*41.0_dp
-       else
-          wmu(i+5) = 2.0_dp*w1*41.0_dp
-       endif
-    enddo
-    w1 = g%hni/140.0_dp
-    wni(1) = w1*41.0_dp
-    do i = 2, nni, 6
-       wni(i) = w1*216.0_dp
-       wni(i+1) = w1*27.0_dp
-       wni(i+2) = w1*272.0_dp
-       wni(i+3) = w1*27.0_dp
-       wni(i+4) = w1*216.0_dp
-       if (i+5 == nni) then
-          wni(i+5) = w1*41.0_dp
-       else
-          wni(i+5) = 2.0_dp*w1*41.0_dp
-       endif
-    enddo
-    ! Jacobians times the mu weights, then the nu weights (initCBAllocArrays.F90:1955-1972)
-    do i = 1, nmu
-       w1 = wmu(i)
-       do j = 1, nni
-          k = (i-1)*nni + j
-          g%suppl(k, 11) = w1*g%suppl(k, 11)
-          g%suppl(k, 12) = w1*g%suppl(k, 12)
-       enddo
-    enddo
-    do i = 1, nmu
-       do j = 1, nni
-          k = (i-1)*nni + j
-          g%suppl(k, 13) = wni(j)*g%suppl(k, 11)
-          g%suppl(k, 14) = wni(j)*g%suppl(k, 12)
-       enddo
-    enddo
+    call put(1, spread(g%vxi2, 1, nni))
+    c(:, 1) = 0.0_dp
+    c(:, 2:) = xi(:, 2:)/sh(:, 2:) - 2.0_dp*sh(:, 2:)/xi(:, 2:)
+    call put(2, c)
+    call put(3, spread(g%veta2, 2, nmu))
+    ! e: the 1/sinh^2 term is left out on the axis row
+    c(:, 1) = 0.0_dp
+    c(:, 2:) = 1.0_dp/(sh(:, 2:)*sh(:, 2:))
+    where (sn < epsilon(1.0_dp))
+       c = 0.0_dp
+    elsewhere
+       c = -c - 1.0_dp/(sn*sn)
+    end where
+    call put(4, c)
+    call put(5, r*(xi*(z1+z2) + eta*(z2-z1)))
+    call put(6, r*r*q/2.0_dp)
+    c(:, 1) = 0.0_dp
+    c(:, 2:) = -r*q(:, 2:)/xi(:, 2:)
+    call put(7, c)
+    call put(8, -2.0_dp/(xi*xi) + g%diag)
+    call put(9, (0.5_dp*r)*xi)
+    call put(10, (-r*r*r*pii*xi/2.0_dp)*q)
 
-    ! 9-point first/second derivative weights (initCBAllocArrays.F90:1979-1990)
-    w1 = 1.0_dp/(840.0_dp)
-    w2 = 1.0_dp/(5040.0_dp)
-    do i = 1, 4
-       g%dmu2(i) = w2*aa2(i)/(g%hmu*g%hmu)
-       g%dmu1(i) = w1*aa1(i)/g%hmu
-       g%dni2(i) = w2*aa2(i)/(g%hni*g%hni)
-       g%dni1(i) = w1*aa1(i)/g%hni
-    enddo
+    ! volume elements with the composite 7-point Newton-Cotes weights
+    ! (initCBAllocArrays.F90:1917-1972)
+    wmu = newton_cotes7(nmu, g%hmu)
+    wni = newton_cotes7(nni, g%hni)
+    c = spread(wmu, 1, nni)*((-r*pii/2.0_dp)*sh*sn)
+    call put(11, c)
+    call put(13, spread(wni, 2, nmu)*c)
+    c = spread(wmu, 1, nni)*((r*r*pii/2.0_dp)*sh*sn*(sh*sh + sn*sn)/xi)
+    call put(12, c)
+    call put(14, spread(wni, 2, nmu)*c)
+
+    ! difference weights (initCBAllocArrays.F90:1979-1990)
+    g%dmu1 = (1.0_dp/840.0_dp)*c1/g%hmu
+    g%dmu2 = (1.0_dp/5040.0_dp)*c2/(g%hmu*g%hmu)
+    g%dni1 = (1.0_dp/840.0_dp)*c1/g%hni
+    g%dni2 = (1.0_dp/5040.0_dp)*c2/(g%hni*g%hni)
     ! boundary extrapolation weights of even functions (initCBAllocArrays.F90:2037-2041)
     g%exeven = [210.0_dp, -120.0_dp, 45.0_dp, -10.0_dp, 1.0_dp]/126.0_dp
-    deallocate(wmu, wni)
+  contains
+    subroutine put(k, f)
+      integer, intent(in) :: k
+      real(dp), intent(in) :: f(:,:)
+      g%suppl(:, k) = reshape(f, [nni*nmu])
+    end subroutine put
   end function grid_build
+
+  ! Weights of the composite 7-point closed Newton-Cotes rule, (41, 216, 27,
+  ! 272, 27, 216, 41) h/140 per panel, over n = 6k+1 nodes spaced h: nodes
+  ! shared by two panels take both panels' end weight.
+  pure function newton_cotes7(n, h) result(w)
+    integer, intent(in) :: n
+    real(dp), intent(in) :: h
+    real(dp) :: w(n)
+    real(dp), parameter :: cycle6(0:5) = [82.0_dp, 216.0_dp, 27.0_dp, 272.0_dp, 27.0_dp, 216.0_dp]
+    integer :: k
+    do k = 1, n
+       w(k) = (h/140.0_dp)*cycle6(mod(k-1, 6))
+    enddo
+    w(1) = (h/140.0_dp)*41.0_dp
+    w(n) = w(1)
+  end function newton_cotes7
 
   ! The 25-double coefficient block of x2g_grid_create.
   subroutine grid_coef(g, coef)
