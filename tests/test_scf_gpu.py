@@ -213,6 +213,35 @@ def test_reference_default_scheme_decks_match_listings(name):
     assert abs(res["etot"] - ref["etot"]) < TOL_E
 
 
+def test_ne2_deck_matches_reference_program():
+    """The reference's Ne2 deck (test-sets/ne2/set-03, 181x151): eight
+    orbitals with pi_g and pi_u among them under the homonuclear g/u parity,
+    a hydrogenic start whose orbitals stay frozen until the potentials
+    settle (read_orbpot's default ienterm4init = 1), SOR in the middle
+    ordering.  The reference program built from its sources runs 621
+    iterations to the saturation stop at E = -258.6296868755868
+    (tests/golden/scf_ne2_set03_181x151.json); the GPU path follows its first
+    40 iterations within 1e-9 relative and stops at the same iteration and
+    energy.  The listing kept beside that deck (E = -255.564 after 1000
+    iterations) was written by a program without the frozen start: its first
+    iteration already moves the orbitals (1-norm 5.2e-3), which the sources'
+    run does not do."""
+    from x2dhf_amd import deck
+    ref = golden("ne2_set03_181x151")
+    d = deck.load(os.path.join(GOLDEN, "decks", "ref_ne2_set-03.data"))
+    assert (d.nni, d.nmu) == (181, 151) and d.ienterm4init == 1
+    print(f"ne2_set-03: {d.nni}x{d.nmu}, {d.notes[-1]}", flush=True)
+    res, worst = run_plain(d.spec(prec=0, schedule=0), ref, check_iters=40)
+    print(f"ne2_set-03: E = {res['etot']:.13f} (ref {ref['etot']:.13f}) dE = {res['etot'] - ref['etot']:.2e}, "
+          f"iterations {res['iterations']} (ref {len(ref['iterations'])}), trajectory rel err {worst:.2e}, "
+          f"stop '{res['stop']}'", flush=True)
+    assert worst < 1e-9
+    assert res["iterations"] == len(ref["iterations"])
+    assert res["stop"] == "saturation"
+    assert abs(res["etot"] - ref["etot"]) < TOL_E
+    assert np.abs(res["eps"] - np.array(ref["eps"])).max() < 1e-9
+
+
 def test_altsweeps_deck_matches_reference_trajectory_and_energy():
     """FH 91x121 with the reference's default schemes (SOR for orbitals and
     potentials, middle ordering) and its altsweeps card: the SOR of the odd
