@@ -139,6 +139,16 @@ const double *x2g_grid_suppl(x2g_grid *g, int k);
  * multiple of 4), <= 0 lets the library choose. */
 int x2g_set_sweep_engine(int engine, int strip_rows);
 
+/* Schedule of ordered sweeps (x2g_relax_ordered, relax items with a method
+ * other than the col-wise MCSOR) for all later calls of this process:
+ * 1 = default: the sweeps pipelined, each a fixed number of dependency
+ * levels behind the one before, one barrier per step of all of them;
+ * 0 = one dependency level of one sweep per barrier (X2G_LEVELS=phase).
+ * Both give the sequential routine's result bit for bit (sormcsor.F90:20-132,
+ * 143-386 with one thread).  No counterpart in the reference: a choice of
+ * this library's schedule. */
+int x2g_set_levels_pipeline(int on);
+
 /* Sweep autotuner (on by default; X2G_AUTOTUNE=0 turns it off): on the first
  * relaxation of a batch shape (precision, ring class of the widest grid,
  * number of ring classes, total rows rounded to a power of sqrt 2) the

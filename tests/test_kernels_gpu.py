@@ -533,7 +533,8 @@ def test_ordered_relaxation_bitexact_vs_reference(ctx, case, pfx):
     """x2g_relax_ordered: sor (10 sweeps, forward and backward) and one-thread
     mcsor (10 and 1 sweeps) in the col-wise and the reference's default middle ordering
     reproduce the reference's own routines bit for bit (the level schedule
-    keeps its sequential order of dependencies; no fused multiply-adds)."""
+    keeps its sequential order of dependencies; no fused multiply-adds), with
+    the sweeps pipelined and one level of one sweep at a time."""
     from x2dhf_amd import gpu
     z = load_golden(case)
     if pfx + "win" not in z:
@@ -544,13 +545,19 @@ def test_ordered_relaxation_bitexact_vs_reference(ctx, case, pfx):
     isym = int(z[pfx + "isym"][0])
     omega, nsw = float(z[pfx + "scal"][0]), int(z[pfx + "scal"][1])
     lhs, rhs = ctx.upload(z[pfx + "lhs"]), ctx.upload(z[pfx + "rhs"])
-    for method, name, ns in ((1, "wout_sor", nsw), (2, "wout_sor_bwd", nsw), (0, "wout_mcsor", nsw),
-                             (0, "wout_mcsor1", 1)):
-        w = ctx.upload(z[pfx + "win"])
-        gpu.check(ctx.lib.x2g_relax_ordered(g.h, isym, 1, w.ptr, lhs.ptr, rhs.ptr, omega, ns, method, ordering))
-        got = w.get()
-        assert np.array_equal(got, z[pfx + name]), (name, relerr(got, z[pfx + name]))
-        w.free()
+    try:
+        for pipe in (1, 0):
+            gpu.check(ctx.lib.x2g_set_levels_pipeline(pipe))
+            for method, name, ns in ((1, "wout_sor", nsw), (2, "wout_sor_bwd", nsw), (0, "wout_mcsor", nsw),
+                                     (0, "wout_mcsor1", 1)):
+                w = ctx.upload(z[pfx + "win"])
+                gpu.check(ctx.lib.x2g_relax_ordered(g.h, isym, 1, w.ptr, lhs.ptr, rhs.ptr, omega, ns, method,
+                                                    ordering))
+                got = w.get()
+                assert np.array_equal(got, z[pfx + name]), (pipe, name, relerr(got, z[pfx + name]))
+                w.free()
+    finally:
+        gpu.check(ctx.lib.x2g_set_levels_pipeline(1))
     g.destroy()
 
 
@@ -575,13 +582,52 @@ def test_other_orderings_bitexact_vs_reference(ctx, ordering):
     isym = int(vec("isym")[0])
     omega, nsw = float(vec("scal")[0]), int(vec("scal")[1])
     lhs, rhs = ctx.upload(vec("lhs")), ctx.upload(vec("rhs"))
-    for method, name, ns in ((1, "wout_sor", nsw), (2, "wout_sor_bwd", nsw), (0, "wout_mcsor1", 1)):
-        w = ctx.upload(vec("win"))
-        gpu.check(ctx.lib.x2g_relax_ordered(g.h, isym, 1, w.ptr, lhs.ptr, rhs.ptr, omega, ns, method,
-                                            ORDER_CODES[ordering]))
-        got = w.get()
-        assert np.array_equal(got, vec(name)), (name, relerr(got, vec(name)))
-        w.free()
+    try:
+        for pipe in (1, 0):
+            gpu.check(ctx.lib.x2g_set_levels_pipeline(pipe))
+            for method, name, ns in ((1, "wout_sor", nsw), (2, "wout_sor_bwd", nsw), (0, "wout_mcsor1", 1)):
+                w = ctx.upload(vec("win"))
+                gpu.check(ctx.lib.x2g_relax_ordered(g.h, isym, 1, w.ptr, lhs.ptr, rhs.ptr, omega, ns, method,
+                                                    ORDER_CODES[ordering]))
+                got = w.get()
+                assert np.array_equal(got, vec(name)), (pipe, name, relerr(got, vec(name)))
+                w.free()
+    finally:
+        gpu.check(ctx.lib.x2g_set_levels_pipeline(1))
+    g.destroy()
+
+
+@pytest.mark.parametrize("shape", [(169, 241), (331, 601), (523, 151)])
+def test_pipelined_ordered_sweeps_vs_oracle(ctx, shape):
+    """The pipelined ordered sweeps at configuration sizes: sor forward and
+    backward (10 sweeps) and one-thread mcsor (3 sweeps), even and odd
+    symmetry, in the default middle, the col-wise and the test2 orderings,
+    equal the oracle's sequential routine bit for bit (and so does one level
+    of one sweep at a time, on the middle ordering)."""
+    from x2dhf_amd import gpu
+    nni, nmu = shape
+    P = synthetic_problem(nni, nmu, seed=3 * nni + nmu)
+    g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
+    lhs, rhs = ctx.upload(P["lhs"]), ctx.upload(P["rhs"])
+    args = (P["lhs"], P["rhs"], P["suppl"][1], P["suppl"][2], P["coef"][:16], P["coef"][16:21], 1.8)
+    try:
+        for pipe, orderings in ((1, ("middle", "col-wise", "test2")), (0, ("middle",))):
+            gpu.check(ctx.lib.x2g_set_levels_pipeline(pipe))
+            for ordering in orderings:
+                for isym in (1, -1):
+                    win = pyoracle.putin(nni, nmu, isym, P["fun"])
+                    for method, ns in ((1, 10), (2, 10), (0, 3)):
+                        print(f"shape={shape} pipe={pipe} ordering={ordering} isym={isym} method={method}",
+                              flush=True)
+                        want = pyoracle.relax_ordered(nni, nmu, isym, win.copy(), *args, ns, method, ordering)
+                        w = ctx.upload(win)
+                        gpu.check(ctx.lib.x2g_relax_ordered(g.h, isym, 1, w.ptr, lhs.ptr, rhs.ptr, 1.8, ns, method,
+                                                            pyoracle.ORDERINGS[ordering]))
+                        got = w.get()
+                        w.free()
+                        assert np.array_equal(got, want), (ordering, isym, method, relerr(got, want))
+    finally:
+        gpu.check(ctx.lib.x2g_set_levels_pipeline(1))
     g.destroy()
 
 

@@ -454,6 +454,8 @@ extern "C" int x2g_grid_destroy(x2g_grid *g)
     for (auto &kv : g->levels) {
         if (kv.second.off) x2g_free(c, kv.second.off);
         if (kv.second.pts) x2g_free(c, kv.second.pts);
+        if (kv.second.poff) x2g_free(c, kv.second.poff);
+        if (kv.second.pops) x2g_free(c, kv.second.pops);
     }
     delete g;
     return X2G_OK;

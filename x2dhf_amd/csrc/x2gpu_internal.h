@@ -97,11 +97,16 @@ struct x2g_ctx {
 
 // Level schedule of an ordered (MC)SOR sweep on one grid (device arrays):
 // phases off[k] .. off[k+1]-1 of pts (unpadded point indices) are mutually
-// independent and run in order.
+// independent and run in order.  The pipelined form: one sweep's point
+// updates and boundary extrapolations on one level scale (levels poff[l] ..
+// poff[l+1]-1 of pops), consecutive sweeps run lag levels apart.
 struct x2g_levels {
     int *off = nullptr;
     int *pts = nullptr;
     int nphase = 0;
+    int *poff = nullptr;
+    int *pops = nullptr;
+    int nlev = 0, lag = 0;
 };
 
 struct x2g_grid {

@@ -1600,6 +1600,17 @@ static int g_autotune = -1;  // -1: from X2G_AUTOTUNE (default on)
 // bumped by every setting above (a captured relaxation graph is only
 // replayed under the settings it was captured with)
 static std::atomic<long> g_plan_gen{0};
+// ordered sweeps: 1 = the pipelined schedule (k_levels_pipe), 0 = one phase
+// of one sweep at a time (k_levels; X2G_LEVELS=phase); -1 = not read yet
+static int g_levels_pipe = -1;
+static bool levels_pipe()
+{
+    if (g_levels_pipe < 0) {
+        const char *e = getenv("X2G_LEVELS");
+        g_levels_pipe = !(e && std::strcmp(e, "phase") == 0);
+    }
+    return g_levels_pipe != 0;
+}
 static void engine_defaults()
 {
     if (g_engine >= 0) return;
@@ -2357,6 +2368,86 @@ static void build_levels(int nni, int nmu, int method, int ordering, std::vector
     }
 }
 
+// The pipelined schedule of the same sweep.  Every operation of one sweep --
+// the point updates in the order the sequential routine makes them, then the
+// boundary extrapolations of sormcsor.F90:94-129 (the mu = 1 row, then the
+// nu = 1 and nu = nni columns) -- gets a level one above every earlier
+// operation it conflicts with (one reads what the other writes): the
+// updates of the four nearest points each way along nu and mu, and for an
+// extrapolation the five points it reads and the points that read the value
+// it writes.  Sweep s then runs level l at step l + s*lag.  Two conflicting
+// operations u, v of consecutive sweeps stay in order when
+// lag > |level(u) - level(v)|; their locations lie at most five points apart
+// along nu or mu, so lag = 1 + the largest level difference of two written
+// locations that close (an upper bound of the conflicting pairs' maximum).
+// Within a step no two operations conflict, and every operation reads the
+// values the sequential order gives it: the result is the sequential one.
+// Codes in ops: an unpadded point index ik >= 0 for a point update, else
+// -1 - (kind << 16 | index) for an extrapolation: kind 0 the mu = 1 row at
+// nu = index, kind 1 / 2 the nu = 1 / nu = nni column at mu = index.
+static void build_pipe(int nni, int nmu, int method, int ordering, std::vector<int> &off, std::vector<int> &ops,
+                       int &lag)
+{
+    std::vector<int> seq;
+    mesh_sequence(nni, nmu, ordering, seq);
+    if (method == X2G_SOR_BACKWARD) std::reverse(seq.begin(), seq.end());
+    const int n = (int)seq.size(), npass = method == X2G_MCSOR ? 5 : 1;
+    // level of the operation writing (nu, mu) in this sweep, -1: not written
+    std::vector<int> lv((size_t)(nni + 2) * (nmu + 2), -1);
+    auto at = [&](int nu, int mu) -> int & { return lv[(size_t)mu * (nni + 2) + nu]; };
+    std::vector<std::pair<int, int>> all;  // (level, code)
+    all.reserve((size_t)n + nni + 2 * nmu);
+    for (int c = 0; c < npass; ++c)
+        for (int k = c; k < n; k += npass) {
+            const int ik = seq[k], mu = ik / nni + 1, nu = ik % nni + 1;
+            int l = 0;
+            for (int j = 1; j <= 4; ++j) {
+                if (nu - j >= 1) l = std::max(l, at(nu - j, mu) + 1);
+                if (nu + j <= nni) l = std::max(l, at(nu + j, mu) + 1);
+                if (mu - j >= 1) l = std::max(l, at(nu, mu - j) + 1);
+                if (mu + j <= nmu) l = std::max(l, at(nu, mu + j) + 1);
+            }
+            at(nu, mu) = l;
+            all.push_back({l, ik});
+        }
+    // an extrapolation reads its five points; the points that read its value
+    // are among them
+    for (int nu = 2; nu <= nni - 1; ++nu) {
+        int l = 0;
+        for (int m = 2; m <= 6; ++m) l = std::max(l, at(nu, m) + 1);
+        at(nu, 1) = l;
+        all.push_back({l, -1 - nu});
+    }
+    for (int side = 1; side <= 2; ++side)
+        for (int mu = 1; mu <= nmu - 4; ++mu) {
+            const int nb = side == 1 ? 1 : nni, dir = side == 1 ? 1 : -1;
+            int l = 0;
+            for (int k = 1; k <= 5; ++k) l = std::max(l, at(nb + k * dir, mu) + 1);
+            at(nb, mu) = l;
+            all.push_back({l, -1 - ((side << 16) | mu)});
+        }
+    int D = 0, maxl = 0;
+    for (int mu = 1; mu <= nmu; ++mu)
+        for (int nu = 1; nu <= nni; ++nu) {
+            const int a = at(nu, mu);
+            if (a < 0) continue;
+            maxl = std::max(maxl, a);
+            for (int k = 1; k <= 5; ++k) {
+                if (nu + k <= nni && at(nu + k, mu) >= 0) D = std::max(D, std::abs(a - at(nu + k, mu)));
+                if (mu + k <= nmu && at(nu, mu + k) >= 0) D = std::max(D, std::abs(a - at(nu, mu + k)));
+            }
+        }
+    lag = D + 1;
+    // by level, in the order made above within a level
+    std::vector<int> cnt(maxl + 2, 0);
+    for (auto &o : all) ++cnt[o.first + 1];
+    for (int l = 0; l <= maxl; ++l) cnt[l + 1] += cnt[l];
+    off.assign(cnt.begin(), cnt.end());
+    ops.assign(all.size(), 0);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (auto &o : all) ops[fill[o.first]++] = o.second;
+}
+
 // The level schedule of (method, ordering) on grid g, built and uploaded on
 // first use.
 static int grid_levels(x2g_grid *g, int method, int ordering, const x2g_levels **out)
@@ -2377,6 +2468,18 @@ static int grid_levels(x2g_grid *g, int method, int ordering, const x2g_levels *
             return rc;
         }
         L.nphase = (int)off.size() - 1;
+        std::vector<int> poff, pops;
+        build_pipe(g->nni, g->nmu, method, ordering, poff, pops, L.lag);
+        rc = x2g_malloc(g->ctx, poff.size() * sizeof(int), (void **)&L.poff);
+        if (!rc) rc = x2g_malloc(g->ctx, pops.size() * sizeof(int), (void **)&L.pops);
+        if (!rc) rc = x2g_h2d(g->ctx, L.poff, poff.data(), poff.size() * sizeof(int));
+        if (!rc) rc = x2g_h2d(g->ctx, L.pops, pops.data(), pops.size() * sizeof(int));
+        if (rc) {
+            for (int *q : {L.off, L.pts, L.poff, L.pops})
+                if (q) x2g_free(g->ctx, q);
+            return rc;
+        }
+        L.nlev = (int)poff.size() - 1;
         it = g->levels.emplace(key, L).first;
     }
     *out = &it->second;
@@ -2387,9 +2490,10 @@ struct LevelDesc {
     double *f;                 // padded array (row stride nni+8), relaxed in place
     const double *lhs, *rhs, *bmu, *dnu;
     const int *off, *pts;      // level schedule
+    const int *poff, *pops;    // pipelined level schedule
     double c[16], ex[5];
     double omega, omega1;
-    int nni, nmu, isym, nphase;
+    int nni, nmu, isym, nphase, nlev, lag;
 };
 
 constexpr int kLevelThreads = 512;
@@ -2448,6 +2552,67 @@ __global__ void __launch_bounds__(kLevelThreads) k_levels(const LevelDesc *__res
     }
 }
 
+constexpr int kPipeThreads = 1024;
+
+// All nsweeps sweeps of one problem per workgroup on the pipelined schedule:
+// step t runs level t - s*lag of every sweep s that has it, then a barrier.
+// The operations of a step are spread over the threads sweep by sweep.  Point
+// updates and extrapolations are the expressions of k_levels (no fused
+// multiply-add), so the result is the sequential routine's bit for bit.
+__global__ void __launch_bounds__(kPipeThreads) k_levels_pipe(const LevelDesc *__restrict__ ld, int nsweeps)
+{
+#pragma clang fp contract(off)
+    const LevelDesc &d = ld[blockIdx.x];
+    const int nni = d.nni, s1 = nni + 8, nlev = d.nlev, lag = d.lag;
+    double *f = d.f;
+    const double *c = d.c, *ex = d.ex;
+    const int *poff = d.poff, *pops = d.pops;
+    const bool even = d.isym == 1;
+    const int nstep = nlev + (nsweeps - 1) * lag;
+    for (int t = 0; t < nstep; ++t) {
+        // sweeps s with 0 <= t - s*lag <= nlev-1
+        const int shi = min(nsweeps - 1, t / lag);
+        const int slo = t < nlev ? 0 : (t - nlev) / lag + 1;
+        int total = 0;
+        for (int s = slo; s <= shi; ++s) total += poff[t - s * lag + 1] - poff[t - s * lag];
+        for (int k = threadIdx.x; k < total; k += blockDim.x) {
+            int r = k, l = t - slo * lag, w = poff[l + 1] - poff[l];
+            while (r >= w) {
+                r -= w;
+                l -= lag;
+                w = poff[l + 1] - poff[l];
+            }
+            const int op = pops[poff[l] + r];
+            if (op >= 0) {
+                const int ik = op;
+                const int mu = ik / nni + 1, nu = ik - (mu - 1) * nni + 1;
+                const int p = (mu + 3) * s1 + nu + 3;
+                const int q1 = s1, q2 = 2 * s1, q3 = 3 * s1, q4 = 4 * s1;
+                const double ddmi2 = c[4] * (f[p - q4] + f[p + q4]) + c[5] * (f[p - q3] + f[p + q3]) +
+                                     c[6] * (f[p - q2] + f[p + q2]) + c[7] * (f[p - q1] + f[p + q1]);
+                const double ddmi1 = c[0] * (f[p - q4] - f[p + q4]) + c[1] * (f[p - q3] - f[p + q3]) +
+                                     c[2] * (f[p - q2] - f[p + q2]) + c[3] * (f[p - q1] - f[p + q1]);
+                const double ddni2 = c[12] * (f[p - 4] + f[p + 4]) + c[13] * (f[p - 3] + f[p + 3]) +
+                                     c[14] * (f[p - 2] + f[p + 2]) + c[15] * (f[p - 1] + f[p + 1]);
+                const double ddni1 = c[8] * (f[p - 4] - f[p + 4]) + c[9] * (f[p - 3] - f[p + 3]) +
+                                     c[10] * (f[p - 2] - f[p + 2]) + c[11] * (f[p - 1] - f[p + 1]);
+                const double cc = ddmi2 + d.bmu[mu - 1] * ddmi1 + ddni2 + d.dnu[nu - 1] * ddni1;
+                f[p] = d.omega * (d.rhs[ik] - cc) / d.lhs[ik] + d.omega1 * f[p];
+            } else {
+                const int b = -1 - op, kind = b >> 16, idx = b & 0xffff;
+                // kind 0: (nu = idx, mu = 1) from mu = 2..6; kind 1 / 2:
+                // (nu = 1 / nni, mu = idx) from nu = 2..6 / nni-1..nni-5
+                const int p = kind == 0 ? 4 * s1 + idx + 3 : (idx + 3) * s1 + (kind == 1 ? 1 : nni) + 3;
+                const int st = kind == 0 ? s1 : (kind == 1 ? 1 : -1);
+                f[p] = even ? ex[0] * f[p + st] + ex[1] * f[p + 2 * st] + ex[2] * f[p + 3 * st] +
+                                  ex[3] * f[p + 4 * st] + ex[4] * f[p + 5 * st]
+                            : 0.0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 static void fill_level_desc(LevelDesc &d, x2g_grid *g, const x2g_levels *L, double *f, const double *lhs,
                             const double *rhs, int bkind, int isym, double omega)
 {
@@ -2466,6 +2631,10 @@ static void fill_level_desc(LevelDesc &d, x2g_grid *g, const x2g_levels *L, doub
     d.nmu = g->nmu;
     d.isym = isym;
     d.nphase = L->nphase;
+    d.poff = L->poff;
+    d.pops = L->pops;
+    d.nlev = L->nlev;
+    d.lag = L->lag;
 }
 
 // nsweeps ordered sweeps of every problem in v (one workgroup each).
@@ -2480,7 +2649,10 @@ static int launch_levels(x2g_ctx *ctx, const std::vector<LevelDesc> &v, int nswe
     for (auto &d : v) bytes += 32.0 * (double)(d.nni - 2) * (double)(d.nmu - 5) * nsweeps;
     hipEvent_t ea, eb;
     if ((rc = stats_begin(ctx, &ea, &eb))) return rc;
-    X2G_LAUNCH(ctx, k_levels, dim3((int)v.size()), dim3(kLevelThreads), 0, dd, nsweeps);
+    if (levels_pipe())
+        X2G_LAUNCH(ctx, k_levels_pipe, dim3((int)v.size()), dim3(kPipeThreads), 0, dd, nsweeps);
+    else
+        X2G_LAUNCH(ctx, k_levels, dim3((int)v.size()), dim3(kLevelThreads), 0, dd, nsweeps);
     X2G_HIP(hipGetLastError());
     return stats_end(ctx, ea, eb, bytes, 0, nsweeps);
 }
@@ -2536,6 +2708,13 @@ extern "C" int x2g_tune_report(char *buf, int len)
     }
     snprintf(buf, (size_t)len, "%s", out.c_str());
     return (int)out.size();
+}
+
+extern "C" int x2g_set_levels_pipeline(int on)
+{
+    g_levels_pipe = on != 0;
+    ++g_plan_gen;
+    return X2G_OK;
 }
 
 extern "C" int x2g_set_sweep_engine(int engine, int strip_rows)

@@ -119,6 +119,7 @@ _PROTOS = {
     "x2g_guess_exchange": (_int, [_vp, _vp, _dbl, _dbl]),
     "x2g_grid_suppl": (_vp, [_vp, _int]),
     "x2g_set_sweep_engine": (_int, [_int, _int]),
+    "x2g_set_levels_pipeline": (_int, [_int]),
     "x2g_putin": (_int, [_vp, _int, _vp, _vp]),
     "x2g_putout": (_int, [_vp, _vp, _vp]),
     "x2g_mcsor": (_int, [_vp, _int, _int, _vp, _vp, _vp, _dbl, _int]),
