@@ -2553,63 +2553,113 @@ __global__ void __launch_bounds__(kLevelThreads) k_levels(const LevelDesc *__res
 }
 
 constexpr int kPipeThreads = 1024;
+constexpr int kPipeChunk = 64;  // sweeps in flight at most: one table entry per lane of a wave
 
 // All nsweeps sweeps of one problem per workgroup on the pipelined schedule:
-// step t runs level t - s*lag of every sweep s that has it, then a barrier.
-// The operations of a step are spread over the threads sweep by sweep.  Point
-// updates and extrapolations are the expressions of k_levels (no fused
-// multiply-add), so the result is the sequential routine's bit for bit.
+// step t runs level t - s*lag of every sweep s that has it, then a barrier
+// (at most kPipeChunk sweeps form one pipeline; longer runs take several,
+// one after another).  The operations of a step are numbered sweep by sweep
+// and spread over the threads; the first wave builds the next step's table
+// (where each sweep's level starts in pops, and the running count) in LDS
+// while the step runs, so a thread finds its operations without walking the
+// level offsets.  A thread takes its operations two at a time: both read
+// everything they need before either writes (no two operations of a step
+// conflict), so their loads are in flight together.  Point updates and
+// extrapolations are the expressions of k_levels (no fused multiply-add):
+// the result is the sequential routine's bit for bit.
 __global__ void __launch_bounds__(kPipeThreads) k_levels_pipe(const LevelDesc *__restrict__ ld, int nsweeps)
 {
 #pragma clang fp contract(off)
+    __shared__ int tb_base[2][kPipeChunk], tb_cum[2][kPipeChunk + 1], tb_n[2];
     const LevelDesc &d = ld[blockIdx.x];
     const int nni = d.nni, s1 = nni + 8, nlev = d.nlev, lag = d.lag;
     double *f = d.f;
     const double *c = d.c, *ex = d.ex;
     const int *poff = d.poff, *pops = d.pops;
     const bool even = d.isym == 1;
-    const int nstep = nlev + (nsweeps - 1) * lag;
-    for (int t = 0; t < nstep; ++t) {
-        // sweeps s with 0 <= t - s*lag <= nlev-1
-        const int shi = min(nsweeps - 1, t / lag);
-        const int slo = t < nlev ? 0 : (t - nlev) / lag + 1;
-        int total = 0;
-        for (int s = slo; s <= shi; ++s) total += poff[t - s * lag + 1] - poff[t - s * lag];
-        for (int k = threadIdx.x; k < total; k += blockDim.x) {
-            int r = k, l = t - slo * lag, w = poff[l + 1] - poff[l];
-            while (r >= w) {
-                r -= w;
-                l -= lag;
-                w = poff[l + 1] - poff[l];
-            }
-            const int op = pops[poff[l] + r];
-            if (op >= 0) {
-                const int ik = op;
-                const int mu = ik / nni + 1, nu = ik - (mu - 1) * nni + 1;
-                const int p = (mu + 3) * s1 + nu + 3;
-                const int q1 = s1, q2 = 2 * s1, q3 = 3 * s1, q4 = 4 * s1;
-                const double ddmi2 = c[4] * (f[p - q4] + f[p + q4]) + c[5] * (f[p - q3] + f[p + q3]) +
-                                     c[6] * (f[p - q2] + f[p + q2]) + c[7] * (f[p - q1] + f[p + q1]);
-                const double ddmi1 = c[0] * (f[p - q4] - f[p + q4]) + c[1] * (f[p - q3] - f[p + q3]) +
-                                     c[2] * (f[p - q2] - f[p + q2]) + c[3] * (f[p - q1] - f[p + q1]);
-                const double ddni2 = c[12] * (f[p - 4] + f[p + 4]) + c[13] * (f[p - 3] + f[p + 3]) +
-                                     c[14] * (f[p - 2] + f[p + 2]) + c[15] * (f[p - 1] + f[p + 1]);
-                const double ddni1 = c[8] * (f[p - 4] - f[p + 4]) + c[9] * (f[p - 3] - f[p + 3]) +
-                                     c[10] * (f[p - 2] - f[p + 2]) + c[11] * (f[p - 1] - f[p + 1]);
-                const double cc = ddmi2 + d.bmu[mu - 1] * ddmi1 + ddni2 + d.dnu[nu - 1] * ddni1;
-                f[p] = d.omega * (d.rhs[ik] - cc) / d.lhs[ik] + d.omega1 * f[p];
-            } else {
-                const int b = -1 - op, kind = b >> 16, idx = b & 0xffff;
-                // kind 0: (nu = idx, mu = 1) from mu = 2..6; kind 1 / 2:
-                // (nu = 1 / nni, mu = idx) from nu = 2..6 / nni-1..nni-5
-                const int p = kind == 0 ? 4 * s1 + idx + 3 : (idx + 3) * s1 + (kind == 1 ? 1 : nni) + 3;
-                const int st = kind == 0 ? s1 : (kind == 1 ? 1 : -1);
-                f[p] = even ? ex[0] * f[p + st] + ex[1] * f[p + 2 * st] + ex[2] * f[p + 3 * st] +
-                                  ex[3] * f[p + 4 * st] + ex[4] * f[p + 5 * st]
-                            : 0.0;
-            }
+    const int lane = threadIdx.x & 63;
+    const bool wave0 = threadIdx.x < 64;
+
+    // table of step u of a pipeline of ns sweeps into buffer bf (first wave)
+    auto build_table = [&](int u, int ns, int bf) {
+        const int shi = min(ns - 1, u / lag);
+        const int slo = u < nlev ? 0 : (u - nlev) / lag + 1;
+        const int na = shi - slo + 1;
+        int base = 0, w = 0;
+        if (lane < na) {
+            const int l = u - (slo + lane) * lag;
+            base = poff[l];
+            w = poff[l + 1] - base;
         }
+        int x = w;  // inclusive running count over the lanes
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane < na) {
+            tb_base[bf][lane] = base;
+            tb_cum[bf][lane + 1] = x;
+        }
+        if (lane == 0) tb_cum[bf][0] = 0;
+        if (lane == 63) tb_n[bf] = x;  // lanes past na add nothing
+    };
+    // value of operation `code` and the padded index it writes
+    auto eval = [&](int code, int &pw) -> double {
+        if (code >= 0) {
+            const int ik = code;
+            const int mu = ik / nni + 1, nu = ik - (mu - 1) * nni + 1;
+            const int p = (mu + 3) * s1 + nu + 3;
+            const int q1 = s1, q2 = 2 * s1, q3 = 3 * s1, q4 = 4 * s1;
+            const double ddmi2 = c[4] * (f[p - q4] + f[p + q4]) + c[5] * (f[p - q3] + f[p + q3]) +
+                                 c[6] * (f[p - q2] + f[p + q2]) + c[7] * (f[p - q1] + f[p + q1]);
+            const double ddmi1 = c[0] * (f[p - q4] - f[p + q4]) + c[1] * (f[p - q3] - f[p + q3]) +
+                                 c[2] * (f[p - q2] - f[p + q2]) + c[3] * (f[p - q1] - f[p + q1]);
+            const double ddni2 = c[12] * (f[p - 4] + f[p + 4]) + c[13] * (f[p - 3] + f[p + 3]) +
+                                 c[14] * (f[p - 2] + f[p + 2]) + c[15] * (f[p - 1] + f[p + 1]);
+            const double ddni1 = c[8] * (f[p - 4] - f[p + 4]) + c[9] * (f[p - 3] - f[p + 3]) +
+                                 c[10] * (f[p - 2] - f[p + 2]) + c[11] * (f[p - 1] - f[p + 1]);
+            const double cc = ddmi2 + d.bmu[mu - 1] * ddmi1 + ddni2 + d.dnu[nu - 1] * ddni1;
+            pw = p;
+            return d.omega * (d.rhs[ik] - cc) / d.lhs[ik] + d.omega1 * f[p];
+        }
+        // kind 0: (nu = idx, mu = 1) from mu = 2..6; kind 1 / 2: (nu = 1 /
+        // nni, mu = idx) from nu = 2..6 / nni-1..nni-5
+        const int b = -1 - code, kind = b >> 16, idx = b & 0xffff;
+        const int p = kind == 0 ? 4 * s1 + idx + 3 : (idx + 3) * s1 + (kind == 1 ? 1 : nni) + 3;
+        const int st = kind == 0 ? s1 : (kind == 1 ? 1 : -1);
+        pw = p;
+        return even ? ex[0] * f[p + st] + ex[1] * f[p + 2 * st] + ex[2] * f[p + 3 * st] + ex[3] * f[p + 4 * st] +
+                          ex[4] * f[p + 5 * st]
+                    : 0.0;
+    };
+
+    for (int s0 = 0; s0 < nsweeps; s0 += kPipeChunk) {
+        const int ns = min(kPipeChunk, nsweeps - s0);
+        const int nstep = nlev + (ns - 1) * lag;
+        if (wave0) build_table(0, ns, 0);
         __syncthreads();
+        for (int t = 0; t < nstep; ++t) {
+            const int bf = t & 1;
+            if (wave0 && t + 1 < nstep) build_table(t + 1, ns, bf ^ 1);
+            const int total = tb_n[bf];
+            const int na = min(ns, t / lag + 1) - (t < nlev ? 0 : (t - nlev) / lag + 1);
+            auto code_of = [&](int k) {
+                int j = 0;
+                for (int i = 1; i < na; ++i) j += k >= tb_cum[bf][i];
+                return pops[tb_base[bf][j] + k - tb_cum[bf][j]];
+            };
+            for (int k = threadIdx.x; k < total; k += 2 * blockDim.x) {
+                const int k2 = k + blockDim.x;
+                const bool two = k2 < total;
+                const int ca = code_of(k), cb = two ? code_of(k2) : ca;
+                int pa, pb;
+                const double va = eval(ca, pa);
+                const double vb = two ? eval(cb, pb) : 0.0;
+                f[pa] = va;
+                if (two) f[pb] = vb;
+            }
+            __syncthreads();
+        }
     }
 }
 
