@@ -71,10 +71,12 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
     ap.add_argument("--cpu-iters", default="61,81", help="SCF iterations of the two timed reference runs")
-    ap.add_argument("--workload", choices=("stream", "n2", "poisson"), default="stream",
+    ap.add_argument("--workload", choices=("stream", "n2", "poisson", "ordered"), default="stream",
                     help="stream: Poisson-arrival SCF requests of closed-shell diatomics (the headline line); "
                          "n2: waves of the N2 331x601 deck; poisson: one batch of seeded synthetic Poisson "
-                         "problems on mixed grids")
+                         "problems on mixed grids; ordered: the reference's default relaxation (10 SOR "
+                         "sweeps, middle ordering) of one 331x601 potential, pipelined against one level "
+                         "per barrier")
     ap.add_argument("--count", type=int, default=None,
                     help="problems (poisson, default 256) or requests per stream (stream, default 64)")
     ap.add_argument("--rate", type=float, default=20.0, help="stream: mean arrivals per second")
@@ -318,6 +320,56 @@ def sweep_roofline(st, prec):
             "kernel": "k_sweep<float>" if k else "k_sweep<double>",
             "algorithmic_bytes_per_launch": nbytes / launches if launches else None,
             "avg_launch_us": ms / launches * 1e3 if launches else None, "launches": launches}
+
+
+def bench_ordered(args):
+    """The reference's default relaxation on one problem: 10 SOR sweeps in the
+    middle ordering (x2g_relax_ordered) of a Coulomb-type potential equation
+    (lhs = f3 + diag) on the 331x601 grid of the He deck test-sets/he/set-03
+    (Z = 2, R = 2 bohr, R_inf 200), with the sweeps pipelined (the default)
+    and with one dependency level of one sweep per barrier; the two results
+    must be identical.  Reported: sweeps per second of the pipelined schedule."""
+    import numpy as np
+    from x2dhf_amd import gpu, scf
+    steps, warm = max(args.steps, 1) * 20, max(args.warmup, 1) * 3
+    ctx = gpu.Context(0)
+    coef, suppl, vxi, veta = scf.grid_build(331, 601, 2.0, 0.0, 2.0, 200.0)
+    g = gpu.Grid(ctx, 331, 601, coef, suppl, vxi, veta)
+    rng = np.random.default_rng(args.seed)
+    X, Y = np.meshgrid(np.arange(331), np.arange(601))
+    rhs = (np.exp(-((X - 160.0) ** 2 + (Y - 40.0) ** 2) / 800.0) + 1e-3 * rng.standard_normal(X.shape)).ravel()
+    lhs_d, rhs_d = ctx.upload(suppl[7]), ctx.upload(rhs)
+    work = ctx.alloc(g.npad)
+    fun = ctx.upload(np.zeros(g.n))
+    res, outs = {}, {}
+    for pipe in (1, 0):
+        gpu.check(ctx.lib.x2g_set_levels_pipeline(pipe))
+        gpu.check(ctx.lib.x2g_putin(g.h, 1, fun.ptr, work.ptr))
+        for _ in range(warm):
+            gpu.check(ctx.lib.x2g_relax_ordered(g.h, 1, 1, work.ptr, lhs_d.ptr, rhs_d.ptr, 1.95, 10, 1, 1))
+        gpu.check(ctx.lib.x2g_ctx_sync(ctx.h))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            gpu.check(ctx.lib.x2g_relax_ordered(g.h, 1, 1, work.ptr, lhs_d.ptr, rhs_d.ptr, 1.95, 10, 1, 1))
+        gpu.check(ctx.lib.x2g_ctx_sync(ctx.h))
+        res[pipe] = (time.perf_counter() - t0) / steps
+        outs[pipe] = work.get()
+    gpu.check(ctx.lib.x2g_set_levels_pipeline(1))
+    identical = bool(np.array_equal(outs[0], outs[1]))
+    line = {
+        "metric": "ordered SOR sweeps/s (one 331x601 problem, middle ordering, bit-exact sequential order)",
+        "value": 10.0 / res[1], "unit": "sweeps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": res[1] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": f"synthetic rhs (seed {args.seed}) on the he/set-03 grid",
+        "config": {"workload": "10 SOR sweeps per call, middle ordering, 331x601, omega 1.95, even symmetry"},
+        "per_level_barrier_ms": res[0] * 1e3, "speedup_vs_per_level": res[0] / res[1],
+        "results_identical": identical,
+    }
+    g.destroy()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    if not identical:
+        sys.exit(1)
 
 
 def bench_poisson(args):
@@ -641,6 +693,9 @@ def main():
         return
     if args.workload == "stream":
         bench_stream(args)
+        return
+    if args.workload == "ordered":
+        bench_ordered(args)
         return
     import ctypes as C
     from x2dhf_amd import gpu, serve
