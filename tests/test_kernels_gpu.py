@@ -631,6 +631,28 @@ def test_pipelined_ordered_sweeps_vs_oracle(ctx, shape):
     g.destroy()
 
 
+def test_pipelined_ordered_sweeps_longer_than_one_pipeline(ctx):
+    """More sweeps than one pipeline holds (64): the kernel runs 64, drains,
+    then the remaining 6 -- still the oracle's sequential sor bit for bit
+    (middle ordering, forward and backward, 91x121)."""
+    from x2dhf_amd import gpu
+    nni, nmu = 91, 121
+    P = synthetic_problem(nni, nmu, seed=11)
+    g = gpu.Grid(ctx, nni, nmu, P["coef"], P["suppl"], P["vxi"], P["veta"])
+    lhs, rhs = ctx.upload(P["lhs"]), ctx.upload(P["rhs"])
+    args = (P["lhs"], P["rhs"], P["suppl"][1], P["suppl"][2], P["coef"][:16], P["coef"][16:21], 1.7)
+    gpu.check(ctx.lib.x2g_set_levels_pipeline(1))
+    win = pyoracle.putin(nni, nmu, 1, P["fun"])
+    for method in (1, 2):
+        want = pyoracle.relax_ordered(nni, nmu, 1, win.copy(), *args, 70, method, "middle")
+        w = ctx.upload(win)
+        gpu.check(ctx.lib.x2g_relax_ordered(g.h, 1, 1, w.ptr, lhs.ptr, rhs.ptr, 1.7, 70, method, 1))
+        got = w.get()
+        w.free()
+        assert np.array_equal(got, want), (method, relerr(got, want))
+    g.destroy()
+
+
 def test_ordered_potential_batch_vs_oracle(ctx):
     """All potentials of the N2 pi orbital in one x2g_relax_batch call with
     SOR in the middle ordering (each a workgroup of the level kernel; every
