@@ -5,8 +5,9 @@ own sources by oracle/ref.mk) on the decks of tests/golden/decks -- all but
 one relax orbitals and potentials with MCSOR in the col-wise order, the
 algorithm of the pipelined GPU sweep; altsweeps_fh_91x121 uses the
 reference's default SOR in the middle ordering with its altsweeps option
-(backward sweeps for odd orbitals), and ne2_set03_181x151 is the reference's
-own Ne2 deck (test-sets/ne2/set-03) -- and keeps, from its listing:
+(backward sweeps for odd orbitals), and ne2_set03_181x151 and gaf_set01 are
+the reference's own Ne2 and GaF decks (test-sets/ne2/set-03, gaf/set-01) --
+and keeps, from its listing:
   * the orbital energy of every orbital after every SCF iteration,
   * the final total energy and its components, orbital energies and the
     number of (MC)SOR sweeps.
@@ -40,6 +41,7 @@ CASES = {
     "config3_fh_481x901": "config3_fh_481x901.data",
     "altsweeps_fh_91x121": "altsweeps_fh_91x121.data",
     "ne2_set03_181x151": "ref_ne2_set-03.data",
+    "gaf_set01": "ref_gaf_set-01.data",
 }
 
 NUM = r"(-?\d\.\d+E[+-]\d+)"

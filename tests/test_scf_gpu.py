@@ -243,6 +243,35 @@ def test_ne2_deck_matches_reference_program():
     assert np.abs(res["eps"] - np.array(ref["eps"])).max() < 1e-9
 
 
+def test_gaf_deck_matches_reference_program():
+    """The reference's GaF deck (test-sets/gaf/set-01): fifteen orbitals (ten
+    sigma, four pi, one delta) around a Z = 31 nucleus, hydrogenic start with
+    frozen orbitals until the energies settle to 1e-2 (orbpot hydrogen 2),
+    SOR in the middle ordering, stopped at the 1e-4 orbital-energy
+    threshold.  The GPU path follows the reference program's run (built from
+    its sources; tests/golden/scf_gaf_set01.json) over the first 40
+    iterations within 1e-9 relative and stops at the same iteration with
+    the same orbital energies (1e-9 Eh).  The total energy (-2022.83 Eh, the
+    listing's to the last digit) is bounded at 1e-8 Eh, 5e-12 relative: its
+    nuclear-attraction and kinetic parts are quadratures of some 5000 Eh
+    over 90k points each, summed in another order than the reference's
+    sequential sums (measured 2.7e-9 Eh)."""
+    from x2dhf_amd import deck
+    ref = golden("gaf_set01")
+    d = deck.load(os.path.join(GOLDEN, "decks", "ref_gaf_set-01.data"))
+    assert len(d.orbitals) == 15 and d.ienterm4init == 2
+    print(f"gaf_set-01: {d.nni}x{d.nmu}, {d.notes[-1]}", flush=True)
+    res, worst = run_plain(d.spec(prec=0, schedule=0), ref, check_iters=40)
+    print(f"gaf_set-01: E = {res['etot']:.13f} (ref {ref['etot']:.13f}) dE = {res['etot'] - ref['etot']:.2e}, "
+          f"iterations {res['iterations']} (ref {len(ref['iterations'])}), trajectory rel err {worst:.2e}, "
+          f"stop '{res['stop']}', max |d eps| {np.abs(res['eps'] - np.array(ref['eps'])).max():.2e}", flush=True)
+    assert worst < 1e-9
+    assert res["iterations"] == len(ref["iterations"])
+    assert res["stop"] == "orbital energy threshold"
+    assert abs(res["etot"] - ref["etot"]) < 1e-8
+    assert np.abs(res["eps"] - np.array(ref["eps"])).max() < 1e-9
+
+
 def test_altsweeps_deck_matches_reference_trajectory_and_energy():
     """FH 91x121 with the reference's default schemes (SOR for orbitals and
     potentials, middle ordering) and its altsweeps card: the SOR of the odd
