@@ -253,9 +253,9 @@ def test_gaf_deck_matches_reference_program():
     iterations within 1e-9 relative and stops at the same iteration with
     the same orbital energies (1e-9 Eh).  The total energy (-2022.83 Eh, the
     listing's to the last digit) is bounded at 1e-8 Eh, 5e-12 relative: its
-    nuclear-attraction and kinetic parts are quadratures of some 5000 Eh
-    over 90k points each, summed in another order than the reference's
-    sequential sums (measured 2.7e-9 Eh)."""
+    nuclear-attraction (-5014 Eh) and kinetic (2023 Eh) parts are
+    quadratures over 90k points, summed in another order than the
+    reference's sequential sums (measured 2.7e-9 Eh)."""
     from x2dhf_amd import deck
     ref = golden("gaf_set01")
     d = deck.load(os.path.join(GOLDEN, "decks", "ref_gaf_set-01.data"))
