@@ -192,12 +192,14 @@ def _listing(name):
         return json.load(f)[name]
 
 
-@pytest.mark.parametrize("name", ["h2_set-01", "he_set-01", "he_set-02", "he_set-03", "ne_set-01", "alf_set-02"])
+@pytest.mark.parametrize("name", ["h2_set-01", "he_set-01", "he_set-02", "he_set-03", "ne_set-01", "alf_set-02",
+                                  "lih-elprop_set-01"])
 def test_reference_default_scheme_decks_match_listings(name):
     """The reference's own decks that relax with its default SOR (and, for
     he/set-02, one-thread MCSOR for the orbital) in the default middle
     ordering (H2, He, He on a 331x601 grid over 932 iterations, Ne with four
-    orbitals, AlF with nine) run through the
+    orbitals, AlF with nine, LiH started without the frozen-orbital phase and
+    stopped by the resets rule) run through the
     same schemes on the GPU (x2g_relax_ordered's level kernel) and stop at
     the total energy their reference listings print, within 1e-10 Eh."""
     from x2dhf_amd import deck, scf
